@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark: batched fabric evolution of an Eulerian CPO field on one MI355X.
+
+Workload (the headline configuration): P = 2^20 parcels at L = 12 (91 complex
+coefficients), per-parcel traceless velocity gradients ~ N(0, 1e-3),
+DDRX rate Gamma0 ~ LogUniform(1e-4, 1e-2), lattice rotation + regularization
++ DDRX, classic RK4 with dt = 0.5.  One "step" is one RK4 step of the whole
+field; the full configuration runs 200 of them, this script times --steps.
+Inputs are synthetic, generated on the GPU from a fixed seed, and resident
+in HBM before the timed region starts.
+
+Prints one JSON line (metric parcel-steps/s) with a roofline object for the
+stepping launch (FP64 vector-ALU bound) and the reference CPU path timed on
+the host's cores on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "parcel-steps/s at L=12 + Eij evals/s, plain vs learned at ≤1e-3 rel err"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def flops_per_parcel_step(L=12, stages=4, ddrx=True):
+    """Algorithmic flops of one RK4 parcel-step in the real basis:
+    stages * (2 * T_r + 4 n + 300 [<D>]) + 10 n (RK combination).
+    T_r = coupling nonzeros of LROT + DDRX: 8,581 at L = 12."""
+    T_r = {4: 679, 8: 3467, 12: 8581, 20: 25773}[L]
+    n = (L + 1) * (L + 2) // 2
+    return stages * (2 * T_r + 4 * n + (300 if ddrx else 0)) + 10 * n
+
+
+def make_field(P, L, seed, device):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    n = (L + 1) * (L + 2) // 2
+    G = torch.randn((P, 3, 3), generator=g, device=device, dtype=torch.float64) * 1e-3
+    tr = G.diagonal(dim1=1, dim2=2).sum(-1)
+    G -= (tr / 3)[:, None, None] * torch.eye(3, device=device, dtype=torch.float64)
+    gam0 = 10.0 ** (torch.rand(P, generator=g, device=device, dtype=torch.float64) * 2 - 4)
+    x = torch.randn((n, P), generator=g, device=device, dtype=torch.float64) * 1e-3
+    x[0] = 1 / math.sqrt(4 * math.pi)
+    return G.contiguous(), gam0.contiguous(), x.contiguous()
+
+
+def cpu_worker(args):
+    """Reference Fortran (or C oracle) on one host core: RK4 of a slice."""
+    kind, P, steps, seed = args
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from specfab_amd import workloads as W
+    f = W.eulerian_field(P, L=12, seed=seed, nsteps=steps)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        from oracle.ref import RefLib
+        r = RefLib()
+        r.init(12)
+        r.evolve(f["nlm"], f["ugrad"], f["gam0"], f["lam"], steps, f["dt"], scheme="rk4", procs=f["procs"])
+    else:
+        from oracle.oracle import Oracle
+        Oracle().evolve(12, f["nlm"], f["ugrad"], f["gam0"], f["lam"], steps, f["dt"], "rk4", f["procs"])
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(target_s=10.0):
+    import multiprocessing as mp
+    from oracle import ref as refmod
+    kind = "reference" if refmod.available() else "port"
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    # calibrate on one core: 1 parcel x 4 steps
+    t1 = cpu_worker((kind, 1, 4, 99)) / 4
+    steps = 8
+    per_proc = max(1, int(target_s / (t1 * steps)))   # each core runs ~target_s of work
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(ncores) as pool:
+        pool.map(cpu_worker, [(kind, per_proc, steps, 1000 + i) for i in range(ncores)])
+    wall = time.perf_counter() - t0
+    total = ncores * per_proc * steps
+    return {"value": total / wall, "unit": "parcel-steps/s", "cores": ncores, "kind": kind,
+            "sample": "%d processes x %d parcels x %d RK4 steps at L=12 (LROT+REG+DDRX), %.1f s wall; "
+                      "single-core rate %.1f parcel-steps/s" % (ncores, per_proc, steps, wall, 1.0 / t1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--parcels", type=int, default=1 << 20)
+    ap.add_argument("--L", type=int, default=12)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the parcel sharding across GPUs is described in "
+              "DESIGN.md and not run here (--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from specfab_amd import batch as B
+    from specfab_amd import _lib
+
+    dev = torch.device("cuda:0")
+    P, L = a.parcels, a.L
+    G, gam0, x = make_field(P, L, 1, dev)
+    fb = B.FabricBatch.__new__(B.FabricBatch)
+    fb.ctx = _lib.default_context()
+    fb.L, fb.n, fb.dev, fb.P = L, (L + 1) * (L + 2) // 2, dev, P
+    fb.x, fb.ugrad, fb.tau, fb.gam0, fb.lam, fb.steps = x, G, None, gam0, None, None
+    fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
+    procs = ("LROT", "REG", "DDRX")
+    for _ in range(a.warmup):
+        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        ev[k][0].record(stream)
+        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / a.steps
+    value = P * a.steps / wall
+    nonfinite = int((fb.status != 0).sum().item())
+    flops = flops_per_parcel_step(L) * P
+    achieved = flops / (launch_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "parcel-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+compensated",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": "Eulerian CPO field: %d parcels, L=%d (%d coeffs), LROT+REG+DDRX, RK4 dt=0.5"
+                               % (P, L, fb.n), "parcels": P, "L": L, "scheme": "rk4", "dt": 0.5,
+                   "procs": list(procs), "full_run_steps": 200, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "sfg_evolve (params + stepping) per RK4 step",
+                     "flop_per_parcel_step": flops_per_parcel_step(L), "launch_ms": launch_ms},
+        "nonfinite_parcels": nonfinite,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/*
+ * specfab_gpu.h -- C-ABI of libspecfab_gpu, the MI355X (gfx950) implementation
+ * of specfab's batched fabric-evolution and homogenization path.
+ *
+ * Every entry point returns an int status (SFG_OK = 0, negative on error; the
+ * message is available from sfg_last_error()) instead of the reference's
+ * Fortran `stop`.  The library keeps no global truncation: L is an argument of
+ * every call, and a context (sfg_ctx) owns the device copies of the coupling
+ * tables, so several truncations can be used from one process.
+ *
+ * Conventions
+ *   - nlm: complex double, interleaved (re, im), specfab coefficient order
+ *     (l = 0, 2, ..., L; m = -l..l), length nlm_len(L) = (L+1)(L+2)/2
+ *     (reference: src/header.f90:23-33).  Batched nlm arrays are (P, n),
+ *     parcel-major, i.e. the layout of specfabpy's *_arr arguments.
+ *   - 3x3 tensors are row-major: A[3*i + j] = A_ij.  Batched tensors are
+ *     (P, 3, 3).  (Fortran callers pass transpose(A); see INTEGRATION.md.)
+ *   - n x n operators are row-major: M[i*n + j], so d nlm_i/dt = M_ij nlm_j.
+ *   - Pointers named *_dev are device (HBM) pointers; all others are host.
+ *   - `stream` is a hipStream_t (NULL = the context's stream).  Batched calls
+ *     are asynchronous on that stream; single-parcel calls synchronise.
+ *   - The device state of a batch is the real ("reality-reduced") SoA array
+ *     x[a * P + p], a = 0..n-1: per even l the real DOFs
+ *     [Re n_l0, Re n_l1, Im n_l1, ..., Re n_ll, Im n_ll] of a real ODF
+ *     (n_l^-m = (-1)^m conj(n_l^m)).  sfg_nlm_to_state / sfg_state_to_nlm
+ *     convert.
+ */
+#ifndef SPECFAB_GPU_H
+#define SPECFAB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFG_OK 0
+#define SFG_ERR_ARG (-1)
+#define SFG_ERR_HIP (-2)
+#define SFG_ERR_L (-3)
+#define SFG_ERR_NOMEM (-4)
+
+/* process bitmask for sfg_evolve_desc.procs */
+#define SFG_LROT 1
+#define SFG_DDRX 2
+#define SFG_CDRX 4
+#define SFG_REG 8
+
+/* time schemes */
+#define SFG_EULER 0
+#define SFG_RK4 1
+
+/* precisions */
+#define SFG_FP64 0
+#define SFG_FP32C 1 /* FP32 stepping with compensated (two-sum) state update */
+
+/* per-parcel status bits written by the batched routines */
+#define SFG_ST_NAN 1          /* non-finite state or output */
+#define SFG_ST_TAYLOR_REG 2   /* Taylor solve needed the regularized normal equations */
+#define SFG_ST_TAYLOR_FAIL 4  /* ... and that failed too (reference: stop) */
+#define SFG_ST_REUSS_SING 8   /* singular 9x9 Reuss matrix (reference: stop) */
+#define SFG_ST_ENVELOPE 16    /* outside the surrogate's training envelope */
+#define SFG_ST_AUDIT 32       /* drawn for the surrogate's random audit */
+
+typedef struct sfg_ctx sfg_ctx;
+
+/* Create / destroy a context on HIP device `device` (replaces initspecfab(L),
+ * src/specfab.f90:37-53; the reference's init(L) of specfabpy,
+ * src/specfabpy.f90:150-160). */
+int sfg_ctx_create(int device, sfg_ctx** ctx);
+int sfg_ctx_destroy(sfg_ctx* ctx);
+const char* sfg_last_error(void);
+
+/* nlm_len (src/header.f90:23) and the (l,m) list lm(2, n) (src/header.f90:30):
+ * lm[2*j] = l, lm[2*j+1] = m. */
+int sfg_nlm_len(int L);
+int sfg_lm(int L, int* lm);
+
+/* ---- single-parcel dense operators (host buffers) ----
+ * Drop-ins for the reference operator builders; M is complex (n x n) or real
+ * (n x n) row-major.  Computed on the GPU from the coupling tables. */
+/* M_LROT(eps, omg, iota, zeta): src/dynamics.f90:52-97, specfabpy.f90:172-180 */
+int sfg_M_LROT(sfg_ctx* ctx, int L, const double eps[9], const double omg[9], double iota, double zeta,
+               double* M);
+/* M_DDRX(nlm, tau): src/dynamics.f90:251-275, specfabpy.f90:182-190 */
+int sfg_M_DDRX(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* M);
+/* M_DDRX_src(tau): src/dynamics.f90:277-298, specfabpy.f90:192-199 */
+int sfg_M_DDRX_src(sfg_ctx* ctx, int L, const double tau[9], double* M);
+/* M_CDRX(): src/dynamics.f90:474-492 (real n x n) */
+int sfg_M_CDRX(sfg_ctx* ctx, int L, double* M);
+/* M_REG(D): src/dynamics.f90:494-518 (real n x n) */
+int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M);
+
+/* ---- batched operators (device buffers), the Eulerian per-node assembly of
+ * src/specfabpy/fenics/CPO.py:200-202: M_dev is (P, n, n) complex. ---- */
+int sfg_M_LROT_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, const double* omg_dev, double iota,
+                     double zeta, double* M_dev, void* stream);
+int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* M_dev, void* stream);
+
+/* ---- state conversion (device buffers) ---- */
+int sfg_nlm_to_state(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* x_dev, void* stream);
+int sfg_state_to_nlm(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* nlm_dev, void* stream);
+
+/* ---- batched evolution ----
+ * Steps P parcels of truncation L in place.  Per parcel: velocity gradient
+ * ugrad (D = sym, W = skew), optional stress tau (NULL: tau := D, the coaxial
+ * choice of src/specfabpy/integrator.py:38-39), DDRX rate Gamma0, CDRX rate
+ * Lambda.  The operator per stage is M_LROT(D,W,iota,zeta) + M_REG(D) +
+ * Lambda M_CDRX + Gamma0 M_DDRX(x, tau), i.e. the sum formed by
+ * src/specfabpy/integrator.py:73-77, with Euler (the reference's scheme) or
+ * classic RK4 (M_DDRX rebuilt from each stage's state).  steps_dev (optional,
+ * int32 per parcel) overrides nsteps for uneven batches. */
+typedef struct sfg_evolve_desc {
+    int L;
+    int64_t P;
+    int procs;           /* SFG_LROT | SFG_DDRX | SFG_CDRX | SFG_REG */
+    int scheme;          /* SFG_EULER or SFG_RK4 */
+    int precision;       /* SFG_FP64 or SFG_FP32C */
+    int nsteps;          /* uniform step count (ignored where steps_dev given) */
+    double dt;
+    double iota, zeta;   /* lattice-rotation parameters (src/dynamics.f90:52-73) */
+    const double* ugrad_dev;  /* (P,3,3) */
+    const double* tau_dev;    /* (P,3,3) or NULL */
+    const double* gam0_dev;   /* (P) or NULL (0) */
+    const double* lam_dev;    /* (P) or NULL (0) */
+    const int* steps_dev;     /* (P) or NULL */
+    double* x_dev;            /* state, real SoA (n, P), FP64 */
+    int32_t* status_dev;      /* (P) or NULL */
+} sfg_evolve_desc;
+int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* desc, void* stream);
+
+/* ---- moments and homogenization on batched states (device buffers) ---- */
+/* a2 (P,3,3), src/moments.f90:37-44 */
+int sfg_a2_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a2_dev, void* stream);
+/* a4 Mandel (P,6,6), src/moments.f90:164-178 */
+int sfg_a4mandel_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4v_dev, void* stream);
+
+/* single-parcel moments (host): a2(nlm) 3x3 (src/moments.f90:37-44) */
+int sfg_a2(sfg_ctx* ctx, int L, const double* nlm, double a2[9]);
+/* <D> = ev_D2(nlm, tau), src/dynamics.f90:402-422 */
+int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* D);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECFAB_GPU_H */

@@ -1,0 +1,177 @@
+"""ctypes face of oracle/_ref/libsfref.so (the reference's own FP64 Fortran,
+built by oracle/ref.mk).  Test infrastructure: used by tests, the golden-vector
+script and bench.py's CPU-baseline leg only, never by specfab_amd.
+
+Array conventions mirror the Fortran: 3x3 tensors and n x n operators are
+exchanged column-major; the helpers below take/return numpy arrays in natural
+(row-major, C) index order and do the transposes.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "_ref", "libsfref.so")
+
+_c = ctypes
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="F_CONTIGUOUS")
+
+
+def available():
+    return os.path.exists(LIBPATH)
+
+
+class RefLib:
+    """One process = one active L (the reference keeps module globals)."""
+
+    def __init__(self, path=LIBPATH):
+        self.lib = _c.CDLL(path)
+        L = self.lib
+        L.sfref_init.argtypes = [_c.c_int]
+        L.sfref_nlm_len.restype = _c.c_int
+        L.sfref_M_LROT.argtypes = [_dp, _dp, _c.c_double, _c.c_double, _cp]
+        L.sfref_M_DDRX.argtypes = [_cp, _dp, _cp]
+        L.sfref_M_DDRX_src.argtypes = [_dp, _cp]
+        L.sfref_M_REG.argtypes = [_dp, _dp]
+        L.sfref_M_CDRX.argtypes = [_dp]
+        L.sfref_ev_D2.argtypes = [_cp, _dp]
+        L.sfref_ev_D2.restype = _c.c_double
+        L.sfref_a2.argtypes = [_cp, _dp]
+        L.sfref_a4.argtypes = [_cp, _dp]
+        L.sfref_ev_ck_Mandel.argtypes = [_cp, _dp, _dp]
+        L.sfref_eig.argtypes = [_cp, _dp, _dp]
+        L.sfref_Eij_tranisotropic.argtypes = [_cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
+        L.sfref_Eij_eigframe_batch.argtypes = [_c.c_int, _cp, _dp, _c.c_double, _c.c_int, _dp, _dp]
+        L.sfref_Cij_to_Lame_tranisotropic.argtypes = [_dp, _dp]
+        L.sfref_Vi_elastic_tranisotropic.argtypes = [_cp, _c.c_double, _dp, _c.c_double, _c.c_int, _dp, _dp, _dp]
+        L.sfref_Qnorm_tranisotropic.argtypes = [_cp, _c.c_double, _dp, _dp]
+        L.sfref_Vi_electromagnetic_tranisotropic.argtypes = [_cp, _c.c_double, _c.c_double, _c.c_double,
+                                                             _c.c_int, _dp, _dp, _dp]
+        L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
+                                   _c.c_double, _dp, _dp, _dp, _cp]
+        self.L = None
+        self.n = 0
+
+    # -- helpers --
+    @staticmethod
+    def _m3(A):
+        return np.asfortranarray(np.asarray(A, dtype=np.float64).reshape(3, 3))
+
+    @staticmethod
+    def _v(x, dtype=np.float64):
+        return np.asfortranarray(np.asarray(x, dtype=dtype).ravel())
+
+    def init(self, L):
+        self.lib.sfref_init(int(L))
+        self.L = int(L)
+        self.n = self.lib.sfref_nlm_len()
+        return self.n
+
+    def M_LROT(self, eps, omg, iota, zeta):
+        M = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        self.lib.sfref_M_LROT(self._m3(eps), self._m3(omg), float(iota), float(zeta), M)
+        return np.ascontiguousarray(M)
+
+    def M_DDRX(self, nlm, tau):
+        M = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        self.lib.sfref_M_DDRX(self._v(nlm, np.complex128), self._m3(tau), M)
+        return np.ascontiguousarray(M)
+
+    def M_DDRX_src(self, tau):
+        M = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        self.lib.sfref_M_DDRX_src(self._m3(tau), M)
+        return np.ascontiguousarray(M)
+
+    def M_REG(self, D):
+        M = np.zeros((self.n, self.n), order="F")
+        self.lib.sfref_M_REG(self._m3(D), M)
+        return np.ascontiguousarray(M)
+
+    def M_CDRX(self):
+        M = np.zeros((self.n, self.n), order="F")
+        self.lib.sfref_M_CDRX(M)
+        return np.ascontiguousarray(M)
+
+    def ev_D2(self, nlm, tau):
+        return self.lib.sfref_ev_D2(self._v(nlm, np.complex128), self._m3(tau))
+
+    def a2(self, nlm):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_a2(self._v(nlm, np.complex128), out)
+        return np.ascontiguousarray(out)
+
+    def a4(self, nlm):
+        out = np.zeros((3, 3, 3, 3), order="F")
+        self.lib.sfref_a4(self._v(nlm, np.complex128), out)
+        return np.ascontiguousarray(out)
+
+    def ev_ck_Mandel(self, nlm):
+        a2v = np.zeros(6, order="F")
+        a4v = np.zeros((6, 6), order="F")
+        self.lib.sfref_ev_ck_Mandel(self._v(nlm, np.complex128), a2v, a4v)
+        return a2v, np.ascontiguousarray(a4v)
+
+    def eig(self, nlm):
+        ei = np.zeros((3, 3), order="F")
+        lami = np.zeros(3, order="F")
+        self.lib.sfref_eig(self._v(nlm, np.complex128), ei, lami)
+        return np.ascontiguousarray(ei), lami.copy()
+
+    def Eij_tranisotropic(self, nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
+        out = np.zeros(6, order="F")
+        self.lib.sfref_Eij_tranisotropic(self._v(nlm, np.complex128), self._v(e1), self._v(e2), self._v(e3),
+                                         self._v(Eij_grain), float(alpha), int(n_grain), out)
+        return out.copy()
+
+    def Eij_eigframe_batch(self, nlm, Eij_grain, alpha, n_grain):
+        """nlm (P, n) -> Eij (P, 6), lami (P, 3)."""
+        P = nlm.shape[0]
+        X = np.asfortranarray(np.asarray(nlm, dtype=np.complex128).T)
+        E = np.zeros((6, P), order="F")
+        lam = np.zeros((3, P), order="F")
+        self.lib.sfref_Eij_eigframe_batch(P, X, self._v(Eij_grain), float(alpha), int(n_grain), E, lam)
+        return np.ascontiguousarray(E.T), np.ascontiguousarray(lam.T)
+
+    def Cij_to_Lame_tranisotropic(self, Cij):
+        out = np.zeros(5, order="F")
+        self.lib.sfref_Cij_to_Lame_tranisotropic(self._v(Cij), out)
+        return out.copy()
+
+    def Vi_elastic_tranisotropic(self, nlm, alpha, Lame, rho, theta, phi):
+        theta = self._v(np.atleast_1d(theta))
+        phi = self._v(np.atleast_1d(phi))
+        out = np.zeros((3, theta.size), order="F")
+        self.lib.sfref_Vi_elastic_tranisotropic(self._v(nlm, np.complex128), float(alpha), self._v(Lame),
+                                                float(rho), theta.size, theta, phi, out)
+        return np.ascontiguousarray(out)
+
+    def Qnorm_tranisotropic(self, nlm, alpha, Lame):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_Qnorm_tranisotropic(self._v(nlm, np.complex128), float(alpha), self._v(Lame), out)
+        return np.ascontiguousarray(out)
+
+    def Vi_electromagnetic_tranisotropic(self, nlm, eps_m, eps_t, mu, theta, phi):
+        theta = self._v(np.atleast_1d(theta))
+        phi = self._v(np.atleast_1d(phi))
+        out = np.zeros((2, theta.size), order="F")
+        self.lib.sfref_Vi_electromagnetic_tranisotropic(self._v(nlm, np.complex128), float(eps_m), float(eps_t),
+                                                        float(mu), theta.size, theta, phi, out)
+        return np.ascontiguousarray(out)
+
+    def evolve(self, nlm, ugrad, gam0, lam, nsteps, dt, scheme="rk4", procs=("LROT", "REG"),
+               iota=1.0, zeta=0.0):
+        """nlm (P, n) complex, ugrad (P, 3, 3) -> advanced nlm (P, n)."""
+        bits = {"LROT": 1, "DDRX": 2, "CDRX": 4, "REG": 8}
+        mask = 0
+        for p in procs:
+            mask |= bits[p]
+        P = nlm.shape[0]
+        X = np.array(np.asarray(nlm, dtype=np.complex128).T, order="F", copy=True)   # advanced in place
+        U = np.asfortranarray(np.transpose(np.asarray(ugrad, dtype=np.float64), (1, 2, 0)))
+        g = self._v(np.broadcast_to(gam0, (P,)))
+        lm = self._v(np.broadcast_to(lam, (P,)))
+        self.lib.sfref_evolve(P, int(nsteps), 0 if scheme == "euler" else 1, mask, float(dt), float(iota),
+                              float(zeta), U, g, lm, X)
+        return np.ascontiguousarray(X.T)

@@ -1,0 +1,239 @@
+! Test-infrastructure driver for the reference's own FP64 Fortran core.
+!
+! This file is part of the oracle: it is compiled together with the
+! reference modules (oracle/ref.mk) into oracle/_ref/libsfref.so and is
+! called only by tests, golden-vector scripts and the CPU-baseline leg of
+! bench.py.  It exposes the reference routines through bind(C) entry points
+! (column-major arrays, plain pointers) and adds the time-stepping driver that
+! the reference leaves to its callers:
+!   * Euler exactly as src/specfabpy/integrator.py:73-77 and
+!     demo/fabric-evolution/run-fabric-evolution-ddrx.f90:136-137
+!     (steady operators built once per parcel, M_DDRX rebuilt from the
+!     current state, dense matmul);
+!   * classic 4-stage RK4 with the same per-stage operator rebuild.
+! One truncation L is active per process (the reference keeps module globals,
+! src/header.f90:16, src/dynamics.f90:17-18).
+module sfref_driver
+
+    use iso_c_binding
+    use header
+    use moments
+    use dynamics
+    use homogenizations
+    use enhancementfactors
+    use frames
+    use elasticities
+    use wavepropagation
+
+    implicit none
+
+    integer, private :: Lcur = -1, ncur = 0
+
+contains
+
+    subroutine sfref_init(L) bind(C, name="sfref_init")
+        integer(c_int), value :: L
+        call initmoments(L)
+        call initdynamics(L)
+        call inithomogenizations(L)
+        Lcur = L
+        ncur = (L+1)*(L+2)/2
+    end subroutine
+
+    function sfref_nlm_len() bind(C, name="sfref_nlm_len") result(n)
+        integer(c_int) :: n
+        n = ncur
+    end function
+
+    ! ---- operators (column-major n x n outputs) ----
+
+    subroutine sfref_M_LROT(eps, omg, iota, zeta, M) bind(C, name="sfref_M_LROT")
+        real(c_double), intent(in) :: eps(3,3), omg(3,3)
+        real(c_double), value :: iota, zeta
+        complex(c_double_complex), intent(out) :: M(ncur,ncur)
+        M = M_LROT(eps, omg, iota, zeta)
+    end subroutine
+
+    subroutine sfref_M_DDRX(nlm, tau, M) bind(C, name="sfref_M_DDRX")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: tau(3,3)
+        complex(c_double_complex), intent(out) :: M(ncur,ncur)
+        M = M_DDRX(nlm, tau)
+    end subroutine
+
+    subroutine sfref_M_DDRX_src(tau, M) bind(C, name="sfref_M_DDRX_src")
+        real(c_double), intent(in) :: tau(3,3)
+        complex(c_double_complex), intent(out) :: M(ncur,ncur)
+        M = M_DDRX_src(tau)
+    end subroutine
+
+    subroutine sfref_M_REG(D, M) bind(C, name="sfref_M_REG")
+        real(c_double), intent(in) :: D(3,3)
+        real(c_double), intent(out) :: M(ncur,ncur)
+        M = M_REG(D)
+    end subroutine
+
+    subroutine sfref_M_CDRX(M) bind(C, name="sfref_M_CDRX")
+        real(c_double), intent(out) :: M(ncur,ncur)
+        M = M_CDRX()
+    end subroutine
+
+    function sfref_ev_D2(nlm, tau) bind(C, name="sfref_ev_D2") result(D)
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: tau(3,3)
+        real(c_double) :: D
+        D = ev_D2(nlm, tau)
+    end function
+
+    ! ---- moments ----
+
+    subroutine sfref_a2(nlm, a2out) bind(C, name="sfref_a2")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: a2out(3,3)
+        a2out = a2(nlm)
+    end subroutine
+
+    subroutine sfref_a4(nlm, a4out) bind(C, name="sfref_a4")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: a4out(3,3,3,3)
+        a4out = a4(nlm)
+    end subroutine
+
+    subroutine sfref_ev_ck_Mandel(nlm, a2v, a4v) bind(C, name="sfref_ev_ck_Mandel")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: a2v(6), a4v(6,6)
+        call f_ev_ck_Mandel(nlm, a2v, a4v)
+    end subroutine
+
+    ! ---- homogenization ----
+
+    subroutine sfref_eig(nlm, ei, lami) bind(C, name="sfref_eig")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: ei(3,3), lami(3)
+        call eig(nlm, ei, lami)
+    end subroutine
+
+    subroutine sfref_Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain, Eij) &
+            bind(C, name="sfref_Eij_tranisotropic")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: e1(3), e2(3), e3(3), Eij_grain(2)
+        real(c_double), value :: alpha
+        integer(c_int), value :: n_grain
+        real(c_double), intent(out) :: Eij(6)
+        Eij = Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain)
+    end subroutine
+
+    ! Eigenframe + Eij for P parcels (nlm(ncur,P)), the reference call pattern
+    ! of src/specfabpy/common.py:13-37 followed by Eij_tranisotropic.
+    subroutine sfref_Eij_eigframe_batch(P, nlm, Eij_grain, alpha, n_grain, Eij, lami) &
+            bind(C, name="sfref_Eij_eigframe_batch")
+        integer(c_int), value :: P, n_grain
+        complex(c_double_complex), intent(in) :: nlm(ncur, P)
+        real(c_double), intent(in) :: Eij_grain(2)
+        real(c_double), value :: alpha
+        real(c_double), intent(out) :: Eij(6, P), lami(3, P)
+        real(c_double) :: ei(3,3)
+        integer :: p_
+        do p_ = 1, P
+            call eig(nlm(:,p_), ei, lami(:,p_))
+            Eij(:,p_) = Eij_tranisotropic(nlm(:,p_), ei(1,:), ei(2,:), ei(3,:), Eij_grain, alpha, n_grain)
+        end do
+    end subroutine
+
+    subroutine sfref_Cij_to_Lame_tranisotropic(Cij, Lame) bind(C, name="sfref_Cij_to_Lame_tranisotropic")
+        real(c_double), intent(in) :: Cij(5)
+        real(c_double), intent(out) :: Lame(5)
+        call Cij_to_Lame_tranisotropic(Cij, Lame)
+    end subroutine
+
+    subroutine sfref_Vi_elastic_tranisotropic(nlm, alpha, Lame, rho, ndir, theta, phi, Vi) &
+            bind(C, name="sfref_Vi_elastic_tranisotropic")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), value :: alpha, rho
+        real(c_double), intent(in) :: Lame(5)
+        integer(c_int), value :: ndir
+        real(c_double), intent(in) :: theta(ndir), phi(ndir)
+        real(c_double), intent(out) :: Vi(3, ndir)
+        Vi = Vi_elastic_tranisotropic(nlm, alpha, Lame, rho, theta, phi)
+    end subroutine
+
+    subroutine sfref_Qnorm_tranisotropic(nlm, alpha, Lame, Q) bind(C, name="sfref_Qnorm_tranisotropic")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), value :: alpha
+        real(c_double), intent(in) :: Lame(5)
+        real(c_double), intent(out) :: Q(3,3)
+        Q = Qnorm_tranisotropic(nlm, alpha, Lame)
+    end subroutine
+
+    subroutine sfref_Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, ndir, theta, phi, Vi) &
+            bind(C, name="sfref_Vi_electromagnetic_tranisotropic")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), value :: eps_m, eps_t, mu
+        integer(c_int), value :: ndir
+        real(c_double), intent(in) :: theta(ndir), phi(ndir)
+        real(c_double), intent(out) :: Vi(2, ndir)
+        Vi = Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi)
+    end subroutine
+
+    ! ---- time stepping ----
+    !
+    ! procs bitmask: 1 = LROT, 2 = DDRX, 4 = CDRX, 8 = REG.  scheme: 0 = Euler,
+    ! 1 = RK4.  ugrad(3,3,P) column-major velocity gradients; D = sym, W = skew,
+    ! stress tau := D (coaxial, src/specfabpy/integrator.py:38-39).
+    ! nlm(ncur,P) is advanced in place by nsteps steps of size dt.
+    subroutine sfref_evolve(P, nsteps, scheme, procs, dt, iota, zeta, ugrad, gam0, lam, nlm) &
+            bind(C, name="sfref_evolve")
+        integer(c_int), value :: P, nsteps, scheme, procs
+        real(c_double), value :: dt, iota, zeta
+        real(c_double), intent(in) :: ugrad(3,3,P), gam0(P), lam(P)
+        complex(c_double_complex), intent(inout) :: nlm(ncur, P)
+        complex(c_double_complex), allocatable :: Mlin(:,:), M(:,:)
+        complex(c_double_complex), allocatable :: x0(:), x(:), k1(:), k2(:), k3(:), k4(:)
+        real(c_double) :: D(3,3), W(3,3)
+        integer :: p_, t_
+
+        allocate(Mlin(ncur,ncur), M(ncur,ncur))
+        allocate(x0(ncur), x(ncur), k1(ncur), k2(ncur), k3(ncur), k4(ncur))
+
+        do p_ = 1, P
+            D = (ugrad(:,:,p_) + transpose(ugrad(:,:,p_)))/2
+            W = (ugrad(:,:,p_) - transpose(ugrad(:,:,p_)))/2
+            Mlin = (0.0d0, 0.0d0)
+            if (iand(procs, 1) /= 0) Mlin = Mlin + M_LROT(D, W, iota, zeta)
+            if (iand(procs, 4) /= 0) Mlin = Mlin + lam(p_)*M_CDRX()
+            if (iand(procs, 8) /= 0) Mlin = Mlin + M_REG(D)
+            x0 = nlm(:, p_)
+            do t_ = 1, nsteps
+                if (scheme == 0) then
+                    call rate(x0, k1)
+                    x0 = x0 + dt*k1
+                else
+                    call rate(x0, k1)
+                    x = x0 + (dt/2)*k1
+                    call rate(x, k2)
+                    x = x0 + (dt/2)*k2
+                    call rate(x, k3)
+                    x = x0 + dt*k3
+                    call rate(x, k4)
+                    x0 = x0 + (dt/6)*(k1 + 2*k2 + 2*k3 + k4)
+                end if
+            end do
+            nlm(:, p_) = x0
+        end do
+
+    contains
+
+        subroutine rate(xin, kout)
+            complex(c_double_complex), intent(in) :: xin(ncur)
+            complex(c_double_complex), intent(out) :: kout(ncur)
+            if (iand(procs, 2) /= 0) then
+                M = Mlin + gam0(p_)*M_DDRX(xin, D)
+                kout = matmul(M, xin)
+            else
+                kout = matmul(Mlin, xin)
+            end if
+        end subroutine
+
+    end subroutine
+
+end module sfref_driver

@@ -1,0 +1,411 @@
+/*
+ * sf_oracle.c -- CPU FP64 restatement of specfab's fabric-evolution and
+ * homogenization path.  TEST INFRASTRUCTURE ONLY: tests/, the smoke check and
+ * bench.py's CPU-baseline leg call it (through ctypes) as the checker; the
+ * product library (specfab_amd) never links, loads or calls it.
+ *
+ * Every routine restates the reference algorithm in plain C99 with complex
+ * arithmetic, dense operators and the reference's numerical data (the
+ * float32-parsed Gaunt tables and the REAL(4) literals noted inline), and
+ * cites the reference file:line it follows.  Layout: nlm is complex double in
+ * specfab order (l = 0,2,..,L; m = -l..l, src/header.f90:30); 3x3 tensors are
+ * row-major C arrays (A[3*i+j] = A(i+1,j+1)); n x n operators are row-major
+ * (M[i*n+j] = M(i+1,j+1), so that d nlm_i/dt = sum_j M[i*n+j] nlm_j).
+ *
+ * Parity status of each function is documented in its comment:
+ *   "pinned: ..." names the check that pins it (reference golden vectors from
+ *   oracle/_ref, closed forms, invariants) in tests/test_oracle.py.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cdbl;
+
+#define PI 3.141592653589793
+
+/* ---------------------------------------------------------------- tables */
+
+typedef struct { uint8_t t; uint8_t i_hi; uint16_t pad; } dummy_t;
+static const struct { int t, i, j, k; uint32_t bits; } gaunt_entries[] = {
+#include "../specfab_amd/csrc/generated/gaunt_L20.inc"
+};
+#define NGAUNT ((int)(sizeof(gaunt_entries) / sizeof(gaunt_entries[0])))
+#define NMAX 231
+
+/* dense tables T[t][i][j][k] (0-based), built once: src/gaunt.f90:14-17 */
+static double *GT = NULL;
+static inline double *gt(int t, int i, int j, int k) {
+    return GT + ((((size_t)t * NMAX + i) * NMAX + j) * 15 + k);
+}
+static void tables_init(void) {
+    if (GT) return;
+    GT = (double *)calloc((size_t)4 * NMAX * NMAX * 15, sizeof(double));
+    for (int e = 0; e < NGAUNT; e++) {
+        float f;
+        memcpy(&f, &gaunt_entries[e].bits, 4);
+        *gt(gaunt_entries[e].t, gaunt_entries[e].i, gaunt_entries[e].j, gaunt_entries[e].k) = (double)f;
+    }
+}
+
+int sfo_nlm_len(int L) { return (L + 1) * (L + 2) / 2; }
+
+/* l of coefficient index (0-based), src/header.f90:30 */
+static int l_of(int idx) {
+    int l = 0;
+    while (sfo_nlm_len(l) <= idx) l += 2;
+    return l;
+}
+
+int sfo_gaunt_count(void) { return NGAUNT; }
+
+/* ---------------------------------------------------------------- quadrics */
+
+/* quad_rr, src/dynamics.f90:563-580.  q[0..4] = m = -2..2.  The l=2, m=0 term
+ * carries the REAL(4) literal 2./3 (:576). */
+static void quad_rr(const double *M, cdbl *q) {
+    const double fsq = sqrt(2 * PI / 15);
+    const double c23 = (double)(2.0f / 3.0f);
+    double xx = M[0], yy = M[4], zz = M[8], xy = M[1], xz = M[2], yz = M[5];
+    q[0] = fsq * ((xx - yy) + 2 * I * xy);
+    q[1] = 2 * fsq * (xz + I * yz);
+    q[2] = -(c23 * sqrt(PI / 5) * (xx + yy - 2 * zz));
+    q[3] = -2 * fsq * (xz - I * yz);
+    q[4] = fsq * ((xx - yy) - 2 * I * xy);
+}
+
+/* quad_tp, src/dynamics.f90:582-593; q[0..2] = m = -1..1; sqrt(2.) is REAL(4). */
+static void quad_tp(const double *M, cdbl *q) {
+    const double fsq1 = sqrt(2 * PI / 3);
+    const double s2 = (double)sqrtf(2.0f);
+    double xy = M[1], xz = M[2], yz = M[5];
+    q[0] = fsq1 * (yz - I * xz);
+    q[1] = fsq1 * (s2 * xy);
+    q[2] = fsq1 * (-yz - I * xz);
+}
+
+/* ---------------------------------------------------------------- operators */
+
+/* M_LROT, src/dynamics.f90:52-97.  pinned: golden vectors (oracle/_ref),
+ * mass conservation (row 0 ~ 0), rotation generator for pure spin. */
+void sfo_M_LROT(int L, const double *eps, const double *omg, double iota, double zeta, cdbl *M) {
+    tables_init();
+    int n = sfo_nlm_len(L);
+    double epssq[9], arg[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += eps[3 * i + k] * eps[3 * k + j];
+            epssq[3 * i + j] = s;
+        }
+    double zetanorm = zeta / sqrt(epssq[0] + epssq[4] + epssq[8]); /* :73, NaN for eps = 0 */
+    for (int i = 0; i < 9; i++) arg[i] = iota * eps[i] + zetanorm * epssq[i];
+    cdbl qe[5], qo[3];
+    quad_rr(arg, qe);
+    quad_tp(omg, qo);
+    const double s3 = (double)sqrtf(3.0f), s6 = (double)sqrtf(6.0f);
+    const double c56 = (double)sqrtf(5.0f / 6.0f), c23 = (double)sqrtf(2.0f / 3.0f), c32 = (double)sqrtf(3.0f / 2.0f);
+    cdbl g0[6], gz[6], gn[6], gp[6];
+    /* rotational (spin) weights, :78-81 */
+    cdbl gz_rot0 = -((I * s3) * qo[1]);
+    cdbl gn_rot0 = -(((I * 6.0) / s6) * qo[0]);
+    cdbl gp_rot0 = ((I * 6.0) / s6) * qo[2];
+    /* Taylor (strain-rate) weights, :83-86 */
+    g0[0] = 0;
+    for (int k = 1; k < 6; k++) g0[k] = 3.0 * qe[k - 1];
+    gz[0] = gz_rot0; gz[1] = -qe[0]; gz[2] = 0; gz[3] = 0; gz[4] = 0; gz[5] = qe[4];
+    gn[0] = gn_rot0 + c56 * qe[1]; gn[1] = 0; gn[2] = qe[0]; gn[3] = c23 * qe[1]; gn[4] = c32 * qe[2]; gn[5] = 2.0 * qe[3];
+    gp[0] = gp_rot0 + c56 * qe[3]; gp[1] = 2.0 * qe[1]; gp[2] = c32 * qe[2]; gp[3] = c23 * qe[3]; gp[4] = qe[4]; gp[5] = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            cdbl s = 0;
+            for (int k = 0; k < 6; k++)
+                s += *gt(0, i, j, k) * g0[k] + *gt(1, i, j, k) * gz[k] + *gt(2, i, j, k) * gn[k] + *gt(3, i, j, k) * gp[k];
+            M[(size_t)i * n + j] = -s;
+        }
+}
+
+/* DDRX harmonic weights g(1:15) scaled by k*5/(tau:tau):
+ * src/include/ddrx-coupling-weights.f90:1-16 and src/dynamics.f90:291-293.
+ * REAL(4) literals: Sqrt((5.0)), Sqrt(1.5), Sqrt((6.0)), Sqrt((14.0)),
+ * Sqrt((7.0)), Sqrt((2.0)), Sqrt((3.0)) and the products -2*Sqrt((14.0)),
+ * -4*Sqrt((7.0)), 3.*Sqrt((5.0)) that Fortran evaluates in REAL(4). */
+void sfo_ddrx_weights(const double *tau, cdbl *g) {
+    cdbl q[5];
+    quad_rr(tau, q);
+    cdbl qm2 = q[0], qm1 = q[1], q0 = q[2], qp1 = q[3], qp2 = q[4];
+    const double k = (3 * sqrt(5 / PI)) / 28.0;
+    const double S5 = sqrtf(5.0f), S15 = sqrtf(1.5f), S6 = sqrtf(6.0f), S2 = sqrtf(2.0f), S3 = sqrtf(3.0f);
+    const double m2S14 = (float)(-2.0f * sqrtf(14.0f)), m4S7 = (float)(-4.0f * sqrtf(7.0f));
+    const double t3S5 = (float)(3.0f * sqrtf(5.0f));
+    g[0] = (7.0 * (q0 * q0 + (-2.0) * qm1 * qp1 + 2.0 * qm2 * qp2)) / S5;
+    g[1] = S15 * (qm1 * qm1) + (-2.0) * q0 * qm2;
+    g[2] = q0 * qm1 + ((-1.0) * S6) * qp1 * qm2;
+    g[3] = q0 * q0 + (-1.0) * qm1 * qp1 + (-2.0) * qm2 * qp2;
+    g[4] = q0 * qp1 + ((-1.0) * S6) * qm1 * qp2;
+    g[5] = S15 * (qp1 * qp1) + (-2.0) * q0 * qp2;
+    g[6] = (m2S14 * (qm2 * qm2)) / 3.0;
+    g[7] = (m4S7 * qm1 * qm2) / 3.0;
+    g[8] = (-4.0 * (S2 * (qm1 * qm1) + S3 * q0 * qm2)) / 3.0;
+    g[9] = (-4.0 * (S6 * q0 * qm1 + qp1 * qm2)) / 3.0;
+    g[10] = (-4.0 * (3.0 * (q0 * q0) + 4.0 * qm1 * qp1 + qm2 * qp2)) / t3S5;
+    g[11] = (-4.0 * (S6 * q0 * qp1 + qm1 * qp2)) / 3.0;
+    g[12] = (-4.0 * (S2 * (qp1 * qp1) + S3 * q0 * qp2)) / 3.0;
+    g[13] = (m4S7 * qp1 * qp2) / 3.0;
+    g[14] = (m2S14 * (qp2 * qp2)) / 3.0;
+    double tt = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) tt += tau[3 * i + j] * tau[3 * j + i];
+    for (int m = 0; m < 15; m++) g[m] = ((k * g[m]) * 5.0) / tt;
+}
+
+/* M_DDRX_src, src/dynamics.f90:277-298.  pinned: golden vectors, row 0 ~ 0. */
+void sfo_M_DDRX_src(int L, const double *tau, cdbl *M) {
+    tables_init();
+    int n = sfo_nlm_len(L);
+    cdbl g[15];
+    sfo_ddrx_weights(tau, g);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            cdbl s = 0;
+            for (int k = 0; k < 15; k++) s += *gt(0, i, j, k) * g[k];
+            M[(size_t)i * n + j] = s;
+        }
+}
+
+/* ---------------------------------------------------------------- moments */
+
+/* f_ev_c2, src/include/ev_c2__body.f90 (normalized a2). pinned: closed forms
+ * (isotropic -> I/3, delta along z -> zz), golden vectors. */
+void sfo_a2(const cdbl *nlm, double *a2) {
+    cdbl n00 = nlm[0];
+    cdbl h0 = nlm[3] / n00, h1 = nlm[4] / n00, h2 = nlm[5] / n00;
+    const double c = sqrt(2.0 / 3);
+    double e[9];
+    e[0] = -0.5 * c * creal(h0) + creal(h2);
+    e[4] = -0.5 * c * creal(h0) - creal(h2);
+    e[8] = c * creal(h0);
+    e[1] = e[3] = -cimag(h2);
+    e[2] = e[6] = -creal(h1);
+    e[5] = e[7] = cimag(h1);
+    const double s = sqrt(2 / 15.0);
+    for (int i = 0; i < 9; i++) a2[i] = s * e[i] + ((i % 4 == 0) ? 1.0 / 3.0 : 0.0);
+}
+
+/* f_ev_c4_Mandel, src/moments.f90:211-218 with
+ * src/include/ev_c4_Mandel__body.f90 (REAL(8) constants throughout).
+ * a4v is row-major 6x6. pinned: golden vectors, isotropic closed form. */
+void sfo_a4_mandel(const cdbl *nlm, double *a4v) {
+    double n00 = creal(nlm[0]);
+#define RE(m) creal(nlm[3 + (m)])
+#define IM(m) cimag(nlm[3 + (m)])
+#define R4(m) creal(nlm[10 + (m)])
+#define I4(m) cimag(nlm[10 + (m)])
+    const double s5 = sqrt(5.0), s6 = sqrt(6.0), s7 = sqrt(7.0), s10 = sqrt(10.0), s30 = sqrt(30.0), s70 = sqrt(70.0);
+    const double s3 = sqrt(3.0), s15 = sqrt(15.0);
+    double e[6][6];
+    e[0][0] = 21.0 * n00 + s5 * (-6.0) * RE(0) + s30 * 6.0 * RE(2) + 3.0 * R4(0) + s10 * (-2.0) * R4(2) + s70 * R4(4);
+    e[0][1] = 7.0 * n00 + (-2.0) * s5 * RE(0) + R4(0) + (-1.0) * s70 * R4(4);
+    e[0][2] = 7.0 * n00 + s5 * (RE(0) + s6 * RE(2)) + (-4.0) * R4(0) + s10 * 2.0 * R4(2);
+    e[0][3] = s10 * (s6 * IM(1) + (-1.0) * I4(1) + s7 * I4(3));
+    e[0][4] = s10 * (-1.0) * (3.0 * s6 * RE(1) + (-3.0) * R4(1) + s7 * R4(3));
+    e[0][5] = (-2.0) * s5 * (pow(3.0, 1.5) * IM(2) + (-1.0) * I4(2) + s7 * I4(4));
+    e[1][0] = e[0][1];
+    e[1][1] = 21.0 * n00 + s5 * (-6.0) * (RE(0) + s6 * RE(2)) + 3.0 * R4(0) + s10 * (2.0 * R4(2) + s7 * R4(4));
+    e[1][2] = 7.0 * n00 + s5 * (RE(0) + (-1.0) * s6 * RE(2)) + (-2.0) * (2.0 * R4(0) + s10 * R4(2));
+    e[1][3] = s10 * (3.0 * s6 * IM(1) + (-3.0) * I4(1) + (-1.0) * s7 * I4(3));
+    e[1][4] = s15 * (-2.0) * RE(1) + s10 * (R4(1) + s7 * R4(3));
+    e[1][5] = 2.0 * s5 * ((-3.0) * s3 * IM(2) + I4(2) + s7 * I4(4));
+    e[2][0] = e[0][2];
+    e[2][1] = e[1][2];
+    e[2][2] = 21.0 * n00 + 12.0 * s5 * RE(0) + 8.0 * R4(0);
+    e[2][3] = s10 * (3.0 * s6 * IM(1) + 4.0 * I4(1));
+    e[2][4] = s10 * (-1.0) * (3.0 * s6 * RE(1) + 4.0 * R4(1));
+    e[2][5] = (-2.0) * s5 * (s3 * IM(2) + 2.0 * I4(2));
+    e[3][0] = e[0][3]; e[3][1] = e[1][3]; e[3][2] = e[2][3];
+    e[3][3] = 2.0 * (7.0 * n00 + s5 * (RE(0) + (-1.0) * s6 * RE(2)) + (-2.0) * (2.0 * R4(0) + s10 * R4(2)));
+    e[3][4] = s10 * (-2.0) * (s3 * IM(2) + 2.0 * I4(2));
+    e[3][5] = s5 * ((-2.0) * s6 * RE(1) + 2.0 * (R4(1) + s7 * R4(3)));
+    e[4][0] = e[0][4]; e[4][1] = e[1][4]; e[4][2] = e[2][4]; e[4][3] = e[3][4];
+    e[4][4] = 2.0 * (7.0 * n00 + s5 * (RE(0) + s6 * RE(2)) + (-4.0) * R4(0) + s10 * 2.0 * R4(2));
+    e[4][5] = 2.0 * s5 * (s6 * IM(1) + (-1.0) * I4(1) + s7 * I4(3));
+    e[5][0] = e[0][5]; e[5][1] = e[1][5]; e[5][2] = e[2][5]; e[5][3] = e[3][5]; e[5][4] = e[4][5];
+    e[5][5] = 2.0 * (7.0 * n00 + (-2.0) * s5 * RE(0) + R4(0) + (-1.0) * s70 * R4(4));
+#undef RE
+#undef IM
+#undef R4
+#undef I4
+    const double k = (2 * sqrt(PI)) / 105.0;
+    const double c0 = creal(sqrt(4 * PI) * nlm[0]); /* f_ev_c0, src/moments.f90:137-142 */
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) a4v[6 * i + j] = e[i][j] * k / c0;
+}
+
+/* mat_to_vec, src/mandel.f90:15-24 */
+static void mat_to_vec(const double *M, double *v) {
+    const double s = sqrt(2.0);
+    v[0] = M[0]; v[1] = M[4]; v[2] = M[8];
+    v[3] = s * M[5]; v[4] = s * M[2]; v[5] = s * M[1];
+}
+
+void sfo_ev_ck_Mandel(const cdbl *nlm, double *a2v, double *a4v) {
+    double a2[9];
+    sfo_a2(nlm, a2);
+    mat_to_vec(a2, a2v);
+    sfo_a4_mandel(nlm, a4v);
+}
+
+/* ev_D2 = <D>, src/dynamics.f90:402-422.  pinned: golden vectors; closed
+ * form 1 for the isotropic state (tau:tau normalisation). */
+double sfo_ev_D2(const cdbl *nlm, const double *tau) {
+    double tauv[6], tausq[9], tsqv[6], a2v[6], a4v[36];
+    mat_to_vec(tau, tauv);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += tau[3 * i + k] * tau[3 * k + j];
+            tausq[3 * i + j] = s;
+        }
+    double norm = tausq[0] + tausq[4] + tausq[8];
+    sfo_ev_ck_Mandel(nlm, a2v, a4v);
+    mat_to_vec(tausq, tsqv);
+    double d1 = 0, d2 = 0;
+    for (int i = 0; i < 6; i++) {
+        d1 += tsqv[i] * a2v[i];
+        double r = 0;
+        for (int j = 0; j < 6; j++) r += a4v[6 * i + j] * tauv[j];
+        d2 += tauv[i] * r;
+    }
+    return 5 * (d1 - d2) / norm;
+}
+
+/* M_DDRX = M_DDRX_src - <D> I, src/dynamics.f90:251-275 */
+void sfo_M_DDRX(int L, const cdbl *nlm, const double *tau, cdbl *M) {
+    int n = sfo_nlm_len(L);
+    sfo_M_DDRX_src(L, tau, M);
+    double D = sfo_ev_D2(nlm, tau);
+    for (int i = 0; i < n; i++) M[(size_t)i * n + i] -= D;
+}
+
+/* REG calibration (expo, nu) per L, src/include/regcalib.f90:1-36 */
+int sfo_regcalib(int L, double *expo, double *nu) {
+    switch (L) {
+    case 4: *expo = 1.700; *nu = 1.9879322126397958; return 0;
+    case 6: *expo = 1.150; *nu = 3.0011508426238862; return 0;
+    case 8: *expo = 1.600; *nu = 5.7498069921352384; return 0;
+    case 10: *expo = 2.000; *nu = 10.7048905312159288; return 0;
+    case 12: *expo = 2.000; *nu = 10.6068117205577668; return 0;
+    case 14: *expo = 2.000; *nu = 13.3591023418822363; return 0;
+    case 16: *expo = 2.500; *nu = 15.3094482670021108; return 0;
+    case 18: *expo = 2.500; *nu = 16.4844589176829217; return 0;
+    case 20: *expo = 3.000; *nu = 19.9467342880730136; return 0;
+    default: return -1;
+    }
+}
+
+/* M_REG diagonal, src/dynamics.f90:494-518.  pinned: closed form. */
+int sfo_M_REG_diag(int L, const double *D, double *diag) {
+    double expo, nu;
+    if (sfo_regcalib(L, &expo, &nu)) return -1;
+    int n = sfo_nlm_len(L);
+    double fro = 0;
+    for (int i = 0; i < 9; i++) fro += D[i] * D[i];
+    double ratemag = nu * sqrt(fro);
+    for (int i = 0; i < n; i++) {
+        int l = l_of(i);
+        double Ld = -(double)(l * (l + 1));
+        diag[i] = -ratemag * pow(fabs(Ld / (L * (L + 1))), expo);
+    }
+    return 0;
+}
+
+/* M_CDRX diagonal, src/dynamics.f90:474-492 (Ldiag, :22) */
+void sfo_M_CDRX_diag(int L, double *diag) {
+    int n = sfo_nlm_len(L);
+    for (int i = 0; i < n; i++) {
+        int l = l_of(i);
+        diag[i] = -(double)(l * (l + 1));
+    }
+}
+
+/* ---------------------------------------------------------------- stepping */
+
+/* Batched evolution, restating the reference call pattern
+ * (src/specfabpy/integrator.py:55-77; demo/fabric-evolution/
+ * run-fabric-evolution-ddrx.f90:123-137): per parcel the steady operator
+ * M_lin = M_LROT(D,W) + Lambda*M_CDRX + M_REG(D) (+ Gamma0*M_DDRX_src(D),
+ * constant for a time-constant stress tau := D) is built once; every stage
+ * applies M_lin - Gamma0*<D>(x) I to the current state (dense matmul).
+ * scheme 0 = explicit Euler, 1 = classic RK4.  procs bitmask 1 = LROT,
+ * 2 = DDRX, 4 = CDRX, 8 = REG.  ugrad row-major (P,3,3); nlm (P,n) in place.
+ * pinned: golden trajectories from oracle/_ref (C1 Euler, RK4 fixtures). */
+int sfo_evolve(int L, int P, int nsteps, int scheme, int procs, double dt, double iota, double zeta,
+               const double *ugrad, const double *gam0, const double *lam, cdbl *nlm) {
+    int n = sfo_nlm_len(L);
+    double expo, nu;
+    if ((procs & 8) && sfo_regcalib(L, &expo, &nu)) return -1;
+    cdbl *M = (cdbl *)malloc(sizeof(cdbl) * n * n);
+    cdbl *T = (cdbl *)malloc(sizeof(cdbl) * n * n);
+    cdbl *x0 = (cdbl *)malloc(sizeof(cdbl) * n * 6);
+    cdbl *x = x0 + n, *k1 = x0 + 2 * n, *k2 = x0 + 3 * n, *k3 = x0 + 4 * n, *k4 = x0 + 5 * n;
+    double *dg = (double *)malloc(sizeof(double) * n);
+    for (int p = 0; p < P; p++) {
+        const double *G = ugrad + 9 * p;
+        double D[9], W[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                D[3 * i + j] = (G[3 * i + j] + G[3 * j + i]) / 2;
+                W[3 * i + j] = (G[3 * i + j] - G[3 * j + i]) / 2;
+            }
+        memset(M, 0, sizeof(cdbl) * n * n);
+        if (procs & 1) {
+            sfo_M_LROT(L, D, W, iota, zeta, T);
+            for (int i = 0; i < n * n; i++) M[i] += T[i];
+        }
+        if (procs & 4) {
+            sfo_M_CDRX_diag(L, dg);
+            for (int i = 0; i < n; i++) M[(size_t)i * n + i] += lam[p] * dg[i];
+        }
+        if (procs & 8) {
+            sfo_M_REG_diag(L, D, dg);
+            for (int i = 0; i < n; i++) M[(size_t)i * n + i] += dg[i];
+        }
+        if (procs & 2) {
+            sfo_M_DDRX_src(L, D, T);
+            for (int i = 0; i < n * n; i++) M[i] += gam0[p] * T[i];
+        }
+        cdbl *st = nlm + (size_t)p * n;
+        memcpy(x0, st, sizeof(cdbl) * n);
+#define RATE(xin, kout)                                                                  \
+    do {                                                                                 \
+        double Dv = (procs & 2) ? gam0[p] * sfo_ev_D2(xin, D) : 0.0;                     \
+        for (int i = 0; i < n; i++) {                                                    \
+            cdbl s = 0;                                                                  \
+            const cdbl *row = M + (size_t)i * n;                                         \
+            for (int j = 0; j < n; j++) s += row[j] * (xin)[j];                          \
+            (kout)[i] = s - Dv * (xin)[i];                                               \
+        }                                                                                \
+    } while (0)
+        for (int t = 0; t < nsteps; t++) {
+            if (scheme == 0) {
+                RATE(x0, k1);
+                for (int i = 0; i < n; i++) x0[i] = x0[i] + dt * k1[i];
+            } else {
+                RATE(x0, k1);
+                for (int i = 0; i < n; i++) x[i] = x0[i] + (dt / 2) * k1[i];
+                RATE(x, k2);
+                for (int i = 0; i < n; i++) x[i] = x0[i] + (dt / 2) * k2[i];
+                RATE(x, k3);
+                for (int i = 0; i < n; i++) x[i] = x0[i] + dt * k3[i];
+                RATE(x, k4);
+                for (int i = 0; i < n; i++) x0[i] = x0[i] + (dt / 6) * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+            }
+        }
+#undef RATE
+        memcpy(st, x0, sizeof(cdbl) * n);
+    }
+    free(M); free(T); free(x0); free(dg);
+    return 0;
+}

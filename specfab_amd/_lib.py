@@ -1,0 +1,117 @@
+"""ctypes binding of libspecfab_gpu.so (C-ABI: include/specfab_gpu.h).
+
+The library is the only compute path: if it is missing or no HIP device is
+present, every entry point raises instead of computing anything on the CPU.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.environ.get("SFG_LIBRARY", os.path.join(HERE, "libspecfab_gpu.so"))
+
+SFG_LROT, SFG_DDRX, SFG_CDRX, SFG_REG = 1, 2, 4, 8
+SFG_EULER, SFG_RK4 = 0, 1
+SFG_FP64, SFG_FP32C = 0, 1
+PROC_BITS = {"LROT": SFG_LROT, "DDRX": SFG_DDRX, "CDRX": SFG_CDRX, "REG": SFG_REG}
+
+# every exported symbol of include/specfab_gpu.h with (restype, argtypes)
+_c = ctypes
+_vp, _i, _i64, _d = _c.c_void_p, _c.c_int, _c.c_int64, _c.c_double
+_dp = _c.POINTER(_c.c_double)
+_ip = _c.POINTER(_c.c_int)
+
+
+class EvolveDesc(_c.Structure):
+    _fields_ = [("L", _c.c_int), ("P", _c.c_int64), ("procs", _c.c_int), ("scheme", _c.c_int),
+                ("precision", _c.c_int), ("nsteps", _c.c_int), ("dt", _c.c_double), ("iota", _c.c_double),
+                ("zeta", _c.c_double), ("ugrad_dev", _c.c_void_p), ("tau_dev", _c.c_void_p),
+                ("gam0_dev", _c.c_void_p), ("lam_dev", _c.c_void_p), ("steps_dev", _c.c_void_p),
+                ("x_dev", _c.c_void_p), ("status_dev", _c.c_void_p)]
+
+
+SIGNATURES = {
+    "sfg_ctx_create": (_i, [_i, _c.POINTER(_vp)]),
+    "sfg_ctx_destroy": (_i, [_vp]),
+    "sfg_last_error": (_c.c_char_p, []),
+    "sfg_nlm_len": (_i, [_i]),
+    "sfg_lm": (_i, [_i, _ip]),
+    "sfg_M_LROT": (_i, [_vp, _i, _dp, _dp, _d, _d, _dp]),
+    "sfg_M_DDRX": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "sfg_M_DDRX_src": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_M_CDRX": (_i, [_vp, _i, _dp]),
+    "sfg_M_REG": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_M_LROT_batch": (_i, [_vp, _i, _i64, _vp, _vp, _d, _d, _vp, _vp]),
+    "sfg_M_DDRX_src_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_nlm_to_state": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_state_to_nlm": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_evolve": (_i, [_vp, _c.POINTER(EvolveDesc), _vp]),
+    "sfg_a2_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_a4mandel_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_a2": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_ev_D2": (_i, [_vp, _i, _dp, _dp, _dp]),
+}
+
+
+class SFGError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libspecfab_gpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise SFGError("libspecfab_gpu.so not found at %s: build it with `python -m specfab_amd.build`"
+                           % LIBPATH)
+        L = _c.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().sfg_last_error()
+        raise SFGError("libspecfab_gpu error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc
+
+
+class Context:
+    """Owns an sfg_ctx on one HIP device (coupling tables, stream, workspace)."""
+
+    def __init__(self, device=0):
+        self.handle = _vp()
+        check(lib().sfg_ctx_create(int(device), _c.byref(self.handle)))
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().sfg_ctx_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def dptr(a):
+    """ctypes double* of a contiguous float64 numpy array."""
+    return a.ctypes.data_as(_dp)

@@ -1,0 +1,118 @@
+"""Batched, device-resident interface: many Lagrangian parcels / Eulerian
+nodes stepped and homogenized on one GPU.
+
+PyTorch is used only for device memory and streams; every computation is a
+call into libspecfab_gpu (HIP).  A batch holds the real SoA state x (n, P) in
+HBM; see include/specfab_gpu.h for the layout.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import EvolveDesc, check, lib
+
+_c = ctypes
+
+
+def _ptr(t):
+    return _c.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return _c.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def nlm_len(L):
+    return (L + 1) * (L + 2) // 2
+
+
+class FabricBatch:
+    """P parcels at truncation L with per-parcel kinematics.
+
+    ugrad: (P,3,3) velocity gradients (D = sym, W = skew); tau: optional
+    (P,3,3) stresses (default tau := D); gam0 / lam: DDRX / CDRX rates (P,).
+    """
+
+    def __init__(self, L, nlm=None, P=None, ugrad=None, tau=None, gam0=None, lam=None, steps=None,
+                 ctx=None, device="cuda:0"):
+        if not torch.cuda.is_available():
+            raise _lib.SFGError("no HIP device available")
+        self.ctx = ctx or _lib.default_context()
+        self.L = int(L)
+        self.n = nlm_len(self.L)
+        self.dev = torch.device(device)
+        if nlm is not None:
+            nlm_t = torch.as_tensor(np.ascontiguousarray(nlm) if isinstance(nlm, np.ndarray) else nlm)
+            nlm_t = nlm_t.to(self.dev, torch.complex128).contiguous()
+            self.P = nlm_t.shape[0]
+            self.x = torch.empty((self.n, self.P), dtype=torch.float64, device=self.dev)
+            self.set_nlm(nlm_t)
+        else:
+            self.P = int(P)
+            self.x = torch.zeros((self.n, self.P), dtype=torch.float64, device=self.dev)
+            self.x[0] = 1 / np.sqrt(4 * np.pi)
+        self.ugrad = self._f64(ugrad, (self.P, 3, 3)) if ugrad is not None else None
+        self.tau = self._f64(tau, (self.P, 3, 3)) if tau is not None else None
+        self.gam0 = self._f64(gam0, (self.P,)) if gam0 is not None else None
+        self.lam = self._f64(lam, (self.P,)) if lam is not None else None
+        self.steps = (torch.as_tensor(steps).to(self.dev, torch.int32).contiguous() if steps is not None else None)
+        self.status = torch.zeros(self.P, dtype=torch.int32, device=self.dev)
+
+    def _f64(self, a, shape):
+        t = torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a)
+        t = t.to(self.dev, torch.float64)
+        t = t.expand(shape) if t.dim() < len(shape) else t
+        return t.reshape(shape).contiguous()
+
+    # ---- state conversion ----
+    def set_nlm(self, nlm):
+        nlm_t = torch.as_tensor(nlm).to(self.dev, torch.complex128).contiguous()
+        assert nlm_t.shape == (self.P, self.n)
+        check(lib().sfg_nlm_to_state(self.ctx.handle, self.L, self.P, _ptr(torch.view_as_real(nlm_t)), _ptr(self.x),
+                                     _stream()))
+
+    def nlm(self):
+        out = torch.empty((self.P, self.n), dtype=torch.complex128, device=self.dev)
+        check(lib().sfg_state_to_nlm(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(torch.view_as_real(out)),
+                                     _stream()))
+        return out
+
+    # ---- evolution ----
+    def evolve(self, nsteps, dt, scheme="rk4", procs=("LROT", "REG"), iota=1.0, zeta=0.0, precision="fp64"):
+        if self.ugrad is None:
+            raise ValueError("ugrad is required")
+        mask = 0
+        for p in procs:
+            mask |= _lib.PROC_BITS[p]
+        d = EvolveDesc()
+        d.L = self.L
+        d.P = self.P
+        d.procs = mask
+        d.scheme = _lib.SFG_EULER if scheme == "euler" else _lib.SFG_RK4
+        d.precision = _lib.SFG_FP64 if precision == "fp64" else _lib.SFG_FP32C
+        d.nsteps = int(nsteps)
+        d.dt = float(dt)
+        d.iota = float(iota)
+        d.zeta = float(zeta)
+        d.ugrad_dev = self.ugrad.data_ptr()
+        d.tau_dev = self.tau.data_ptr() if self.tau is not None else None
+        d.gam0_dev = self.gam0.data_ptr() if self.gam0 is not None else None
+        d.lam_dev = self.lam.data_ptr() if self.lam is not None else None
+        d.steps_dev = self.steps.data_ptr() if self.steps is not None else None
+        d.x_dev = self.x.data_ptr()
+        d.status_dev = self.status.data_ptr()
+        check(lib().sfg_evolve(self.ctx.handle, _c.byref(d), _stream()))
+        return self
+
+    # ---- moments ----
+    def a2(self):
+        out = torch.empty((self.P, 3, 3), dtype=torch.float64, device=self.dev)
+        check(lib().sfg_a2_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(out), _stream()))
+        return out
+
+    def a4_mandel(self):
+        out = torch.empty((self.P, 6, 6), dtype=torch.float64, device=self.dev)
+        check(lib().sfg_a4mandel_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(out), _stream()))
+        return out

@@ -1,0 +1,75 @@
+"""Build libspecfab_gpu.so in-tree (hipcc, gfx950) and its generated sources.
+
+    python -m specfab_amd.build      (or __graft_entry__.build())
+
+Steps: (1) the coupling tables (specfab_amd/tables/make_gaunt.py) are
+generated into csrc/generated/ unless present; (2) the register-resident
+steppers are generated (specfab_amd/tables/gen_fast.py); (3) all HIP/C++
+sources are compiled into specfab_amd/libspecfab_gpu.so.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "generated")
+LIB = os.path.join(HERE, "libspecfab_gpu.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SFG_ARCH", "gfx950")
+
+SOURCES = ["sfg_basis.cpp", "sfg_api.cpp", "sfg_kernels.hip", "sfg_step_fast.hip"]
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument", "-Wno-unused-result"]
+
+
+def run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, **kw)
+
+
+def gen_tables(force=False):
+    os.makedirs(GEN, exist_ok=True)
+    inc = os.path.join(GEN, "gaunt_L20.inc")
+    if force or not os.path.exists(inc):
+        run([sys.executable, os.path.join(HERE, "tables", "make_gaunt.py"), inc])
+    gen_fast = os.path.join(HERE, "tables", "gen_fast.py")
+    if os.path.exists(gen_fast):
+        run([sys.executable, gen_fast, GEN])
+
+
+def compile_lib(jobs=8):
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = list(SOURCES)
+    for f in sorted(os.listdir(GEN)):
+        if f.endswith(".hip"):
+            srcs.append(os.path.join("generated", f))
+    procs, objs = [], []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [HIPCC] + HIPFLAGS + ["-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
+        while len([p for _, p in procs if p.poll() is None]) >= jobs:
+            import time
+            time.sleep(0.2)
+    bad = [s for s, p in procs if p.wait() != 0]
+    if bad:
+        raise RuntimeError("compile failed: %s" % bad)
+    tmp = LIB + ".tmp"
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def main():
+    gen_tables()
+    print("built", compile_lib())
+
+
+if __name__ == "__main__":
+    main()

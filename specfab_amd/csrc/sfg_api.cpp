@@ -1,0 +1,482 @@
+// C-ABI layer of libspecfab_gpu (declared in include/specfab_gpu.h).
+//
+// A context owns: the device copies of the per-L real-basis operators and of
+// the coupling-table CSR (uploaded lazily, once per L), a HIP stream, and a
+// grow-only device workspace for per-parcel parameters and RK scratch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/specfab_gpu.h"
+#include "sfg_internal.h"
+#include "sfg_kernels.h"
+#include "sfg_fast.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(SFG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+hipError_t upload(T** dptr, const std::vector<T>& v) {
+    size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    hipError_t e = hipMalloc((void**)dptr, bytes);
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+struct DevOp {
+    int L = 0, n = 0, nblk = 0, nq = 0;
+    double reg_nu = 0;
+    bool reg_defined = false;
+    int2* rows = nullptr;
+    int* row_blk = nullptr;
+    int* grp_param = nullptr;
+    int2* grp_rng = nullptr;
+    int* tcol = nullptr;
+    double* tval = nullptr;
+    double* A2 = nullptr;
+    double* A4 = nullptr;
+    double* reg_shape = nullptr;
+    int* r2c = nullptr;
+    int* rpart = nullptr;
+    int *cre = nullptr, *cim = nullptr, *csgn = nullptr, *cneg = nullptr;
+    int* cl = nullptr;   // l of each complex index
+    std::vector<void*> owned;
+};
+
+}  // namespace
+
+struct sfg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<int, DevOp> ops;
+    // coupling-table CSR by output row i (all four tables)
+    int* csr_row_ptr = nullptr;
+    int4* csr_ent = nullptr;
+    double* csr_val = nullptr;
+    // workspace
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+namespace {
+
+int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
+    if (L < 0 || L > sfg::LMAX_TABLE || (L % 2)) return fail(SFG_ERR_L, "L must be even and 0 <= L <= 20");
+    auto it = ctx->ops.find(L);
+    if (it != ctx->ops.end()) {
+        *out = &it->second;
+        return SFG_OK;
+    }
+    const sfg::RealOperator& op = sfg::real_operator(L);
+    DevOp d;
+    d.L = L;
+    d.n = op.n;
+    d.nblk = L / 2 + 1;
+    d.nq = std::min(op.n, 15);
+    d.reg_nu = op.reg_nu;
+    d.reg_defined = op.reg_defined;
+    std::vector<int2> rows;
+    std::vector<int> row_blk, gparam, tcol, r2c, rpart;
+    std::vector<int2> grng;
+    std::vector<double> tval;
+    for (auto& r : op.rows) { rows.push_back(make_int2(r.g0, r.g1)); row_blk.push_back(r.l / 2); }
+    for (auto& g : op.groups) { gparam.push_back(g.param); grng.push_back(make_int2(g.t0, g.t1)); }
+    for (auto& t : op.terms) { tcol.push_back(t.col); tval.push_back(t.val); }
+    auto lm = sfg::lm_list(L);
+    for (auto& ri : op.ridx) {
+        int j = -1;
+        for (int q = 0; q < op.n; q++)
+            if (lm[q].l == ri.l && lm[q].m == ri.m) j = q;
+        r2c.push_back(j);
+        rpart.push_back(ri.c);
+    }
+    std::vector<double> A2(6 * 15), A4(36 * 15);
+    for (int i = 0; i < 6; i++) for (int q = 0; q < 15; q++) A2[i * 15 + q] = op.A2[i][q];
+    for (int i = 0; i < 36; i++) for (int q = 0; q < 15; q++) A4[i * 15 + q] = op.A4[i][q];
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(upload(&d.rows, rows));
+    HIPCHK(upload(&d.row_blk, row_blk));
+    HIPCHK(upload(&d.grp_param, gparam));
+    HIPCHK(upload(&d.grp_rng, grng));
+    HIPCHK(upload(&d.tcol, tcol));
+    HIPCHK(upload(&d.tval, tval));
+    HIPCHK(upload(&d.A2, A2));
+    HIPCHK(upload(&d.A4, A4));
+    HIPCHK(upload(&d.reg_shape, op.reg_shape));
+    HIPCHK(upload(&d.r2c, r2c));
+    HIPCHK(upload(&d.rpart, rpart));
+    HIPCHK(upload(&d.cre, op.cre));
+    HIPCHK(upload(&d.cim, op.cim));
+    HIPCHK(upload(&d.csgn, op.csgn));
+    HIPCHK(upload(&d.cneg, op.cneg));
+    std::vector<int> cl;
+    for (auto& v : lm) cl.push_back(v.l);
+    HIPCHK(upload(&d.cl, cl));
+    d.owned = {d.rows, d.row_blk, d.grp_param, d.grp_rng, d.tcol, d.tval, d.A2, d.A4, d.reg_shape,
+               d.r2c, d.rpart, d.cre, d.cim, d.csgn, d.cneg, d.cl};
+    auto res = ctx->ops.emplace(L, std::move(d));
+    *out = &res.first->second;
+    return SFG_OK;
+}
+
+int ensure_csr(sfg_ctx* ctx) {
+    if (ctx->csr_row_ptr) return SFG_OK;
+    const auto& ents = sfg::gaunt_entries();
+    std::vector<const sfg::GauntEntry*> v;
+    for (auto& e : ents) v.push_back(&e);
+    std::stable_sort(v.begin(), v.end(), [](const sfg::GauntEntry* a, const sfg::GauntEntry* b) {
+        if (a->i != b->i) return a->i < b->i;
+        if (a->j != b->j) return a->j < b->j;
+        if (a->t != b->t) return a->t < b->t;
+        return a->k < b->k;
+    });
+    std::vector<int> rp(sfg::NMAX + 1, 0);
+    std::vector<int4> en;
+    std::vector<double> va;
+    for (auto* e : v) {
+        rp[e->i + 1]++;
+        en.push_back(make_int4(e->t, e->j, e->k, 0));
+        va.push_back(e->v);
+    }
+    for (int i = 0; i < sfg::NMAX; i++) rp[i + 1] += rp[i];
+    HIPCHK(upload(&ctx->csr_row_ptr, rp));
+    HIPCHK(upload(&ctx->csr_ent, en));
+    HIPCHK(upload(&ctx->csr_val, va));
+    return SFG_OK;
+}
+
+int workspace(sfg_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->ws_bytes) {
+        if (ctx->ws) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            HIPCHK(hipFree(ctx->ws));
+            ctx->ws = nullptr;
+            ctx->ws_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&ctx->ws, bytes));
+        ctx->ws_bytes = bytes;
+    }
+    *out = ctx->ws;
+    return SFG_OK;
+}
+
+hipStream_t pick(sfg_ctx* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
+
+}  // namespace
+
+extern "C" {
+
+const char* sfg_last_error(void) { return g_err.c_str(); }
+
+int sfg_nlm_len(int L) {
+    if (L < 0 || (L % 2)) return fail(SFG_ERR_L, "L must be even and >= 0");
+    return sfg::nlm_len(L);
+}
+
+int sfg_lm(int L, int* lm) {
+    if (!lm) return fail(SFG_ERR_ARG, "lm is NULL");
+    if (L < 0 || (L % 2)) return fail(SFG_ERR_L, "L must be even and >= 0");
+    int j = 0;
+    for (auto& v : sfg::lm_list(L)) { lm[2 * j] = v.l; lm[2 * j + 1] = v.m; j++; }
+    return SFG_OK;
+}
+
+int sfg_ctx_create(int device, sfg_ctx** ctx) {
+    if (!ctx) return fail(SFG_ERR_ARG, "ctx is NULL");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SFG_ERR_ARG, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new sfg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(SFG_ERR_HIP, hipGetErrorString(e)); }
+    *ctx = c;
+    return SFG_OK;
+}
+
+int sfg_ctx_destroy(sfg_ctx* ctx) {
+    if (!ctx) return SFG_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->ops)
+        for (void* p : kv.second.owned) hipFree(p);
+    hipFree(ctx->csr_row_ptr);
+    hipFree(ctx->csr_ent);
+    hipFree(ctx->csr_val);
+    if (ctx->ws) hipFree(ctx->ws);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SFG_OK;
+}
+
+// ---------------------------------------------------------------- conversions
+
+int sfg_nlm_to_state(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* x_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!nlm_dev || !x_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_nlm_to_state(P, op->n, op->r2c, op->rpart, nlm_dev, x_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_state_to_nlm(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* nlm_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!nlm_dev || !x_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_state_to_nlm(P, op->n, op->cre, op->cim, op->csgn, op->cneg, x_dev, nlm_dev,
+                                    pick(ctx, stream)));
+    return SFG_OK;
+}
+
+// ---------------------------------------------------------------- evolution
+
+int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
+    if (!ctx || !d) return fail(SFG_ERR_ARG, "ctx/desc is NULL");
+    if (d->P < 0 || (d->P > 0 && (!d->ugrad_dev || !d->x_dev))) return fail(SFG_ERR_ARG, "ugrad_dev/x_dev missing");
+    if (d->scheme != SFG_EULER && d->scheme != SFG_RK4) return fail(SFG_ERR_ARG, "unknown scheme");
+    if (d->precision != SFG_FP64 && d->precision != SFG_FP32C) return fail(SFG_ERR_ARG, "unknown precision");
+    if (d->nsteps < 0) return fail(SFG_ERR_ARG, "nsteps < 0");
+    if (d->procs & ~(SFG_LROT | SFG_DDRX | SFG_CDRX | SFG_REG)) return fail(SFG_ERR_ARG, "unknown process bit");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, d->L, &op);
+    if (rc) return rc;
+    if ((d->procs & SFG_REG) && !op->reg_defined)
+        return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
+    if (d->P == 0) return SFG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = pick(ctx, stream);
+    const int64_t P = d->P;
+    const int n = op->n;
+    size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
+    size_t nscr = (size_t)3 * n * P;
+    void* ws;
+    rc = workspace(ctx, (nth + nom + nsl + nscr) * sizeof(double), &ws);
+    if (rc) return rc;
+    double* theta = (double*)ws;
+    double* omega = theta + nth;
+    double* sl = omega + nom;
+    double* scr = sl + nsl;
+
+    sfg::ParamArgs pa{};
+    pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
+    pa.iota = d->iota; pa.zeta = d->zeta; pa.reg_nu = op->reg_nu;
+    pa.ugrad = d->ugrad_dev; pa.tau = d->tau_dev; pa.gam0 = d->gam0_dev; pa.lam = d->lam_dev;
+    pa.A2 = op->A2; pa.A4 = op->A4; pa.reg_shape = op->reg_shape;
+    pa.theta = theta; pa.omega = omega; pa.sl = sl;
+    HIPCHK(sfg::launch_params(pa, s));
+
+    sfg::EvolveArgs ea{};
+    ea.P = P; ea.n = n; ea.nq = op->nq; ea.scheme = d->scheme; ea.nsteps = d->nsteps;
+    ea.use_ddrx = (d->procs & SFG_DDRX) ? 1 : 0;
+    ea.steps = d->steps_dev; ea.dt = d->dt;
+    ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
+    ea.tcol = op->tcol; ea.tval = op->tval;
+    ea.theta = theta; ea.omega = omega; ea.sl = sl;
+    ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
+    ea.status = d->status_dev;
+    if (d->precision == SFG_FP64 && sfg::fast_available(d->L, 0)) {
+        HIPCHK(sfg::launch_evolve_fast(d->L, 0, ea, s));
+    } else if (d->precision == SFG_FP32C && sfg::fast_available(d->L, 1)) {
+        HIPCHK(sfg::launch_evolve_fast(d->L, 1, ea, s));
+    } else if (d->precision == SFG_FP64) {
+        HIPCHK(sfg::launch_evolve_generic(ea, s));
+    } else {
+        return fail(SFG_ERR_ARG, "FP32C stepping is not available for this L");
+    }
+    return SFG_OK;
+}
+
+// ---------------------------------------------------------------- moments
+
+int sfg_a2_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a2_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !a2_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (L < 2) return fail(SFG_ERR_L, "a2 needs L >= 2");
+    HIPCHK(sfg::launch_a2(P, x_dev, a2_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_a4mandel_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4v_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !a4v_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_a4mandel(P, op->nq, op->A4, x_dev, a4v_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+// ---------------------------------------------------------------- dense operators
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+
+int dense_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const double* A_dev, const double* B_dev, double iota,
+                double zeta, double* M_dev, hipStream_t s) {
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    rc = ensure_csr(ctx);
+    if (rc) return rc;
+    sfg::DenseArgs a{};
+    a.P = P; a.n = op->n; a.mode = mode; a.iota = iota; a.zeta = zeta;
+    a.A = A_dev; a.B = B_dev; a.row_ptr = ctx->csr_row_ptr; a.ent = ctx->csr_ent; a.val = ctx->csr_val; a.M = M_dev;
+    HIPCHK(sfg::launch_dense_op(a, s));
+    return SFG_OK;
+}
+
+// single-parcel helper: stage 3x3 inputs, build on the GPU, copy M back
+int dense_single(sfg_ctx* ctx, int L, int mode, const double* A, const double* B, double iota, double zeta,
+                 double* M) {
+    if (!ctx || !A || !M || (mode == 0 && !B)) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = sfg::nlm_len(L);
+    DevBuf in, out;
+    HIPCHK(hipMalloc(&in.p, 18 * sizeof(double)));
+    HIPCHK(hipMalloc(&out.p, (size_t)n * n * 2 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(in.p, A, 9 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (B) HIPCHK(hipMemcpyAsync((double*)in.p + 9, B, 9 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = dense_batch(ctx, L, 1, mode, (double*)in.p, (double*)in.p + 9, iota, zeta, (double*)out.p, ctx->stream);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(M, out.p, (size_t)n * n * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SFG_OK;
+}
+
+}  // namespace
+
+int sfg_M_LROT_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, const double* omg_dev, double iota,
+                     double zeta, double* M_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!eps_dev || !omg_dev || !M_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return dense_batch(ctx, L, P, 0, eps_dev, omg_dev, iota, zeta, M_dev, pick(ctx, stream));
+}
+
+int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* M_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!tau_dev || !M_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return dense_batch(ctx, L, P, 1, tau_dev, nullptr, 0, 0, M_dev, pick(ctx, stream));
+}
+
+int sfg_M_LROT(sfg_ctx* ctx, int L, const double eps[9], const double omg[9], double iota, double zeta, double* M) {
+    return dense_single(ctx, L, 0, eps, omg, iota, zeta, M);
+}
+
+int sfg_M_DDRX_src(sfg_ctx* ctx, int L, const double tau[9], double* M) {
+    return dense_single(ctx, L, 1, tau, nullptr, 0, 0, M);
+}
+
+// <D> and a2 of one host nlm, evaluated on the GPU (state conversion + kernel)
+static int single_state_eval(sfg_ctx* ctx, int L, const double* nlm, const double* tau, double* D, double* a2) {
+    if (!ctx || !nlm) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(2 * n + n + 9 + 9 + 1) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* dt = dx + n;
+    double* da2 = dt + 9;
+    double* dD = da2 + 9;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    if (D) {
+        HIPCHK(hipMemcpyAsync(dt, tau, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(sfg::launch_evD2(1, op->nq, op->A2, op->A4, dx, dt, dD, s));
+        HIPCHK(hipMemcpyAsync(D, dD, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (a2) {
+        HIPCHK(sfg::launch_a2(1, dx, da2, s));
+        HIPCHK(hipMemcpyAsync(a2, da2, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* D) {
+    if (!tau || !D) return fail(SFG_ERR_ARG, "NULL argument");
+    if (L < 4) return fail(SFG_ERR_L, "<D> needs L >= 4");
+    return single_state_eval(ctx, L, nlm, tau, D, nullptr);
+}
+
+int sfg_a2(sfg_ctx* ctx, int L, const double* nlm, double a2[9]) {
+    if (!a2) return fail(SFG_ERR_ARG, "NULL argument");
+    if (L < 2) return fail(SFG_ERR_L, "a2 needs L >= 2");
+    return single_state_eval(ctx, L, nlm, nullptr, nullptr, a2);
+}
+
+int sfg_M_DDRX(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* M) {
+    if (!nlm) return fail(SFG_ERR_ARG, "nlm is NULL");
+    int rc = dense_single(ctx, L, 1, tau, nullptr, 0, 0, M);
+    if (rc) return rc;
+    // M_DDRX = M_DDRX_src - <D> I (src/dynamics.f90:266-273)
+    double D = 0;
+    rc = sfg_ev_D2(ctx, L, nlm, tau, &D);
+    if (rc) return rc;
+    int n = sfg::nlm_len(L);
+    for (int i = 0; i < n; i++) M[((size_t)i * n + i) * 2] -= D;
+    return SFG_OK;
+}
+
+static int diag_single(sfg_ctx* ctx, int L, int mode, const double* D, double* M) {
+    if (!ctx || !M || (mode == 1 && !D)) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    if (mode == 1 && !op->reg_defined) return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20");
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, ((size_t)n * n + 9) * sizeof(double)));
+    double* dM = (double*)buf.p;
+    double* dD = dM + (size_t)n * n;
+    hipStream_t s = ctx->stream;
+    if (D) HIPCHK(hipMemcpyAsync(dD, D, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_diag_op(1, n, mode, op->cl, op->reg_shape, op->reg_nu, dD, dM, s));
+    HIPCHK(hipMemcpyAsync(M, dM, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_M_CDRX(sfg_ctx* ctx, int L, double* M) { return diag_single(ctx, L, 0, nullptr, M); }
+
+int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M) { return diag_single(ctx, L, 1, D, M); }
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+"""Drop-in mirror of the reference's `specfabpy.specfab` module (the f2py
+binding, src/specfabpy.f90) for the fabric-evolution and homogenization path,
+backed by libspecfab_gpu on the GPU.
+
+Usage mirrors the reference:
+
+    from specfab_amd import specfabpy as sf
+    lm, nlm_len = sf.init(12)
+    M = sf.M_LROT(nlm, D, W, 1, 0) + sf.M_REG(nlm, D)
+
+Like the reference, `init(L)` selects the module's active truncation; unlike
+the reference, the library itself is reentrant (L travels with each call).
+Inputs are numpy arrays in the reference's index order (3x3 tensors A[i, j],
+nlm complex in specfab order); outputs are numpy arrays.  Batched, device-
+resident work goes through `specfab_amd.batch.FabricBatch` instead.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, lib
+
+_c = ctypes
+_state = {"L": None, "n": None}
+
+L2len = 6
+L4len = 15
+L6len = 28
+L8len = 45
+I20 = 3   # index of n_2^0 (0-based), src/specfabpy.f90:139-142
+I40 = 10
+I60 = 21
+I80 = 36
+
+
+def _ctx():
+    return _lib.default_context().handle
+
+
+def _L_of(nlm):
+    n = len(nlm)
+    L = int(round((-3 + np.sqrt(1 + 8 * n)) / 2))
+    if (L + 1) * (L + 2) // 2 != n:
+        raise ValueError("len(nlm)=%d is not an nlm length" % n)
+    return L
+
+
+def _m3(A):
+    return np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3))
+
+
+def _z(nlm):
+    return np.ascontiguousarray(np.asarray(nlm, dtype=np.complex128))
+
+
+def _zp(a):
+    return a.view(np.float64).ctypes.data_as(_lib._dp)
+
+
+def init(L):
+    """init(L) -> (lm (2, n), nlm_len): src/specfabpy.f90:150-160."""
+    n = check_n(lib().sfg_nlm_len(int(L)))
+    lm = np.zeros(2 * n, dtype=np.int32)
+    check(lib().sfg_lm(int(L), lm.ctypes.data_as(_lib._ip)))
+    _state["L"], _state["n"] = int(L), n
+    _lib.default_context()
+    return lm.reshape(n, 2).T.copy(), n
+
+
+def check_n(n):
+    if n < 0:
+        check(n)
+    return n
+
+
+def nlm_len():
+    return _state["n"]
+
+
+def M_LROT(nlm, eps, omg, iota, zeta):
+    """Lattice rotation operator, src/specfabpy.f90:172-180 -> dynamics.f90:52-97."""
+    L = _L_of(nlm)
+    n = len(nlm)
+    M = np.zeros((n, n), dtype=np.complex128)
+    check(lib().sfg_M_LROT(_ctx(), L, dptr(_m3(eps)), dptr(_m3(omg)), float(iota), float(zeta), _zp(M)))
+    return M
+
+
+def M_DDRX(nlm, tau):
+    """DDRX operator (Gamma/Gamma0), src/specfabpy.f90:182-190 -> dynamics.f90:251-275."""
+    L = _L_of(nlm)
+    n = len(nlm)
+    M = np.zeros((n, n), dtype=np.complex128)
+    z = _z(nlm)
+    check(lib().sfg_M_DDRX(_ctx(), L, _zp(z), dptr(_m3(tau)), _zp(M)))
+    return M
+
+
+def M_DDRX_src(nlm, tau):
+    """DDRX source operator, src/specfabpy.f90:192-199 -> dynamics.f90:277-298."""
+    L = _L_of(nlm)
+    n = len(nlm)
+    M = np.zeros((n, n), dtype=np.complex128)
+    check(lib().sfg_M_DDRX_src(_ctx(), L, dptr(_m3(tau)), _zp(M)))
+    return M
+
+
+def M_CDRX(nlm):
+    """CDRX operator (unscaled Laplacian), src/specfabpy.f90:228-235."""
+    L = _L_of(nlm)
+    n = len(nlm)
+    M = np.zeros((n, n))
+    check(lib().sfg_M_CDRX(_ctx(), L, dptr(M)))
+    return M.astype(np.complex128)
+
+
+def M_REG(nlm, eps):
+    """Regularization operator, src/specfabpy.f90:237-245 -> dynamics.f90:494-518."""
+    L = _L_of(nlm)
+    n = len(nlm)
+    M = np.zeros((n, n))
+    check(lib().sfg_M_REG(_ctx(), L, dptr(_m3(eps)), dptr(M)))
+    return M
+
+
+def a2(nlm):
+    """a2 = <c^2>, src/moments.f90:37-44."""
+    L = _L_of(nlm)
+    out = np.zeros(9)
+    z = _z(nlm)
+    check(lib().sfg_a2(_ctx(), L, _zp(z), dptr(out)))
+    return out.reshape(3, 3)
+
+
+def ev_D2(nlm, tau):
+    L = _L_of(nlm)
+    out = np.zeros(1)
+    z = _z(nlm)
+    check(lib().sfg_ev_D2(_ctx(), L, _zp(z), dptr(_m3(tau)), dptr(out)))
+    return float(out[0])
+
+
+def ugrad_to_D_and_W(ugrad):
+    """src/deformationmodes.f90:29-35 (kinematics helper, host arithmetic only)."""
+    G = np.asarray(ugrad, dtype=np.float64)
+    return (G + G.T) / 2, (G - G.T) / 2

@@ -1,0 +1,175 @@
+"""Pins the C oracle (oracle/sf_oracle.c) before it is trusted as the checker:
+  * against the reference's own FP64 Fortran outputs (tests/golden/*.npz,
+    written by tests/golden/make_golden.py from oracle/_ref);
+  * against closed forms and invariants that a dropped term, a wrong sign or
+    a transposed operand would break (isotropic and delta fabrics, mass
+    conservation, the pure-spin rotation generator, Jeffery's a2 equation).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, relerr
+
+ISO = 1 / np.sqrt(4 * np.pi)
+
+
+@pytest.mark.parametrize("L", [4, 8, 12])
+def test_operators_match_reference(oracle, L):
+    g = load_golden("ops_L%d.npz" % L)
+    for k in range(len(g["iota"])):
+        assert relerr(oracle.M_LROT(L, g["eps"][k], g["omg"][k], g["iota"][k], g["zeta"][k]), g["M_LROT"][k]) < 1e-13
+        assert relerr(oracle.M_DDRX_src(L, g["tau"][k]), g["M_DDRX_src"][k]) < 1e-13
+        assert relerr(oracle.M_DDRX(L, g["nlm"][k], g["tau"][k]), g["M_DDRX"][k]) < 1e-13
+        assert relerr(oracle.M_REG(L, g["eps"][k]), g["M_REG"][k]) < 1e-14
+        assert abs(oracle.ev_D2(g["nlm"][k], g["tau"][k]) - g["ev_D2"][k]) < 1e-13 * abs(g["ev_D2"][k])
+        assert relerr(oracle.a2(g["nlm"][k]), g["a2"][k]) < 1e-14
+        a2v, a4v = oracle.ev_ck_Mandel(g["nlm"][k])
+        assert relerr(a2v, g["a2v"][k]) < 1e-14
+        assert relerr(a4v, g["a4v"][k]) < 1e-14
+    assert relerr(oracle.M_CDRX(L), g["M_CDRX"]) == 0.0
+
+
+def test_c1_trajectory_matches_reference(oracle):
+    g = load_golden("c1_traj.npz")
+    L = int(g["L"])
+    x = g["nlm0"].copy()
+    for t in range(1, g["traj"].shape[0]):
+        x = oracle.evolve(L, x, g["ugrad"], 0.0, 0.0, int(g["every"]), float(g["dt"]), "euler", ("LROT", "REG"))
+        assert relerr(x[0], g["traj"][t]) < 1e-12, t
+
+
+def test_field_rk4_matches_reference(oracle):
+    g = load_golden("field_L12.npz")
+    out = oracle.evolve(12, g["nlm0"], g["ugrad"], g["gam0"], 0.0, int(g["nsteps"]), float(g["dt"]), "rk4",
+                        ("LROT", "REG", "DDRX"))
+    for p in range(out.shape[0]):
+        assert relerr(out[p], g["nlm"][p]) < 1e-12
+
+
+def test_uneven_matches_reference(oracle):
+    g = load_golden("uneven.npz")
+    for L in (4, 8, 12, 20):
+        for k in range(len(g["steps_L%d" % L])):
+            out = oracle.evolve(L, g["nlm0_L%d" % L][k:k + 1], g["ugrad_L%d" % L][k:k + 1], g["gam0_L%d" % L][k:k + 1],
+                                g["lam_L%d" % L][k:k + 1], int(g["steps_L%d" % L][k]), 0.5, "rk4",
+                                ("LROT", "REG", "DDRX", "CDRX"))
+            assert relerr(out[0], g["nlm_L%d" % L][k]) < 1e-12, (L, k)
+
+
+def test_euler_ddrx_matches_reference(oracle):
+    g = load_golden("euler_ddrx_L8.npz")
+    out = oracle.evolve(8, g["nlm0"], g["ugrad"], g["gam0"], 0.0, int(g["nsteps"]), float(g["dt"]), "euler",
+                        ("LROT", "REG", "DDRX"))
+    assert relerr(out, g["nlm"]) < 1e-12
+
+
+# ---------------------------------------------------------------- closed forms
+
+
+def delta_z(L):
+    """n(r) = delta(r - z) + delta(r + z) normalised to n00 = 1/sqrt(4 pi):
+    n_l0 = sqrt(2l+1)/sqrt(4 pi), m != 0 zero (src/idealstate.f90:15-32)."""
+    lm = [(l, m) for l in range(0, L + 1, 2) for m in range(-l, l + 1)]
+    return np.array([np.sqrt(2 * l + 1) * ISO if m == 0 else 0.0 for (l, m) in lm], dtype=complex)
+
+
+def test_moments_closed_forms(oracle):
+    iso = np.zeros(45, complex)
+    iso[0] = ISO
+    assert np.allclose(oracle.a2(iso), np.eye(3) / 3, atol=1e-15)
+    _, a4v = oracle.ev_ck_Mandel(iso)
+    # isotropic a4 in Mandel form: diag block 1/5, off 1/15, shear 2/15
+    ref = np.zeros((6, 6))
+    ref[:3, :3] = 1 / 15
+    np.fill_diagonal(ref[:3, :3], 1 / 5)
+    ref[3:, 3:] = np.eye(3) * 2 / 15
+    assert np.allclose(a4v, ref, atol=1e-15)
+    dz = delta_z(8)
+    assert np.allclose(oracle.a2(dz), np.diag([0, 0, 1.0]), atol=1e-15)
+    _, a4v = oracle.ev_ck_Mandel(dz)
+    assert abs(a4v[2, 2] - 1) < 1e-14 and np.abs(a4v).sum() - 1 < 1e-13
+
+
+def test_deformability_closed_forms(oracle):
+    rng = np.random.default_rng(3)
+    iso = np.zeros(15, complex)
+    iso[0] = ISO
+    for _ in range(3):
+        t = rng.normal(size=(3, 3))
+        t = (t + t.T) / 2
+        t -= np.trace(t) / 3 * np.eye(3)
+        # isotropic fabric: <D> = 1 by the tau:tau/5 normalisation
+        assert abs(oracle.ev_D2(iso, t) - 1) < 1e-14
+    # delta fabric along z under uniaxial compression along z: basal shear stress vanishes
+    tz = np.diag([0.5, 0.5, -1.0])
+    assert abs(oracle.ev_D2(delta_z(4), tz)) < 1e-14
+    # ... and under xz shear it is maximal: 5 (tau.tau):zz - tau:zzzz:tau over tau:tau/5 = 5/2
+    txz = np.zeros((3, 3))
+    txz[0, 2] = txz[2, 0] = 1.0
+    assert abs(oracle.ev_D2(delta_z(4), txz) - 2.5) < 1e-14
+
+
+@pytest.mark.parametrize("L", [8, 12])
+def test_mass_conservation(oracle, L):
+    """Row 0 (d n00/dt) of the LROT and DDRX source operators vanishes up to
+    the float32 rounding of the coupling tables (docs/fabdyn-DDRX.md:45)."""
+    rng = np.random.default_rng(L)
+    G = rng.normal(size=(3, 3))
+    G -= np.trace(G) / 3 * np.eye(3)
+    D, W = (G + G.T) / 2, (G - G.T) / 2
+    M = oracle.M_LROT(L, D, W, 1.0, 0.0)
+    assert np.abs(M[0]).max() < 2e-6 * np.abs(M).max()
+    # DDRX: the source row 0 is balanced by the <D> sink, (M_DDRX n)_0 = 0
+    nlm = np.zeros(M.shape[0], complex)
+    nlm[0] = ISO
+    nlm[3] = 0.1
+    nlm[10] = 0.05
+    S = oracle.M_DDRX(L, nlm, D)
+    Dv = oracle.ev_D2(nlm, D)
+    assert abs((S @ nlm)[0]) < 5e-6 * abs(Dv * nlm[0])
+
+
+def test_pure_spin_is_rotation_generator(oracle):
+    """iota = zeta = 0: M_LROT is the generator of a rigid rotation about z,
+    d n_lm/dt = -i m w n_lm (n(r, t) = n0(R_z(-w t) r))."""
+    L = 8
+    w = 0.37
+    W = np.array([[0, -w, 0], [w, 0, 0], [0, 0, 0.0]])
+    eps = np.diag([1.0, -1.0, 0.0])  # any nonzero strain rate; iota = 0 switches it off
+    M = oracle.M_LROT(L, eps, W, 0.0, 0.0)
+    lm = [(l, m) for l in range(0, L + 1, 2) for m in range(-l, l + 1)]
+    expect = np.diag([-1j * m * w for (l, m) in lm])
+    assert np.abs(M - expect).max() < 2e-6 * w * L
+
+
+def test_jeffery_a2_equation(oracle):
+    """The l <= 2 rows of M_LROT(D, W, iota=1) reproduce the tensorial lattice
+    rotation of a2 (src/tensorialdynamics.f90:14-29):
+      da2/dt = W a2 - a2 W - (D a2 + a2 D - 2 a4:D)."""
+    L = 8
+    rng = np.random.default_rng(7)
+    G = rng.normal(size=(3, 3))
+    G -= np.trace(G) / 3 * np.eye(3)
+    D, W = (G + G.T) / 2, (G - G.T) / 2
+    lm = [(l, m) for l in range(0, L + 1, 2) for m in range(-l, l + 1)]
+    nlm = np.zeros(len(lm), complex)
+    nlm[0] = ISO
+    for i, (l, m) in enumerate(lm):
+        if 1 <= l <= 4 and m >= 0:
+            v = rng.normal(0, 0.05) + (1j * rng.normal(0, 0.05) if m > 0 else 0)
+            nlm[i] = v
+            if m > 0:
+                nlm[lm.index((l, -m))] = (-1) ** m * np.conj(v)
+    M = oracle.M_LROT(L, D, W, 1.0, 0.0)
+    dn = M @ nlm
+    h = 1e-6
+    da2 = (oracle.a2(nlm + h * dn) - oracle.a2(nlm - h * dn)) / (2 * h)
+    a2 = oracle.a2(nlm)
+    _, a4v = oracle.ev_ck_Mandel(nlm)
+    s = np.sqrt(2)
+    Dv = np.array([D[0, 0], D[1, 1], D[2, 2], s * D[1, 2], s * D[0, 2], s * D[0, 1]])
+    a4D = a4v @ Dv
+    a4D = np.array([[a4D[0], a4D[5] / s, a4D[4] / s], [a4D[5] / s, a4D[1], a4D[3] / s],
+                    [a4D[4] / s, a4D[3] / s, a4D[2]]])
+    expect = W @ a2 - a2 @ W - (D @ a2 + a2 @ D - 2 * a4D)
+    assert np.abs(da2 - expect).max() < 1e-5 * np.abs(expect).max()
