@@ -17,8 +17,9 @@
  *     (P, 3, 3).  (Fortran callers pass transpose(A); see INTEGRATION.md.)
  *   - n x n operators are row-major: M[i*n + j], so d nlm_i/dt = M_ij nlm_j.
  *   - Pointers named *_dev are device (HBM) pointers; all others are host.
- *   - `stream` is a hipStream_t (NULL = the context's stream).  Batched calls
- *     are asynchronous on that stream; single-parcel calls synchronise.
+ *   - `stream` is a hipStream_t (NULL = HIP's default stream).  Batched calls
+ *     are asynchronous on that stream; single-parcel calls use the context's
+ *     own stream and synchronise before returning.
  *   - The device state of a batch is the real ("reality-reduced") SoA array
  *     x[a * P + p], a = 0..n-1: per even l the real DOFs
  *     [Re n_l0, Re n_l1, Im n_l1, ..., Re n_ll, Im n_ll] of a real ODF
@@ -134,6 +135,24 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* desc, void* stream);
 int sfg_a2_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a2_dev, void* stream);
 /* a4 Mandel (P,6,6), src/moments.f90:164-178 */
 int sfg_a4mandel_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4v_dev, void* stream);
+
+/* Enhancement factors Eij (P,6) of transversely isotropic grains with a
+ * linear Sachs/Taylor mix, Eij_tranisotropic (src/enhancementfactors.f90:23-69,
+ * specfabpy.f90:390-400 and the *_arr loop :474-486), for P states x_dev.
+ * frame: 0 = the a2 eigenframe of each parcel (eig, src/frames.f90:14-22;
+ * eigenvalues -> lami_dev (P,3), eigenvectors -> ei_dev (P,3,3) rows e1,e2,e3,
+ * largest eigenvalue first), 1 = caller frames frames_dev (P,3,3),
+ * 2 = Cartesian (x, y, z).  Eij_grain = (Ecc, Eca), alpha = Taylor weight,
+ * n_grain = 1 or -3.  status_dev bits: SFG_ST_TAYLOR_REG / _FAIL / _NAN. */
+int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame,
+                                const double* frames_dev, const double Eij_grain[2], double alpha, int n_grain,
+                                double* Eij_dev, double* lami_dev, double* ei_dev, int32_t* status_dev, void* stream);
+
+/* single-parcel eig(nlm) -> ei (3x3 rows), lami (3): src/frames.f90:14-22 */
+int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]);
+/* single-parcel Eij_tranisotropic(nlm, e1,e2,e3, Eij_grain, alpha, n_grain) */
+int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e1[3], const double e2[3],
+                          const double e3[3], const double Eij_grain[2], double alpha, int n_grain, double Eij[6]);
 
 /* single-parcel moments (host): a2(nlm) 3x3 (src/moments.f90:37-44) */
 int sfg_a2(sfg_ctx* ctx, int L, const double* nlm, double a2[9]);

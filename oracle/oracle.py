@@ -53,20 +53,22 @@ class Oracle:
         self._extra()
 
     def _extra(self):
-        """Homogenization entry points (present once the oracle has them)."""
+        """Homogenization entry points (registered where the oracle has them)."""
         lib = self.lib
-        self.has_homog = hasattr(lib, "sfo_Eij_tranisotropic")
-        if not self.has_homog:
-            return
-        lib.sfo_eig.argtypes = [_z, _d, _d]
-        lib.sfo_Eij_tranisotropic.argtypes = [_z, _d, _d, _d, _d, _c.c_double, _c.c_int, _d]
-        lib.sfo_Eij_tranisotropic.restype = _c.c_int
-        lib.sfo_a4.argtypes = [_z, _d]
-        lib.sfo_Cij_to_Lame_tranisotropic.argtypes = [_d, _d]
-        lib.sfo_Vi_elastic_tranisotropic.argtypes = [_z, _c.c_double, _d, _c.c_double, _c.c_int, _d, _d, _d]
-        lib.sfo_Vi_elastic_tranisotropic.restype = _c.c_int
-        lib.sfo_Vi_electromagnetic_tranisotropic.argtypes = [_z, _c.c_double, _c.c_double, _c.c_double,
-                                                             _c.c_int, _d, _d, _d]
+        sig = {
+            "sfo_eig": [_z, _d, _d],
+            "sfo_eig3": [_d, _d, _d],
+            "sfo_Eij_tranisotropic": [_z, _d, _d, _d, _d, _c.c_double, _c.c_int, _d],
+            "sfo_a4": [_z, _d],
+            "sfo_Cij_to_Lame_tranisotropic": [_d, _d],
+            "sfo_Vi_elastic_tranisotropic": [_z, _c.c_double, _d, _c.c_double, _c.c_int, _d, _d, _d],
+            "sfo_Vi_electromagnetic_tranisotropic": [_z, _c.c_double, _c.c_double, _c.c_double, _c.c_int, _d, _d, _d],
+        }
+        for name, args in sig.items():
+            if hasattr(lib, name):
+                getattr(lib, name).argtypes = args
+                getattr(lib, name).restype = _c.c_int if name.startswith("sfo_E") or name.startswith("sfo_Vi_el") \
+                    else None
 
     @staticmethod
     def _m3(A):

@@ -49,6 +49,7 @@ class RefLib:
         L.sfref_Qnorm_tranisotropic.argtypes = [_cp, _c.c_double, _dp, _dp]
         L.sfref_Vi_electromagnetic_tranisotropic.argtypes = [_cp, _c.c_double, _c.c_double, _c.c_double,
                                                              _c.c_int, _dp, _dp, _dp]
+        L.sfref_rotate_nlm4.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
         L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                    _c.c_double, _dp, _dp, _dp, _cp]
         self.L = None
@@ -146,6 +147,11 @@ class RefLib:
         self.lib.sfref_Vi_elastic_tranisotropic(self._v(nlm, np.complex128), float(alpha), self._v(Lame),
                                                 float(rho), theta.size, theta, phi, out)
         return np.ascontiguousarray(out)
+
+    def rotate_nlm4(self, nlm4, theta, phi):
+        out = np.zeros(15, dtype=np.complex128, order="F")
+        self.lib.sfref_rotate_nlm4(self._v(nlm4[:15], np.complex128), float(theta), float(phi), out)
+        return out.copy()
 
     def Qnorm_tranisotropic(self, nlm, alpha, Lame):
         out = np.zeros((3, 3), order="F")

@@ -24,6 +24,7 @@ module sfref_driver
     use frames
     use elasticities
     use wavepropagation
+    use rotation
 
     implicit none
 
@@ -173,6 +174,13 @@ contains
         real(c_double), intent(in) :: theta(ndir), phi(ndir)
         real(c_double), intent(out) :: Vi(2, ndir)
         Vi = Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi)
+    end subroutine
+
+    subroutine sfref_rotate_nlm4(nlm4, theta, phi, out) bind(C, name="sfref_rotate_nlm4")
+        complex(c_double_complex), intent(in) :: nlm4(15)
+        real(c_double), value :: theta, phi
+        complex(c_double_complex), intent(out) :: out(15)
+        out = rotate_nlm4(nlm4, theta, phi)
     end subroutine
 
     ! ---- time stepping ----

@@ -409,3 +409,221 @@ int sfo_evolve(int L, int P, int nsteps, int scheme, int procs, double dt, doubl
     free(M); free(T); free(x0); free(dg);
     return 0;
 }
+
+/* ---------------------------------------------------------------- eigenframe */
+
+/* Symmetric 3x3 eigenpairs by cyclic Jacobi rotations (textbook; the
+ * reference calls LAPACK dsyev, src/frames.f90:62-80).  Output ordered with
+ * the largest eigenvalue first (src/frames.f90:77-79); ei rows are the
+ * eigenvectors.  Eigenvector signs are arbitrary (as with dsyev); every
+ * consumer here (Eij) is sign-invariant.  pinned: reference eigenvalues and
+ * |e_i . e_i^ref| = 1 on golden states. */
+void sfo_eig3(const double *M, double *ei, double *lami) {
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (M[3 * i + j] + M[3 * j + i]);
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        if (off == 0.0) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                double theta = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double vkp = v[k][p], vkq = v[k][q];
+                    v[k][p] = c * vkp - s * vkq;
+                    v[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (a[idx[j]][idx[j]] > a[idx[i]][idx[i]]) { int t = idx[i]; idx[i] = idx[j]; idx[j] = t; }
+    for (int i = 0; i < 3; i++) {
+        lami[i] = a[idx[i]][idx[i]];
+        for (int k = 0; k < 3; k++) ei[3 * i + k] = v[k][idx[i]];
+    }
+}
+
+/* eig(nlm): eigenframe of a2, src/frames.f90:14-22 */
+void sfo_eig(const cdbl *nlm, double *ei, double *lami) {
+    double a2[9];
+    sfo_a2(nlm, a2);
+    sfo_eig3(a2, ei, lami);
+}
+
+/* ---------------------------------------------------------------- Eij */
+
+/* rheo_params_tranisotropic, src/rheologies.f90:123-132 (d = 3) */
+static void rheo_params(const double *Eg, double n, int ef, double *cI, double *cM, double *cL) {
+    double ne = (double)(ef * 2) / (n + 1);
+    *cI = (pow(Eg[0], ne) - 1) / 2;
+    *cM = (3 * (pow(Eg[0], ne) + 1) - 2) / 2 - 2 * pow(Eg[1], ne);
+    *cL = pow(Eg[1], ne) - 1;
+}
+
+static void vec_to_mat(const double *v, double *M) {
+    const double s = sqrt(2.0);
+    M[0] = v[0]; M[1] = v[5] / s; M[2] = v[4] / s;
+    M[3] = v[5] / s; M[4] = v[1]; M[5] = v[3] / s;
+    M[6] = v[4] / s; M[7] = v[3] / s; M[8] = v[2];
+}
+
+static double dinner22(const double *A, const double *B) { /* A_ij B_ji */
+    double s = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) s += A[3 * i + j] * B[3 * j + i];
+    return s;
+}
+
+/* Sachs homogenization, n' = 1 or -3: src/homogenizations.f90:69-116 */
+static void sachs(const double *tau, const double *a2v, const double *a4v, const double *Eg, int ng, double *eps) {
+    double cA, cB, cC;
+    rheo_params(Eg, (double)ng, 1, &cA, &cB, &cC);
+    double I2 = dinner22(tau, tau);
+    double e0 = (ng == 1) ? 1.0 : I2;
+    double et2[9], tv[6], a4t[6], a4tau[9];
+    vec_to_mat(a2v, et2);
+    mat_to_vec(tau, tv);
+    for (int i = 0; i < 6; i++) {
+        double s = 0;
+        for (int j = 0; j < 6; j++) s += a4v[6 * i + j] * tv[j];
+        a4t[i] = s;
+    }
+    vec_to_mat(a4t, a4tau);
+    if (ng != 1)
+        for (int i = 0; i < 9; i++) { et2[i] *= I2; a4tau[i] *= I2; }
+    double tr = dinner22(et2, tau);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double ac = 0;
+            for (int k = 0; k < 3; k++) ac += tau[3 * i + k] * et2[3 * k + j] + et2[3 * i + k] * tau[3 * k + j];
+            eps[3 * i + j] = e0 * tau[3 * i + j] - cA * tr * (i == j) + cB * a4tau[3 * i + j] + cC * ac;
+        }
+}
+
+/* anticommutator_Mandel, src/homogenizations.f90:238-258 */
+static void anticomm_mandel(const double *a, double L[6][6]) {
+    const double s = sqrt(2.0);
+    double r[6][6] = {
+        {2 * a[0], 0, 0, 0, s * a[2], s * a[1]},
+        {0, 2 * a[4], 0, s * a[5], 0, s * a[1]},
+        {0, 0, 2 * a[8], s * a[5], s * a[2], 0},
+        {0, s * a[5], s * a[5], a[4] + a[8], a[1], a[2]},
+        {s * a[2], 0, s * a[2], a[1], a[0] + a[8], a[5]},
+        {s * a[1], s * a[1], 0, a[2], a[5], a[0] + a[4]}};
+    memcpy(L, r, sizeof(r));
+}
+
+/* Cholesky solve reading only the lower triangle of A (LAPACK dposv('L')
+ * semantics); returns 0 on success, j+1 if the leading minor j is not PD. */
+static int chol_solve_lower(const double A[6][6], double *b) {
+    double Lf[6][6] = {{0}};
+    for (int j = 0; j < 6; j++) {
+        double d = A[j][j];
+        for (int k = 0; k < j; k++) d -= Lf[j][k] * Lf[j][k];
+        if (!(d > 0)) return j + 1;
+        Lf[j][j] = sqrt(d);
+        for (int i = j + 1; i < 6; i++) {
+            double s = A[i][j];
+            for (int k = 0; k < j; k++) s -= Lf[i][k] * Lf[j][k];
+            Lf[i][j] = s / Lf[j][j];
+        }
+    }
+    for (int i = 0; i < 6; i++) {
+        double s = b[i];
+        for (int k = 0; k < i; k++) s -= Lf[i][k] * b[k];
+        b[i] = s / Lf[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double s = b[i];
+        for (int k = i + 1; k < 6; k++) s -= Lf[k][i] * b[k];
+        b[i] = s / Lf[i][i];
+    }
+    return 0;
+}
+
+/* Taylor homogenization, n' = 1: src/homogenizations.f90:145-189.
+ * status |= 2 when the regularized normal equations were needed, 4 if they
+ * failed too (the reference stops).  The regularized branch uses the
+ * unfactored P (the reference passes the partially factored P left behind by
+ * the failed dposv; parity unpinned for that branch). */
+static void taylor(const double *tau, const double *a2v, const double *a4v, const double *Eg, int ng, double *eps,
+                   int *status) {
+    double cA, cB, cC;
+    rheo_params(Eg, (double)ng, -1, &cA, &cB, &cC);
+    double a2m[9], Lm[6][6], P[6][6], x[6];
+    vec_to_mat(a2v, a2m);
+    anticomm_mandel(a2m, Lm);
+    const double idv[6] = {1, 1, 1, 0, 0, 0};
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++)
+            P[i][j] = (i == j ? 1.0 : 0.0) - cA * (idv[i] * a2v[j]) + cB * a4v[6 * i + j] + cC * Lm[i][j];
+    mat_to_vec(tau, x);
+    if (chol_solve_lower(P, x) != 0) {
+        *status |= 2;
+        double R[6][6], rhs[6], t[6];
+        mat_to_vec(tau, t);
+        for (int i = 0; i < 6; i++) {
+            rhs[i] = 0;
+            for (int k = 0; k < 6; k++) rhs[i] += P[k][i] * t[k];
+            for (int j = 0; j < 6; j++) {
+                double s = 0;
+                for (int k = 0; k < 6; k++) s += P[k][i] * P[k][j];
+                R[i][j] = s + (i == j ? (double)1e-6f : 0.0);
+            }
+        }
+        if (chol_solve_lower(R, rhs) != 0) *status |= 4;
+        memcpy(x, rhs, sizeof(x));
+    }
+    vec_to_mat(x, eps);
+}
+
+/* Evw (src/enhancementfactors.f90:47-69) for the six stress states
+ * tau_vv, tau_vw (:398-413) along (e1,e2,e3): Eij_tranisotropic, :23-45.
+ * pinned: golden vectors (eigenframe and Cartesian frames), isotropic -> 1,
+ * Sachs/Taylor endpoints alpha = 0, 1. */
+int sfo_Eij_tranisotropic(const cdbl *nlm, const double *e1, const double *e2, const double *e3, const double *Eg,
+                          double alpha, int ng, double *Eij) {
+    if (ng != 1 && ng != -3) return -1;
+    double a2v[6], a4v[36];
+    sfo_ev_ck_Mandel(nlm, a2v, a4v);
+    const double *ev[3] = {e1, e2, e3};
+    const int pv[6] = {0, 1, 2, 1, 0, 0}, pw[6] = {0, 1, 2, 2, 2, 1};
+    double cA, cB, cC, sachs_iso, taylor_iso;
+    rheo_params(Eg, (double)ng, 1, &cA, &cB, &cC);
+    sachs_iso = (1 + 2.0 / 15 * cB + 2.0 / 3 * cC);
+    rheo_params(Eg, (double)ng, -1, &cA, &cB, &cC);
+    taylor_iso = 1.0 / (1 + 2.0 / 15 * cB + 2.0 / 3 * cC);
+    int status = 0;
+    for (int q = 0; q < 6; q++) {
+        const double *v = ev[pv[q]], *w = ev[pw[q]];
+        double tau[9], vw[9], es[9], et[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                vw[3 * i + j] = v[i] * w[j];
+                tau[3 * i + j] = (q < 3) ? ((i == j) / 3.0 - v[i] * v[j]) : (v[i] * w[j] + w[i] * v[j]);
+            }
+        double I2 = dinner22(tau, tau);
+        sachs(tau, a2v, a4v, Eg, ng, es);
+        taylor(tau, a2v, a4v, Eg, ng, et, &status);
+        double siso = (ng == 1) ? sachs_iso : I2;   /* __isotropic, src/homogenizations.f90:191-236 */
+        double ts = dinner22(es, vw) / (siso * dinner22(tau, vw));
+        double tt = dinner22(et, vw) / (taylor_iso * dinner22(tau, vw));
+        Eij[q] = (1 - alpha) * ts + alpha * tt;
+    }
+    return status;
+}

@@ -48,6 +48,9 @@ SIGNATURES = {
     "sfg_a2_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_a4mandel_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_a2": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_Eij_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _i, _vp, _dp, _d, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sfg_eig": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "sfg_Eij_tranisotropic": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _d, _i, _dp]),
     "sfg_ev_D2": (_i, [_vp, _i, _dp, _dp, _dp]),
 }
 

@@ -116,3 +116,21 @@ class FabricBatch:
         out = torch.empty((self.P, 6, 6), dtype=torch.float64, device=self.dev)
         check(lib().sfg_a4mandel_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(out), _stream()))
         return out
+
+    # ---- homogenization ----
+    def Eij(self, Eij_grain=(1.0, 1e3), alpha=0.0125, n_grain=1, frame="eig", frames=None, status=None):
+        """Enhancement factors (P, 6) of transversely isotropic grains
+        (Eij_tranisotropic, src/enhancementfactors.f90:23-45) in the a2
+        eigenframe ("eig", returns also eigenvalues (P,3) and eigenvectors
+        (P,3,3)), the Cartesian frame ("xyz"), or caller frames (P,3,3)."""
+        mode = {"eig": 0, "frames": 1, "xyz": 2}[frame if frames is None else "frames"]
+        fr = self._f64(frames, (self.P, 3, 3)) if frames is not None else None
+        E = torch.empty((self.P, 6), dtype=torch.float64, device=self.dev)
+        lam = torch.empty((self.P, 3), dtype=torch.float64, device=self.dev) if mode == 0 else None
+        ei = torch.empty((self.P, 3, 3), dtype=torch.float64, device=self.dev) if mode == 0 else None
+        st = status if status is not None else self.status
+        Eg = (_c.c_double * 2)(*Eij_grain)
+        check(lib().sfg_Eij_tranisotropic_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), mode, _ptr(fr), Eg,
+                                                float(alpha), int(n_grain), _ptr(E), _ptr(lam), _ptr(ei), _ptr(st),
+                                                _stream()))
+        return (E, lam, ei) if mode == 0 else E

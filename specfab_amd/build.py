@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libspecfab_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SFG_ARCH", "gfx950")
 
-SOURCES = ["sfg_basis.cpp", "sfg_api.cpp", "sfg_kernels.hip", "sfg_step_fast.hip"]
+SOURCES = ["sfg_basis.cpp", "sfg_api.cpp", "sfg_kernels.hip", "sfg_homog.hip"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", "-Wno-unused-result"]
@@ -66,9 +66,29 @@ def compile_lib(jobs=8):
     return LIB
 
 
+def compile_fortran():
+    """Fortran face (ISO_C_BINDING module specfab_gpu) and its test program,
+    built with amdflang against libspecfab_gpu.so."""
+    fc = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        print("amdflang not found; skipping the Fortran module")
+        return None
+    out = os.path.join(HERE, "fortran", "_build")
+    os.makedirs(out, exist_ok=True)
+    rocm_lib = "/opt/rocm/lib"
+    exe = os.path.join(out, "c1_parcel")
+    run([fc, "-O2", "-J", out, "-c", os.path.join(HERE, "fortran", "specfab_gpu.f90"), "-o",
+         os.path.join(out, "specfab_gpu.o")])
+    run([fc, "-O2", "-I", out, os.path.join(os.path.dirname(HERE), "tests", "fortran", "c1_parcel.f90"),
+         os.path.join(out, "specfab_gpu.o"), "-L", HERE, "-lspecfab_gpu", "-L", rocm_lib, "-lamdhip64",
+         "-Wl,-rpath," + HERE, "-Wl,-rpath," + rocm_lib, "-o", exe])
+    return exe
+
+
 def main():
     gen_tables()
     print("built", compile_lib())
+    compile_fortran()
 
 
 if __name__ == "__main__":

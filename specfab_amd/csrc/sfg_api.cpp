@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -18,6 +19,7 @@
 #include "sfg_internal.h"
 #include "sfg_kernels.h"
 #include "sfg_fast.h"
+#include "sfg_homog.h"
 
 namespace {
 
@@ -53,6 +55,7 @@ struct DevOp {
     int2* grp_rng = nullptr;
     int* tcol = nullptr;
     double* tval = nullptr;
+    float* tval_f = nullptr;
     double* A2 = nullptr;
     double* A4 = nullptr;
     double* reg_shape = nullptr;
@@ -67,6 +70,8 @@ struct DevOp {
 
 struct sfg_ctx {
     int device = 0;
+    int stepper = 0;      // SFG_STEPPER: 0 = fastest available, 1 = generic only
+    bool sync_check = false;  // SFG_SYNC_CHECK=1: synchronise and check after each launch
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::map<int, DevOp> ops;
@@ -121,6 +126,8 @@ int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
     HIPCHK(upload(&d.grp_rng, grng));
     HIPCHK(upload(&d.tcol, tcol));
     HIPCHK(upload(&d.tval, tval));
+    std::vector<float> tvalf(tval.begin(), tval.end());
+    HIPCHK(upload(&d.tval_f, tvalf));
     HIPCHK(upload(&d.A2, A2));
     HIPCHK(upload(&d.A4, A4));
     HIPCHK(upload(&d.reg_shape, op.reg_shape));
@@ -133,7 +140,7 @@ int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
     std::vector<int> cl;
     for (auto& v : lm) cl.push_back(v.l);
     HIPCHK(upload(&d.cl, cl));
-    d.owned = {d.rows, d.row_blk, d.grp_param, d.grp_rng, d.tcol, d.tval, d.A2, d.A4, d.reg_shape,
+    d.owned = {d.rows, d.row_blk, d.grp_param, d.grp_rng, d.tcol, d.tval, d.tval_f, d.A2, d.A4, d.reg_shape,
                d.r2c, d.rpart, d.cre, d.cim, d.csgn, d.cneg, d.cl};
     auto res = ctx->ops.emplace(L, std::move(d));
     *out = &res.first->second;
@@ -181,7 +188,9 @@ int workspace(sfg_ctx* ctx, size_t bytes, void** out) {
     return SFG_OK;
 }
 
-hipStream_t pick(sfg_ctx* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
+// Batched calls run on the caller's stream; NULL is HIP's default stream (the
+// convention of the HIP runtime and of PyTorch's default stream).
+hipStream_t pick(sfg_ctx*, void* stream) { return (hipStream_t)stream; }
 
 }  // namespace
 
@@ -210,6 +219,8 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
     HIPCHK(hipSetDevice(device));
     auto* c = new sfg_ctx();
     c->device = device;
+    if (const char* e = std::getenv("SFG_STEPPER")) c->stepper = (std::string(e) == "generic") ? 1 : 0;
+    if (const char* e = std::getenv("SFG_SYNC_CHECK")) c->sync_check = std::string(e) == "1";
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(SFG_ERR_HIP, hipGetErrorString(e)); }
     *ctx = c;
@@ -297,11 +308,12 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     ea.use_ddrx = (d->procs & SFG_DDRX) ? 1 : 0;
     ea.steps = d->steps_dev; ea.dt = d->dt;
     ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
-    ea.tcol = op->tcol; ea.tval = op->tval;
+    ea.tcol = op->tcol; ea.tval = op->tval; ea.tval_f = op->tval_f;
     ea.theta = theta; ea.omega = omega; ea.sl = sl;
     ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
     ea.status = d->status_dev;
-    if (d->precision == SFG_FP64 && sfg::fast_available(d->L, 0)) {
+    ea.zero = 0;
+    if (d->precision == SFG_FP64 && ctx->stepper == 0 && sfg::fast_available(d->L, 0)) {
         HIPCHK(sfg::launch_evolve_fast(d->L, 0, ea, s));
     } else if (d->precision == SFG_FP32C && sfg::fast_available(d->L, 1)) {
         HIPCHK(sfg::launch_evolve_fast(d->L, 1, ea, s));
@@ -309,6 +321,10 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
         HIPCHK(sfg::launch_evolve_generic(ea, s));
     } else {
         return fail(SFG_ERR_ARG, "FP32C stepping is not available for this L");
+    }
+    if (ctx->sync_check) {
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGetLastError());
     }
     return SFG_OK;
 }
@@ -476,6 +492,82 @@ static int diag_single(sfg_ctx* ctx, int L, int mode, const double* D, double* M
 }
 
 int sfg_M_CDRX(sfg_ctx* ctx, int L, double* M) { return diag_single(ctx, L, 0, nullptr, M); }
+
+// ---------------------------------------------------------------- enhancement factors
+
+static int eij_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame, const double* frames_dev,
+                            const double Eg[2], double alpha, int n_grain, double* Eij_dev, double* lami_dev,
+                            double* ei_dev, int32_t* status_dev, hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "Eij needs the l <= 4 coefficients (L >= 4)");
+    if (n_grain != 1 && n_grain != -3) return fail(SFG_ERR_ARG, "n_grain must be 1 or -3 (n'=3 needs a6/a8)");
+    if (frame < 0 || frame > 2 || (frame == 1 && !frames_dev)) return fail(SFG_ERR_ARG, "bad frame selection");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    sfg::EijArgs a{};
+    a.P = P; a.nq = op->nq; a.frame = frame; a.n_grain = n_grain;
+    a.Ecc = Eg[0]; a.Eca = Eg[1]; a.alpha = alpha;
+    a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4; a.frames = frames_dev;
+    a.Eij = Eij_dev; a.lami = lami_dev; a.ei = ei_dev; a.status = status_dev;
+    HIPCHK(sfg::launch_eij(a, s));
+    return SFG_OK;
+}
+
+int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame,
+                                const double* frames_dev, const double Eij_grain[2], double alpha, int n_grain,
+                                double* Eij_dev, double* lami_dev, double* ei_dev, int32_t* status_dev, void* stream) {
+    if (!ctx || !Eij_grain || P < 0 || (P > 0 && (!x_dev || !Eij_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    return eij_batch_locked(ctx, L, P, x_dev, frame, frames_dev, Eij_grain, alpha, n_grain, Eij_dev, lami_dev, ei_dev,
+                            status_dev, pick(ctx, stream));
+}
+
+// single parcel: host nlm -> device state -> kernel -> host outputs
+static int eij_single(sfg_ctx* ctx, int L, const double* nlm, int frame, const double* frame9, const double Eg[2],
+                      double alpha, int ng, double* Eij, double* ei, double* lami) {
+    if (!ctx || !nlm) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(2 * n + n + 9 + 6 + 3 + 9 + 1) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* df = dx + n;
+    double* dE = df + 9;
+    double* dl = dE + 6;
+    double* de = dl + 3;
+    int32_t* dst = (int32_t*)(de + 9);
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    if (frame9) HIPCHK(hipMemcpyAsync(df, frame9, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(dst, 0, sizeof(int32_t), s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    double Eg0[2] = {Eg ? Eg[0] : 1.0, Eg ? Eg[1] : 1.0};
+    rc = eij_batch_locked(ctx, L, 1, dx, frame, df, Eg0, alpha, ng, dE, dl, de, dst, s);
+    if (rc) return rc;
+    if (Eij) HIPCHK(hipMemcpyAsync(Eij, dE, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ei) HIPCHK(hipMemcpyAsync(ei, de, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (lami) HIPCHK(hipMemcpyAsync(lami, dl, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]) {
+    if (!ei || !lami) return fail(SFG_ERR_ARG, "NULL argument");
+    return eij_single(ctx, L, nlm, 0, nullptr, nullptr, 0.0, 1, nullptr, ei, lami);
+}
+
+int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e1[3], const double e2[3],
+                          const double e3[3], const double Eij_grain[2], double alpha, int n_grain, double Eij[6]) {
+    if (!e1 || !e2 || !e3 || !Eij_grain || !Eij) return fail(SFG_ERR_ARG, "NULL argument");
+    double f[9] = {e1[0], e1[1], e1[2], e2[0], e2[1], e2[2], e3[0], e3[1], e3[2]};
+    return eij_single(ctx, L, nlm, 1, f, Eij_grain, alpha, n_grain, Eij, nullptr, nullptr);
+}
 
 int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M) { return diag_single(ctx, L, 1, D, M); }
 

@@ -2,7 +2,11 @@
 // the per-parcel weight formulas of the reference, evaluated in FP64 with the
 // reference's REAL(4) literals where it has them.
 #pragma once
+#ifdef SFG_HOST_EMUL
+#include "sfg_emul.h"
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace sfg {

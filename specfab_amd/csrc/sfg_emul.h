@@ -1,0 +1,22 @@
+// Minimal host stand-ins for the HIP device vocabulary used by the generated
+// stepping kernels, so that tests can compile and run a generated kernel on
+// the CPU lane by lane (one lane's program order at a time) and compare it
+// with the oracle.  Used only by the CPU emulation test build (SFG_HOST_EMUL).
+#pragma once
+#include <math.h>
+#include <cmath>
+#include <cstdint>
+
+struct int2 { int x, y; };
+struct int4 { int x, y, z, w; };
+struct sfg_dim3 { unsigned x, y, z; };
+extern thread_local sfg_dim3 threadIdx, blockIdx, blockDim;
+typedef int hipError_t;
+typedef void* hipStream_t;
+#define __global__
+#define __device__
+#define __launch_bounds__(n)
+#define __shared__ static
+#define __forceinline__ inline
+using std::fma;
+using std::isfinite;

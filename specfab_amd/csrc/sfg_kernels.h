@@ -1,7 +1,19 @@
 // Kernel argument blocks and launchers (host-visible declarations).
 #pragma once
+#ifdef SFG_HOST_EMUL
+#include "sfg_emul.h"   // CPU build of the generated kernels (tests only)
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
+
+// Constant address space for read-only coupling tables: loads through such a
+// pointer with a uniform address become scalar (SGPR) loads.
+#ifdef SFG_HOST_EMUL
+#define SFG_CONST_AS
+#else
+#define SFG_CONST_AS __attribute__((address_space(4)))
+#endif
 
 namespace sfg {
 
@@ -36,6 +48,7 @@ struct EvolveArgs {
     const int2* grp_rng;
     const int* tcol;
     const double* tval;
+    const float* tval_f;   // the same coupling values rounded to FP32 (FP32 stepping)
     const double* theta;
     const double* omega;
     const double* sl;
@@ -44,6 +57,7 @@ struct EvolveArgs {
     double* XA;
     double* XB;
     int* status;
+    int64_t zero;          // always 0 (an opaque stride for the generated kernels)
 };
 
 struct DenseArgs {

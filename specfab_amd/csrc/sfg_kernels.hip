@@ -8,10 +8,8 @@
 //
 // The register-resident fast steppers (generated per L) live in
 // sfg_step_fast.hip.
-#include <hip/hip_runtime.h>
-
-#include "sfg_device.h"
 #include "sfg_kernels.h"
+#include "sfg_device.h"
 
 namespace sfg {
 
@@ -336,6 +334,7 @@ __global__ void dense_op_kernel(DenseArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
+#ifndef SFG_HOST_EMUL
 
 static inline unsigned nblocks(int64_t work, int bs) { return (unsigned)((work + bs - 1) / bs); }
 
@@ -398,5 +397,7 @@ hipError_t launch_dense_op(const DenseArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(dense_op_kernel, dim3(nblocks(a.P * a.n, 64)), dim3(64), 0, s, a);
     return hipGetLastError();
 }
+
+#endif  // SFG_HOST_EMUL
 
 }  // namespace sfg

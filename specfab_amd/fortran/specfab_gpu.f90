@@ -1,0 +1,277 @@
+! specfab_gpu: Fortran face of libspecfab_gpu (include/specfab_gpu.h) through
+! ISO_C_BINDING, mirroring the names of the reference's `use specfab` module
+! API for the fabric-evolution path (src/specfab.f90, src/dynamics.f90,
+! src/moments.f90, src/frames.f90, src/enhancementfactors.f90) so that a
+! Fortran host code can switch by changing its `use` line:
+!
+!     use specfab_gpu
+!     call initspecfab(12)                  ! as src/specfab.f90:37
+!     M = M_LROT(eps, omg, 1d0, 0d0)        ! complex(dp) (nlm_len, nlm_len)
+!     M = M + Gam0*M_DDRX(nlm, tau)
+!
+! plus the batched entry point sfg_evolve_parcels for many parcels at once.
+! The C-ABI is row-major for 3x3 tensors and n x n operators, so the wrappers
+! transpose (a Fortran array A(3,3) passed as-is is A^T in C order).
+module specfab_gpu
+
+    use iso_c_binding
+    implicit none
+
+    integer, parameter, private :: dp = 8
+    type(c_ptr), private, save :: ctx = c_null_ptr
+    integer, save :: Lcap = -1
+    integer, save :: nlm_len = 0
+
+    integer(c_int), parameter :: SFG_LROT = 1, SFG_DDRX = 2, SFG_CDRX = 4, SFG_REG = 8
+    integer(c_int), parameter :: SFG_EULER = 0, SFG_RK4 = 1, SFG_FP64 = 0, SFG_FP32C = 1
+
+    type, bind(C) :: sfg_evolve_desc
+        integer(c_int) :: L
+        integer(c_int64_t) :: P
+        integer(c_int) :: procs, scheme, precision, nsteps
+        real(c_double) :: dt, iota, zeta
+        type(c_ptr) :: ugrad_dev, tau_dev, gam0_dev, lam_dev, steps_dev, x_dev, status_dev
+    end type
+
+    interface
+        integer(c_int) function sfg_ctx_create(device, pctx) bind(C, name="sfg_ctx_create")
+            import :: c_int, c_ptr
+            integer(c_int), value :: device
+            type(c_ptr) :: pctx
+        end function
+        integer(c_int) function sfg_nlm_len(L) bind(C, name="sfg_nlm_len")
+            import :: c_int
+            integer(c_int), value :: L
+        end function
+        integer(c_int) function c_M_LROT(c, L, eps, omg, iota, zeta, M) bind(C, name="sfg_M_LROT")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            real(c_double) :: eps(9), omg(9)
+            real(c_double), value :: iota, zeta
+            complex(c_double_complex) :: M(*)
+        end function
+        integer(c_int) function c_M_DDRX(c, L, nlm, tau, M) bind(C, name="sfg_M_DDRX")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*), M(*)
+            real(c_double) :: tau(9)
+        end function
+        integer(c_int) function c_M_REG(c, L, D, M) bind(C, name="sfg_M_REG")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            real(c_double) :: D(9), M(*)
+        end function
+        integer(c_int) function c_M_CDRX(c, L, M) bind(C, name="sfg_M_CDRX")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            real(c_double) :: M(*)
+        end function
+        integer(c_int) function c_a2(c, L, nlm, a2) bind(C, name="sfg_a2")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: a2(9)
+        end function
+        integer(c_int) function c_eig(c, L, nlm, ei, lami) bind(C, name="sfg_eig")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: ei(9), lami(3)
+        end function
+        integer(c_int) function c_Eij(c, L, nlm, e1, e2, e3, Eg, alpha, ng, Eij) bind(C, name="sfg_Eij_tranisotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ng
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: e1(3), e2(3), e3(3), Eg(2), Eij(6)
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_evolve(c, desc, stream) bind(C, name="sfg_evolve")
+            import :: c_int, c_ptr, sfg_evolve_desc
+            type(c_ptr), value :: c
+            type(sfg_evolve_desc) :: desc
+            type(c_ptr), value :: stream
+        end function
+        integer(c_int) function c_nlm_to_state(c, L, P, nlm, x, stream) bind(C, name="sfg_nlm_to_state")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: c, nlm, x, stream
+            integer(c_int), value :: L
+            integer(c_int64_t), value :: P
+        end function
+        integer(c_int) function c_state_to_nlm(c, L, P, x, nlm, stream) bind(C, name="sfg_state_to_nlm")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: c, x, nlm, stream
+            integer(c_int), value :: L
+            integer(c_int64_t), value :: P
+        end function
+        ! HIP runtime (device buffers for the batched path)
+        integer(c_int) function hipMalloc(p, bytes) bind(C, name="hipMalloc")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr) :: p
+            integer(c_size_t), value :: bytes
+        end function
+        integer(c_int) function hipFree(p) bind(C, name="hipFree")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: p
+        end function
+        integer(c_int) function hipMemcpy(dst, src, bytes, kind) bind(C, name="hipMemcpy")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: bytes
+            integer(c_int), value :: kind
+        end function
+        integer(c_int) function hipDeviceSynchronize() bind(C, name="hipDeviceSynchronize")
+            import :: c_int
+        end function
+    end interface
+
+contains
+
+    subroutine check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: what
+        if (rc /= 0) then
+            print *, 'specfab_gpu error in ', what, ': ', rc
+            error stop 'specfab_gpu: libspecfab_gpu call failed'
+        end if
+    end subroutine
+
+    ! initspecfab(L): src/specfab.f90:37-53 (context on HIP device 0)
+    subroutine initspecfab(L)
+        integer, intent(in) :: L
+        if (.not. c_associated(ctx)) call check(sfg_ctx_create(0_c_int, ctx), 'sfg_ctx_create')
+        Lcap = L
+        nlm_len = sfg_nlm_len(int(L, c_int))
+    end subroutine
+
+    ! M_LROT(eps, omg, iota, zeta): src/dynamics.f90:52-97
+    function M_LROT(eps, omg, iota, zeta) result(M)
+        real(dp), intent(in) :: eps(3,3), omg(3,3), iota, zeta
+        complex(dp) :: M(nlm_len, nlm_len), Mc(nlm_len, nlm_len)
+        real(c_double) :: e(9), o(9)
+        e = reshape(transpose(eps), [9])
+        o = reshape(transpose(omg), [9])
+        call check(c_M_LROT(ctx, int(Lcap, c_int), e, o, iota, zeta, Mc), 'M_LROT')
+        M = transpose(Mc)
+    end function
+
+    ! M_DDRX(nlm, tau): src/dynamics.f90:251-275
+    function M_DDRX(nlm, tau) result(M)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(in) :: tau(3,3)
+        complex(dp) :: M(nlm_len, nlm_len), Mc(nlm_len, nlm_len), z(nlm_len)
+        real(c_double) :: t(9)
+        z = nlm(1:nlm_len)
+        t = reshape(transpose(tau), [9])
+        call check(c_M_DDRX(ctx, int(Lcap, c_int), z, t, Mc), 'M_DDRX')
+        M = transpose(Mc)
+    end function
+
+    ! M_REG(D): src/dynamics.f90:494-518
+    function M_REG(D) result(M)
+        real(dp), intent(in) :: D(3,3)
+        real(dp) :: M(nlm_len, nlm_len), Mc(nlm_len, nlm_len)
+        real(c_double) :: t(9)
+        t = reshape(transpose(D), [9])
+        call check(c_M_REG(ctx, int(Lcap, c_int), t, Mc), 'M_REG')
+        M = transpose(Mc)
+    end function
+
+    ! M_CDRX(): src/dynamics.f90:474-492
+    function M_CDRX() result(M)
+        real(dp) :: M(nlm_len, nlm_len), Mc(nlm_len, nlm_len)
+        call check(c_M_CDRX(ctx, int(Lcap, c_int), Mc), 'M_CDRX')
+        M = transpose(Mc)
+    end function
+
+    ! a2(nlm): src/moments.f90:37-44
+    function a2(nlm) result(A)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp) :: A(3,3)
+        real(c_double) :: v(9)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len)
+        call check(c_a2(ctx, int(Lcap, c_int), z, v), 'a2')
+        A = transpose(reshape(v, [3,3]))
+    end function
+
+    ! eig(nlm, ei, lami): src/frames.f90:14-22 (ei(i,:) = i-th eigenvector)
+    subroutine eig(nlm, ei, lami)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(out) :: ei(3,3), lami(3)
+        real(c_double) :: v(9)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len)
+        call check(c_eig(ctx, int(Lcap, c_int), z, v, lami), 'eig')
+        ei = transpose(reshape(v, [3,3]))
+    end subroutine
+
+    ! Eij_tranisotropic(nlm, e1,e2,e3, Eij_grain, alpha, n_grain): src/enhancementfactors.f90:23-45
+    function Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain) result(Eij)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(in) :: e1(3), e2(3), e3(3), Eij_grain(2), alpha
+        integer, intent(in) :: n_grain
+        real(dp) :: Eij(6), a1(3), a2_(3), a3(3), Eg(2)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len)
+        a1 = e1; a2_ = e2; a3 = e3; Eg = Eij_grain
+        call check(c_Eij(ctx, int(Lcap, c_int), z, a1, a2_, a3, Eg, alpha, int(n_grain, c_int), Eij), 'Eij')
+    end function
+
+    ! Batched evolution of P parcels on the GPU (host arrays in, host arrays out):
+    ! nlm(nlm_len, P) advanced in place by nsteps steps of size dt; ugrad(3,3,P)
+    ! velocity gradients, gam0(P) DDRX and lam(P) CDRX rates; procs a bitmask
+    ! of SFG_LROT/SFG_DDRX/SFG_CDRX/SFG_REG, scheme SFG_EULER or SFG_RK4.
+    subroutine sfg_evolve_parcels(nlm, ugrad, gam0, lam, nsteps, dt, procs, scheme, iota, zeta)
+        complex(dp), intent(inout), target :: nlm(:,:)
+        real(dp), intent(in) :: ugrad(:,:,:), dt, iota, zeta
+        real(dp), intent(in), target :: gam0(:), lam(:)
+        integer, intent(in) :: nsteps, procs, scheme
+        integer(c_int64_t) :: P
+        real(c_double), allocatable, target :: G(:), nlmT(:,:)
+        type(c_ptr) :: dG, dg0, dlam, dx, dnlm, dst
+        type(sfg_evolve_desc) :: d
+        integer :: p_, i, j, n
+        integer(c_size_t) :: nb
+        n = size(nlm, 1)
+        P = size(nlm, 2)
+        allocate(G(9*P))
+        do p_ = 1, int(P)
+            do i = 1, 3
+                do j = 1, 3
+                    G(9*(p_-1) + 3*(i-1) + j) = ugrad(i, j, p_)   ! row-major per parcel
+                end do
+            end do
+        end do
+        call check(hipMalloc(dG, int(9*P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dg0, int(P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dlam, int(P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dx, int(n*P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dnlm, int(2*n*P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dst, int(P*4, c_size_t)), 'hipMalloc')
+        call check(hipMemcpy(dG, c_loc(G), int(9*P*8, c_size_t), 1_c_int), 'hipMemcpy')
+        call check(hipMemcpy(dg0, c_loc(gam0), int(P*8, c_size_t), 1_c_int), 'hipMemcpy')
+        call check(hipMemcpy(dlam, c_loc(lam), int(P*8, c_size_t), 1_c_int), 'hipMemcpy')
+        ! nlm(:, p) is parcel p's coefficient vector: the (P, n) parcel-major C layout
+        nb = int(2*n*P*8, c_size_t)
+        call check(hipMemcpy(dnlm, c_loc(nlm), nb, 1_c_int), 'hipMemcpy')
+        call check(c_nlm_to_state(ctx, int(Lcap, c_int), P, dnlm, dx, c_null_ptr), 'nlm_to_state')
+        d%L = Lcap; d%P = P; d%procs = procs; d%scheme = scheme; d%precision = SFG_FP64
+        d%nsteps = nsteps; d%dt = dt; d%iota = iota; d%zeta = zeta
+        d%ugrad_dev = dG; d%tau_dev = c_null_ptr; d%gam0_dev = dg0; d%lam_dev = dlam
+        d%steps_dev = c_null_ptr; d%x_dev = dx; d%status_dev = c_null_ptr
+        call check(c_evolve(ctx, d, c_null_ptr), 'sfg_evolve')
+        call check(c_state_to_nlm(ctx, int(Lcap, c_int), P, dx, dnlm, c_null_ptr), 'state_to_nlm')
+        call check(hipDeviceSynchronize(), 'hipDeviceSynchronize')
+        call check(hipMemcpy(c_loc(nlm), dnlm, nb, 2_c_int), 'hipMemcpy')
+        call check(hipFree(dG), 'hipFree'); call check(hipFree(dg0), 'hipFree'); call check(hipFree(dlam), 'hipFree')
+        call check(hipFree(dx), 'hipFree'); call check(hipFree(dnlm), 'hipFree'); call check(hipFree(dst), 'hipFree')
+    end subroutine
+
+end module specfab_gpu

@@ -145,3 +145,37 @@ def ugrad_to_D_and_W(ugrad):
     """src/deformationmodes.f90:29-35 (kinematics helper, host arithmetic only)."""
     G = np.asarray(ugrad, dtype=np.float64)
     return (G + G.T) / 2, (G - G.T) / 2
+
+
+def eig(nlm):
+    """eig(nlm) -> (ei (3,3) rows e1,e2,e3, lami (3)), largest first: src/frames.f90:14-22."""
+    L = _L_of(nlm)
+    ei = np.zeros(9)
+    lam = np.zeros(3)
+    z = _z(nlm)
+    check(lib().sfg_eig(_ctx(), L, _zp(z), dptr(ei), dptr(lam)))
+    return ei.reshape(3, 3), lam
+
+
+def Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Enhancement factors w.r.t. (e1,e2,e3): src/specfabpy.f90:390-400."""
+    L = _L_of(nlm)
+    out = np.zeros(6)
+    z = _z(nlm)
+    v = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    check(lib().sfg_Eij_tranisotropic(_ctx(), L, _zp(z), dptr(v(e1)), dptr(v(e2)), dptr(v(e3)), dptr(v(Eij_grain)),
+                                      float(alpha), int(n_grain), dptr(out)))
+    return out
+
+
+def Eij_tranisotropic_arr(nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Batched Eij (src/specfabpy.f90:474-486): nlm (N, n), e_i (N, 3) -> (N, 6),
+    evaluated in one GPU launch."""
+    import torch
+    from .batch import FabricBatch
+    nlm = np.asarray(nlm)
+    frames = np.stack([np.asarray(e1), np.asarray(e2), np.asarray(e3)], axis=1)
+    b = FabricBatch(_L_of(nlm[0]), nlm=nlm)
+    E = b.Eij(Eij_grain=Eij_grain, alpha=alpha, n_grain=n_grain, frames=frames)
+    torch.cuda.synchronize()
+    return E.cpu().numpy()

@@ -1,0 +1,66 @@
+// CPU emulation driver (tests only): compiles the parameter kernel, the
+// generic stepper and one generated register-resident stepper as plain C++
+// (-DSFG_HOST_EMUL, see specfab_amd/csrc/sfg_emul.h) and runs them lane by
+// lane, so the generated code's indexing and arithmetic can be checked against
+// the oracle without a GPU.
+#include <cstdint>
+#include <vector>
+
+#include "sfg_internal.h"
+#include "sfg_kernels.h"
+
+thread_local sfg_dim3 threadIdx, blockIdx, blockDim;
+
+namespace sfg {
+__global__ void params_kernel(ParamArgs a);
+__global__ void evolve_generic(EvolveArgs a);
+void KERNEL_NAME(EvolveArgs a);
+}
+
+template <class F>
+static void grid(int64_t P, int bs, F f) {
+    for (int64_t b = 0; b < (P + bs - 1) / bs; b++)
+        for (int t = 0; t < bs; t++) {
+            blockIdx.x = (unsigned)b;
+            blockDim.x = (unsigned)bs;
+            threadIdx.x = (unsigned)t;
+            f();
+        }
+}
+
+// x: real SoA state (n, P), advanced in place.  which: 0 = generated kernel,
+// 1 = generic kernel.  Returns 0.
+extern "C" int emul_evolve(int L, int64_t P, int procs, int scheme, int nsteps, double dt, double iota, double zeta,
+                           const double* ugrad, const double* gam0, const double* lam, double* x, int which,
+                           int* status) {
+    const sfg::RealOperator& op = sfg::real_operator(L);
+    const int n = op.n, nblk = L / 2 + 1, nq = n < 15 ? n : 15;
+    std::vector<double> A2(6 * 15), A4(36 * 15), tval;
+    std::vector<float> tvalf;
+    for (int i = 0; i < 6; i++) for (int q = 0; q < 15; q++) A2[i * 15 + q] = op.A2[i][q];
+    for (int i = 0; i < 36; i++) for (int q = 0; q < 15; q++) A4[i * 15 + q] = op.A4[i][q];
+    std::vector<int2> rows, grng;
+    std::vector<int> row_blk, gparam, tcol;
+    for (auto& r : op.rows) { rows.push_back({r.g0, r.g1}); row_blk.push_back(r.l / 2); }
+    for (auto& g : op.groups) { gparam.push_back(g.param); grng.push_back({g.t0, g.t1}); }
+    for (auto& t : op.terms) { tcol.push_back(t.col); tval.push_back(t.val); tvalf.push_back((float)t.val); }
+    std::vector<double> theta((size_t)sfg::NPAR_K * P), omega((size_t)15 * P), sl((size_t)nblk * P);
+    std::vector<double> scr((size_t)3 * n * P, 0.0);
+    sfg::ParamArgs pa{};
+    pa.P = P; pa.nblk = nblk; pa.nq = nq; pa.procs = procs; pa.iota = iota; pa.zeta = zeta; pa.reg_nu = op.reg_nu;
+    pa.ugrad = ugrad; pa.tau = nullptr; pa.gam0 = gam0; pa.lam = lam;
+    pa.A2 = A2.data(); pa.A4 = A4.data(); pa.reg_shape = op.reg_shape.data();
+    pa.theta = theta.data(); pa.omega = omega.data(); pa.sl = sl.data();
+    grid(P, 256, [&] { sfg::params_kernel(pa); });
+    sfg::EvolveArgs ea{};
+    ea.P = P; ea.n = n; ea.nq = nq; ea.scheme = scheme; ea.nsteps = nsteps; ea.use_ddrx = (procs & 2) ? 1 : 0;
+    ea.steps = nullptr; ea.dt = dt;
+    ea.rows = rows.data(); ea.row_blk = row_blk.data(); ea.grp_param = gparam.data(); ea.grp_rng = grng.data();
+    ea.tcol = tcol.data(); ea.tval = tval.data(); ea.tval_f = tvalf.data();
+    ea.theta = theta.data(); ea.omega = omega.data(); ea.sl = sl.data();
+    ea.X = x; ea.YS = scr.data(); ea.XA = scr.data() + (size_t)n * P; ea.XB = scr.data() + (size_t)2 * n * P;
+    ea.status = status; ea.zero = 0;
+    if (which == 0) grid(P, 64, [&] { sfg::KERNEL_NAME(ea); });
+    else grid(P, 64, [&] { sfg::evolve_generic(ea); });
+    return 0;
+}

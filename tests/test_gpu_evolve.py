@@ -109,3 +109,42 @@ def test_edge_cases(gpu_ctx):
     b = FabricBatch(2, P=2, ugrad=np.ones((2, 3, 3)) * 1e-3)
     with pytest.raises(_lib.SFGError):
         b.evolve(1, 1.0, procs=("LROT", "REG"))
+
+
+def test_fp32c_field_error_vs_fp64(gpu_ctx, oracle):
+    """FP32 stepping with the FP64-accumulated state: error against the FP64
+    oracle on the L = 12 field (LROT + REG + DDRX, RK4, 200 steps)."""
+    g = load_golden("field_L12.npz")
+    out, st = run(12, g["nlm0"], g["ugrad"], g["gam0"], 0.0, int(g["nsteps"]), float(g["dt"]), "rk4",
+                  ("LROT", "REG", "DDRX"), precision="fp32c")
+    err = max(relerr(out[p], g["nlm"][p]) for p in range(out.shape[0]))
+    print("fp32c max rel err vs FP64 reference: %.3e" % err)
+    assert err < 1e-5
+    assert not st.any()
+
+
+def test_fast_vs_generic_stepper(gpu_ctx):
+    """The generated register-resident stepper and the table-driven generic
+    stepper agree on the GPU (same inputs, FP64)."""
+    from specfab_amd import _lib
+    from specfab_amd import workloads as W
+    from specfab_amd.batch import FabricBatch
+    rng = np.random.default_rng(77)
+    for L in (4, 8, 12):
+        P = 130
+        nlm0 = W.perturbed_isotropic(rng, P, L, sigma=1e-2)
+        ug = W.traceless_gradients(rng, P)
+        g0 = 10.0 ** rng.uniform(-4, -2, P)
+        outs = []
+        for stepper in ("fast", "generic"):
+            ctx = _lib.Context(0)
+            if stepper == "generic":
+                import os
+                os.environ["SFG_STEPPER"] = "generic"
+                ctx = _lib.Context(0)
+                del os.environ["SFG_STEPPER"]
+            b = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0, ctx=ctx)
+            b.evolve(7, 0.5, scheme="rk4", procs=("LROT", "REG", "DDRX"))
+            outs.append(b.nlm().cpu().numpy())
+        print("L=%d fast vs generic %.3e" % (L, relerr(outs[0], outs[1])))
+        assert relerr(outs[0], outs[1]) < 1e-12, L
