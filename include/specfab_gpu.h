@@ -72,6 +72,13 @@ int sfg_ctx_create(int device, sfg_ctx** ctx);
 int sfg_ctx_destroy(sfg_ctx* ctx);
 const char* sfg_last_error(void);
 
+/* Profiling hook, no counterpart in the reference: while enabled, every
+ * sfg_evolve brackets its stepping kernel with HIP events on the launch
+ * stream; sfg_ctx_read_timing waits for them, returns the summed kernel time
+ * (ms) and the number of launches, and clears the record. */
+int sfg_ctx_set_timing(sfg_ctx* ctx, int enable);
+int sfg_ctx_read_timing(sfg_ctx* ctx, double* ms_total, int* count);
+
 /* nlm_len (src/header.f90:23) and the (l,m) list lm(2, n) (src/header.f90:30):
  * lm[2*j] = l, lm[2*j+1] = m. */
 int sfg_nlm_len(int L);

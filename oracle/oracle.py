@@ -63,12 +63,14 @@ class Oracle:
             "sfo_Cij_to_Lame_tranisotropic": [_d, _d],
             "sfo_Vi_elastic_tranisotropic": [_z, _c.c_double, _d, _c.c_double, _c.c_int, _d, _d, _d],
             "sfo_Vi_electromagnetic_tranisotropic": [_z, _c.c_double, _c.c_double, _c.c_double, _c.c_int, _d, _d, _d],
+            "sfo_rotate_nlm4": [_z, _c.c_double, _c.c_double, _z],
+            "sfo_Qnorm_tranisotropic": [_z, _c.c_double, _d, _d],
         }
         for name, args in sig.items():
             if hasattr(lib, name):
                 getattr(lib, name).argtypes = args
-                getattr(lib, name).restype = _c.c_int if name.startswith("sfo_E") or name.startswith("sfo_Vi_el") \
-                    else None
+                getattr(lib, name).restype = _c.c_int if name in ("sfo_Eij_tranisotropic", "sfo_Vi_elastic_tranisotropic",
+                                                                 "sfo_Qnorm_tranisotropic") else None
 
     @staticmethod
     def _m3(A):
@@ -169,15 +171,25 @@ class Oracle:
     def Vi_elastic_tranisotropic(self, nlm, alpha, Lame, rho, theta, phi):
         th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
         ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
-        out = np.zeros((3, th.size))
+        out = np.zeros((th.size, 3))
         self.lib.sfo_Vi_elastic_tranisotropic(self._z(nlm), float(alpha), np.ascontiguousarray(Lame, dtype=np.float64),
                                               float(rho), th.size, th, ph, out)
+        return out.T.copy()
+
+    def Qnorm_tranisotropic(self, nlm, alpha, Lame):
+        out = np.zeros(9)
+        self.lib.sfo_Qnorm_tranisotropic(self._z(nlm), float(alpha), np.ascontiguousarray(Lame, dtype=np.float64), out)
+        return out.reshape(3, 3)
+
+    def rotate_nlm4(self, nlm4, theta, phi):
+        out = np.zeros(15, dtype=np.complex128)
+        self.lib.sfo_rotate_nlm4(self._z(nlm4[:15]), float(theta), float(phi), out)
         return out
 
     def Vi_electromagnetic_tranisotropic(self, nlm, eps_m, eps_t, mu, theta, phi):
         th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
         ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
-        out = np.zeros((2, th.size))
+        out = np.zeros((th.size, 2))
         self.lib.sfo_Vi_electromagnetic_tranisotropic(self._z(nlm), float(eps_m), float(eps_t), float(mu),
                                                       th.size, th, ph, out)
-        return out
+        return out.T.copy()

@@ -627,3 +627,358 @@ int sfo_Eij_tranisotropic(const cdbl *nlm, const double *e1, const double *e2, c
     }
     return status;
 }
+
+/* ------------------------------------------------------- full a4 tensor */
+
+/* f_ev_c4, src/moments.f90:202-209 with src/include/ev_c4__body.f90.
+ * The body is a Mathematica export whose literals are REAL(4): every
+ * coefficient is a float32 product (e.g. (-12.0)*Sqrt((5.0)) is rounded to
+ * float32 before it meets the complex(8) coefficient), the outer i*Sqrt((2.0))
+ * is a complex(4) factor and the divisor Sqrt((5.0)) is a float32 sqrt.  Each
+ * of the 15 index-sorted entries is a table row below: terms (l, m, c) with
+ * c = a (exact), a*sqrtf(b) or a*powf(b, 1.5) formed in float32.
+ * The body assigns a4(3,2,1,2) = a4(1,2,3,3) (ev_c4__body.f90, the a4(3,2,...)
+ * block), not the symmetric a4(1,2,2,3); that entry is reproduced as is.
+ * pinned: oracle/_ref a4 (all 81 entries), closed forms (isotropic
+ * a4_iijj sums, delta along z -> a4_3333 = 1) in tests/test_oracle.py. */
+typedef struct { signed char l, m, kind; float a, b; } a4term;
+enum { A4_OUT_1 = 0, A4_OUT_I = 1, A4_OUT_IS2 = 2 };
+static const struct { int outer, div5, nterm; a4term t[9]; } A4FULL[15] = {
+    /* 1111 */ {A4_OUT_1, 1, 9, {{0,0,0,42,0},{2,0,1,-12,5},{2,-2,1,6,30},{2,2,1,6,30},{4,0,0,6,0},
+                                {4,-2,1,-2,10},{4,2,1,-2,10},{4,-4,1,1,70},{4,4,1,1,70}}},
+    /* 1112 */ {A4_OUT_IS2, 0, 6, {{2,-2,1,-3,3},{2,2,2,1,3},{4,-2,0,1,0},{4,2,0,-1,0},{4,-4,1,-1,7},{4,4,1,1,7}}},
+    /* 1113 */ {A4_OUT_1, 0, 6, {{2,-1,1,3,6},{2,1,1,-3,6},{4,-1,0,-3,0},{4,1,0,3,0},{4,-3,1,1,7},{4,3,1,-1,7}}},
+    /* 1122 */ {A4_OUT_1, 1, 5, {{0,0,0,14,0},{2,0,1,-4,5},{4,0,0,2,0},{4,-4,1,-1,70},{4,4,1,-1,70}}},
+    /* 1123 */ {A4_OUT_I, 0, 6, {{2,-1,1,-1,6},{2,1,1,-1,6},{4,-1,0,1,0},{4,1,0,1,0},{4,-3,1,-1,7},{4,3,1,-1,7}}},
+    /* 1133 */ {A4_OUT_1, 1, 7, {{0,0,0,14,0},{2,0,1,2,5},{2,-2,1,1,30},{2,2,1,1,30},{4,0,0,-8,0},
+                                {4,-2,1,2,10},{4,2,1,2,10}}},
+    /* 1222 */ {A4_OUT_IS2, 0, 6, {{2,-2,1,-3,3},{2,2,2,1,3},{4,-2,0,1,0},{4,2,0,-1,0},{4,-4,1,1,7},{4,4,1,-1,7}}},
+    /* 1223 */ {A4_OUT_1, 0, 6, {{2,-1,1,1,6},{2,1,1,-1,6},{4,-1,0,-1,0},{4,1,0,1,0},{4,-3,1,-1,7},{4,3,1,1,7}}},
+    /* 1233 */ {A4_OUT_IS2, 0, 4, {{2,-2,1,-1,3},{2,2,1,1,3},{4,-2,0,-2,0},{4,2,0,2,0}}},
+    /* 1333 */ {A4_OUT_1, 0, 4, {{2,-1,1,3,6},{2,1,1,-3,6},{4,-1,0,4,0},{4,1,0,-4,0}}},
+    /* 2222 */ {A4_OUT_1, 1, 9, {{0,0,0,42,0},{2,0,1,-12,5},{2,-2,1,-6,30},{2,2,1,-6,30},{4,0,0,6,0},
+                                {4,-2,1,2,10},{4,2,1,2,10},{4,-4,1,1,70},{4,4,1,1,70}}},
+    /* 2223 */ {A4_OUT_I, 0, 6, {{2,-1,1,-3,6},{2,1,1,-3,6},{4,-1,0,3,0},{4,1,0,3,0},{4,-3,1,1,7},{4,3,1,1,7}}},
+    /* 2233 */ {A4_OUT_1, 1, 7, {{0,0,0,14,0},{2,0,1,2,5},{2,-2,1,-1,30},{2,2,1,-1,30},{4,0,0,-8,0},
+                                {4,-2,1,-2,10},{4,2,1,-2,10}}},
+    /* 2333 */ {A4_OUT_I, 0, 4, {{2,-1,1,-3,6},{2,1,1,-3,6},{4,-1,0,-4,0},{4,1,0,-4,0}}},
+    /* 3333 */ {A4_OUT_1, 1, 3, {{0,0,0,42,0},{2,0,1,24,5},{4,0,0,16,0}}},
+};
+
+static double a4coef(const a4term *t) {
+    float c;
+    if (t->kind == 0) c = t->a;
+    else if (t->kind == 1) c = t->a * sqrtf(t->b);
+    else c = t->a * powf(t->b, 1.5f);
+    return (double)c;
+}
+
+/* index (0..14) of the sorted quadruple i<=j<=k<=l (0-based components) */
+static int a4_sorted_index(int i, int j, int k, int l) {
+    int v[4] = {i, j, k, l};
+    for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++)
+            if (v[b] < v[a]) { int t = v[a]; v[a] = v[b]; v[b] = t; }
+    int idx = 0;
+    for (int a = 0; a <= 2; a++)
+        for (int b = a; b <= 2; b++)
+            for (int c = b; c <= 2; c++)
+                for (int d = c; d <= 2; d++) {
+                    if (a == v[0] && b == v[1] && c == v[2] && d == v[3]) return idx;
+                    idx++;
+                }
+    return -1;
+}
+
+/* 15 sorted entries u[] of a4 from nlm (l <= 4 used) */
+static void a4_unique(const cdbl *nlm, double *u) {
+    const double k = sqrt(PI / 5.0) / 21.0;
+    const double c0 = creal(sqrt(4 * PI) * nlm[0]);
+    const double s2 = (double)sqrtf(2.0f), s5 = (double)sqrtf(5.0f);
+    for (int e = 0; e < 15; e++) {
+        cdbl s = 0;
+        for (int q = 0; q < A4FULL[e].nterm; q++) {
+            const a4term *t = &A4FULL[e].t[q];
+            int idx = (t->l == 0) ? 0 : (t->l == 2 ? 3 : 10) + t->m;
+            s += a4coef(t) * nlm[idx];
+        }
+        if (A4FULL[e].outer == A4_OUT_I) s = I * s;
+        else if (A4FULL[e].outer == A4_OUT_IS2) s = (I * s2) * s;
+        if (A4FULL[e].div5) s = s / s5;
+        u[e] = creal(s) * k / c0;
+    }
+}
+
+void sfo_a4(const cdbl *nlm, double *a4) {
+    double u[15];
+    a4_unique(nlm, u);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) a4[27 * i + 9 * j + 3 * k + l] = u[a4_sorted_index(i, j, k, l)];
+    a4[27 * 2 + 9 * 1 + 3 * 0 + 1] = u[a4_sorted_index(0, 1, 2, 2)]; /* a4(3,2,1,2) := a4(1,2,3,3) */
+}
+
+/* ------------------------------------------------------------ rotation */
+
+static double fact(int n) {
+    double f = 1;
+    for (int i = 2; i <= n; i++) f *= i;
+    return f;
+}
+
+/* sqrt((l+m)!(l-m)!(l+n)!(l-n)!) as the reference's Mathematica-exported
+ * Wigner-D body writes it (src/include/Dlmn_L4.f90): the square-free part s of
+ * the factorial product enters as a float32 sqrt (Sqrt((7.0)), Sqrt(17.5) =
+ * sqrtf(70)/2, (2.0)**1.5 = 2 sqrtf(2), ...), except that for l = 4,
+ * {|m|,|n|} = {2,1} the body divides by Sqrt((2.0)) instead of multiplying by
+ * sqrt(2), so there 2/sqrtf(2) stands for sqrt(2). */
+static double dnorm_f32(int l, int m, int n) {
+    long long P = (long long)(fact(l + m) * fact(l - m) * fact(l + n) * fact(l - n) + 0.5);
+    long long s = 1, r = 1, rem = P;
+    for (long long p = 2; p * p <= rem; p++) {
+        int e = 0;
+        while (rem % p == 0) { rem /= p; e++; }
+        for (int q = 0; q < e / 2; q++) r *= p;
+        if (e % 2) s *= p;
+    }
+    s *= rem; /* leftover prime (exponent 1) */
+    if (s == 1) return (double)r;
+    int am = m < 0 ? -m : m, an = n < 0 ? -n : n;
+    if (l == 4 && ((am == 2 && an == 1) || (am == 1 && an == 2))) return r * (2.0 / (double)sqrtf(2.0f));
+    return r * (double)sqrtf((float)s);
+}
+
+/* Wigner small-d d^l_{mn}(beta) (sum formula) with the reference's float32
+ * normalisation. */
+static double wigner_d(int l, int m, int n, double beta) {
+    double c = cos(beta / 2), s = sin(beta / 2), sum = 0;
+    for (int k = 0; k <= 2 * l; k++) {
+        int a = l + n - k, b = m - n + k, d = l - m - k;
+        if (a < 0 || b < 0 || d < 0) continue;
+        double sign = ((m - n + k) % 2 == 0) ? 1.0 : -1.0;
+        sum += sign * pow(c, 2 * l + n - m - 2 * k) * pow(s, m - n + 2 * k) / (fact(a) * fact(k) * fact(b) * fact(d));
+    }
+    return dnorm_f32(l, m, n) * sum;
+}
+
+/* rotate_nlm4, src/rotation.f90:14-27: nlm_rot(l,m) = sum_n D^l(m,n) nlm(l,n)
+ * with D^l(m,n) = exp(-i n phi) d^l_{mn}(theta) (the convention of
+ * src/include/Dlmn_L4.f90).  pinned: oracle/_ref rotate_nlm4 on random
+ * angles; rotation by (theta, 0) of a z-delta -> delta along the rotated axis
+ * (a2 closed form). */
+void sfo_rotate_nlm4(const cdbl *nlm4, double theta, double phi, cdbl *out) {
+    out[0] = nlm4[0];
+    for (int l = 2; l <= 4; l += 2) {
+        int off = (l == 2) ? 3 : 10;
+        for (int m = -l; m <= l; m++) {
+            cdbl s = 0;
+            for (int n = -l; n <= l; n++) s += cexp(-I * (double)n * phi) * wigner_d(l, m, n, theta) * nlm4[off + n];
+            out[off + m] = s;
+        }
+    }
+}
+
+/* -------------------------------------------------- elastic velocities */
+
+/* Cij_to_Lame_tranisotropic, src/elasticities.f90:107-120 */
+void sfo_Cij_to_Lame_tranisotropic(const double *Cij, double *Lame) {
+    double C66 = (Cij[0] - Cij[3]) / 2;
+    Lame[0] = Cij[3];
+    Lame[1] = C66;
+    Lame[2] = Cij[4] / Cij[3];
+    Lame[3] = Cij[2] / C66;
+    Lame[4] = Cij[1] / Cij[0];
+}
+
+/* elas_revparams_tranisotropic, src/elasticities.f90:64-78 */
+static void elas_rev(const double *Lm, double *k) {
+    double lam = Lm[0], mu = Lm[1], El = Lm[2], Em = Lm[3], Eg = Lm[4], gam = lam + 2 * mu;
+    k[0] = lam;
+    k[1] = 2 * mu;
+    k[2] = -lam * (1 - El);
+    k[3] = ((1 + Eg) * gam - 2 * (El * lam + 2 * Em * mu));
+    k[4] = -2 * mu * (1 - Em);
+}
+
+/* elas_fwdparams_tranisotropic, src/elasticities.f90:80-96 */
+static void elas_fwd(const double *Lm, double *k) {
+    double lam = Lm[0], mu = Lm[1], El = Lm[2], Em = Lm[3], Eg = Lm[4], gam = lam + 2 * mu;
+    double xi = 2 * Eg * gam * (lam + mu) - 2 * (El * lam) * (El * lam);
+    k[0] = ((El * lam) * (El * lam) - Eg * gam * lam) / (2 * mu * xi);
+    k[1] = 1 / (2 * mu);
+    k[2] = (Eg * gam * lam - El * lam * (El * lam + 2 * mu)) / (2 * mu * xi);
+    k[3] = (4 * mu * lam * (1 + El) + 4 * mu * mu + Eg * gam * gam - El * El * lam * lam) / (2 * mu * xi) - 1 / (Em * mu);
+    k[4] = (1 - Em) / (2 * Em * mu);
+}
+
+/* Gauss-Jordan inverse with partial pivoting (the matinv of
+ * src/wavepropagation.f90:326-361 uses LAPACK dgetrf/dgetri); returns
+ * nonzero if singular. */
+static int invert(int n, double *A, double *Ainv) {
+    double M[9][18];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 2 * n; j++) M[i][j] = (j < n) ? A[n * i + j] : (j - n == i ? 1.0 : 0.0);
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        for (int r = c + 1; r < n; r++)
+            if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
+        if (M[p][c] == 0.0) return 1;
+        if (p != c)
+            for (int j = 0; j < 2 * n; j++) { double t = M[c][j]; M[c][j] = M[p][j]; M[p][j] = t; }
+        double piv = M[c][c];
+        for (int j = 0; j < 2 * n; j++) M[c][j] /= piv;
+        for (int r = 0; r < n; r++) {
+            if (r == c || M[r][c] == 0.0) continue;
+            double f = M[r][c];
+            for (int j = 0; j < 2 * n; j++) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Ainv[n * i + j] = M[i][n + j];
+    return 0;
+}
+
+/* Qnorm_tranisotropic_Reuss, src/wavepropagation.f90:189-211: the 9x9
+ * compliance P (strain = P:stress, src/include/P_Reuss.f90) is the Reuss
+ * average of elas_fwd_tranisotropic (src/elasticities.f90:39-50):
+ *   P_ijkl = k1 d_ij d_kl + k2 d_ik d_jl + k3 (d_ij a2_kl + a2_ij d_kl)
+ *          + k4 a4_ijkl + k5 (d_ik a2_lj + a2_ik d_jl),
+ * flat index 3i+j; a4 entries are the index-sorted ones. */
+static int qnorm_reuss(const cdbl *nlm, const double *Lame, double *Q) {
+    double k[5], a2[9], u[15], P[81], Pi_[81];
+    elas_fwd(Lame, k);
+    sfo_a2(nlm, a2);
+    a4_unique(nlm, u);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int kk = 0; kk < 3; kk++)
+                for (int l = 0; l < 3; l++) {
+                    double dij = i == j, dkl = kk == l, dik = i == kk, djl = j == l;
+                    P[9 * (3 * i + j) + 3 * kk + l] = k[0] * dij * dkl + k[1] * dik * djl +
+                                                      k[2] * (dij * a2[3 * kk + l] + a2[3 * i + j] * dkl) +
+                                                      k[3] * u[a4_sorted_index(i, j, kk, l)] +
+                                                      k[4] * (dik * a2[3 * l + j] + a2[3 * i + kk] * djl);
+                }
+    if (invert(9, P, Pi_)) return 1;
+    for (int r = 0; r < 3; r++) {
+        int R = 3 * r + 2; /* rows 3, 6, 9 */
+        Q[3 * r + 0] = 0.5 * (Pi_[9 * R + 2] + Pi_[9 * R + 6]);
+        Q[3 * r + 1] = 0.5 * (Pi_[9 * R + 5] + Pi_[9 * R + 7]);
+        Q[3 * r + 2] = 0.5 * (2 * Pi_[9 * R + 8]);
+    }
+    return 0;
+}
+
+/* Qnorm_tranisotropic_Voigt, src/wavepropagation.f90:170-187 (k = z) */
+static void qnorm_voigt(const cdbl *nlm, const double *Lame, double *Q) {
+    double k[5], a2v[6], a4v[36], a2[9], ksq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 1}, KM[9], kv[6], a4k[6], a4km[9];
+    elas_rev(Lame, k);
+    sfo_ev_ck_Mandel(nlm, a2v, a4v);
+    vec_to_mat(a2v, a2);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int m = 0; m < 3; m++) s += ksq[3 * i + m] * a2[3 * m + j] + a2[3 * i + m] * ksq[3 * m + j];
+            KM[3 * i + j] = s;
+        }
+    mat_to_vec(ksq, kv);
+    for (int i = 0; i < 6; i++) {
+        double s = 0;
+        for (int j = 0; j < 6; j++) s += a4v[6 * i + j] * kv[j];
+        a4k[i] = s;
+    }
+    vec_to_mat(a4k, a4km);
+    double a2k = dinner22(a2, ksq);
+    for (int i = 0; i < 9; i++) {
+        double id = (i % 4 == 0);
+        Q[i] = k[1] / 2 * id + (k[0] + k[1] / 2) * ksq[i] + k[2] * KM[i] + k[3] * a4km[i] +
+               0.5 * k[4] * (a2[i] + a2k * id + KM[i]);
+    }
+}
+
+/* Qnorm_tranisotropic, src/wavepropagation.f90:146-168 */
+int sfo_Qnorm_tranisotropic(const cdbl *nlm, double alpha, const double *Lame, double *Q) {
+    double QR[9] = {0}, QV[9] = {0};
+    int st = 0;
+    if (alpha < 1.0) st = qnorm_reuss(nlm, Lame, QR);
+    if (alpha > 0.0) qnorm_voigt(nlm, Lame, QV);
+    for (int i = 0; i < 9; i++) Q[i] = (1 - alpha) * QR[i] + alpha * QV[i];
+    return st;
+}
+
+/* Qnorm_to_vi with detQ_polycoefs, src/wavepropagation.f90:267-324: Cardano
+ * roots of the depressed cubic in complex arithmetic.  The exponents
+ * 0.6666666666666666, 0.3333333333333333 and 1.3333333333333333 are REAL(4)
+ * literals, widened to double. */
+static void qnorm_to_vi(const double *Q, double rho, double *vi) {
+    const double om = 1.0, w = om * om * rho;
+    double I1 = Q[0] + Q[4] + Q[8];
+    double I2 = -Q[1] * Q[1] - Q[2] * Q[2] + Q[0] * Q[4] - Q[5] * Q[5] + Q[0] * Q[8] + Q[4] * Q[8];
+    double I3 = -Q[2] * Q[2] * Q[4] + 2 * Q[1] * Q[2] * Q[5] - Q[0] * Q[5] * Q[5] - Q[1] * Q[1] * Q[8] +
+                Q[0] * Q[4] * Q[8];
+    double r0 = -w * w * w, r2 = I1 * w * w, r4 = -(I2 * w), r6 = I3;
+    const double e23 = (double)0.6666666666666666f, e13 = (double)0.3333333333333333f,
+                 e43 = (double)1.3333333333333333f;
+    cdbl a = (3.0 * r6 * r2 - r4 * r4) / (3.0 * r6 * r6);
+    cdbl b = (2.0 * r4 * r4 * r4 - 9.0 * r6 * r4 * r2 + 27.0 * r6 * r6 * r0) / (27.0 * r6 * r6 * r6);
+    cdbl root = csqrt(12.0 * cpow(a, 3.0) + 81.0 * cpow(b, 2.0));
+    cdbl denom = cpow(6.0, e23) * cpow(-9.0 * b + root, e13);
+    cdbl D = cpow(-9.0 * b + root, e23);
+    cdbl sh = -r4 / (3.0 * r6);
+    cdbl x1 = sh + (-2.0 * cpow(3.0, e13) * a + cpow(2.0, e13) * D) / denom;
+    cdbl x2 = sh + cpow(-1.0, e13) * (2.0 * cpow(3.0, e13) * a + cpow(-2.0, e13) * D) / denom;
+    cdbl x3 = sh + cpow(-1.0, e43) * (2.0 * cpow(-3.0, e13) * a + cpow(2.0, e13) * D) / denom;
+    vi[0] = om / cabs(csqrt(x3));
+    vi[1] = om / cabs(csqrt(x1));
+    vi[2] = om / cabs(csqrt(x2));
+}
+
+/* Vi_elastic_tranisotropic, src/wavepropagation.f90:121-144: per direction,
+ * rotate the l <= 4 block by (-theta, -phi), homogenize the acoustic tensor
+ * for k = z, solve for (qS1, qS2, qP).  Vi is (ndir, 3) row-major.  pinned:
+ * oracle/_ref on random fabrics and directions; delta fabric along z with
+ * k = z gives vP = sqrt(C33/rho), vS = sqrt(C55/rho) (both Reuss and Voigt). */
+int sfo_Vi_elastic_tranisotropic(const cdbl *nlm, double alpha, const double *Lame, double rho, int ndir,
+                                 const double *theta, const double *phi, double *Vi) {
+    int st = 0;
+    for (int d = 0; d < ndir; d++) {
+        cdbl r[15];
+        double Q[9];
+        sfo_rotate_nlm4(nlm, -theta[d], -phi[d], r);
+        st |= sfo_Qnorm_tranisotropic(r, alpha, Lame, Q);
+        qnorm_to_vi(Q, rho, Vi + 3 * d);
+    }
+    return st;
+}
+
+/* Vi_electromagnetic_tranisotropic, src/wavepropagation.f90:217-261: the two
+ * roots q^2 of eps33 q^4 + b q^2 + det(eps) w^2 = 0 (b = (eps13^2 + eps23^2 -
+ * (eps11 + eps22) eps33) w, w = mu/c^2 for omega = 1), S1 = smaller root.
+ * Vi is (ndir, 2).
+ * pinned: oracle/_ref; delta fabric along z -> both velocities
+ * c/sqrt(mu eps_t) (closed form). */
+void sfo_Vi_electromagnetic_tranisotropic(const cdbl *nlm, double eps_m, double eps_t, double mu, int ndir,
+                                          const double *theta, const double *phi, double *Vi) {
+    const double c = 299792458.0, om = 1.0, s2 = sqrt(2.0);
+    for (int d = 0; d < ndir; d++) {
+        cdbl r[15];
+        double a2[9], e[9];
+        sfo_rotate_nlm4(nlm, -theta[d], -phi[d], r);
+        sfo_a2(r, a2);
+        for (int i = 0; i < 9; i++) {
+            double id = (i % 4 == 0);
+            e[i] = (2 * eps_t + eps_m) * id / 3.0 + (eps_m - eps_t) * (a2[i] - id / 3.0);
+        }
+        double w = (om / c) * (om / c) * mu;
+        double e11 = e[0], e12 = e[1], e13 = e[2], e22 = e[4], e23 = e[5], e33 = e[8];
+        double bq = (e13 * e13 + e23 * e23 - (e11 + e22) * e33) * w;
+        double t = e13 * e13 + e23 * e23;
+        double disc = (t * t - 2.0 * (e13 * e13 * (e11 - e22) + 4.0 * e12 * e13 * e23 + (e22 - e11) * e23 * e23) * e33 +
+                       (4.0 * e12 * e12 + (e11 - e22) * (e11 - e22)) * e33 * e33) * (w * w);
+        double q2 = sqrt(-(1.0 / e33) * (bq + sqrt(disc))) / s2;
+        double q4 = sqrt((1.0 / e33) * (-bq + sqrt(disc))) / s2;
+        Vi[2 * d + 0] = om / q2;
+        Vi[2 * d + 1] = om / q4;
+    }
+}

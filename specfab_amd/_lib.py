@@ -33,6 +33,8 @@ SIGNATURES = {
     "sfg_ctx_create": (_i, [_i, _c.POINTER(_vp)]),
     "sfg_ctx_destroy": (_i, [_vp]),
     "sfg_last_error": (_c.c_char_p, []),
+    "sfg_ctx_set_timing": (_i, [_vp, _i]),
+    "sfg_ctx_read_timing": (_i, [_vp, _dp, _ip]),
     "sfg_nlm_len": (_i, [_i]),
     "sfg_lm": (_i, [_i, _ip]),
     "sfg_M_LROT": (_i, [_vp, _i, _dp, _dp, _d, _d, _dp]),

@@ -79,9 +79,15 @@ struct sfg_ctx {
     int* csr_row_ptr = nullptr;
     int4* csr_ent = nullptr;
     double* csr_val = nullptr;
-    // workspace
+    // workspace, shared by all calls on this context: every user waits for
+    // the previous user's stream through ws_event before touching it
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    hipEvent_t ws_event = nullptr;
+    bool ws_event_recorded = false;
+    // stepping-kernel timing (sfg_ctx_set_timing)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
 };
 
 namespace {
@@ -173,10 +179,13 @@ int ensure_csr(sfg_ctx* ctx) {
     return SFG_OK;
 }
 
-int workspace(sfg_ctx* ctx, size_t bytes, void** out) {
+// Acquire the context workspace for work enqueued on stream s; the caller
+// enqueues its kernels and then calls ws_release(ctx, s).
+int ws_acquire(sfg_ctx* ctx, size_t bytes, hipStream_t s, void** out) {
     if (bytes > ctx->ws_bytes) {
         if (ctx->ws) {
-            HIPCHK(hipStreamSynchronize(ctx->stream));
+            // the old buffer may still be read by work on any stream
+            HIPCHK(hipDeviceSynchronize());
             HIPCHK(hipFree(ctx->ws));
             ctx->ws = nullptr;
             ctx->ws_bytes = 0;
@@ -184,7 +193,14 @@ int workspace(sfg_ctx* ctx, size_t bytes, void** out) {
         HIPCHK(hipMalloc(&ctx->ws, bytes));
         ctx->ws_bytes = bytes;
     }
+    if (ctx->ws_event_recorded) HIPCHK(hipStreamWaitEvent(s, ctx->ws_event, 0));
     *out = ctx->ws;
+    return SFG_OK;
+}
+
+int ws_release(sfg_ctx* ctx, hipStream_t s) {
+    HIPCHK(hipEventRecord(ctx->ws_event, s));
+    ctx->ws_event_recorded = true;
     return SFG_OK;
 }
 
@@ -197,6 +213,31 @@ hipStream_t pick(sfg_ctx*, void* stream) { return (hipStream_t)stream; }
 extern "C" {
 
 const char* sfg_last_error(void) { return g_err.c_str(); }
+
+int sfg_ctx_set_timing(sfg_ctx* ctx, int enable) {
+    if (!ctx) return fail(SFG_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->timing = enable != 0;
+    return SFG_OK;
+}
+
+int sfg_ctx_read_timing(sfg_ctx* ctx, double* ms_total, int* count) {
+    if (!ctx || !ms_total || !count) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    double tot = 0;
+    for (auto& t : ctx->tev) {
+        HIPCHK(hipEventSynchronize(t.second));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, t.first, t.second));
+        tot += ms;
+    }
+    *ms_total = tot;
+    *count = (int)ctx->tev.size();
+    for (auto& t : ctx->tev) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
+    ctx->tev.clear();
+    return SFG_OK;
+}
 
 int sfg_nlm_len(int L) {
     if (L < 0 || (L % 2)) return fail(SFG_ERR_L, "L must be even and >= 0");
@@ -222,6 +263,7 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
     if (const char* e = std::getenv("SFG_STEPPER")) c->stepper = (std::string(e) == "generic") ? 1 : 0;
     if (const char* e = std::getenv("SFG_SYNC_CHECK")) c->sync_check = std::string(e) == "1";
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming);
     if (e != hipSuccess) { delete c; return fail(SFG_ERR_HIP, hipGetErrorString(e)); }
     *ctx = c;
     return SFG_OK;
@@ -230,7 +272,9 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
 int sfg_ctx_destroy(sfg_ctx* ctx) {
     if (!ctx) return SFG_OK;
     hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    hipDeviceSynchronize();
+    for (auto& t : ctx->tev) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
+    if (ctx->ws_event) hipEventDestroy(ctx->ws_event);
     for (auto& kv : ctx->ops)
         for (void* p : kv.second.owned) hipFree(p);
     hipFree(ctx->csr_row_ptr);
@@ -288,7 +332,7 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
     size_t nscr = (size_t)3 * n * P;
     void* ws;
-    rc = workspace(ctx, (nth + nom + nsl + nscr) * sizeof(double), &ws);
+    rc = ws_acquire(ctx, (nth + nom + nsl + nscr) * sizeof(double), s, &ws);
     if (rc) return rc;
     double* theta = (double*)ws;
     double* omega = theta + nth;
@@ -313,15 +357,27 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
     ea.status = d->status_dev;
     ea.zero = 0;
-    if (d->precision == SFG_FP64 && ctx->stepper == 0 && sfg::fast_available(d->L, 0)) {
-        HIPCHK(sfg::launch_evolve_fast(d->L, 0, ea, s));
-    } else if (d->precision == SFG_FP32C && sfg::fast_available(d->L, 1)) {
-        HIPCHK(sfg::launch_evolve_fast(d->L, 1, ea, s));
-    } else if (d->precision == SFG_FP64) {
-        HIPCHK(sfg::launch_evolve_generic(ea, s));
-    } else {
+    const bool fast64 = d->precision == SFG_FP64 && ctx->stepper == 0 && sfg::fast_available(d->L, 0);
+    const bool fast32 = d->precision == SFG_FP32C && sfg::fast_available(d->L, 1);
+    if (!fast64 && !fast32 && d->precision != SFG_FP64) {
+        ws_release(ctx, s);
         return fail(SFG_ERR_ARG, "FP32C stepping is not available for this L");
     }
+    std::pair<hipEvent_t, hipEvent_t> tv{nullptr, nullptr};
+    if (ctx->timing) {
+        HIPCHK(hipEventCreate(&tv.first));
+        HIPCHK(hipEventCreate(&tv.second));
+        HIPCHK(hipEventRecord(tv.first, s));
+    }
+    if (fast64) HIPCHK(sfg::launch_evolve_fast(d->L, 0, ea, s));
+    else if (fast32) HIPCHK(sfg::launch_evolve_fast(d->L, 1, ea, s));
+    else HIPCHK(sfg::launch_evolve_generic(ea, s));
+    if (ctx->timing) {
+        HIPCHK(hipEventRecord(tv.second, s));
+        ctx->tev.push_back(tv);
+    }
+    rc = ws_release(ctx, s);
+    if (rc) return rc;
     if (ctx->sync_check) {
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(hipGetLastError());
