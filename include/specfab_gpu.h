@@ -166,6 +166,42 @@ int sfg_a2(sfg_ctx* ctx, int L, const double* nlm, double a2[9]);
 /* <D> = ev_D2(nlm, tau), src/dynamics.f90:402-422 */
 int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* D);
 
+/* ---- full fourth-order structure tensor and wave velocities ---- */
+
+/* a4(nlm) (src/moments.f90:46-55, body src/include/ev_c4__body.f90 with its
+ * REAL(4) literals): 3x3x3x3 row-major, a4[27i+9j+3k+l] = a4(i+1,j+1,k+1,l+1);
+ * entry (3,2,1,2) is the reference body's a4(1,2,3,3).  Batched: a4_dev is
+ * (P, 81). */
+int sfg_a4(sfg_ctx* ctx, int L, const double* nlm, double a4[81]);
+int sfg_a4_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4_dev, void* stream);
+
+/* Cij_to_Lame_tranisotropic (src/elasticities.f90:107-120): (C11, C33, C55,
+ * C12, C13) -> (lam, mu, Elam, Emu, Egam).  Host only. */
+int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]);
+
+/* Vi_elastic_tranisotropic (src/wavepropagation.f90:121-144; specfabpy
+ * src/specfabpy.f90:999-1008): phase velocities (qS1, qS2, qP) along the
+ * directions (theta[d], phi[d]) (host arrays), grains with Lame parameters
+ * Lame[5], Reuss-Voigt weight alpha, density rho.  Vi is (ndir, 3) row-major
+ * (the transpose of the reference's Vi(3, ndir)).  Batched: Vi_dev is
+ * (P, ndir, 3); a singular Reuss compliance sets status bit
+ * SFG_ST_REUSS_SING and NaN velocities (the reference stops). */
+int sfg_Vi_elastic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double alpha, const double Lame[5],
+                                 double rho, int ndir, const double* theta, const double* phi, double* Vi);
+int sfg_Vi_elastic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
+                                       const double Lame[5], double rho, int ndir, const double* theta,
+                                       const double* phi, double* Vi_dev, int32_t* status_dev, void* stream);
+
+/* Vi_electromagnetic_tranisotropic (src/wavepropagation.f90:217-261): EM
+ * phase velocities (S1 fast, S2 slow) for grain permittivities eps_m
+ * (parallel to c), eps_t (perpendicular) and permeability mu.  Vi is
+ * (ndir, 2); batched Vi_dev is (P, ndir, 2). */
+int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double eps_m, double eps_t,
+                                         double mu, int ndir, const double* theta, const double* phi, double* Vi);
+int sfg_Vi_electromagnetic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double eps_m,
+                                               double eps_t, double mu, int ndir, const double* theta,
+                                               const double* phi, double* Vi_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,13 @@ SIGNATURES = {
     "sfg_eig": (_i, [_vp, _i, _dp, _dp, _dp]),
     "sfg_Eij_tranisotropic": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _d, _i, _dp]),
     "sfg_ev_D2": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "sfg_a4": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_a4_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_Cij_to_Lame_tranisotropic": (_i, [_dp, _dp]),
+    "sfg_Vi_elastic_tranisotropic": (_i, [_vp, _i, _dp, _d, _dp, _d, _i, _dp, _dp, _dp]),
+    "sfg_Vi_elastic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _dp, _d, _i, _dp, _dp, _vp, _vp, _vp]),
+    "sfg_Vi_electromagnetic_tranisotropic": (_i, [_vp, _i, _dp, _d, _d, _d, _i, _dp, _dp, _dp]),
+    "sfg_Vi_electromagnetic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _d, _d, _i, _dp, _dp, _vp, _vp]),
 }
 
 

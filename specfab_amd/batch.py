@@ -134,3 +134,34 @@ class FabricBatch:
                                                 float(alpha), int(n_grain), _ptr(E), _ptr(lam), _ptr(ei), _ptr(st),
                                                 _stream()))
         return (E, lam, ei) if mode == 0 else E
+
+    # ---- full a4 and wave velocities ----
+    def a4(self):
+        """Full a4 (P, 3, 3, 3, 3) with the reference body's REAL(4) literals."""
+        out = torch.empty((self.P, 3, 3, 3, 3), dtype=torch.float64, device=self.dev)
+        check(lib().sfg_a4_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(out), _stream()))
+        return out
+
+    def Vi_elastic(self, alpha, Lame_grain, rho, theta, phi, status=None):
+        """Elastic phase velocities (P, ndir, 3) = (qS1, qS2, qP) per direction
+        (Vi_elastic_tranisotropic, src/wavepropagation.f90:121-144)."""
+        th = np.ascontiguousarray(np.atleast_1d(np.asarray(theta, dtype=np.float64)).ravel())
+        ph = np.ascontiguousarray(np.atleast_1d(np.asarray(phi, dtype=np.float64)).ravel())
+        Lm = np.ascontiguousarray(np.asarray(Lame_grain, dtype=np.float64).reshape(5))
+        out = torch.empty((self.P, th.size, 3), dtype=torch.float64, device=self.dev)
+        st = status if status is not None else self.status
+        check(lib().sfg_Vi_elastic_tranisotropic_batch(
+            self.ctx.handle, self.L, self.P, _ptr(self.x), float(alpha), _lib.dptr(Lm), float(rho), th.size,
+            _lib.dptr(th), _lib.dptr(ph), _ptr(out), _ptr(st), _stream()))
+        return out
+
+    def Vi_electromagnetic(self, eps_m, eps_t, mu, theta, phi):
+        """EM phase velocities (P, ndir, 2) = (S1, S2)
+        (Vi_electromagnetic_tranisotropic, src/wavepropagation.f90:217-261)."""
+        th = np.ascontiguousarray(np.atleast_1d(np.asarray(theta, dtype=np.float64)).ravel())
+        ph = np.ascontiguousarray(np.atleast_1d(np.asarray(phi, dtype=np.float64)).ravel())
+        out = torch.empty((self.P, th.size, 2), dtype=torch.float64, device=self.dev)
+        check(lib().sfg_Vi_electromagnetic_tranisotropic_batch(
+            self.ctx.handle, self.L, self.P, _ptr(self.x), float(eps_m), float(eps_t), float(mu), th.size,
+            _lib.dptr(th), _lib.dptr(ph), _ptr(out), _stream()))
+        return out

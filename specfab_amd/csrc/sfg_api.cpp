@@ -20,6 +20,7 @@
 #include "sfg_kernels.h"
 #include "sfg_fast.h"
 #include "sfg_homog.h"
+#include "sfg_wave.h"
 
 namespace {
 
@@ -58,6 +59,7 @@ struct DevOp {
     float* tval_f = nullptr;
     double* A2 = nullptr;
     double* A4 = nullptr;
+    double* A4F = nullptr;
     double* reg_shape = nullptr;
     int* r2c = nullptr;
     int* rpart = nullptr;
@@ -85,6 +87,9 @@ struct sfg_ctx {
     size_t ws_bytes = 0;
     hipEvent_t ws_event = nullptr;
     bool ws_event_recorded = false;
+    // rotation blocks of recently used direction sets (wave velocities)
+    struct DirSet { std::vector<double> th, ph; double* R = nullptr; };
+    std::vector<DirSet> dirs;
     // stepping-kernel timing (sfg_ctx_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
@@ -136,6 +141,14 @@ int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
     HIPCHK(upload(&d.tval_f, tvalf));
     HIPCHK(upload(&d.A2, A2));
     HIPCHK(upload(&d.A4, A4));
+    {
+        double m[15][15];
+        sfg::a4full_map(m);
+        std::vector<double> A4F(15 * 15);
+        for (int e = 0; e < 15; e++)
+            for (int q = 0; q < 15; q++) A4F[e * 15 + q] = q < d.nq ? m[e][q] : 0.0;
+        HIPCHK(upload(&d.A4F, A4F));
+    }
     HIPCHK(upload(&d.reg_shape, op.reg_shape));
     HIPCHK(upload(&d.r2c, r2c));
     HIPCHK(upload(&d.rpart, rpart));
@@ -146,7 +159,7 @@ int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
     std::vector<int> cl;
     for (auto& v : lm) cl.push_back(v.l);
     HIPCHK(upload(&d.cl, cl));
-    d.owned = {d.rows, d.row_blk, d.grp_param, d.grp_rng, d.tcol, d.tval, d.tval_f, d.A2, d.A4, d.reg_shape,
+    d.owned = {d.rows, d.row_blk, d.grp_param, d.grp_rng, d.tcol, d.tval, d.tval_f, d.A2, d.A4, d.A4F, d.reg_shape,
                d.r2c, d.rpart, d.cre, d.cim, d.csgn, d.cneg, d.cl};
     auto res = ctx->ops.emplace(L, std::move(d));
     *out = &res.first->second;
@@ -204,6 +217,42 @@ int ws_release(sfg_ctx* ctx, hipStream_t s) {
     return SFG_OK;
 }
 
+// Device rotation blocks (ROT4_STRIDE doubles per direction) of the direction
+// set (theta, phi); each direction rotates by (-theta, -phi) as
+// Vi_*_tranisotropic do (src/wavepropagation.f90:139, 235).  Recent sets are
+// kept on the context.
+int get_dirs(sfg_ctx* ctx, int ndir, const double* theta, const double* phi, const double** R) {
+    for (auto& ds : ctx->dirs)
+        if ((int)ds.th.size() == ndir && std::equal(ds.th.begin(), ds.th.end(), theta) &&
+            std::equal(ds.ph.begin(), ds.ph.end(), phi)) {
+            *R = ds.R;
+            return SFG_OK;
+        }
+    std::vector<double> host((size_t)ndir * sfg::ROT4_STRIDE);
+    for (int d = 0; d < ndir; d++) {
+        double R2[5][5], R4[9][9];
+        sfg::rotation_l4(-theta[d], -phi[d], R2, R4);
+        double* o = host.data() + (size_t)d * sfg::ROT4_STRIDE;
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) o[5 * i + j] = R2[i][j];
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) o[25 + 9 * i + j] = R4[i][j];
+    }
+    if (ctx->dirs.size() >= 16) {
+        // the oldest set may still be read by queued kernels
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipFree(ctx->dirs.front().R));
+        ctx->dirs.erase(ctx->dirs.begin());
+    }
+    sfg_ctx::DirSet ds;
+    ds.th.assign(theta, theta + ndir);
+    ds.ph.assign(phi, phi + ndir);
+    HIPCHK(upload(&ds.R, host));
+    ctx->dirs.push_back(ds);
+    *R = ctx->dirs.back().R;
+    return SFG_OK;
+}
+
 // Batched calls run on the caller's stream; NULL is HIP's default stream (the
 // convention of the HIP runtime and of PyTorch's default stream).
 hipStream_t pick(sfg_ctx*, void* stream) { return (hipStream_t)stream; }
@@ -234,7 +283,7 @@ int sfg_ctx_read_timing(sfg_ctx* ctx, double* ms_total, int* count) {
     }
     *ms_total = tot;
     *count = (int)ctx->tev.size();
-    for (auto& t : ctx->tev) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
+    for (auto& t : ctx->tev) { (void)hipEventDestroy(t.first); (void)hipEventDestroy(t.second); }
     ctx->tev.clear();
     return SFG_OK;
 }
@@ -271,17 +320,18 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
 
 int sfg_ctx_destroy(sfg_ctx* ctx) {
     if (!ctx) return SFG_OK;
-    hipSetDevice(ctx->device);
-    hipDeviceSynchronize();
-    for (auto& t : ctx->tev) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
-    if (ctx->ws_event) hipEventDestroy(ctx->ws_event);
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto& t : ctx->tev) { (void)hipEventDestroy(t.first); (void)hipEventDestroy(t.second); }
+    if (ctx->ws_event) (void)hipEventDestroy(ctx->ws_event);
     for (auto& kv : ctx->ops)
-        for (void* p : kv.second.owned) hipFree(p);
-    hipFree(ctx->csr_row_ptr);
-    hipFree(ctx->csr_ent);
-    hipFree(ctx->csr_val);
-    if (ctx->ws) hipFree(ctx->ws);
-    hipStreamDestroy(ctx->stream);
+        for (void* p : kv.second.owned) (void)hipFree(p);
+    (void)hipFree(ctx->csr_row_ptr);
+    (void)hipFree(ctx->csr_ent);
+    (void)hipFree(ctx->csr_val);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    for (auto& ds : ctx->dirs) (void)hipFree(ds.R);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SFG_OK;
 }
@@ -410,7 +460,7 @@ namespace {
 
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
 int dense_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const double* A_dev, const double* B_dev, double iota,
@@ -626,5 +676,154 @@ int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e
 }
 
 int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M) { return diag_single(ctx, L, 1, D, M); }
+
+// ---------------------------------------------------------------- full a4, wave velocities
+
+int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]) {
+    if (!Cij || !Lame) return fail(SFG_ERR_ARG, "NULL argument");
+    // src/elasticities.f90:107-120
+    const double C66 = (Cij[0] - Cij[3]) / 2;
+    Lame[0] = Cij[3];
+    Lame[1] = C66;
+    Lame[2] = Cij[4] / Cij[3];
+    Lame[3] = Cij[2] / C66;
+    Lame[4] = Cij[1] / Cij[0];
+    return SFG_OK;
+}
+
+static int a4_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4_dev, hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "a4 needs the l <= 4 coefficients (L >= 4)");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    sfg::A4FullArgs a{};
+    a.P = P; a.nq = op->nq; a.x = x_dev; a.A4F = op->A4F; a.a4 = a4_dev;
+    HIPCHK(sfg::launch_a4full(a, s));
+    return SFG_OK;
+}
+
+int sfg_a4_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !a4_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return a4_batch_locked(ctx, L, P, x_dev, a4_dev, pick(ctx, stream));
+}
+
+static int vi_el_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha, const double Lame[5],
+                        double rho, int ndir, const double* theta, const double* phi, double* Vi_dev,
+                        int32_t* status_dev, hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "wave velocities need the l <= 4 coefficients (L >= 4)");
+    if (ndir < 0 || (ndir > 0 && (!theta || !phi)) || !Lame) return fail(SFG_ERR_ARG, "bad direction arguments");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    if (P == 0 || ndir == 0) return SFG_OK;
+    sfg::ViElasticArgs a{};
+    rc = get_dirs(ctx, ndir, theta, phi, &a.R);
+    if (rc) return rc;
+    a.P = P; a.ndir = ndir; a.nq = op->nq;
+    a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4; a.A4F = op->A4F;
+    a.alpha = alpha; a.rho = rho;
+    sfg::elas_params(Lame, a.kf, a.kr);
+    a.Vi = Vi_dev; a.status = status_dev;
+    HIPCHK(sfg::launch_vi_elastic(a, s));
+    return SFG_OK;
+}
+
+static int vi_em_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double eps_m, double eps_t, double mu,
+                        int ndir, const double* theta, const double* phi, double* Vi_dev, hipStream_t s) {
+    if (L < 2) return fail(SFG_ERR_L, "EM velocities need the l <= 2 coefficients (L >= 2)");
+    if (ndir < 0 || (ndir > 0 && (!theta || !phi))) return fail(SFG_ERR_ARG, "bad direction arguments");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    if (P == 0 || ndir == 0) return SFG_OK;
+    sfg::ViEMArgs a{};
+    rc = get_dirs(ctx, ndir, theta, phi, &a.R);
+    if (rc) return rc;
+    a.P = P; a.ndir = ndir; a.x = x_dev; a.A2 = op->A2;
+    a.eps_m = eps_m; a.eps_t = eps_t; a.mu = mu; a.Vi = Vi_dev;
+    HIPCHK(sfg::launch_vi_em(a, s));
+    return SFG_OK;
+}
+
+int sfg_Vi_elastic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
+                                       const double Lame[5], double rho, int ndir, const double* theta,
+                                       const double* phi, double* Vi_dev, int32_t* status_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return vi_el_locked(ctx, L, P, x_dev, alpha, Lame, rho, ndir, theta, phi, Vi_dev, status_dev, pick(ctx, stream));
+}
+
+int sfg_Vi_electromagnetic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double eps_m,
+                                               double eps_t, double mu, int ndir, const double* theta,
+                                               const double* phi, double* Vi_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return vi_em_locked(ctx, L, P, x_dev, eps_m, eps_t, mu, ndir, theta, phi, Vi_dev, pick(ctx, stream));
+}
+
+// single parcel: host nlm -> device state -> kernel -> host output
+namespace {
+int single_parcel(sfg_ctx* ctx, int L, const double* nlm, size_t out_doubles, double* out,
+                  int (*body)(sfg_ctx*, int, const double*, double*, void*), void* user) {
+    if (!ctx || !nlm || !out) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(3 * n + out_doubles + 1) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* dout = dx + n;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    rc = body(ctx, L, dx, dout, user);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, out_doubles * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+struct ViElUser { double alpha, rho; const double* Lame; int ndir; const double *th, *ph; };
+struct ViEmUser { double eps_m, eps_t, mu; int ndir; const double *th, *ph; };
+}  // namespace
+
+int sfg_a4(sfg_ctx* ctx, int L, const double* nlm, double a4[81]) {
+    return single_parcel(ctx, L, nlm, 81, a4,
+                         [](sfg_ctx* c, int L_, const double* dx, double* dout, void*) {
+                             return a4_batch_locked(c, L_, 1, dx, dout, c->stream);
+                         },
+                         nullptr);
+}
+
+int sfg_Vi_elastic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double alpha, const double Lame[5],
+                                 double rho, int ndir, const double* theta, const double* phi, double* Vi) {
+    if (ndir <= 0) return ndir == 0 ? SFG_OK : fail(SFG_ERR_ARG, "ndir < 0");
+    ViElUser u{alpha, rho, Lame, ndir, theta, phi};
+    return single_parcel(ctx, L, nlm, (size_t)3 * ndir, Vi,
+                         [](sfg_ctx* c, int L_, const double* dx, double* dout, void* up) {
+                             auto* v = (ViElUser*)up;
+                             return vi_el_locked(c, L_, 1, dx, v->alpha, v->Lame, v->rho, v->ndir, v->th, v->ph,
+                                                 dout, nullptr, c->stream);
+                         },
+                         &u);
+}
+
+int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double eps_m, double eps_t,
+                                         double mu, int ndir, const double* theta, const double* phi, double* Vi) {
+    if (ndir <= 0) return ndir == 0 ? SFG_OK : fail(SFG_ERR_ARG, "ndir < 0");
+    ViEmUser u{eps_m, eps_t, mu, ndir, theta, phi};
+    return single_parcel(ctx, L, nlm, (size_t)2 * ndir, Vi,
+                         [](sfg_ctx* c, int L_, const double* dx, double* dout, void* up) {
+                             auto* v = (ViEmUser*)up;
+                             return vi_em_locked(c, L_, 1, dx, v->eps_m, v->eps_t, v->mu, v->ndir, v->th, v->ph,
+                                                 dout, c->stream);
+                         },
+                         &u);
+}
 
 }  // extern "C"

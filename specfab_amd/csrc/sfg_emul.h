@@ -18,5 +18,11 @@ typedef void* hipStream_t;
 #define __launch_bounds__(n)
 #define __shared__ static
 #define __forceinline__ inline
+#define __constant__
 using std::fma;
 using std::isfinite;
+inline int atomicOr(int* p, int v) {
+    int o = *p;
+    *p |= v;
+    return o;
+}

@@ -64,6 +64,18 @@ struct RealOperator {
 
 const RealOperator& real_operator(int L);  // built once per L, thread-safe
 
+// Real l <= 4 block (15 DOFs, real basis order): q -> (l, |m|, re/im part).
+void real4_index(int q, int* l, int* am, int* c);
+// Full fourth-order structure tensor (src/moments.f90:202-209, REAL(4)
+// literals of src/include/ev_c4__body.f90): index-sorted entries
+// (1111,1112,1113,1122,1123,1133,1222,1223,1233,1333,2222,2223,2233,2333,3333)
+// of a state x are (A4F x) / x0.
+void a4full_map(double A4F[15][15]);
+// rotate_nlm4 (src/rotation.f90:14-27) as real 5x5 / 9x9 blocks.
+void rotation_l4(double theta, double phi, double R2[5][5], double R4[9][9]);
+// Reuss (forward) and Voigt (reverse) elastic coefficients k1..k5.
+void elas_params(const double Lame[5], double kf[5], double kr[5]);
+
 // LROT harmonic weights (g0, gz, gn, gp)(k = 0..5) for given quadric
 // coefficients qe(-2..2), qo(-1..1); src/dynamics.f90:78-91.
 void lrot_weights(const std::complex<double> qe[5], const std::complex<double> qo[3],

@@ -179,3 +179,53 @@ def Eij_tranisotropic_arr(nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
     E = b.Eij(Eij_grain=Eij_grain, alpha=alpha, n_grain=n_grain, frames=frames)
     torch.cuda.synchronize()
     return E.cpu().numpy()
+
+
+def a4(nlm):
+    """a4 = <c^4> (3,3,3,3), src/moments.f90:46-55 (REAL(4) literals of the body)."""
+    L = _L_of(nlm)
+    out = np.zeros(81)
+    z = _z(nlm)
+    check(lib().sfg_a4(_ctx(), L, _zp(z), dptr(out)))
+    return out.reshape(3, 3, 3, 3)
+
+
+def Cij_to_Lame_tranisotropic(Cij):
+    """(C11, C33, C55, C12, C13) -> (lam, mu, Elam, Emu, Egam), src/elasticities.f90:107-120."""
+    C = np.ascontiguousarray(np.asarray(Cij, dtype=np.float64).reshape(5))
+    out = np.zeros(5)
+    check(lib().sfg_Cij_to_Lame_tranisotropic(dptr(C), dptr(out)))
+    return out
+
+
+def _dirs(theta, phi):
+    th = np.ascontiguousarray(np.atleast_1d(np.asarray(theta, dtype=np.float64)).ravel())
+    ph = np.ascontiguousarray(np.atleast_1d(np.asarray(phi, dtype=np.float64)).ravel())
+    if th.size != ph.size:
+        raise ValueError("theta and phi must have the same length")
+    return th, ph
+
+
+def Vi_elastic_tranisotropic(nlm, alpha, Lame_grain, rho, theta, phi):
+    """Elastic phase velocities (qS1, qS2, qP) x ndir, src/specfabpy.f90:999-1008
+    -> src/wavepropagation.f90:121-144.  Returns shape (3, ndir) like the reference."""
+    L = _L_of(nlm)
+    th, ph = _dirs(theta, phi)
+    out = np.zeros((th.size, 3))
+    Lm = np.ascontiguousarray(np.asarray(Lame_grain, dtype=np.float64).reshape(5))
+    z = _z(nlm)
+    check(lib().sfg_Vi_elastic_tranisotropic(_ctx(), L, _zp(z), float(alpha), dptr(Lm), float(rho), th.size,
+                                             dptr(th), dptr(ph), dptr(out)))
+    return out.T.copy()
+
+
+def Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi):
+    """EM phase velocities (S1 fast, S2 slow) x ndir, src/specfabpy.f90:1024-1033
+    -> src/wavepropagation.f90:217-261.  Returns shape (2, ndir)."""
+    L = _L_of(nlm)
+    th, ph = _dirs(theta, phi)
+    out = np.zeros((th.size, 2))
+    z = _z(nlm)
+    check(lib().sfg_Vi_electromagnetic_tranisotropic(_ctx(), L, _zp(z), float(eps_m), float(eps_t), float(mu),
+                                                     th.size, dptr(th), dptr(ph), dptr(out)))
+    return out.T.copy()

@@ -98,3 +98,53 @@ def test_generated_kernel_vs_oracle(oracle, L, prec, scheme):
         out = to_nlm(x, L)
         assert relerr(out, ref) < tol, (which, relerr(out, ref))
         assert not st.any()
+
+
+_wave = {}
+
+
+def emul_wave_lib():
+    if "lib" in _wave:
+        return _wave["lib"]
+    out = os.path.join(tempfile.gettempdir(), "sfg_emul_wave_%d.so" % os.getpid())
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DSFG_HOST_EMUL", "-I", CSRC, "-x", "c++",
+           os.path.join(ROOT, "tests", "emul_wave.cpp"), os.path.join(CSRC, "sfg_wave.hip"),
+           os.path.join(CSRC, "sfg_wave_host.cpp"), os.path.join(CSRC, "sfg_basis.cpp"), "-o", out]
+    if not os.path.exists(os.path.join(GEN, "..", "generated", "gaunt_L20.inc")):
+        pytest.skip("generated tables not present (run the build)")
+    subprocess.check_call(cmd)
+    lib = ctypes.CDLL(out)
+    d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    c = ctypes
+    lib.emul_vi_elastic.argtypes = [c.c_int, c.c_int64, d, c.c_double, d, c.c_double, c.c_int, d, d, d, i]
+    lib.emul_vi_em.argtypes = [c.c_int, c.c_int64, d, c.c_double, c.c_double, c.c_double, c.c_int, d, d, d]
+    lib.emul_a4.argtypes = [c.c_int, c.c_int64, d, d]
+    _wave["lib"] = lib
+    return lib
+
+
+def test_wave_kernels_vs_reference():
+    """Elastic/EM velocity and full-a4 kernels (the 6x6 Mandel Reuss solve in
+    place of the reference's 9x9 inverse) against the reference fixture."""
+    from conftest import load_golden
+    g = load_golden("wave_L8.npz")
+    lib = emul_wave_lib()
+    X = to_state(g["nlm"], 8)
+    P = X.shape[1]
+    th, ph = g["theta"], g["phi"]
+    nd = len(th)
+    for tag, alpha in (("reuss", 0.0), ("hill", 0.5), ("voigt", 1.0)):
+        V = np.zeros((P, nd, 3))
+        st = np.zeros(P, np.int32)
+        lib.emul_vi_elastic(8, P, X, alpha, g["Lame"], float(g["rho"]), nd, th, ph, V, st)
+        ref = np.transpose(g["Vi_" + tag], (0, 2, 1))
+        assert relerr(V[:24], ref[:24]) < 1e-12, tag
+        assert relerr(V[24:], ref[24:]) < 1e-7, tag
+        assert not st.any()
+    V = np.zeros((P, nd, 2))
+    lib.emul_vi_em(8, P, X, float(g["eps_m"]), float(g["eps_t"]), float(g["mu"]), nd, th, ph, V)
+    assert relerr(V, np.transpose(g["Vi_em"], (0, 2, 1))) < 1e-13
+    A = np.zeros((P, 81))
+    lib.emul_a4(8, P, X, A)
+    assert relerr(A.reshape(P, 3, 3, 3, 3), g["a4"]) < 1e-14

@@ -173,3 +173,96 @@ def test_jeffery_a2_equation(oracle):
                     [a4D[4] / s, a4D[3] / s, a4D[2]]])
     expect = W @ a2 - a2 @ W - (D @ a2 + a2 @ D - 2 * a4D)
     assert np.abs(da2 - expect).max() < 1e-5 * np.abs(expect).max()
+
+
+# ---------------------------------------------------------------- wave velocities
+# States 0..23 of wave_L8.npz are anisotropic; 24 is isotropic and 25 a
+# z-delta.  For the last two the two S-wave roots coincide and the
+# reference's Cardano formula (src/wavepropagation.f90:283-291) takes the
+# cube root of a discriminant that vanishes: a rounding difference eps in
+# the inputs moves those roots by ~sqrt(eps) ~ 1e-8 relative, which is the
+# tolerance used for them.
+NONDEG = slice(0, 24)
+
+
+def test_wave_matches_reference(oracle):
+    g = load_golden("wave_L8.npz")
+    X, th, ph = g["nlm"], g["theta"], g["phi"]
+    assert relerr(oracle.Cij_to_Lame_tranisotropic(g["Cij"]), g["Lame"]) == 0.0
+    for k, x in enumerate(X):
+        assert relerr(oracle.a4(x), g["a4"][k]) < 1e-14, k
+        for d in range(len(th)):
+            assert relerr(oracle.rotate_nlm4(x, th[d], ph[d]), g["rot"][k, d]) < 1e-13, (k, d)
+        tol = 1e-12 if k < 24 else 1e-7
+        for tag, alpha in (("reuss", 0.0), ("hill", 0.5), ("voigt", 1.0)):
+            v = oracle.Vi_elastic_tranisotropic(x, alpha, g["Lame"], float(g["rho"]), th, ph)
+            assert relerr(v, g["Vi_" + tag][k]) < tol, (k, tag)
+        v = oracle.Vi_electromagnetic_tranisotropic(x, float(g["eps_m"]), float(g["eps_t"]), float(g["mu"]), th, ph)
+        assert relerr(v, g["Vi_em"][k]) < 1e-13, k
+
+
+def _delta_z():
+    d = np.zeros(15, complex)
+    d[0], d[3], d[10] = ISO, np.sqrt(5 / (4 * np.pi)), np.sqrt(9 / (4 * np.pi))
+    return d
+
+
+def test_wave_closed_forms(oracle):
+    """Delta fabric (all c-axes along z) = a single crystal: k along z gives
+    vP = sqrt(C33/rho), vS = sqrt(C55/rho); k along x gives sqrt(C11/rho),
+    sqrt(C55/rho) and sqrt(C66/rho), C66 = (C11 - C12)/2, for Reuss and Voigt
+    alike.  EM: k along z sees eps_t twice, k along x sees eps_t and eps_m.
+    Rotations: a z-delta rotated by (theta, phi) points along
+    (sin theta, 0, cos theta); at theta = 0 the rotation is the phase
+    exp(-i m phi); every rotation keeps the power per l.  Tolerances 1e-7
+    cover the reference's REAL(4) constants (a4 body, Wigner-D body)."""
+    C = np.array([14.060e9, 15.240e9, 3.060e9, 7.150e9, 5.880e9])
+    rho = 917.0
+    Lm = oracle.Cij_to_Lame_tranisotropic(C)
+    d = _delta_z()
+    for alpha in (0.0, 0.5, 1.0):
+        vz = oracle.Vi_elastic_tranisotropic(d, alpha, Lm, rho, [0.0], [0.0])[:, 0]
+        assert np.allclose(np.sort(vz), np.sqrt(np.sort([C[2], C[2], C[1]]) / rho), rtol=1e-7), alpha
+        vx = oracle.Vi_elastic_tranisotropic(d, alpha, Lm, rho, [np.pi / 2], [0.0])[:, 0]
+        assert np.allclose(np.sort(vx), np.sqrt(np.sort([C[2], (C[0] - C[3]) / 2, C[0]]) / rho), rtol=1e-7), alpha
+    c = 299792458.0
+    em = oracle.Vi_electromagnetic_tranisotropic(d, 3.17, 3.13, 1.0, [0.0, np.pi / 2], [0.0, 0.0])
+    assert np.allclose(em[:, 0], c / np.sqrt(3.13), rtol=1e-7)
+    assert np.allclose(em[:, 1], [c / np.sqrt(3.13), c / np.sqrt(3.17)], rtol=1e-7)
+    # rotations
+    for th, ph in ((0.7, 1.1), (2.0, 4.0)):
+        r = oracle.rotate_nlm4(d, th, ph)
+        n = np.array([np.sin(th), 0.0, np.cos(th)])
+        assert np.allclose(oracle.a2(r), np.outer(n, n), atol=1e-7)
+    rng = np.random.default_rng(7)
+    x = np.zeros(15, complex)
+    x[0] = ISO
+    for l, c0 in ((2, 3), (4, 10)):
+        for m in range(1, l + 1):
+            v = rng.normal() + 1j * rng.normal()
+            x[c0 + m], x[c0 - m] = v, (-1) ** m * np.conj(v)
+        x[c0] = rng.normal()
+    r = oracle.rotate_nlm4(x, 0.0, 0.8)
+    m = np.array([0] + list(range(-2, 3)) + list(range(-4, 5)))
+    assert np.allclose(r, x * np.exp(-1j * m * 0.8), rtol=0, atol=1e-14)
+    r = oracle.rotate_nlm4(x, 1.3, 0.4)
+    for a, b in ((1, 6), (6, 15)):
+        assert abs(np.sum(abs(r[a:b]) ** 2) / np.sum(abs(x[a:b]) ** 2) - 1) < 1e-7
+
+
+def test_a4_full_closed_forms(oracle):
+    """Isotropic a4 (delta_ij delta_kl + perms)/15; z-delta a4_3333 = 1;
+    the full body's a4(3,2,1,2) equals its a4(1,2,3,3) (not the symmetric
+    a4(1,2,2,3)); trace a4_iikk = 1."""
+    iso = np.zeros(15, complex)
+    iso[0] = ISO
+    A = oracle.a4(iso)
+    I = np.eye(3)
+    ref = (np.einsum("ij,kl->ijkl", I, I) + np.einsum("ik,jl->ijkl", I, I) + np.einsum("il,jk->ijkl", I, I)) / 15
+    assert np.allclose(A, ref, atol=1e-7)
+    assert abs(oracle.a4(_delta_z())[2, 2, 2, 2] - 1) < 1e-7
+    g = load_golden("wave_L8.npz")
+    for x in g["nlm"][:5]:
+        A = oracle.a4(x)
+        assert A[2, 1, 0, 1] == A[0, 1, 2, 2]
+        assert abs(np.einsum("iikk->", A) - 1) < 1e-7
