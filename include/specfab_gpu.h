@@ -202,6 +202,74 @@ int sfg_Vi_electromagnetic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, c
                                                double eps_t, double mu, int ndir, const double* theta,
                                                const double* phi, double* Vi_dev, void* stream);
 
+
+/* ---- learned homogenization (no counterpart in the reference) ----
+ *
+ * A multilayer perceptron 36 -> 256 -> 256 -> 256 -> 9 (GELU), bf16 weights
+ * and activations with FP32 accumulation on the matrix cores, replaces the
+ * per-parcel Sachs/Taylor homogenization and the elastic-velocity solve:
+ *   inputs  Re, Im of n_l^m / n_0^0 for l <= 4 in specfab order (30) and the
+ *           unit strain rate D/|D| in Mandel order (6);
+ *   outputs log E_ij in the Cartesian frame (E11, E22, E33, E23, E13, E12)
+ *           and log of the elastic phase velocities along z, sorted
+ *           ascending (slower S, faster S, P).
+ * Inputs and outputs are normalised inside the network ((v - mean)/std, set
+ * by sfg_mlp_set_normalization).  Parameters are a flat FP32 array of
+ * sfg_mlp_nparams() = 143,369 values: W1 (36 x 256, [in][out]), b1, W2
+ * (256 x 256), b2, W3, b3, W4 (256 x 9), b4.  Training runs on the GPU (MSE on
+ * normalised outputs, Adam on FP32 master weights). */
+typedef struct sfg_mlp sfg_mlp;
+
+typedef struct {
+    double Eij_grain[2]; /* (Ecc, Eca) */
+    double alpha;        /* Sachs-Taylor weight of Eij */
+    int n_grain;         /* 1 or -3 */
+    double Lame[5];      /* grain elastic parameters (Cij_to_Lame_tranisotropic) */
+    double rho;          /* density */
+    double vi_alpha;     /* Reuss-Voigt weight of the velocities */
+} sfg_homog_params;
+
+typedef struct {
+    int64_t n_fallback;  /* parcels outside the envelope: plain homogenization */
+    int64_t n_audit;     /* audited parcels (surrogate kept, plain computed too) */
+    double audit_max_err_Eij, audit_mean_err_Eij;  /* max_ij |E - E_plain| / |E_plain| */
+    double audit_max_err_Vi, audit_mean_err_Vi;
+} sfg_surrogate_stats;
+
+int64_t sfg_mlp_nparams(void);
+int sfg_mlp_create(sfg_ctx* ctx, uint64_t seed, sfg_mlp** net);
+int sfg_mlp_destroy(sfg_mlp* net);
+int sfg_mlp_get_params(sfg_mlp* net, float* params);
+int sfg_mlp_set_params(sfg_mlp* net, const float* params);
+int sfg_mlp_set_normalization(sfg_mlp* net, const float in_mean[36], const float in_std[36], const float out_mean[9],
+                              const float out_std[9]);
+int sfg_mlp_get_normalization(sfg_mlp* net, float norm[90]);
+/* training envelope: sorted a2 eigenvalues lo[i] <= lambda_i <= hi[i] */
+int sfg_mlp_set_envelope(sfg_mlp* net, const double lo[3], const double hi[3]);
+/* one Adam step on B rows: X_dev (B x 36) raw features, T_dev (B x 9) raw
+ * targets; *loss (if non-NULL, synchronises) = batch MSE before the update */
+int sfg_mlp_train_step(sfg_mlp* net, int64_t B, const float* X_dev, const float* T_dev, float lr, float* loss,
+                       void* stream);
+/* gradients of the batch loss w.r.t. all parameters (host, nparams), no update */
+int sfg_mlp_gradients(sfg_mlp* net, int64_t B, const float* X_dev, const float* T_dev, float* grad, float* loss);
+/* inference: X_dev (B x 36) -> Y_dev (B x 9), raw units */
+int sfg_mlp_forward(sfg_mlp* net, int64_t B, const float* X_dev, float* Y_dev, void* stream);
+/* features (P x 36) of a batch state; L >= 4 */
+int sfg_surrogate_features(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* ugrad_dev,
+                           float* feat_dev, void* stream);
+/* training targets (P x 9) from plain outputs: Eij (P x 6, Cartesian) and
+ * Vi (P x 3, along z) */
+int sfg_surrogate_targets(int64_t P, const double* Eij_dev, const double* Vi_dev, float* T_dev, void* stream);
+/* Eij (P x 6, Cartesian) and sorted Vi (P x 3, along z) of a batch by the
+ * network; parcels whose a2 eigenvalues leave the envelope get the plain
+ * homogenization (status bit SFG_ST_ENVELOPE), a seeded fraction audit_frac of
+ * the others is also computed by the plain method (status bit SFG_ST_AUDIT)
+ * and the relative errors are reported in *stats.  Synchronises once. */
+int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const double* x_dev,
+                             const double* ugrad_dev, const sfg_homog_params* hp, double audit_frac, uint64_t seed,
+                             double* Eij_dev, double* Vi_dev, int32_t* status_dev, sfg_surrogate_stats* stats,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

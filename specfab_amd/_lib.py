@@ -29,6 +29,19 @@ class EvolveDesc(_c.Structure):
                 ("x_dev", _c.c_void_p), ("status_dev", _c.c_void_p)]
 
 
+class HomogParams(_c.Structure):
+    _fields_ = [("Eij_grain", _c.c_double * 2), ("alpha", _c.c_double), ("n_grain", _c.c_int),
+                ("Lame", _c.c_double * 5), ("rho", _c.c_double), ("vi_alpha", _c.c_double)]
+
+
+class SurrogateStats(_c.Structure):
+    _fields_ = [("n_fallback", _c.c_int64), ("n_audit", _c.c_int64), ("audit_max_err_Eij", _c.c_double),
+                ("audit_mean_err_Eij", _c.c_double), ("audit_max_err_Vi", _c.c_double),
+                ("audit_mean_err_Vi", _c.c_double)]
+
+
+_fp = _c.POINTER(_c.c_float)
+
 SIGNATURES = {
     "sfg_ctx_create": (_i, [_i, _c.POINTER(_vp)]),
     "sfg_ctx_destroy": (_i, [_vp]),
@@ -61,6 +74,21 @@ SIGNATURES = {
     "sfg_Vi_elastic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _dp, _d, _i, _dp, _dp, _vp, _vp, _vp]),
     "sfg_Vi_electromagnetic_tranisotropic": (_i, [_vp, _i, _dp, _d, _d, _d, _i, _dp, _dp, _dp]),
     "sfg_Vi_electromagnetic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _d, _d, _i, _dp, _dp, _vp, _vp]),
+    "sfg_mlp_nparams": (_i64, []),
+    "sfg_mlp_create": (_i, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "sfg_mlp_destroy": (_i, [_vp]),
+    "sfg_mlp_get_params": (_i, [_vp, _fp]),
+    "sfg_mlp_set_params": (_i, [_vp, _fp]),
+    "sfg_mlp_set_normalization": (_i, [_vp, _fp, _fp, _fp, _fp]),
+    "sfg_mlp_get_normalization": (_i, [_vp, _fp]),
+    "sfg_mlp_set_envelope": (_i, [_vp, _dp, _dp]),
+    "sfg_mlp_train_step": (_i, [_vp, _i64, _vp, _vp, _c.c_float, _fp, _vp]),
+    "sfg_mlp_gradients": (_i, [_vp, _i64, _vp, _vp, _fp, _fp]),
+    "sfg_mlp_forward": (_i, [_vp, _i64, _vp, _vp, _vp]),
+    "sfg_surrogate_features": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp]),
+    "sfg_surrogate_targets": (_i, [_i64, _vp, _vp, _vp, _vp]),
+    "sfg_surrogate_homogenize": (_i, [_vp, _vp, _i, _i64, _vp, _vp, _c.POINTER(HomogParams), _d, _c.c_uint64,
+                                      _vp, _vp, _vp, _c.POINTER(SurrogateStats), _vp]),
 }
 
 

@@ -21,6 +21,8 @@
 #include "sfg_fast.h"
 #include "sfg_homog.h"
 #include "sfg_wave.h"
+#include "sfg_mlp_net.h"
+#include "sfg_surrogate.h"
 
 namespace {
 
@@ -749,7 +751,7 @@ static int vi_em_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, dou
 int sfg_Vi_elastic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
                                        const double Lame[5], double rho, int ndir, const double* theta,
                                        const double* phi, double* Vi_dev, int32_t* status_dev, void* stream) {
-    if (!ctx || P < 0 || (P > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (!ctx || P < 0 || (P > 0 && ndir > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     return vi_el_locked(ctx, L, P, x_dev, alpha, Lame, rho, ndir, theta, phi, Vi_dev, status_dev, pick(ctx, stream));
 }
@@ -757,7 +759,7 @@ int sfg_Vi_elastic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const dou
 int sfg_Vi_electromagnetic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double eps_m,
                                                double eps_t, double mu, int ndir, const double* theta,
                                                const double* phi, double* Vi_dev, void* stream) {
-    if (!ctx || P < 0 || (P > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (!ctx || P < 0 || (P > 0 && ndir > 0 && (!x_dev || !Vi_dev))) return fail(SFG_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     return vi_em_locked(ctx, L, P, x_dev, eps_m, eps_t, mu, ndir, theta, phi, Vi_dev, pick(ctx, stream));
 }
@@ -824,6 +826,292 @@ int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm,
                                                  dout, c->stream);
                          },
                          &u);
+}
+
+// ---------------------------------------------------------------- surrogate network
+
+}  // extern "C"
+
+struct sfg_mlp {
+    sfg_ctx* ctx = nullptr;
+    sfg::MlpNet net;
+    // surrogate workspace (grow-only)
+    int64_t wcap = 0;
+    float* feat = nullptr;
+    float* y = nullptr;
+    int* flags = nullptr;
+    int* idx = nullptr;
+    int* count = nullptr;
+    double *Ep = nullptr, *Vp = nullptr, *errE = nullptr, *errV = nullptr;
+    std::vector<void*> wbufs;
+};
+
+namespace {
+
+int mlp_ws(sfg_mlp* m, int64_t P) {
+    if (P <= m->wcap) return SFG_OK;
+    HIPCHK(hipDeviceSynchronize());
+    for (void* p : m->wbufs) (void)hipFree(p);
+    m->wbufs.clear();
+    auto A = [&](void** p, size_t bytes) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+        if (e == hipSuccess) m->wbufs.push_back(*p);
+        return e;
+    };
+    const size_t Pz = (size_t)P;
+    HIPCHK(A((void**)&m->feat, Pz * 36 * 4));
+    HIPCHK(A((void**)&m->y, Pz * 9 * 4));
+    HIPCHK(A((void**)&m->flags, Pz * 4));
+    HIPCHK(A((void**)&m->idx, Pz * 4));
+    HIPCHK(A((void**)&m->count, 64));
+    HIPCHK(A((void**)&m->Ep, Pz * 6 * 8));
+    HIPCHK(A((void**)&m->Vp, Pz * 3 * 8));
+    HIPCHK(A((void**)&m->errE, Pz * 8));
+    HIPCHK(A((void**)&m->errV, Pz * 8));
+    m->wcap = P;
+    return SFG_OK;
+}
+
+constexpr int64_t MLP_CHUNK = 1 << 16;   // rows per forward pass of the surrogate
+
+}  // namespace
+
+extern "C" {
+
+int64_t sfg_mlp_nparams(void) { return sfg::MLP_NPARAM; }
+
+int sfg_mlp_create(sfg_ctx* ctx, uint64_t seed, sfg_mlp** net) {
+    if (!ctx || !net) return fail(SFG_ERR_ARG, "NULL argument");
+    auto* m = new sfg_mlp();
+    m->ctx = ctx;
+    hipError_t e = m->net.init(ctx->device, seed);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(SFG_ERR_HIP, std::string("sfg_mlp_create: ") + hipGetErrorString(e));
+    }
+    *net = m;
+    return SFG_OK;
+}
+
+int sfg_mlp_destroy(sfg_mlp* net) {
+    if (!net) return SFG_OK;
+    (void)hipSetDevice(net->net.device);
+    (void)hipDeviceSynchronize();
+    for (void* p : net->wbufs) (void)hipFree(p);
+    delete net;
+    return SFG_OK;
+}
+
+int sfg_mlp_get_params(sfg_mlp* net, float* params) {
+    if (!net || !params) return fail(SFG_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(net->net.device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(params, net->net.w, (size_t)sfg::MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    return SFG_OK;
+}
+
+int sfg_mlp_set_params(sfg_mlp* net, const float* params) {
+    if (!net || !params) return fail(SFG_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(net->net.device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(net->net.set_params(params));
+    net->net.trained = true;
+    return SFG_OK;
+}
+
+int sfg_mlp_set_normalization(sfg_mlp* net, const float in_mean[36], const float in_std[36], const float out_mean[9],
+                              const float out_std[9]) {
+    if (!net || !in_mean || !in_std || !out_mean || !out_std) return fail(SFG_ERR_ARG, "NULL argument");
+    std::vector<float> nm(2 * (sfg::MLP_IN + sfg::MLP_OUT));
+    for (int i = 0; i < sfg::MLP_IN; i++) {
+        if (!(in_std[i] > 0)) return fail(SFG_ERR_ARG, "in_std must be positive");
+        nm[i] = in_mean[i];
+        nm[sfg::MLP_IN + i] = in_std[i];
+    }
+    for (int i = 0; i < sfg::MLP_OUT; i++) {
+        if (!(out_std[i] > 0)) return fail(SFG_ERR_ARG, "out_std must be positive");
+        nm[2 * sfg::MLP_IN + i] = out_mean[i];
+        nm[2 * sfg::MLP_IN + sfg::MLP_OUT + i] = out_std[i];
+    }
+    HIPCHK(hipSetDevice(net->net.device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(net->net.norm, nm.data(), nm.size() * 4, hipMemcpyHostToDevice));
+    net->net.norm_host = nm;
+    return SFG_OK;
+}
+
+int sfg_mlp_get_normalization(sfg_mlp* net, float* norm90) {
+    if (!net || !norm90) return fail(SFG_ERR_ARG, "NULL argument");
+    std::copy(net->net.norm_host.begin(), net->net.norm_host.end(), norm90);
+    return SFG_OK;
+}
+
+int sfg_mlp_set_envelope(sfg_mlp* net, const double lo[3], const double hi[3]) {
+    if (!net || !lo || !hi) return fail(SFG_ERR_ARG, "NULL argument");
+    for (int i = 0; i < 3; i++) {
+        net->net.env_lo[i] = lo[i];
+        net->net.env_hi[i] = hi[i];
+    }
+    net->net.trained = true;
+    return SFG_OK;
+}
+
+static int mlp_loss(sfg_mlp* net, int64_t B, hipStream_t s, float* loss) {
+    std::vector<float> rows((size_t)B);
+    HIPCHK(hipMemcpyAsync(rows.data(), net->net.loss_rows, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double acc = 0;
+    for (float r : rows) acc += r;
+    *loss = (float)(acc / ((double)B * sfg::MLP_OUT));
+    return SFG_OK;
+}
+
+int sfg_mlp_train_step(sfg_mlp* net, int64_t B, const float* X_dev, const float* T_dev, float lr, float* loss,
+                       void* stream) {
+    if (!net || B <= 0 || !X_dev || !T_dev) return fail(SFG_ERR_ARG, "bad argument");
+    if (B > INT32_MAX) return fail(SFG_ERR_ARG, "batch too large");
+    HIPCHK(hipSetDevice(net->net.device));
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(net->net.ensure_batch(B));
+    HIPCHK(net->net.forward(B, X_dev, s));
+    HIPCHK(net->net.backward(B, T_dev, s));
+    if (loss) {
+        int rc = mlp_loss(net, B, s, loss);
+        if (rc) return rc;
+    }
+    HIPCHK(net->net.adam(lr, s));
+    return SFG_OK;
+}
+
+int sfg_mlp_gradients(sfg_mlp* net, int64_t B, const float* X_dev, const float* T_dev, float* grad, float* loss) {
+    if (!net || B <= 0 || !X_dev || !T_dev || !grad) return fail(SFG_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(net->net.device));
+    hipStream_t s = net->ctx->stream;
+    HIPCHK(net->net.ensure_batch(B));
+    HIPCHK(net->net.forward(B, X_dev, s));
+    HIPCHK(net->net.backward(B, T_dev, s));
+    if (loss) {
+        int rc = mlp_loss(net, B, s, loss);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(grad, net->net.g, (size_t)sfg::MLP_NPARAM * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_mlp_forward(sfg_mlp* net, int64_t B, const float* X_dev, float* Y_dev, void* stream) {
+    if (!net || B < 0 || (B > 0 && (!X_dev || !Y_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(net->net.device));
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(net->net.ensure_batch(std::min<int64_t>(B, MLP_CHUNK)));
+    for (int64_t off = 0; off < B; off += MLP_CHUNK) {
+        const int64_t n = std::min<int64_t>(MLP_CHUNK, B - off);
+        HIPCHK(net->net.forward(n, X_dev + off * sfg::MLP_IN, s));
+        HIPCHK(sfg::launch_denormalize_out(n, net->net.yout, net->net.norm, Y_dev + off * sfg::MLP_OUT, s));
+    }
+    return SFG_OK;
+}
+
+int sfg_surrogate_features(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* ugrad_dev,
+                           float* feat_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !ugrad_dev || !feat_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (L < 4) return fail(SFG_ERR_L, "surrogate features need the l <= 4 coefficients (L >= 4)");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    sfg::SurrogateFeatArgs a{};
+    a.P = P; a.x = x_dev; a.ugrad = ugrad_dev; a.A2 = op->A2; a.feat = feat_dev; a.flags = nullptr;
+    HIPCHK(sfg::launch_features(a, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_surrogate_targets(int64_t P, const double* Eij_dev, const double* Vi_dev, float* T_dev, void* stream) {
+    if (P < 0 || (P > 0 && (!Eij_dev || !Vi_dev || !T_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    HIPCHK(sfg::launch_targets(P, Eij_dev, Vi_dev, T_dev, (hipStream_t)stream));
+    return SFG_OK;
+}
+
+int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const double* x_dev,
+                             const double* ugrad_dev, const sfg_homog_params* hp, double audit_frac, uint64_t seed,
+                             double* Eij_dev, double* Vi_dev, int32_t* status_dev, sfg_surrogate_stats* stats,
+                             void* stream) {
+    if (!ctx || !net || !hp || P < 0 || (P > 0 && (!x_dev || !ugrad_dev || !Eij_dev || !Vi_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    if (L < 4) return fail(SFG_ERR_L, "the surrogate needs the l <= 4 coefficients (L >= 4)");
+    if (hp->n_grain != 1 && hp->n_grain != -3) return fail(SFG_ERR_ARG, "n_grain must be 1 or -3");
+    if (!(audit_frac >= 0 && audit_frac <= 1)) return fail(SFG_ERR_ARG, "audit_frac must be in [0, 1]");
+    if (!net->net.trained) return fail(SFG_ERR_ARG, "surrogate network has no trained parameters/envelope");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    sfg_surrogate_stats st{};
+    if (P == 0) {
+        if (stats) *stats = st;
+        return SFG_OK;
+    }
+    hipStream_t s = pick(ctx, stream);
+    rc = mlp_ws(net, P);
+    if (rc) return rc;
+    // 1. features, envelope and audit flags
+    sfg::SurrogateFeatArgs fa{};
+    fa.P = P; fa.x = x_dev; fa.ugrad = ugrad_dev; fa.A2 = op->A2; fa.feat = net->feat; fa.flags = net->flags;
+    for (int i = 0; i < 3; i++) { fa.env_lo[i] = net->net.env_lo[i]; fa.env_hi[i] = net->net.env_hi[i]; }
+    fa.audit_frac = audit_frac; fa.seed = seed;
+    HIPCHK(sfg::launch_features(fa, s));
+    // 2. network (chunks), 3. decode
+    HIPCHK(net->net.ensure_batch(std::min<int64_t>(P, MLP_CHUNK)));
+    for (int64_t off = 0; off < P; off += MLP_CHUNK) {
+        const int64_t n = std::min<int64_t>(MLP_CHUNK, P - off);
+        HIPCHK(net->net.forward(n, net->feat + off * sfg::MLP_IN, s));
+        HIPCHK(sfg::launch_denormalize_out(n, net->net.yout, net->net.norm, net->y + off * sfg::MLP_OUT, s));
+    }
+    HIPCHK(sfg::launch_decode(P, net->y, Eij_dev, Vi_dev, s));
+    // 4. plain homogenization of the flagged parcels
+    HIPCHK(hipMemsetAsync(net->count, 0, sizeof(int), s));
+    HIPCHK(sfg::launch_select(P, net->flags, net->idx, net->count, s));
+    int nsel = 0;
+    HIPCHK(hipMemcpyAsync(&nsel, net->count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (nsel > 0) {
+        sfg::EijArgs ea{};
+        ea.P = P; ea.nq = op->nq; ea.frame = 2; ea.n_grain = hp->n_grain;
+        ea.Ecc = hp->Eij_grain[0]; ea.Eca = hp->Eij_grain[1]; ea.alpha = hp->alpha;
+        ea.x = x_dev; ea.A2 = op->A2; ea.A4 = op->A4; ea.Eij = net->Ep; ea.idx = net->idx; ea.nidx = nsel;
+        HIPCHK(sfg::launch_eij(ea, s));
+        sfg::ViElasticArgs va{};
+        const double zero = 0.0;
+        rc = get_dirs(ctx, 1, &zero, &zero, &va.R);
+        if (rc) return rc;
+        va.P = P; va.ndir = 1; va.nq = op->nq; va.x = x_dev; va.A2 = op->A2; va.A4 = op->A4; va.A4F = op->A4F;
+        va.alpha = hp->vi_alpha; va.rho = hp->rho;
+        sfg::elas_params(hp->Lame, va.kf, va.kr);
+        va.Vi = net->Vp; va.idx = net->idx; va.nidx = nsel;
+        HIPCHK(sfg::launch_vi_elastic(va, s));
+        HIPCHK(sfg::launch_merge(nsel, net->idx, net->flags, net->Ep, net->Vp, Eij_dev, Vi_dev, status_dev,
+                                 net->errE, net->errV, s));
+        std::vector<double> eE((size_t)nsel), eV((size_t)nsel);
+        HIPCHK(hipMemcpyAsync(eE.data(), net->errE, (size_t)nsel * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(eV.data(), net->errV, (size_t)nsel * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double sE = 0, sV = 0;
+        for (int t = 0; t < nsel; t++) {
+            if (eE[t] < 0) { st.n_fallback++; continue; }
+            st.n_audit++;
+            st.audit_max_err_Eij = std::max(st.audit_max_err_Eij, eE[t]);
+            st.audit_max_err_Vi = std::max(st.audit_max_err_Vi, eV[t]);
+            sE += eE[t];
+            sV += eV[t];
+        }
+        if (st.n_audit) {
+            st.audit_mean_err_Eij = sE / st.n_audit;
+            st.audit_mean_err_Vi = sV / st.n_audit;
+        }
+    }
+    if (stats) *stats = st;
+    return SFG_OK;
 }
 
 }  // extern "C"

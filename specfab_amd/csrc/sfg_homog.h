@@ -19,6 +19,10 @@ struct EijArgs {
     double* lami;         // (P, 3) or null
     double* ei;           // (P, 3, 3) or null
     int* status;          // (P) or null
+    // optional work list: items t < nidx evaluate state column idx[t] and
+    // write Eij/lami/ei at row t (status stays at the parcel's own row)
+    const int* idx;
+    int64_t nidx;
 };
 
 hipError_t launch_eij(const EijArgs& a, hipStream_t s);

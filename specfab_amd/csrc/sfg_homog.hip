@@ -245,8 +245,10 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
 }
 
 __global__ void eij_kernel(EijArgs a) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.idx ? a.nidx : a.P)) return;
+    const int64_t p = a.idx ? (int64_t)a.idx[t] : t;   // state column
+    const int64_t o = t;                               // output row
     double a2v[6], a4v[36];
     d_moments(a.x, a.P, p, a.nq, a.A2, a.A4, a2v, a4v);
     double e[9], lam[3] = {0, 0, 0};
@@ -261,15 +263,16 @@ __global__ void eij_kernel(EijArgs a) {
     }
     double E[6];
     int st = d_Eij(a2v, a4v, e, a.Ecc, a.Eca, a.alpha, a.n_grain, E);
-    for (int i = 0; i < 6; i++) a.Eij[p * 6 + i] = E[i];
-    if (a.lami) for (int i = 0; i < 3; i++) a.lami[p * 3 + i] = lam[i];
-    if (a.ei) for (int i = 0; i < 9; i++) a.ei[p * 9 + i] = e[i];
-    if (a.status) a.status[p] |= st;
+    for (int i = 0; i < 6; i++) a.Eij[o * 6 + i] = E[i];
+    if (a.lami) for (int i = 0; i < 3; i++) a.lami[o * 3 + i] = lam[i];
+    if (a.ei) for (int i = 0; i < 9; i++) a.ei[o * 9 + i] = e[i];
+    if (a.status && st) atomicOr(a.status + p, st);
 }
 
 hipError_t launch_eij(const EijArgs& a, hipStream_t s) {
-    if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(eij_kernel, dim3((unsigned)((a.P + 127) / 128)), dim3(128), 0, s, a);
+    const int64_t n = a.idx ? a.nidx : a.P;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(eij_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,29 @@
+// Surrogate homogenization kernels (see sfg_surrogate.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfg {
+
+enum { SUR_FALLBACK = 1, SUR_AUDIT = 2 };
+
+struct SurrogateFeatArgs {
+    int64_t P;
+    const double* x;       // real SoA state (n, P), n >= 15
+    const double* ugrad;   // (P, 3, 3)
+    const double* A2;      // (6, 15) Mandel a2 map
+    float* feat;           // (P, 36)
+    int* flags;            // (P) or null: SUR_FALLBACK / SUR_AUDIT
+    double env_lo[3], env_hi[3];
+    double audit_frac;
+    uint64_t seed;
+};
+
+hipError_t launch_features(const SurrogateFeatArgs& a, hipStream_t s);
+hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float* T, hipStream_t s);
+hipError_t launch_decode(int64_t n, const float* y, double* Eij, double* Vi, hipStream_t s);
+hipError_t launch_select(int64_t P, const int* flags, int* idx, int* count, hipStream_t s);
+hipError_t launch_merge(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp, double* Eij,
+                        double* Vi, int* status, double* errE, double* errV, hipStream_t s);
+
+}  // namespace sfg

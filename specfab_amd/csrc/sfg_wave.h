@@ -24,6 +24,10 @@ struct ViElasticArgs {
     double kr[5];         // elas_revparams (Voigt)
     double* Vi;           // (P, ndir, 3): qS1, qS2, qP
     int* status;          // (P) or null
+    // optional work list: items t < nidx evaluate state column idx[t] and
+    // write Vi at row t
+    const int* idx;
+    int64_t nidx;
 };
 
 struct ViEMArgs {

@@ -209,9 +209,11 @@ __device__ void qnorm_to_vi(const double* Q, double rho, double* vi) {
 
 __global__ void __launch_bounds__(128) vi_elastic_kernel(ViElasticArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.P * a.ndir) return;
-    const int64_t p = t % a.P;
-    const int d = (int)(t / a.P);
+    const int64_t nwork = a.idx ? a.nidx : a.P;
+    if (t >= nwork * a.ndir) return;
+    const int64_t item = t % nwork;
+    const int d = (int)(t / nwork);
+    const int64_t p = a.idx ? (int64_t)a.idx[item] : item;   // state column
     double xr[15];
     rotated_block(a.x, a.P, p, a.nq, a.R + (int64_t)d * ROT4_STRIDE, xr);
     const double x0 = xr[0];
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(128) vi_elastic_kernel(ViElasticArgs a) {
         vi[0] = vi[1] = vi[2] = __builtin_nan("");
         if (a.status) atomicOr(a.status + p, 8);
     }
-    double* o = a.Vi + (p * a.ndir + d) * 3;
+    double* o = a.Vi + (item * a.ndir + d) * 3;
     o[0] = vi[0];
     o[1] = vi[1];
     o[2] = vi[2];
@@ -337,8 +339,9 @@ __global__ void __launch_bounds__(128) a4full_kernel(A4FullArgs a) {
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s) {
-    if (a.P * a.ndir == 0) return hipSuccess;
-    hipLaunchKernelGGL(vi_elastic_kernel, dim3(nblk(a.P * a.ndir, 128)), dim3(128), 0, s, a);
+    const int64_t n = (a.idx ? a.nidx : a.P) * a.ndir;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(vi_elastic_kernel, dim3(nblk(n, 128)), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,142 @@
+"""GPU checks of the surrogate network's hand-written kernels against the CPU
+restatement oracle/mlp_ref.py (bf16 rounding emulated at the same points):
+
+  * forward pass (this also fixes the v_mfma_f32_16x16x32_bf16 operand and
+    accumulator lane layout assumed by the GEMM kernel: a wrong layout gives
+    O(1) errors);
+  * loss and gradients of every parameter block;
+  * training reduces the loss of a teacher-student problem;
+  * the trained surrogate against the plain homogenization on an evolved
+    field, with the envelope fallback and the audit.
+
+Tolerances: forward outputs 2e-3 of their range and gradients 1e-2 of each
+block's norm.  The GPU accumulates bf16 products in FP32 in a different order
+and evaluates GELU in FP32, so an activation can round to the neighbouring
+bf16 value (2^-8 relative) where the FP64 reference does not."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_net(rng):
+    n = sum(M.DIMS[i] * M.DIMS[i + 1] + M.DIMS[i + 1] for i in range(4))
+    p = np.concatenate([np.concatenate([rng.normal(0, np.sqrt(2.0 / M.DIMS[i]), M.DIMS[i] * M.DIMS[i + 1]),
+                                        rng.normal(0, 0.05, M.DIMS[i + 1])]) for i in range(4)]).astype(np.float32)
+    assert p.size == n
+    norm = np.concatenate([rng.normal(0, 0.1, 36), rng.uniform(0.5, 2, 36), rng.normal(0, 0.1, 9),
+                           rng.uniform(0.5, 2, 9)]).astype(np.float32)
+    return p, norm
+
+
+def _net(gpu_ctx, p, norm):
+    from specfab_amd.surrogate import Surrogate
+    net = Surrogate(seed=5)
+    net.set_normalization(norm[:36], norm[36:72], norm[72:81], norm[81:90])
+    net.set_params(p)
+    return net
+
+
+def test_forward_matches_cpu(gpu_ctx):
+    import torch
+    rng = np.random.default_rng(1)
+    p, norm = _rand_net(rng)
+    net = _net(gpu_ctx, p, norm)
+    for B in (1, 63, 300):
+        X = rng.normal(0, 1, (B, 36)).astype(np.float32)
+        Y = net.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+        yn, _, _ = M.forward(p, X, norm.astype(np.float64), bf=True)
+        ref = yn * norm[81:90] + norm[72:81]
+        err = np.abs(Y - ref).max() / (np.abs(ref).max() + 1e-30)
+        assert err < 2e-3, (B, err)
+
+
+def test_gradients_match_cpu(gpu_ctx):
+    import torch
+    rng = np.random.default_rng(2)
+    p, norm = _rand_net(rng)
+    net = _net(gpu_ctx, p, norm)
+    B = 200
+    X = rng.normal(0, 1, (B, 36)).astype(np.float32)
+    T = rng.normal(0, 1, (B, 9)).astype(np.float32)
+    g, loss = net.gradients(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+    lref, gref = M.loss_and_grad(p, X, T, norm.astype(np.float64), bf=True)
+    assert abs(loss - lref) < 2e-3 * abs(lref)
+    off = 0
+    for i in range(4):
+        for n in (M.DIMS[i] * M.DIMS[i + 1], M.DIMS[i + 1]):
+            a, b = g[off:off + n], gref[off:off + n]
+            assert np.linalg.norm(a - b) < 1e-2 * np.linalg.norm(b), (i, n)
+            off += n
+    assert off == g.size
+
+
+def test_training_reduces_loss(gpu_ctx):
+    """Teacher-student: targets from a fixed random network."""
+    import torch
+    rng = np.random.default_rng(3)
+    pt, norm = _rand_net(rng)
+    N = 16384
+    X = rng.normal(0, 1, (N, 36)).astype(np.float32)
+    yn, _, _ = M.forward(pt, X[:], norm.astype(np.float64), bf=False)
+    T = (yn * norm[81:90] + norm[72:81]).astype(np.float32)
+    from specfab_amd.surrogate import Surrogate
+    net = Surrogate(seed=9)
+    net.set_normalization(norm[:36], norm[36:72], norm[72:81], norm[81:90])
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    hist = net.fit(Xd, Td, epochs=12, batch=1024, lr=1e-3)
+    assert hist[-1] < 0.1 * hist[0], hist
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "specfab_amd", "data",
+                       "surrogate_ice_L12.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_surrogate_vs_plain_on_field(gpu_ctx):
+    """Evolved L=12 field (LROT+REG+DDRX, RK4 dt=0.5, 200 steps): every
+    in-envelope parcel audited against the plain homogenization."""
+    import torch
+    from specfab_amd import surrogate as S
+    net = S.Surrogate().load(WEIGHTS)
+    fb = _field(4096, 200, seed=5)
+    E, V, st = net.homogenize(fb, audit_frac=1.0, seed=3)
+    torch.cuda.synchronize()
+    assert st["n_audit"] + st["n_fallback"] == fb.P
+    assert st["n_audit"] > 0.8 * fb.P
+    assert st["audit_max_err_Eij"] < 1e-2
+    assert st["audit_mean_err_Eij"] < 1e-3
+    assert st["audit_max_err_Vi"] < 1e-3
+    # fallback parcels carry the plain values exactly
+    Tp, Ep, Vp = S.plain_targets(fb)
+    flags = fb.status.cpu().numpy()
+    fbk = (flags & 16) != 0
+    if fbk.any():
+        assert np.abs(E.cpu().numpy()[fbk] - Ep.cpu().numpy()[fbk]).max() == 0.0
+
+
+def _field(P, steps, seed):
+    import math
+    import torch
+    from specfab_amd import _lib
+    from specfab_amd import surrogate as S
+    from specfab_amd.batch import FabricBatch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    L, n = 12, 91
+    fb = FabricBatch.__new__(FabricBatch)
+    fb.ctx = _lib.default_context()
+    fb.L, fb.n, fb.dev, fb.P = L, n, dev, P
+    fb.ugrad = S.random_gradients(P, g, dev)
+    fb.gam0 = (10.0 ** (torch.rand(P, generator=g, device=dev, dtype=torch.float64) * 2 - 4)).contiguous()
+    x = torch.randn((n, P), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    x[0] = 1 / math.sqrt(4 * math.pi)
+    fb.x, fb.tau, fb.lam, fb.steps = x.contiguous(), None, None, None
+    fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
+    fb.evolve(steps, 0.5, scheme="rk4", procs=("LROT", "REG", "DDRX"))
+    return fb
