@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the parcel sharding across GPUs is described in "
@@ -113,6 +115,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if a.copy_probe:
+        n = 1 << 27
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copy probe: 3 copies of %d bytes" % (n * 8), flush=True)
+        return
     from specfab_amd import batch as B
     from specfab_amd import _lib
 

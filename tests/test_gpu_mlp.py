@@ -76,20 +76,21 @@ def test_gradients_match_cpu(gpu_ctx):
 
 
 def test_training_reduces_loss(gpu_ctx):
-    """Teacher-student: targets from a fixed random network."""
+    """A smooth target (linear plus a bounded nonlinear part) of 36 inputs:
+    the loss of the GPU training loop falls by more than 20x."""
     import torch
     rng = np.random.default_rng(3)
-    pt, norm = _rand_net(rng)
     N = 16384
     X = rng.normal(0, 1, (N, 36)).astype(np.float32)
-    yn, _, _ = M.forward(pt, X[:], norm.astype(np.float64), bf=False)
-    T = (yn * norm[81:90] + norm[72:81]).astype(np.float32)
+    A = rng.normal(0, 1 / 6, (36, 9))
+    C = rng.normal(0, 1 / 6, (36, 9))
+    T = (X @ A + 0.3 * np.tanh(X @ C)).astype(np.float32)
     from specfab_amd.surrogate import Surrogate
     net = Surrogate(seed=9)
-    net.set_normalization(norm[:36], norm[36:72], norm[72:81], norm[81:90])
+    net.set_normalization(np.zeros(36), np.ones(36), T.mean(0), T.std(0))
     Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
-    hist = net.fit(Xd, Td, epochs=12, batch=1024, lr=1e-3)
-    assert hist[-1] < 0.1 * hist[0], hist
+    hist = net.fit(Xd, Td, epochs=12, batch=512, lr=1e-3)
+    assert hist[-1] < 0.05 * hist[0], hist
 
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "specfab_amd", "data",
