@@ -136,6 +136,7 @@ def main():
     fb.x, fb.ugrad, fb.tau, fb.gam0, fb.lam, fb.steps = x, G, None, gam0, None, None
     fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
     procs = ("LROT", "REG", "DDRX")
+    variant, variant_ms = fb.autotune(a.precision)
     for _ in range(a.warmup):
         fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
     torch.cuda.synchronize()
@@ -198,9 +199,10 @@ def main():
                      "frac": achieved / peak, "traffic": None,
                      "compute": "%s vector ALU (sparse FMA; MI355X %s peak %.1f TFLOP/s)"
                                 % ("fp64" if a.precision == "fp64" else "fp32", a.precision, peak),
-                     "kernel": "evolve_fast_L%d_%s (one RK4 step of the field)"
-                               % (L, "f64" if a.precision == "fp64" else "f32"),
+                     "kernel": "evolve_fast_L%d_%s_v%d (one RK4 step of the field)"
+                               % (L, "f64" if a.precision == "fp64" else "f32", variant),
                      "flop_per_parcel_step": flops_per_parcel_step(L), "launch_ms": launch_ms},
+        "autotune": {"variant": variant, "ms_per_step_by_variant": variant_ms, "probe_parcels": min(P, 1 << 18)},
         "eij_evals_per_s": eij_rate,
         "vi_elastic_evals_per_s": vi_rate,
         "nonfinite_parcels": nonfinite,

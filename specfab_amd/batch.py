@@ -106,6 +106,19 @@ class FabricBatch:
         check(lib().sfg_evolve(self.ctx.handle, _c.byref(d), _stream()))
         return self
 
+    def autotune(self, precision="fp64", P=None):
+        """Time the generated stepper variants for this L and precision on a
+        synthetic batch (default: min(P, 2^18) parcels) and keep the fastest
+        for the context.  Returns (chosen variant, per-variant ms per step)."""
+        Pp = int(P or min(self.P, 1 << 18))
+        best = _c.c_int(-1)
+        ms = (_c.c_double * 16)()
+        prec = _lib.SFG_FP64 if precision == "fp64" else _lib.SFG_FP32C
+        check(lib().sfg_autotune(self.ctx.handle, self.L, prec, Pp, _stream(), _c.byref(best), ms, 16))
+        nv = max(0, best.value + 1)
+        times = [ms[i] for i in range(16) if ms[i] > 0]
+        return best.value, times
+
     # ---- moments ----
     def a2(self):
         out = torch.empty((self.P, 3, 3), dtype=torch.float64, device=self.dev)

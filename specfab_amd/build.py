@@ -51,7 +51,12 @@ def compile_lib(jobs=8):
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + HIPFLAGS + ["-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        extra = []
+        if src.startswith("generated" + os.sep + "sfg_fast_L"):
+            # the steppers are one long basic block per row chunk: keep the
+            # literal constants next to their uses (no loop-invariant hoisting)
+            extra = ["-mllvm", "-disable-machine-licm"]
+        cmd = [HIPCC] + HIPFLAGS + extra + ["-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd)))
         while len([p for _, p in procs if p.poll() is None]) >= jobs:

@@ -48,6 +48,11 @@ hipError_t upload(T** dptr, const std::vector<T>& v) {
     return e;
 }
 
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
 struct DevOp {
     int L = 0, n = 0, nblk = 0, nq = 0;
     double reg_nu = 0;
@@ -75,6 +80,8 @@ struct DevOp {
 struct sfg_ctx {
     int device = 0;
     int stepper = 0;      // SFG_STEPPER: 0 = fastest available, 1 = generic only
+    int variant_env = -1; // SFG_VARIANT: force one generated stepper variant
+    std::map<std::pair<int, int>, int> variant;   // (L, precision) -> tuned variant
     bool sync_check = false;  // SFG_SYNC_CHECK=1: synchronise and check after each launch
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -313,6 +320,7 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
     c->device = device;
     if (const char* e = std::getenv("SFG_STEPPER")) c->stepper = (std::string(e) == "generic") ? 1 : 0;
     if (const char* e = std::getenv("SFG_SYNC_CHECK")) c->sync_check = std::string(e) == "1";
+    if (const char* e = std::getenv("SFG_VARIANT")) c->variant_env = std::atoi(e);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming);
     if (e != hipSuccess) { delete c; return fail(SFG_ERR_HIP, hipGetErrorString(e)); }
@@ -363,14 +371,18 @@ int sfg_state_to_nlm(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double
 
 // ---------------------------------------------------------------- evolution
 
-int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
-    if (!ctx || !d) return fail(SFG_ERR_ARG, "ctx/desc is NULL");
-    if (d->P < 0 || (d->P > 0 && (!d->ugrad_dev || !d->x_dev))) return fail(SFG_ERR_ARG, "ugrad_dev/x_dev missing");
-    if (d->scheme != SFG_EULER && d->scheme != SFG_RK4) return fail(SFG_ERR_ARG, "unknown scheme");
-    if (d->precision != SFG_FP64 && d->precision != SFG_FP32C) return fail(SFG_ERR_ARG, "unknown precision");
-    if (d->nsteps < 0) return fail(SFG_ERR_ARG, "nsteps < 0");
-    if (d->procs & ~(SFG_LROT | SFG_DDRX | SFG_CDRX | SFG_REG)) return fail(SFG_ERR_ARG, "unknown process bit");
-    std::lock_guard<std::mutex> lk(ctx->mu);
+namespace {
+
+int variant_of(sfg_ctx* ctx, int L, int prec) {
+    const int nv = sfg::fast_variants(L, prec);
+    if (ctx->variant_env >= 0 && ctx->variant_env < nv) return ctx->variant_env;
+    auto it = ctx->variant.find({L, prec});
+    return (it != ctx->variant.end() && it->second < nv) ? it->second : 0;
+}
+
+// body of sfg_evolve; ctx->mu held.  force_variant >= 0 selects a generated
+// stepper variant (autotuner).
+int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int force_variant) {
     const DevOp* op;
     int rc = get_op(ctx, d->L, &op);
     if (rc) return rc;
@@ -378,7 +390,6 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
         return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
     if (d->P == 0) return SFG_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    hipStream_t s = pick(ctx, stream);
     const int64_t P = d->P;
     const int n = op->n;
     size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
@@ -409,11 +420,14 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
     ea.status = d->status_dev;
     ea.zero = 0;
-    const bool fast64 = d->precision == SFG_FP64 && ctx->stepper == 0 && sfg::fast_available(d->L, 0);
-    const bool fast32 = d->precision == SFG_FP32C && sfg::fast_available(d->L, 1);
-    if (!fast64 && !fast32 && d->precision != SFG_FP64) {
+    // the register-resident steppers address SoA rows with 32-bit buffer
+    // offsets (and use offsets >= 2^31 as "out of range"): n * P * 8 < 2^31
+    const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
+    const int prec = d->precision == SFG_FP64 ? 0 : 1;
+    const bool fast = fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0);
+    if (!fast && prec == 1) {
         ws_release(ctx, s);
-        return fail(SFG_ERR_ARG, "FP32C stepping is not available for this L");
+        return fail(SFG_ERR_ARG, "FP32C stepping needs L <= 16 and n * P * 8 < 2^31 (split the batch)");
     }
     std::pair<hipEvent_t, hipEvent_t> tv{nullptr, nullptr};
     if (ctx->timing) {
@@ -421,9 +435,12 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
         HIPCHK(hipEventCreate(&tv.second));
         HIPCHK(hipEventRecord(tv.first, s));
     }
-    if (fast64) HIPCHK(sfg::launch_evolve_fast(d->L, 0, ea, s));
-    else if (fast32) HIPCHK(sfg::launch_evolve_fast(d->L, 1, ea, s));
-    else HIPCHK(sfg::launch_evolve_generic(ea, s));
+    if (fast) {
+        const int v = force_variant >= 0 ? force_variant : variant_of(ctx, d->L, prec);
+        HIPCHK(sfg::launch_evolve_fast(d->L, prec, v, ea, s));
+    } else {
+        HIPCHK(sfg::launch_evolve_generic(ea, s));
+    }
     if (ctx->timing) {
         HIPCHK(hipEventRecord(tv.second, s));
         ctx->tev.push_back(tv);
@@ -434,6 +451,90 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(hipGetLastError());
     }
+    return SFG_OK;
+}
+
+}  // namespace
+
+int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
+    if (!ctx || !d) return fail(SFG_ERR_ARG, "ctx/desc is NULL");
+    if (d->P < 0 || (d->P > 0 && (!d->ugrad_dev || !d->x_dev))) return fail(SFG_ERR_ARG, "ugrad_dev/x_dev missing");
+    if (d->scheme != SFG_EULER && d->scheme != SFG_RK4) return fail(SFG_ERR_ARG, "unknown scheme");
+    if (d->precision != SFG_FP64 && d->precision != SFG_FP32C) return fail(SFG_ERR_ARG, "unknown precision");
+    if (d->nsteps < 0) return fail(SFG_ERR_ARG, "nsteps < 0");
+    if (d->procs & ~(SFG_LROT | SFG_DDRX | SFG_CDRX | SFG_REG)) return fail(SFG_ERR_ARG, "unknown process bit");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return evolve_locked(ctx, d, pick(ctx, stream), -1);
+}
+
+int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, void* stream, int* best, double* ms, int ms_len) {
+    if (!ctx || P <= 0 || (precision != SFG_FP64 && precision != SFG_FP32C)) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int prec = precision == SFG_FP64 ? 0 : 1;
+    const int nv = sfg::fast_variants(L, prec);
+    if (best) *best = -1;
+    if (nv == 0) return SFG_OK;
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = pick(ctx, stream);
+    const int n = op->n;
+    // synthetic batch: perturbed isotropic states, traceless gradients
+    std::vector<double> hx((size_t)n * P), hg((size_t)9 * P), hgam((size_t)P);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return ((double)(st >> 11) * (1.0 / 9007199254740992.0)) - 0.5;
+    };
+    for (int64_t p = 0; p < P; p++) {
+        hx[p] = 0.28209479177387814;
+        for (int a = 1; a < n; a++) hx[(size_t)a * P + p] = 2e-3 * rnd();
+        double t = 0;
+        for (int i = 0; i < 9; i++) hg[9 * p + i] = 2e-3 * rnd();
+        for (int i = 0; i < 3; i++) t += hg[9 * p + 4 * i];
+        for (int i = 0; i < 3; i++) hg[9 * p + 4 * i] -= t / 3;
+        hgam[p] = 1e-3;
+    }
+    DevBuf bx, bg, bgam, bst;
+    HIPCHK(hipMalloc(&bx.p, hx.size() * 8));
+    HIPCHK(hipMalloc(&bg.p, hg.size() * 8));
+    HIPCHK(hipMalloc(&bgam.p, hgam.size() * 8));
+    HIPCHK(hipMalloc(&bst.p, (size_t)P * 4));
+    HIPCHK(hipMemcpy(bx.p, hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bg.p, hg.data(), hg.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bgam.p, hgam.data(), hgam.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(bst.p, 0, (size_t)P * 4));
+    sfg_evolve_desc d{};
+    d.L = L; d.P = P; d.procs = SFG_LROT | SFG_REG | SFG_DDRX; d.scheme = SFG_RK4; d.precision = precision;
+    d.nsteps = 1; d.dt = 0.5; d.iota = 1.0; d.zeta = 0.0;
+    d.ugrad_dev = (double*)bg.p; d.gam0_dev = (double*)bgam.p; d.x_dev = (double*)bx.p; d.status_dev = (int32_t*)bst.p;
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double tbest = 1e300;
+    int vbest = 0;
+    for (int v = 0; v < nv; v++) {
+        rc = evolve_locked(ctx, &d, s, v);   // warm-up (code object load)
+        if (rc) break;
+        HIPCHK(hipEventRecord(e0, s));
+        for (int k = 0; k < 3 && !rc; k++) rc = evolve_locked(ctx, &d, s, v);
+        if (rc) break;
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        if (ms && v < ms_len) ms[v] = t / 3;
+        if (t < tbest) { tbest = t; vbest = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timing = timing;
+    if (rc) return rc;
+    ctx->variant[{L, prec}] = vbest;
+    if (best) *best = vbest;
     return SFG_OK;
 }
 
@@ -460,10 +561,6 @@ int sfg_a4mandel_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, doub
 
 namespace {
 
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
 
 int dense_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const double* A_dev, const double* B_dev, double iota,
                 double zeta, double* M_dev, hipStream_t s) {

@@ -15,7 +15,8 @@ typedef int hipError_t;
 typedef void* hipStream_t;
 #define __global__
 #define __device__
-#define __launch_bounds__(n)
+#define __launch_bounds__(...)
+#define __builtin_amdgcn_sched_barrier(m) ((void)0)
 #define __shared__ static
 #define __forceinline__ inline
 #define __constant__

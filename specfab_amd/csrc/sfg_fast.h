@@ -1,11 +1,14 @@
-// Register-resident steppers (generated per truncation L): availability and launch.
+// Register-resident steppers (generated per truncation L and variant):
+// availability and launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "sfg_kernels.h"
 
 namespace sfg {
-// prec: 0 = FP64, 1 = FP32 with compensated state update
-bool fast_available(int L, int prec);
-hipError_t launch_evolve_fast(int L, int prec, const EvolveArgs& a, hipStream_t s);
+// prec: 0 = FP64, 1 = FP32 with compensated state update.  Number of
+// generated variants of (L, prec) (0: no register-resident stepper).
+int fast_variants(int L, int prec);
+hipError_t launch_evolve_fast(int L, int prec, int variant, const EvolveArgs& a, hipStream_t s);
+inline bool fast_available(int L, int prec) { return fast_variants(L, prec) > 0; }
 }  // namespace sfg

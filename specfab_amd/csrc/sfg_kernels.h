@@ -17,6 +17,49 @@
 
 namespace sfg {
 
+// Raw buffer access for the generated steppers: a wave-uniform resource
+// (base, byte size) in SGPRs, one shared VGPR lane offset and a scalar row
+// offset, so that SoA rows cost no per-access vector address arithmetic.
+// Reads beyond the resource's size return 0.
+#ifndef SFG_HOST_EMUL
+typedef __amdgpu_buffer_rsrc_t sfg_rsrc;
+__device__ __forceinline__ sfg_rsrc make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double bld64(sfg_rsrc r, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ void bst64(sfg_rsrc r, uint32_t vo, uint32_t so, double v) {
+    typedef int i32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, v), r, vo, so, 0);
+}
+__device__ __forceinline__ float bld32(sfg_rsrc r, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+__device__ __forceinline__ void bst32(sfg_rsrc r, uint32_t vo, uint32_t so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vo, so, 0);
+}
+#else
+struct sfg_rsrc { char* base; uint32_t bytes; };
+inline sfg_rsrc make_rsrc(const void* p, uint32_t bytes) { return sfg_rsrc{(char*)p, bytes}; }
+template <class T>
+inline T bld(sfg_rsrc r, uint32_t vo, uint32_t so) {
+    uint64_t o = (uint64_t)vo + so;
+    T v{};
+    if (o + sizeof(T) <= r.bytes) __builtin_memcpy(&v, r.base + o, sizeof(T));
+    return v;
+}
+template <class T>
+inline void bst(sfg_rsrc r, uint32_t vo, uint32_t so, T v) {
+    uint64_t o = (uint64_t)vo + so;
+    if (o + sizeof(T) <= r.bytes) __builtin_memcpy(r.base + o, &v, sizeof(T));
+}
+inline double bld64(sfg_rsrc r, uint32_t vo, uint32_t so) { return bld<double>(r, vo, so); }
+inline void bst64(sfg_rsrc r, uint32_t vo, uint32_t so, double v) { bst<double>(r, vo, so, v); }
+inline float bld32(sfg_rsrc r, uint32_t vo, uint32_t so) { return bld<float>(r, vo, so); }
+inline void bst32(sfg_rsrc r, uint32_t vo, uint32_t so, float v) { bst<float>(r, vo, so, v); }
+#endif
+
 constexpr int NPAR_K = 23;   // 8 LROT + 15 DDRX parameters
 
 struct ParamArgs {

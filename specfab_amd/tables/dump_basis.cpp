@@ -1,12 +1,11 @@
 // Build-time helper: writes the sparsity structure of the real-basis operator
 // of one truncation L (rows -> parameter groups -> (column, term index)) as
-// text, for specfab_amd/tables/gen_fast.py.  The term values are not needed
-// by the generator: the generated kernels read them at run time from the same
-// host-built term array (sfg::real_operator(L).terms) that this dumps.
+// text, for specfab_amd/tables/gen_fast.py, which compiles the term values
+// into the generated kernels as literal constants.
 //
 //   dump_basis L > basis_L.txt
 // Format:  "L n nrows ngroups nterms"  then per row "R l ngroups", per group
-// "G param nterms", per term "T col termindex".
+// "G param nterms", per term "T col termindex value" (value as a C hex float).
 #include <cstdio>
 #include <cstdlib>
 
@@ -22,7 +21,7 @@ int main(int argc, char** argv) {
         for (int g = r.g0; g < r.g1; g++) {
             const auto& G = op.groups[g];
             std::printf("G %d %d\n", G.param, G.t1 - G.t0);
-            for (int t = G.t0; t < G.t1; t++) std::printf("T %d %d\n", op.terms[t].col, t);
+            for (int t = G.t0; t < G.t1; t++) std::printf("T %d %d %a\n", op.terms[t].col, t, op.terms[t].val);
         }
     }
     return 0;

@@ -18,16 +18,16 @@ GEN = os.path.join(CSRC, "generated")
 _cache = {}
 
 
-def emul_lib(L, prec):
-    key = (L, prec)
+def emul_lib(L, prec, v=0):
+    key = (L, prec, v)
     if key in _cache:
         return _cache[key]
-    src = os.path.join(GEN, "sfg_fast_L%d_%s.hip" % (L, prec))
+    src = os.path.join(GEN, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, v))
     if not os.path.exists(src):
         pytest.skip("generated kernels not present (run the build)")
-    out = os.path.join(tempfile.gettempdir(), "sfg_emul_%d_%s_%d.so" % (L, prec, os.getpid()))
+    out = os.path.join(tempfile.gettempdir(), "sfg_emul_%d_%s_v%d_%d.so" % (L, prec, v, os.getpid()))
     cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DSFG_HOST_EMUL",
-           "-DKERNEL_NAME=evolve_fast_L%d_%s" % (L, prec), "-I", CSRC, "-x", "c++",
+           "-DKERNEL_NAME=evolve_fast_L%d_%s_v%d" % (L, prec, v), "-I", CSRC, "-x", "c++",
            os.path.join(ROOT, "tests", "emul_driver.cpp"), os.path.join(CSRC, "sfg_kernels.hip"),
            os.path.join(CSRC, "sfg_basis.cpp"), src, "-o", out]
     subprocess.check_call(cmd)
@@ -74,9 +74,18 @@ def to_nlm(x, L):
 PROCS = {"LROT": 1, "DDRX": 2, "CDRX": 4, "REG": 8}
 
 
-@pytest.mark.parametrize("L,prec", [(4, "f64"), (8, "f64"), (12, "f64"), (8, "f32")])
+def _variants(L, prec):
+    return sorted(int(f.split("_v")[1].split(".")[0]) for f in os.listdir(GEN)
+                  if f.startswith("sfg_fast_L%d_%s_v" % (L, prec))) if os.path.isdir(GEN) else [0]
+
+
+CASES = [(L, prec, v) for (L, prec) in [(4, "f64"), (8, "f64"), (12, "f64"), (8, "f32"), (12, "f32")]
+         for v in (_variants(L, prec) if L == 8 else [0])]
+
+
+@pytest.mark.parametrize("L,prec,v", CASES)
 @pytest.mark.parametrize("scheme", ["rk4", "euler"])
-def test_generated_kernel_vs_oracle(oracle, L, prec, scheme):
+def test_generated_kernel_vs_oracle(oracle, L, prec, v, scheme):
     from specfab_amd import workloads as W
     rng = np.random.default_rng(L * 10 + (scheme == "rk4"))
     P = 70   # one full and one partial wavefront
@@ -88,7 +97,7 @@ def test_generated_kernel_vs_oracle(oracle, L, prec, scheme):
     mask = sum(PROCS[p] for p in procs)
     steps, dt = 6, 0.5
     ref = oracle.evolve(L, nlm0, ug, g0, lam, steps, dt, scheme, procs)
-    lib = emul_lib(L, prec)
+    lib = emul_lib(L, prec, v)
     tol = 1e-12 if prec == "f64" else 2e-6
     for which in (0, 1) if prec == "f64" else (0,):
         x = to_state(nlm0, L)

@@ -148,3 +148,36 @@ def test_fast_vs_generic_stepper(gpu_ctx):
             outs.append(b.nlm().cpu().numpy())
         print("L=%d fast vs generic %.3e" % (L, relerr(outs[0], outs[1])))
         assert relerr(outs[0], outs[1]) < 1e-12, L
+
+
+@pytest.mark.parametrize("L", [8, 12])
+def test_every_stepper_variant(gpu_ctx, oracle, L):
+    """Every generated stepper variant (FP64 and FP32 operator) against the
+    oracle, and the autotuner's choice is one of them."""
+    import os
+    from specfab_amd import _lib
+    from specfab_amd import workloads as W
+    from specfab_amd.batch import FabricBatch
+    rng = np.random.default_rng(300 + L)
+    P = 200
+    nlm0 = W.perturbed_isotropic(rng, P, L, sigma=1e-2)
+    ug = W.traceless_gradients(rng, P)
+    g0 = 10.0 ** rng.uniform(-4, -2, P)
+    procs = ("LROT", "REG", "DDRX")
+    ref = oracle.evolve(L, nlm0, ug, g0, 0.0, 5, 0.5, "rk4", procs)
+    for prec, tol in (("fp64", 1e-12), ("fp32c", 2e-6)):
+        b0 = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0)
+        best, times = b0.autotune(prec, P=4096)
+        print("L=%d %s autotune: variant %d, ms %s" % (L, prec, best, ["%.3f" % t for t in times]))
+        assert 0 <= best < len(times)
+        for v in range(len(times)):
+            os.environ["SFG_VARIANT"] = str(v)
+            try:
+                ctx = _lib.Context(0)
+            finally:
+                del os.environ["SFG_VARIANT"]
+            b = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0, ctx=ctx)
+            b.evolve(5, 0.5, scheme="rk4", procs=procs, precision=prec)
+            err = relerr(b.nlm().cpu().numpy(), ref)
+            assert err < tol, (prec, v, err)
+            ctx.close()
