@@ -97,6 +97,93 @@ def cpu_baseline(target_s=10.0):
                       % (ncores, per_proc, steps, slowest, 1.0 / t1)}
 
 
+def bench_uneven(a):
+    """C4: 2^18 parcels, L ~ U{4, 8, 12, 20}, steps ~ U[50, 500], traceless
+    N(0, 1e-3) gradients, Gamma0 ~ 10^U(-4,-2), Lambda ~ 10^U(-4,-3),
+    LROT+REG+DDRX+CDRX, RK4 dt = 0.5, whole time loop as one graph launch."""
+    import numpy as np
+    import torch
+    from specfab_amd import batch as B
+    P = 1 << 18
+    rng = np.random.default_rng(4)
+    Ls = rng.choice([4, 8, 12, 20], P)
+    steps = rng.integers(50, 501, P)
+    G = rng.normal(0, 1e-3, (P, 3, 3))
+    G -= np.trace(G, axis1=1, axis2=2)[:, None, None] / 3 * np.eye(3)
+    g0 = 10.0 ** rng.uniform(-4, -2, P)
+    lam = 10.0 ** rng.uniform(-4, -3, P)
+    nlm = []
+    for L in Ls:
+        n = (L + 1) * (L + 2) // 2
+        x = np.zeros(n, complex)
+        x[0] = 1 / math.sqrt(4 * math.pi)
+        nlm.append(x)
+    ub = B.UnevenBatch(Ls, nlm, G, g0, lam, steps)
+    for L, idx, fb, rem in ub.buckets:
+        fb.autotune(a.precision)
+    ub.make_plan(0.5, "rk4", ("LROT", "REG", "DDRX", "CDRX"), precision=a.precision, chunk=50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ub.run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    work = int(steps.sum())
+    flops = sum(flops_per_parcel_step(int(L), 4, True) * int(s) for L, s in zip(Ls, steps)) \
+        if False else sum(flops_per_parcel_step(L) * int(steps[Ls == L].sum()) for L in (4, 8, 12, 20))
+    per_L = {int(L): int(steps[Ls == L].sum()) for L in (4, 8, 12, 20)}
+    out = {"metric": "parcel-steps/s, uneven batch (mixed L and step counts) in one graph launch",
+           "value": work / wall, "unit": "parcel-steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if a.precision == "fp64" else "f32+compensated", "data": "synthetic (seeded)",
+           "config": {"workload": "uneven batch: 2^18 parcels, L ~ U{4,8,12,20}, steps ~ U[50,500], "
+                                  "LROT+REG+DDRX+CDRX, RK4 dt=0.5, one graph launch", "parcels": P,
+                      "parcel_steps": work, "parcel_steps_by_L": per_L},
+           "achieved_tflops": flops / wall / 1e12,
+           "nonfinite_parcels": int((ub.status() != 0).sum())}
+    print(json.dumps(out), flush=True)
+
+
+def bench_surrogate(a):
+    """C5 in-loop homogenization on the 2^20-parcel L = 12 field after 200 RK4
+    steps: plain FP64 Eij (Cartesian) + Vi (z) vs the trained network with
+    envelope fallback and a 1 % audit."""
+    import torch
+    from specfab_amd import surrogate as S
+    P = a.parcels
+    X, T, fb = S.field_pairs(P, steps=200, seed=11)
+    del X, T
+    net = S.Surrogate().load()
+    hp = S.ice_params()
+    E0, E1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    S.plain_targets(fb, hp)
+    torch.cuda.synchronize()
+    E0.record()
+    for _ in range(3):
+        S.plain_targets(fb, hp)
+    E1.record()
+    torch.cuda.synchronize()
+    t_plain = E0.elapsed_time(E1) / 3
+    net.homogenize(fb, hp, audit_frac=0.01)
+    torch.cuda.synchronize()
+    E0.record()
+    for _ in range(3):
+        E, V, st = net.homogenize(fb, hp, audit_frac=0.01)
+    E1.record()
+    torch.cuda.synchronize()
+    t_sur = E0.elapsed_time(E1) / 3
+    _, Ep, Vp = S.plain_targets(fb, hp)
+    err = ((E - Ep).abs() / Ep.abs()).max(1).values
+    out = {"metric": "Eij + Vi evaluations/s on the L=12 field, plain FP64 vs learned (bf16 MFMA)",
+           "value": P / (t_sur * 1e-3), "unit": "parcel-evals/s", "n_gpus": 1, "higher_is_better": True,
+           "plain_evals_per_s": P / (t_plain * 1e-3), "speedup_vs_plain": t_plain / t_sur,
+           "ms_plain": t_plain, "ms_surrogate": t_sur, "dtype": "bf16 (fp32 accumulate)",
+           "audit": st, "Eij_rel_err_vs_plain": {"mean": err.mean().item(), "p99": err.quantile(0.99).item(),
+                                                   "max": err.max().item()},
+           "config": {"workload": "Eulerian CPO field, %d parcels, L=12, after 200 RK4 steps; ice linear "
+                                  "Sachs/Taylor Eij (Cartesian) + Hill Vi along z" % P, "parcels": P}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +193,9 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate"],
+                    help="field: the L=12 Eulerian field (default, the headline); uneven: 2^18 parcels with mixed "
+                         "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs learned")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -127,6 +217,10 @@ def main():
     from specfab_amd import batch as B
     from specfab_amd import _lib
 
+    if a.workload == "uneven":
+        return bench_uneven(a)
+    if a.workload == "surrogate":
+        return bench_surrogate(a)
     dev = torch.device("cuda:0")
     P, L = a.parcels, a.L
     G, gam0, x = make_field(P, L, 1, dev)

@@ -139,7 +139,8 @@ typedef struct sfg_evolve_desc {
     const double* tau_dev;    /* (P,3,3) or NULL */
     const double* gam0_dev;   /* (P) or NULL (0) */
     const double* lam_dev;    /* (P) or NULL (0) */
-    const int* steps_dev;     /* (P) or NULL */
+    int* steps_dev;           /* (P) or NULL; sfg_evolve only reads it, plans
+                                 decrement it (remaining steps) */
     double* x_dev;            /* state, real SoA (n, P), FP64 */
     int32_t* status_dev;      /* (P) or NULL */
 } sfg_evolve_desc;
@@ -210,6 +211,24 @@ int sfg_Vi_electromagnetic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, c
                                                double eps_t, double mu, int ndir, const double* theta,
                                                const double* phi, double* Vi_dev, void* stream);
 
+
+/* ---- uneven batches (no counterpart in the reference) ----
+ *
+ * A plan steps several buckets -- each one truncation L with its own state,
+ * kinematics and per-parcel step counts -- through a whole time loop with one
+ * graph launch.  Each bucket's desc.steps_dev holds the parcels' remaining
+ * step counts (int32, decremented as steps are taken; desc.nsteps is
+ * ignored); the time loop is `nchunks` chunks of `chunk` steps, captured once
+ * as a HIP graph in which a bucket's chunk c depends only on its chunk c - 1,
+ * so buckets run concurrently.  Per-parcel operator weights are computed at
+ * creation.  The stepper variant per (L, precision) is the context's
+ * autotuned one (sfg_autotune).  Sorting parcels by step count within a
+ * bucket keeps wavefronts busy (specfab_amd.batch.UnevenBatch does). */
+typedef struct sfg_plan sfg_plan;
+int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, int chunk, int nchunks,
+                    void* stream, sfg_plan** plan);
+int sfg_plan_run(sfg_plan* plan, void* stream);
+int sfg_plan_destroy(sfg_plan* plan);
 
 /* ---- learned homogenization (no counterpart in the reference) ----
  *

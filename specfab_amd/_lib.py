@@ -75,6 +75,9 @@ SIGNATURES = {
     "sfg_Vi_elastic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _dp, _d, _i, _dp, _dp, _vp, _vp, _vp]),
     "sfg_Vi_electromagnetic_tranisotropic": (_i, [_vp, _i, _dp, _d, _d, _d, _i, _dp, _dp, _dp]),
     "sfg_Vi_electromagnetic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _d, _d, _i, _dp, _dp, _vp, _vp]),
+    "sfg_plan_create": (_i, [_vp, _i, _c.POINTER(EvolveDesc), _i, _i, _vp, _c.POINTER(_vp)]),
+    "sfg_plan_run": (_i, [_vp, _vp]),
+    "sfg_plan_destroy": (_i, [_vp]),
     "sfg_mlp_nparams": (_i64, []),
     "sfg_mlp_create": (_i, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
     "sfg_mlp_destroy": (_i, [_vp]),

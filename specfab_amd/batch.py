@@ -178,3 +178,96 @@ class FabricBatch:
             self.ctx.handle, self.L, self.P, _ptr(self.x), float(eps_m), float(eps_t), float(mu), th.size,
             _lib.dptr(th), _lib.dptr(ph), _ptr(out), _stream()))
         return out
+
+
+class UnevenBatch:
+    """Parcels of different truncations L and step counts (SURVEY config C4),
+    stepped through their whole time loop by one graph launch.
+
+    Ls: (P,) truncations; nlm: sequence of P complex arrays (length
+    nlm_len(L_p)); ugrad (P, 3, 3); gam0, lam (P,) or scalars; steps (P,)
+    step counts.  Parcels are bucketed by L and sorted by step count inside a
+    bucket (wavefronts of similar length); results come back in the input
+    order."""
+
+    def __init__(self, Ls, nlm, ugrad, gam0=0.0, lam=0.0, steps=None, ctx=None, device="cuda:0"):
+        self.ctx = ctx or _lib.default_context()
+        Ls = np.asarray(Ls, dtype=np.int64)
+        P = Ls.size
+        ug = np.asarray(ugrad, dtype=np.float64).reshape(P, 3, 3)
+        g0 = np.broadcast_to(np.asarray(gam0, dtype=np.float64), (P,))
+        lm = np.broadcast_to(np.asarray(lam, dtype=np.float64), (P,))
+        st = np.asarray(steps, dtype=np.int64).reshape(P)
+        self.P = P
+        self.buckets = []   # (L, original indices, FabricBatch, remaining-steps tensor)
+        for L in sorted(set(Ls.tolist())):
+            idx = np.nonzero(Ls == L)[0]
+            idx = idx[np.argsort(st[idx], kind="stable")]
+            X = np.array([np.asarray(nlm[i], dtype=np.complex128) for i in idx])
+            fb = FabricBatch(L, nlm=X, ugrad=ug[idx], gam0=g0[idx], lam=lm[idx], ctx=self.ctx, device=device)
+            rem = torch.as_tensor(st[idx].astype(np.int32)).to(fb.dev).contiguous()
+            self.buckets.append((L, idx, fb, rem))
+        self.max_steps = int(st.max()) if P else 0
+        self.plan = None
+
+    def make_plan(self, dt, scheme="rk4", procs=("LROT", "REG", "DDRX", "CDRX"), iota=1.0, zeta=0.0,
+                  precision="fp64", chunk=50):
+        mask = 0
+        for p in procs:
+            mask |= _lib.PROC_BITS[p]
+        descs = (EvolveDesc * len(self.buckets))()
+        for d, (L, idx, fb, rem) in zip(descs, self.buckets):
+            d.L, d.P, d.procs = L, fb.P, mask
+            d.scheme = _lib.SFG_EULER if scheme == "euler" else _lib.SFG_RK4
+            d.precision = _lib.SFG_FP64 if precision == "fp64" else _lib.SFG_FP32C
+            d.nsteps, d.dt, d.iota, d.zeta = 0, float(dt), float(iota), float(zeta)
+            d.ugrad_dev = fb.ugrad.data_ptr()
+            d.tau_dev = None
+            d.gam0_dev = fb.gam0.data_ptr() if fb.gam0 is not None else None
+            d.lam_dev = fb.lam.data_ptr() if fb.lam is not None else None
+            d.steps_dev = rem.data_ptr()
+            d.x_dev = fb.x.data_ptr()
+            d.status_dev = fb.status.data_ptr()
+        nchunks = max(1, -(-self.max_steps // chunk))
+        h = _c.c_void_p()
+        check(lib().sfg_plan_create(self.ctx.handle, len(self.buckets), descs, int(chunk), nchunks, _stream(),
+                                    _c.byref(h)))
+        self.close()
+        self.plan = h
+        return self
+
+    def run(self):
+        """One graph launch: every parcel takes all its remaining steps."""
+        check(lib().sfg_plan_run(self.plan, _stream()))
+        return self
+
+    def close(self):
+        if self.plan:
+            lib().sfg_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nlm(self):
+        out = [None] * self.P
+        for (L, idx, fb, rem) in self.buckets:
+            X = fb.nlm().cpu().numpy()
+            for k, i in enumerate(idx):
+                out[i] = X[k]
+        return out
+
+    def status(self):
+        s = np.zeros(self.P, np.int32)
+        for (L, idx, fb, rem) in self.buckets:
+            s[idx] = fb.status.cpu().numpy()
+        return s
+
+    def remaining(self):
+        r = np.zeros(self.P, np.int64)
+        for (L, idx, fb, rem) in self.buckets:
+            r[idx] = rem.cpu().numpy()
+        return r

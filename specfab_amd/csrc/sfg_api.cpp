@@ -1211,4 +1211,143 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     return SFG_OK;
 }
 
+// ---------------------------------------------------------------- uneven batches: graph plans
+
+}  // extern "C"
+
+struct sfg_plan {
+    sfg_ctx* ctx = nullptr;
+    struct Bucket {
+        sfg::EvolveArgs ea{};
+        const void* fn = nullptr;
+        unsigned grid = 0, block = 64;
+        void* ws = nullptr;
+    };
+    std::vector<Bucket> b;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int nchunks = 0;
+};
+
+extern "C" {
+
+int sfg_plan_destroy(sfg_plan* plan) {
+    if (!plan) return SFG_OK;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    for (auto& bk : plan->b) if (bk.ws) (void)hipFree(bk.ws);
+    delete plan;
+    return SFG_OK;
+}
+
+int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, int chunk, int nchunks,
+                    void* stream, sfg_plan** out) {
+    if (!ctx || !out || nbuckets <= 0 || !buckets || chunk <= 0 || nchunks <= 0)
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = pick(ctx, stream);
+    auto* plan = new sfg_plan();
+    plan->ctx = ctx;
+    plan->nchunks = nchunks;
+    auto bail = [&](int rc) {
+        for (auto& bk : plan->b) if (bk.ws) (void)hipFree(bk.ws);
+        delete plan;
+        return rc;
+    };
+    for (int i = 0; i < nbuckets; i++) {
+        const sfg_evolve_desc* d = buckets + i;
+        if (d->P < 0 || (d->P > 0 && (!d->ugrad_dev || !d->x_dev || !d->steps_dev)))
+            return bail(fail(SFG_ERR_ARG, "bucket needs ugrad_dev, x_dev and steps_dev (remaining steps)"));
+        if (d->scheme != SFG_EULER && d->scheme != SFG_RK4) return bail(fail(SFG_ERR_ARG, "unknown scheme"));
+        if (d->precision != SFG_FP64 && d->precision != SFG_FP32C) return bail(fail(SFG_ERR_ARG, "unknown precision"));
+        const DevOp* op;
+        int rc = get_op(ctx, d->L, &op);
+        if (rc) return bail(rc);
+        if ((d->procs & SFG_REG) && !op->reg_defined) return bail(fail(SFG_ERR_L, "M_REG needs L = 4..20"));
+        if (d->P == 0) continue;
+        sfg_plan::Bucket bk;
+        const int64_t P = d->P;
+        const int n = op->n;
+        const size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
+        const size_t nscr = (size_t)3 * n * P;
+        hipError_t e = hipMalloc(&bk.ws, (nth + nom + nsl + nscr) * sizeof(double));
+        if (e != hipSuccess) return bail(fail(SFG_ERR_HIP, hipGetErrorString(e)));
+        plan->b.push_back(bk);
+        sfg_plan::Bucket& B = plan->b.back();
+        double* theta = (double*)B.ws;
+        double* omega = theta + nth;
+        double* sl = omega + nom;
+        double* scr = sl + nsl;
+        sfg::ParamArgs pa{};
+        pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
+        pa.iota = d->iota; pa.zeta = d->zeta; pa.reg_nu = op->reg_nu;
+        pa.ugrad = d->ugrad_dev; pa.tau = d->tau_dev; pa.gam0 = d->gam0_dev; pa.lam = d->lam_dev;
+        pa.A2 = op->A2; pa.A4 = op->A4; pa.reg_shape = op->reg_shape;
+        pa.theta = theta; pa.omega = omega; pa.sl = sl;
+        e = sfg::launch_params(pa, s);
+        if (e != hipSuccess) return bail(fail(SFG_ERR_HIP, hipGetErrorString(e)));
+        sfg::EvolveArgs& ea = B.ea;
+        ea.P = P; ea.n = n; ea.nq = op->nq; ea.scheme = d->scheme; ea.nsteps = chunk;
+        ea.use_ddrx = (d->procs & SFG_DDRX) ? 1 : 0;
+        ea.steps = d->steps_dev; ea.steps_dec = 1; ea.dt = d->dt;
+        ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
+        ea.tcol = op->tcol; ea.tval = op->tval; ea.tval_f = op->tval_f;
+        ea.theta = theta; ea.omega = omega; ea.sl = sl;
+        ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
+        ea.status = d->status_dev;
+        ea.zero = 0;
+        const int prec = d->precision == SFG_FP64 ? 0 : 1;
+        const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
+        if (fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0)) {
+            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec));
+            B.block = 64;
+        } else if (prec == 0) {
+            B.fn = sfg::generic_kernel();
+            B.block = sfg::GENERIC_BLOCK;
+        } else {
+            return bail(fail(SFG_ERR_ARG, "FP32C stepping needs L <= 16 and n * P * 8 < 2^31"));
+        }
+        B.grid = (unsigned)((P + B.block - 1) / B.block);
+    }
+    // the time loop as one graph: chunk c of every bucket depends on chunk
+    // c - 1 of the same bucket only, so the buckets run concurrently
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGraphCreate(&plan->graph, 0);
+    std::vector<hipGraphNode_t> prev(plan->b.size(), nullptr);
+    for (int c = 0; c < nchunks && e == hipSuccess; c++) {
+        for (size_t i = 0; i < plan->b.size() && e == hipSuccess; i++) {
+            sfg_plan::Bucket& B = plan->b[i];
+            void* args[] = {&B.ea};
+            hipKernelNodeParams kp{};
+            kp.func = const_cast<void*>(B.fn);
+            kp.gridDim = dim3(B.grid);
+            kp.blockDim = dim3(B.block);
+            kp.sharedMemBytes = 0;
+            kp.kernelParams = args;
+            kp.extra = nullptr;
+            hipGraphNode_t node;
+            e = hipGraphAddKernelNode(&node, plan->graph, prev[i] ? &prev[i] : nullptr, prev[i] ? 1 : 0, &kp);
+            prev[i] = node;
+        }
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (plan->graph) (void)hipGraphDestroy(plan->graph);
+        return bail(fail(SFG_ERR_HIP, std::string("sfg_plan_create: ") + hipGetErrorString(e)));
+    }
+    *out = plan;
+    return SFG_OK;
+}
+
+int sfg_plan_run(sfg_plan* plan, void* stream) {
+    if (!plan) return fail(SFG_ERR_ARG, "plan is NULL");
+    HIPCHK(hipSetDevice(plan->ctx->device));
+    HIPCHK(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    if (plan->ctx->sync_check) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return SFG_OK;
+}
+
 }  // extern "C"

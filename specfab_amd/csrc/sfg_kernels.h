@@ -83,7 +83,11 @@ struct ParamArgs {
 struct EvolveArgs {
     int64_t P;
     int n, nq, scheme, nsteps, use_ddrx;
-    const int* steps;
+    // per-parcel step counts (or null: nsteps for all).  steps_dec = 1: steps
+    // holds the remaining count; a launch takes min(remaining, nsteps) steps
+    // and stores the new remaining count.
+    int* steps;
+    int steps_dec;
     double dt;
     const int2* rows;      // (n) group ranges
     const int* row_blk;    // (n) l/2

@@ -9,6 +9,9 @@
 // The register-resident fast steppers (generated per L) live in
 // sfg_step_fast.hip.
 #include "sfg_kernels.h"
+#ifndef SFG_HOST_EMUL
+#include "sfg_fast.h"
+#endif
 #include "sfg_device.h"
 
 namespace sfg {
@@ -127,7 +130,8 @@ __global__ void evolve_generic(EvolveArgs a) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.P) return;
     const int64_t P = a.P;
-    const int ns = a.steps ? a.steps[p] : a.nsteps;
+    const int rem = a.steps ? a.steps[p] : a.nsteps;
+    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;
     const double h = a.dt;
     double* X = a.X;
     double* YS = a.YS;
@@ -162,6 +166,7 @@ __global__ void evolve_generic(EvolveArgs a) {
             });
         }
     }
+    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;
     if (a.status) {
         int st = 0;
         for (int r = 0; r < a.n; r++) {
@@ -343,6 +348,8 @@ hipError_t launch_params(const ParamArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(params_kernel, dim3(nblocks(a.P, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
+
+const void* generic_kernel() { return (const void*)&evolve_generic; }
 
 hipError_t launch_evolve_generic(const EvolveArgs& a, hipStream_t s) {
     if (a.P == 0) return hipSuccess;

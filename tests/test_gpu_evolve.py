@@ -181,3 +181,38 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
             err = relerr(b.nlm().cpu().numpy(), ref)
             assert err < tol, (prec, v, err)
             ctx.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32c", 2e-6)])
+def test_uneven_plan_vs_reference(gpu_ctx, precision, tol):
+    """Uneven batch (L in {4, 8, 12, 20} mixed, 50..500 steps per parcel,
+    LROT+REG+DDRX+CDRX, RK4) through one graph launch, in a shuffled parcel
+    order, against the reference trajectories.  L = 20 has no FP32 stepper:
+    the FP32C plan covers L <= 16."""
+    from specfab_amd.batch import UnevenBatch
+    g = load_golden("uneven.npz")
+    Ls_all = (4, 8, 12, 20) if precision == "fp64" else (4, 8, 12)
+    Ls, nlm0, ug, g0, lam, steps, ref = [], [], [], [], [], [], []
+    for L in Ls_all:
+        for k in range(g["nlm0_L%d" % L].shape[0]):
+            Ls.append(L)
+            nlm0.append(g["nlm0_L%d" % L][k])
+            ug.append(g["ugrad_L%d" % L][k])
+            g0.append(g["gam0_L%d" % L][k])
+            lam.append(g["lam_L%d" % L][k])
+            steps.append(g["steps_L%d" % L][k])
+            ref.append(g["nlm_L%d" % L][k])
+    perm = np.random.default_rng(0).permutation(len(Ls))
+    sel = lambda a: [a[i] for i in perm]
+    ub = UnevenBatch(sel(Ls), sel(nlm0), np.array(sel(ug)), np.array(sel(g0)), np.array(sel(lam)),
+                     np.array(sel(steps)))
+    ub.make_plan(0.5, "rk4", ("LROT", "REG", "DDRX", "CDRX"), precision=precision, chunk=64).run()
+    out = ub.nlm()
+    for k, i in enumerate(perm):
+        assert relerr(out[k], ref[i]) < tol, (Ls[i], precision, relerr(out[k], ref[i]))
+    assert (ub.remaining() == 0).all()
+    assert not ub.status().any()
+    # a second launch does nothing (no steps remain)
+    ub.run()
+    out2 = ub.nlm()
+    assert all(np.array_equal(a, b) for a, b in zip(out, out2))
