@@ -128,8 +128,7 @@ def bench_uneven(a):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     work = int(steps.sum())
-    flops = sum(flops_per_parcel_step(int(L), 4, True) * int(s) for L, s in zip(Ls, steps)) \
-        if False else sum(flops_per_parcel_step(L) * int(steps[Ls == L].sum()) for L in (4, 8, 12, 20))
+    flops = sum(flops_per_parcel_step(L) * int(steps[Ls == L].sum()) for L in (4, 8, 12, 20))
     per_L = {int(L): int(steps[Ls == L].sum()) for L in (4, 8, 12, 20)}
     out = {"metric": "parcel-steps/s, uneven batch (mixed L and step counts) in one graph launch",
            "value": work / wall, "unit": "parcel-steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,

@@ -1302,8 +1302,9 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         const int prec = d->precision == SFG_FP64 ? 0 : 1;
         const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
         if (fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0)) {
-            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec));
-            B.block = 64;
+            int blk = 64;
+            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec), &blk);
+            B.block = (unsigned)blk;
         } else if (prec == 0) {
             B.fn = sfg::generic_kernel();
             B.block = sfg::GENERIC_BLOCK;

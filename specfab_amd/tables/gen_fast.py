@@ -49,12 +49,16 @@ FAST_L = {"f64": (4, 6, 8, 10, 12), "f32": (4, 6, 8, 10, 12, 14, 16)}
 #   waves   waves per SIMD requested from the register allocator
 #   bfree   branch-free RK update (stage flags folded into buffer offsets)
 #   fence   scheduling fence after every (row, parameter) group
-V = lambda mode, chunk, waves, bfree, fence: dict(mode=mode, chunk=chunk, waves=waves, bfree=bfree, fence=fence)
+#   block   threads per workgroup (64 = one wave; 256: four waves dispatched
+#           together, which then tend to share instruction-cache lines)
+V = lambda mode, chunk, waves, bfree, fence, block=64: dict(mode=mode, chunk=chunk, waves=waves, bfree=bfree,
+                                                            fence=fence, block=block)
 VARIANTS = {
-    ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False), V("lit", 4, 2, False, True),
-                       V("lit", 16, 1, False, False)],
+    ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False), V("lit", 8, 2, False, False, 256),
+                       V("lit", 4, 2, False, True, 256)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
-    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), V("lit", 8, 2, False, False)],
+    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), V("lit", 8, 2, True, False, 256),
+                       V("lit", 8, 3, True, False, 256)],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
 }
 TUNED_L = (8, 12)
@@ -118,9 +122,10 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
        ", ".join("%s=%s" % kv for kv in sorted(cfg.items()))))
     w('#include "../sfg_kernels.h"')
     w("namespace sfg {")
-    w("__global__ void __launch_bounds__(64, %d) %s(EvolveArgs a) {" % (cfg["waves"], name))
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (cfg["block"], cfg["waves"], name))
     w("    const int64_t P = a.P;")
-    w("    const int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x;")
+    B = cfg["block"]
+    w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
     w("    if (p >= P) return;")
     w("    // SoA rows through buffer resources: row r of an array with element")
     w("    // size w is at byte (r * P * w) [scalar] + p * w [one lane offset]")
@@ -137,7 +142,7 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("    const double h = a.dt;")
     w("    // per-parcel operator parameters in LDS ([param][lane]), read once per")
     w("    // (row chunk, parameter)")
-    w("    __shared__ %s thl[%d][64];" % (T, len(used_p)))
+    w("    __shared__ %s thl[%d][%d];" % (T, len(used_p), B))
     pidx = {q: i for i, q in enumerate(used_p)}
     for q in used_p:
         w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
@@ -228,7 +233,7 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("}")
     w("#ifndef SFG_HOST_EMUL")
     w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
-    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + 63) / 64)), dim3(64), 0, s, a);" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + %d) / %d)), dim3(%d), 0, s, a);" % (name, B - 1, B, B))
     w("    return hipGetLastError();")
     w("}")
     w("#endif")
@@ -237,33 +242,34 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
 
 
 def gen_dispatch(kinds):
-    """kinds: list of (L, prec, nvariants)."""
+    """kinds: list of (L, prec, variant configs)."""
     o = ["// Generated by specfab_amd/tables/gen_fast.py -- do not edit.",
          '#include "../sfg_fast.h"', "namespace sfg {"]
-    for (L, prec, nv) in kinds:
-        for v in range(nv):
+    for (L, prec, cfgs) in kinds:
+        for v in range(len(cfgs)):
             o.append("hipError_t launch_evolve_fast_L%d_%s_v%d(const EvolveArgs& a, hipStream_t s);" % (L, prec, v))
     o.append("int fast_variants(int L, int prec) {")
-    for (L, prec, nv) in kinds:
-        o.append("    if (L == %d && prec == %d) return %d;" % (L, 0 if prec == "f64" else 1, nv))
+    for (L, prec, cfgs) in kinds:
+        o.append("    if (L == %d && prec == %d) return %d;" % (L, 0 if prec == "f64" else 1, len(cfgs)))
     o.append("    return 0;")
     o.append("}")
     o.append("#ifndef SFG_HOST_EMUL")
-    for (L, prec, nv) in kinds:
-        for v in range(nv):
+    for (L, prec, cfgs) in kinds:
+        for v in range(len(cfgs)):
             o.append("__global__ void evolve_fast_L%d_%s_v%d(EvolveArgs a);" % (L, prec, v))
-    o.append("const void* fast_kernel(int L, int prec, int variant) {")
-    for (L, prec, nv) in kinds:
-        for v in range(nv):
-            o.append("    if (L == %d && prec == %d && variant == %d) return (const void*)&evolve_fast_L%d_%s_v%d;"
-                     % (L, 0 if prec == "f64" else 1, v, L, prec, v))
+    o.append("const void* fast_kernel(int L, int prec, int variant, int* block) {")
+    for (L, prec, cfgs) in kinds:
+        for v, cfg in enumerate(cfgs):
+            o.append("    if (L == %d && prec == %d && variant == %d) { *block = %d; "
+                     "return (const void*)&evolve_fast_L%d_%s_v%d; }"
+                     % (L, 0 if prec == "f64" else 1, v, cfg["block"], L, prec, v))
     o.append("    return nullptr;")
     o.append("}")
     o.append("#endif")
     o.append("hipError_t launch_evolve_fast(int L, int prec, int variant, const EvolveArgs& a, hipStream_t s) {")
     o.append("    if (a.P == 0) return hipSuccess;")
-    for (L, prec, nv) in kinds:
-        for v in range(nv):
+    for (L, prec, cfgs) in kinds:
+        for v in range(len(cfgs)):
             o.append("    if (L == %d && prec == %d && variant == %d) return launch_evolve_fast_L%d_%s_v%d(a, s);"
                      % (L, 0 if prec == "f64" else 1, v, L, prec, v))
     o.append("    return hipErrorInvalidValue;")
@@ -295,14 +301,14 @@ def main():
             for vid, cfg in enumerate(vs):
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
                                  gen_kernel(L, prec, n, rows, cfg, vid))
-            kinds.append((L, prec, len(vs)))
+            kinds.append((L, prec, vs))
     # drop generated kernels of an older variant list
-    keep = {"sfg_fast_L%d_%s_v%d.hip" % (L, prec, v) for (L, prec, nv) in kinds for v in range(nv)}
+    keep = {"sfg_fast_L%d_%s_v%d.hip" % (L, prec, v) for (L, prec, cfgs) in kinds for v in range(len(cfgs))}
     for f in os.listdir(outdir):
         if f.startswith("sfg_fast_L") and f.endswith(".hip") and f not in keep:
             os.remove(os.path.join(outdir, f))
     write_if_changed(os.path.join(outdir, "sfg_fast_dispatch.hip"), gen_dispatch(kinds))
-    print("generated %d fast kernels" % sum(nv for (_, _, nv) in kinds))
+    print("generated %d fast kernels" % sum(len(c) for (_, _, c) in kinds))
 
 
 if __name__ == "__main__":

@@ -60,7 +60,7 @@ extern "C" int emul_evolve(int L, int64_t P, int procs, int scheme, int nsteps, 
     ea.theta = theta.data(); ea.omega = omega.data(); ea.sl = sl.data();
     ea.X = x; ea.YS = scr.data(); ea.XA = scr.data() + (size_t)n * P; ea.XB = scr.data() + (size_t)2 * n * P;
     ea.status = status; ea.zero = 0;
-    if (which == 0) grid(P, 64, [&] { sfg::KERNEL_NAME(ea); });
+    if (which == 0) grid(P, KERNEL_BLOCK, [&] { sfg::KERNEL_NAME(ea); });
     else grid(P, 64, [&] { sfg::evolve_generic(ea); });
     return 0;
 }
