@@ -17,6 +17,7 @@ typedef void* hipStream_t;
 #define __device__
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(m) ((void)0)
+#define __syncthreads() ((void)0)
 #define __shared__ static
 #define __forceinline__ inline
 #define __constant__

@@ -50,15 +50,20 @@ FAST_L = {"f64": (4, 6, 8, 10, 12), "f32": (4, 6, 8, 10, 12, 14, 16)}
 #   bfree   branch-free RK update (stage flags folded into buffer offsets)
 #   fence   scheduling fence after every (row, parameter) group
 #   block   threads per workgroup (64 = one wave; 256: four waves dispatched
-#           together, which then tend to share instruction-cache lines)
-V = lambda mode, chunk, waves, bfree, fence, block=64: dict(mode=mode, chunk=chunk, waves=waves, bfree=bfree,
-                                                            fence=fence, block=block)
+#           together)
+#   sync    workgroup barrier at every row chunk (uniform step counts only):
+#           the waves of a workgroup then walk the ~100 KB stage code in
+#           lockstep and share its instruction-cache lines
+V = lambda mode, chunk, waves, bfree, fence, block=64, sync=False: dict(
+    mode=mode, chunk=chunk, waves=waves, bfree=bfree, fence=fence, block=block, sync=sync)
 VARIANTS = {
-    ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False), V("lit", 8, 2, False, False, 256),
-                       V("lit", 4, 2, False, True, 256)],
+    ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
+                       V("lit", 8, 2, False, False, 256, True), V("lit", 8, 1, False, False, 256, True),
+                       V("lit", 8, 2, False, False, 512, True)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
-    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), V("lit", 8, 2, True, False, 256),
-                       V("lit", 8, 3, True, False, 256)],
+    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), V("lit", 8, 2, True, False, 256, True),
+                       V("lit", 8, 3, True, False, 256, True), V("lit", 8, 2, True, False, 512, True),
+                       V("lit", 8, 3, True, False, 768, True)],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
 }
 TUNED_L = (8, 12)
@@ -126,18 +131,32 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("    const int64_t P = a.P;")
     B = cfg["block"]
     w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
-    w("    if (p >= P) return;")
+    SYNC = cfg["sync"]
+    if SYNC:
+        w("    // lockstep mode: with uniform step counts every lane of the workgroup")
+        w("    // (also those past P) runs the whole loop nest and meets every barrier;")
+        w("    // lanes past P address the buffers out of range (loads 0, stores dropped)")
+        w("    const bool bsync = (a.steps == nullptr);")
+        w("    const bool live = p < P;")
+        w("    if (!bsync && !live) return;")
+        w("    const int64_t pc = live ? p : P - 1;")
+    else:
+        w("    if (p >= P) return;")
+        w("    const int64_t pc = p;")
     w("    // SoA rows through buffer resources: row r of an array with element")
     w("    // size w is at byte (r * P * w) [scalar] + p * w [one lane offset]")
     w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
-    w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
+    if SYNC:
+        w("    const uint32_t v8 = live ? (uint32_t)p * 8u : 0x80000000u, vT = live ? (uint32_t)p * %du : 0x80000000u;" % W)
+    else:
+        w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
     w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
     w("    const sfg_rsrc RYS = make_rsrc(a.YS, PT * %du);" % n)
     w("    const sfg_rsrc RXN = make_rsrc(a.XA, PT * %du);" % n)
     w("    const sfg_rsrc ROM = make_rsrc(a.omega, P8 * 15u);")
     w("    const sfg_rsrc RSL = make_rsrc(a.sl, P8 * %du);" % nblk)
     w("    const int nstage = a.scheme ? 4 : 1;")
-    w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
+    w("    const int rem = a.steps ? a.steps[pc] : a.nsteps;")
     w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
     w("    const double h = a.dt;")
     w("    // per-parcel operator parameters in LDS ([param][lane]), read once per")
@@ -145,7 +164,7 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("    __shared__ %s thl[%d][%d];" % (T, len(used_p), B))
     pidx = {q: i for i, q in enumerate(used_p)}
     for q in used_p:
-        w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
+        w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + pc]" % q)))
     w("    %s %s;" % (T, ", ".join("x%d" % b for b in range(n))))
     w("    for (int step = 0; step < ns; step++) {")
     for b in range(n):
@@ -158,9 +177,11 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("            const %s cy = %s;" % (T, cv("euler ? h : ((s == 0 || s == 3) ? h / 6 : h / 3)")))
     w("            const %s cx = %s;" % (T, cv("(s == 2) ? h : h / 2")))
     if BRANCHFREE:
-        w("            // out-of-range offsets (buffer reads return 0, writes are dropped)")
-        w("            const uint32_t yoff = (first || euler) ? 0x80000000u : 0u;")
-        w("            const uint32_t woff = (last || euler) ? 0x80000000u : 0u;")
+        w("            // stage flags as lane offsets: 0x80000000 is past every buffer (whose size")
+        w("            // is < 2^31), so the first stage reads YS as 0 and the last one drops its")
+        w("            // YS / XN stores; the row offset stays in the scalar offset (< 2^31)")
+        w("            const uint32_t vY = (first || euler) ? 0x80000000u : vT;")
+        w("            const uint32_t vW = (last || euler) ? 0x80000000u : vT;")
     w("            %s gD = (%s)0;" % (T, T))
     w("            if (a.use_ddrx) {")
     terms = " + ".join("%s * x%d" % (cv("bld64(ROM, v8, %du * P8)" % q), q) for q in range(nq))
@@ -169,6 +190,8 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     for c0 in range(0, n, CHUNK):
         chunk = list(range(c0, min(n, c0 + CHUNK)))
         w("            {   // rows %d..%d" % (chunk[0], chunk[-1]))
+        if SYNC:
+            w("                if (bsync) __syncthreads();")
         w("                %s %s;" % (T, ", ".join("acc%d = (%s)0" % (r, T) for r in chunk)))
         byp = {}
         for r in chunk:
@@ -200,10 +223,10 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
                 # uniform stage flags folded into the offsets: the first stage
                 # reads YS beyond the buffer (returns 0), the last stage writes
                 # YS / XN beyond it (dropped), other stages write X back unchanged
-                w("                    const %s ys = fma(cy, f, %s(RYS, vT, %du * PT + yoff));" % (T, LD, r))
+                w("                    const %s ys = fma(cy, f, %s(RYS, vY, %du * PT));" % (T, LD, r))
                 w("                    bst64(RX, v8, %du * P8, (last || euler) ? x0a + (double)ys : x0a);" % r)
-                w("                    %s(RYS, vT, %du * PT + woff, ys);" % (ST, r))
-                w("                    %s(RXN, vT, %du * PT + woff, fma(cx, f, (%s)x0a));" % (ST, r, T))
+                w("                    %s(RYS, vW, %du * PT, ys);" % (ST, r))
+                w("                    %s(RXN, vW, %du * PT, fma(cx, f, (%s)x0a));" % (ST, r, T))
             else:
                 w("                    if (euler) {")
                 w("                        bst64(RX, v8, %du * P8, x0a + (double)(cy * f));" % r)
@@ -224,6 +247,8 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     w("            }")
     w("        }")
     w("    }")
+    if SYNC:
+        w("    if (!live) return;")
     w("    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;")
     w("    if (a.status) {")
     w("        int st = 0;")
