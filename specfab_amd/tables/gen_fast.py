@@ -54,16 +54,20 @@ FAST_L = {"f64": (4, 6, 8, 10, 12), "f32": (4, 6, 8, 10, 12, 14, 16)}
 #   sync    workgroup barrier at every row chunk (uniform step counts only):
 #           the waves of a workgroup then walk the ~100 KB stage code in
 #           lockstep and share its instruction-cache lines
+#   xn      ("reg" mode) where the next-stage input lives: "reg" (registers,
+#           with an FP32 copy of the step-start state) or "mem" (XN buffer)
 V = lambda mode, chunk, waves, bfree, fence, block=64, sync=False: dict(
     mode=mode, chunk=chunk, waves=waves, bfree=bfree, fence=fence, block=block, sync=sync)
+#   split   ("reg" mode) the last stage as a separate copy of the stage code
+#           (else one stage loop that tests the stage kind per row chunk)
+R = lambda chunk, waves, xn, split=True, block=64: dict(mode="reg", chunk=chunk, waves=waves, xn=xn,
+                                                      split=split, block=block)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
-                       V("lit", 8, 2, False, False, 256, True), V("lit", 8, 1, False, False, 256, True),
-                       V("lit", 8, 2, False, False, 512, True)],
+                       V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
-    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), V("lit", 8, 2, True, False, 256, True),
-                       V("lit", 8, 3, True, False, 256, True), V("lit", 8, 2, True, False, 512, True),
-                       V("lit", 8, 3, True, False, 768, True)],
+    ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
+                       R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg")],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
 }
 TUNED_L = (8, 12)
@@ -266,6 +270,201 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     return "\n".join(o) + "\n"
 
 
+def gen_kernel_reg(L, prec, n, rows, cfg, vid):
+    """Register-state stepper ("reg" mode).
+
+    Differences from gen_kernel: the RK4 increment sum y stays in registers
+    for the whole step; with xn="reg" the next-stage input and the FP32 copy
+    of the step-start state x0f are registers too, so a step touches HBM only
+    to read and write the FP64 state once (last stage); with xn="mem" the
+    next-stage input goes through XN and x0 is read per row chunk.  theta,
+    the 15 DDRX moment weights omega and the per-l diagonal s_l are staged
+    once in LDS.  Loads of a row chunk's x0 are issued at the top of the
+    chunk so that their latency overlaps the chunk's FMA block.  The last RK
+    stage (or the single Euler stage) is a separate copy of the stage code,
+    so no per-row branch on the stage kind remains.
+    """
+    T = "double" if prec == "f64" else "float"
+    LD = "bld64" if prec == "f64" else "bld32"
+    ST = "bst64" if prec == "f64" else "bst32"
+    W = 8 if prec == "f64" else 4
+    cv = (lambda e: e) if prec == "f64" else (lambda e: "(float)(%s)" % e)
+    name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
+    CHUNK, B = cfg["chunk"], cfg["block"]
+    XREG = cfg["xn"] == "reg"
+    used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
+    NPU = len(used_p)
+    nq = min(n, 15)
+    nblk = L // 2 + 1
+    OMC, SLC = NPU, NPU + nq          # LDS columns of omega and s_l
+    pidx = {q: i for i, q in enumerate(used_p)}
+    o = []
+    w = o.append
+    w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
+    w("// Register-state stepper, L = %d (n = %d), %s; variant %d: %s." %
+      (L, n, "FP64" if prec == "f64" else "FP32 stepping, FP64 state", vid,
+       ", ".join("%s=%s" % kv for kv in sorted(cfg.items()))))
+    w('#include "../sfg_kernels.h"')
+    w("namespace sfg {")
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, cfg["waves"], name))
+    w("    const int64_t P = a.P;")
+    w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
+    w("    if (p >= P) return;")
+    w("    // SoA rows through buffer resources: row r of an array with element")
+    w("    // size w is at byte (r * P * w) [scalar] + p * w [one lane offset]")
+    w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
+    w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
+    w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
+    if not XREG:
+        w("    const sfg_rsrc RXN = make_rsrc(a.XA, PT * %du);" % n)
+    w("    const int nstage = a.scheme ? 4 : 1;")
+    w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
+    w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
+    w("    const double h = a.dt;")
+    w("    // per-parcel constants of the launch in LDS ([column][lane]): theta_p,")
+    w("    // then the DDRX moment weights omega_q, then the diagonal s_l")
+    w("    __shared__ %s thl[%d][%d];" % (T, NPU + nq + nblk, B))
+    for q in used_p:
+        w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
+    for q in range(nq):
+        w("    thl[%d][threadIdx.x] = %s;" % (OMC + q, cv("a.omega[(int64_t)%d * P + p]" % q)))
+    for b in range(nblk):
+        w("    thl[%d][threadIdx.x] = %s;" % (SLC + b, cv("a.sl[(int64_t)%d * P + p]" % b)))
+    w("    %s %s;" % (T, ", ".join("x%d" % b for b in range(n))))
+    w("    %s %s;" % (T, ", ".join("y%d = (%s)0" % (b, T) for b in range(n))))
+    if XREG:
+        w("    %s %s;" % (T, ", ".join("xn%d" % b for b in range(n))))
+        w("    %s %s;" % (T, ", ".join("xf%d" % b for b in range(n))))
+        for b in range(n):
+            w("    x%d = %s; xf%d = x%d;" % (b, cv("bld64(RX, v8, %du * P8)" % b), b, b))
+    zc = [0]
+
+    def zk():
+        zc[0] += 1
+        return zc[0]
+
+    def stage(last):
+        # one stage's code; `last`: the final RK4 stage or the Euler stage;
+        # None: any stage, the kind tested at run time per row chunk
+        if last is None:
+            w("            const bool first = (s == 0), last = (s == nstage - 1);")
+            w("            const %s cy = %s;" % (T, cv("(nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3)")))
+            w("            const %s cx = %s;" % (T, cv("(s == 2) ? h : h / 2")))
+        elif last:
+            w("            const bool first = (nstage == 1);")
+            w("            const %s cy = %s;" % (T, cv("first ? h : h / 6")))
+        else:
+            w("            const bool first = (s == 0);")
+            w("            const %s cy = %s;" % (T, cv("(s == 0) ? h / 6 : h / 3")))
+            w("            const %s cx = %s;" % (T, cv("(s == 2) ? h : h / 2")))
+        w("            // a.zero is 0 at run time but unknown to the compiler: it keeps the")
+        w("            // LDS reads of different row chunks and stages apart")
+        w("            const int z = (int)a.zero * (step * 4 + %s);" % ("3" if last else "s"))
+        dyn = last is None
+        w("            %s gD = (%s)0;" % (T, T))
+        w("            if (a.use_ddrx) {")
+        k = zk()
+        terms = " + ".join("thl[%d][threadIdx.x + z * %d] * x%d" % (OMC + q, k, q) for q in range(nq))
+        w("                gD = (%s) / x0;" % terms)
+        w("            }")
+        for c0 in range(0, n, CHUNK):
+            chunk = list(range(c0, min(n, c0 + CHUNK)))
+            k = zk()
+            w("            {   // rows %d..%d" % (chunk[0], chunk[-1]))
+            if last or dyn or not XREG:
+                w("                const double %s;" % ", ".join("xo%d = bld64(RX, v8, %du * P8)" % (r, r) for r in chunk))
+            w("                %s %s;" % (T, ", ".join("acc%d = (%s)0" % (r, T) for r in chunk)))
+            byp = {}
+            for r in chunk:
+                for (pp, tl) in rows[r][1]:
+                    byp.setdefault(pp, []).append((r, tl))
+            for pp in sorted(byp):
+                w("                {")
+                w("                    const %s th = thl[%d][threadIdx.x + z * %d];" % (T, pidx[pp], k))
+                for (r, tl) in byp[pp]:
+                    col, ti, v = tl[0]
+                    w("                    %s u%d = %s * x%d;" % (T, r, lit(v, prec), col))
+                    for (col, ti, v) in tl[1:]:
+                        w("                    u%d = fma(%s, x%d, u%d);" % (r, lit(v, prec), col, r))
+                    w("                    acc%d = fma(th, u%d, acc%d);" % (r, r, r))
+                w("                }")
+            def epilogue(last, r):
+                l = rows[r][0]
+                w("                {")
+                w("                    const %s f = fma(thl[%d][threadIdx.x + z * %d] - gD, x%d, acc%d);"
+                  % (T, SLC + l // 2, k, r, r))
+                w("                    const %s yv = fma(cy, f, first ? (%s)0 : y%d);" % (T, T, r))
+                if last:
+                    w("                    const double xs = xo%d + (double)yv;" % r)
+                    w("                    bst64(RX, v8, %du * P8, xs);" % r)
+                    if XREG:
+                        w("                    xf%d = %s;" % (r, cv("xs")))
+                else:
+                    w("                    y%d = yv;" % r)
+                    if XREG:
+                        w("                    xn%d = fma(cx, f, xf%d);" % (r, r))
+                    else:
+                        w("                    %s(RXN, vT, %du * PT, fma(cx, f, (%s)xo%d));" % (ST, r, T, r))
+                w("                }")
+            if dyn:
+                w("                if (last) {")
+                for r in chunk:
+                    epilogue(True, r)
+                w("                } else {")
+                for r in chunk:
+                    epilogue(False, r)
+                w("                }")
+            else:
+                for r in chunk:
+                    epilogue(last, r)
+            w("            }")
+        # next stage input
+        if dyn:
+            assert not XREG
+            w("            if (!last) {")
+            for b in range(n):
+                w("                x%d = %s(RXN, vT, %du * PT);" % (b, LD, b))
+            w("            }")
+        elif XREG:
+            src = "xf" if last else "xn"
+            w("            %s" % " ".join("x%d = %s%d;" % (b, src, b) for b in range(n)))
+        elif not last:
+            for b in range(n):
+                w("            x%d = %s(RXN, vT, %du * PT);" % (b, LD, b))
+
+    w("    for (int step = 0; step < ns; step++) {")
+    if not XREG:
+        for b in range(n):
+            w("        x%d = %s;" % (b, cv("bld64(RX, v8, %du * P8)" % b)))
+    if cfg.get("split", True):
+        w("        for (int s = 0; s < nstage - 1; s++) {")
+        stage(False)
+        w("        }")
+        w("        {")
+        stage(True)
+        w("        }")
+    else:
+        w("        for (int s = 0; s < nstage; s++) {")
+        stage(None)
+        w("        }")
+    w("    }")
+    w("    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;")
+    w("    if (a.status) {")
+    w("        int st = 0;")
+    w("        for (int r = 0; r < %d; r++) st |= isfinite(bld64(RX, v8, (uint32_t)r * P8)) ? 0 : 1;" % n)
+    w("        a.status[p] |= st;")
+    w("    }")
+    w("}")
+    w("#ifndef SFG_HOST_EMUL")
+    w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + %d) / %d)), dim3(%d), 0, s, a);" % (name, B - 1, B, B))
+    w("    return hipGetLastError();")
+    w("}")
+    w("#endif")
+    w("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
 def gen_dispatch(kinds):
     """kinds: list of (L, prec, variant configs)."""
     o = ["// Generated by specfab_amd/tables/gen_fast.py -- do not edit.",
@@ -324,8 +523,9 @@ def main():
             n, rows = dump(L, exe)
             vs = VARIANTS[(prec, "tuned" if L in TUNED_L else "default")]
             for vid, cfg in enumerate(vs):
+                gen = gen_kernel_reg if cfg["mode"] == "reg" else gen_kernel
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
-                                 gen_kernel(L, prec, n, rows, cfg, vid))
+                                 gen(L, prec, n, rows, cfg, vid))
             kinds.append((L, prec, vs))
     # drop generated kernels of an older variant list
     keep = {"sfg_fast_L%d_%s_v%d.hip" % (L, prec, v) for (L, prec, cfgs) in kinds for v in range(len(cfgs))}
