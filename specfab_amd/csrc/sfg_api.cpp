@@ -934,8 +934,6 @@ struct sfg_mlp {
     sfg::MlpNet net;
     // surrogate workspace (grow-only)
     int64_t wcap = 0;
-    float* feat = nullptr;
-    float* y = nullptr;
     int* flags = nullptr;
     int* idx = nullptr;
     int* count = nullptr;
@@ -956,8 +954,6 @@ int mlp_ws(sfg_mlp* m, int64_t P) {
         return e;
     };
     const size_t Pz = (size_t)P;
-    HIPCHK(A((void**)&m->feat, Pz * 36 * 4));
-    HIPCHK(A((void**)&m->y, Pz * 9 * 4));
     HIPCHK(A((void**)&m->flags, Pz * 4));
     HIPCHK(A((void**)&m->idx, Pz * 4));
     HIPCHK(A((void**)&m->count, 64));
@@ -1152,21 +1148,16 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     hipStream_t s = pick(ctx, stream);
     rc = mlp_ws(net, P);
     if (rc) return rc;
-    // 1. features, envelope and audit flags
-    sfg::SurrogateFeatArgs fa{};
-    fa.P = P; fa.x = x_dev; fa.ugrad = ugrad_dev; fa.A2 = op->A2; fa.feat = net->feat; fa.flags = net->flags;
+    // 1. features, envelope / audit flags, network and decoding in one kernel
+    sfg::SurrogateInferArgs ia{};
+    sfg::SurrogateFeatArgs& fa = ia.fa;
+    fa.P = P; fa.x = x_dev; fa.ugrad = ugrad_dev; fa.A2 = op->A2; fa.feat = nullptr; fa.flags = net->flags;
     for (int i = 0; i < 3; i++) { fa.env_lo[i] = net->net.env_lo[i]; fa.env_hi[i] = net->net.env_hi[i]; }
     fa.audit_frac = audit_frac; fa.seed = seed;
-    HIPCHK(sfg::launch_features(fa, s));
-    // 2. network (chunks), 3. decode
-    HIPCHK(net->net.ensure_batch(std::min<int64_t>(P, MLP_CHUNK)));
-    for (int64_t off = 0; off < P; off += MLP_CHUNK) {
-        const int64_t n = std::min<int64_t>(MLP_CHUNK, P - off);
-        HIPCHK(net->net.forward(n, net->feat + off * sfg::MLP_IN, s));
-        HIPCHK(sfg::launch_denormalize_out(n, net->net.yout, net->net.norm, net->y + off * sfg::MLP_OUT, s));
-    }
-    HIPCHK(sfg::launch_decode(P, net->y, Eij_dev, Vi_dev, s));
-    // 4. plain homogenization of the flagged parcels
+    for (int i = 0; i < sfg::MLP_NLAYER; i++) { ia.wf[i] = net->net.wf[i]; ia.bias[i] = net->net.bias_pad[i]; }
+    ia.norm = net->net.norm; ia.Eij = Eij_dev; ia.Vi = Vi_dev;
+    HIPCHK(sfg::launch_surrogate_infer(ia, s));
+    // 2. plain homogenization of the flagged parcels
     HIPCHK(hipMemsetAsync(net->count, 0, sizeof(int), s));
     HIPCHK(sfg::launch_select(P, net->flags, net->idx, net->count, s));
     int nsel = 0;

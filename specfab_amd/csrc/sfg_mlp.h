@@ -34,6 +34,12 @@ struct GemmArgs {
 
 hipError_t launch_gemm_bf16(const GemmArgs& a, hipStream_t s);
 
+// weights (FP32 [in][out]) -> bf16 MFMA fragment order of the fused inference
+// kernel (in_pad a multiple of 32, out_pad of 16); perm: permuted K order of
+// layers fed by accumulators
+hipError_t launch_pack_frag(int in, int out, int in_pad, int out_pad, int perm, const float* W, __bf16* wf,
+                            hipStream_t s);
+
 // elementwise helpers
 hipError_t launch_gelu_fwd(int64_t n, const float* z, __bf16* act, hipStream_t s);
 hipError_t launch_gelu_bwd(int64_t n, const float* dact, const float* z, float* dz, __bf16* dz_bf, hipStream_t s);

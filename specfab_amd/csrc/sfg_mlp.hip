@@ -134,8 +134,33 @@ __global__ void mse_grad_kernel(int rows, int cols, int ld, const float* y, cons
     loss_acc[r] = s;
 }
 
+// weights of one layer (FP32 master, [in][out]) in the fragment order of the
+// fused inference kernel: element ((m * KS + s) * 64 + lane) * 8 + j holds
+// W[k][16 m + lane % 16] with k = 32 s + 8 (lane / 16) + j (input layer) or,
+// for layers fed by a previous layer's accumulators, the permuted
+// k = 32 s + 16 (j / 4) + 4 (lane / 16) + j % 4; zero outside in x out.
+__global__ void pack_frag_kernel(int in, int out, int KS, int MT, int perm, const float* W, __bf16* wf) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)MT * KS * 512) return;
+    const int j = (int)(e & 7), lane = (int)((e >> 3) & 63);
+    const int64_t f = e >> 9;
+    const int s = (int)(f % KS), m = (int)(f / KS);
+    const int g = lane >> 4;
+    const int k = perm ? 32 * s + 16 * (j >> 2) + 4 * g + (j & 3) : 32 * s + 8 * g + j;
+    const int o = 16 * m + (lane & 15);
+    wf[e] = (__bf16)((k < in && o < out) ? W[(int64_t)k * out + o] : 0.0f);
+}
+
 namespace {
 unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+}
+
+hipError_t launch_pack_frag(int in, int out, int in_pad, int out_pad, int perm, const float* W, __bf16* wf,
+                            hipStream_t s) {
+    const int KS = in_pad / 32, MT = out_pad / 16;
+    const int64_t n = (int64_t)MT * KS * 512;
+    hipLaunchKernelGGL(pack_frag_kernel, dim3(nb(n, 256)), dim3(256), 0, s, in, out, KS, MT, perm, W, wf);
+    return hipGetLastError();
 }
 
 hipError_t launch_gemm_bf16(const GemmArgs& a, hipStream_t s) {

@@ -59,6 +59,7 @@ hipError_t MlpNet::init(int device_, uint64_t seed) {
         Layer L = layer(i);
         if ((e = alloc((void**)&wb[i], (size_t)L.in_pad * L.out_pad * 2)) != hipSuccess) return e;
         if ((e = alloc((void**)&bias_pad[i], (size_t)L.out_pad * 4)) != hipSuccess) return e;
+        if ((e = alloc((void**)&wf[i], (size_t)L.in_pad * L.out_pad * 2)) != hipSuccess) return e;
     }
     // He-uniform init of W, zero biases; identity normalisation
     std::mt19937_64 rng(seed);
@@ -103,6 +104,8 @@ hipError_t MlpNet::refresh_bf16(hipStream_t s) {
         e = hipMemsetAsync(bias_pad[i], 0, (size_t)L.out_pad * 4, s);
         if (e != hipSuccess) return e;
         e = hipMemcpyAsync(bias_pad[i], w + L.b_off, (size_t)L.out * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        e = launch_pack_frag(L.in, L.out, L.in_pad, L.out_pad, i > 0 ? 1 : 0, w + L.w_off, wf[i], s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -19,6 +19,7 @@ struct MlpNet {
     std::vector<float> norm_host;
     __bf16* wb[MLP_NLAYER] = {};    // bf16 weights [in_pad][out_pad]
     float* bias_pad[MLP_NLAYER] = {};
+    __bf16* wf[MLP_NLAYER] = {};    // bf16 weights in the fused inference kernel's fragment order
     // batch buffers
     int64_t cap = 0;
     float* xin = nullptr;           // normalised input (B x 64)

@@ -19,9 +19,19 @@ struct SurrogateFeatArgs {
     uint64_t seed;
 };
 
+// fused inference: state -> features -> network -> Eij (P, 6), Vi (P, 3)
+struct SurrogateInferArgs {
+    SurrogateFeatArgs fa;      // state, strain rate, envelope / audit flags (fa.feat unused)
+    const void* wf[4];         // bf16 weights in MFMA fragment order (MlpNet::wf)
+    const float* bias[4];      // padded biases (256, 256, 256, 16)
+    const float* norm;         // in_mean (36), in_std (36), out_mean (9), out_std (9)
+    double* Eij;
+    double* Vi;
+};
+
 hipError_t launch_features(const SurrogateFeatArgs& a, hipStream_t s);
+hipError_t launch_surrogate_infer(const SurrogateInferArgs& a, hipStream_t s);
 hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float* T, hipStream_t s);
-hipError_t launch_decode(int64_t n, const float* y, double* Eij, double* Vi, hipStream_t s);
 hipError_t launch_select(int64_t P, const int* flags, int* idx, int* count, hipStream_t s);
 hipError_t launch_merge(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp, double* Eij,
                         double* Vi, int* status, double* errE, double* errV, hipStream_t s);

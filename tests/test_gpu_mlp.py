@@ -55,6 +55,46 @@ def test_forward_matches_cpu(gpu_ctx):
         assert err < 2e-3, (B, err)
 
 
+def test_fused_inference_matches_layers(gpu_ctx):
+    """The fused inference kernel behind Surrogate.homogenize (state -> Eij, Vi
+    in one launch: weights in permuted-K MFMA fragment order, activations kept
+    in registers, polynomial GELU) against the layer-by-layer kernels and the
+    CPU restatement, on random states (features that differ from parcel to
+    parcel, so a parcel mix-up shows) whose counts are not multiples of the
+    64-parcel wave tile.  Bars: against the CPU restatement with the same
+    polynomial GELU, 2e-3 of the output range or 1.5x the error of the
+    layer-by-layer kernels against theirs (exact GELU) on the same features;
+    against the layer-by-layer kernels, 1e-2: on a random untrained network a
+    GELU difference of <= 3.7e-5 flips bf16 roundings of activations that the
+    layers amplify (about 2e-3 of the range in the FP64 restatement)."""
+    import torch
+    from specfab_amd import surrogate as S
+    rng = np.random.default_rng(4)
+    p, norm = _rand_net(rng)
+    net = _net(gpu_ctx, p, norm)
+    net.set_envelope([-10.0] * 3, [10.0] * 3)
+    for P in (1, 1000, 4133):
+        fb = _field(P, 0, seed=P, spread=0.05)
+        X = S.features(fb)
+        Y = net.forward(X).double().cpu().numpy()
+        E, V, st = net.homogenize(fb, audit_frac=0.0)
+        torch.cuda.synchronize()
+        assert st["n_fallback"] == 0 and st["n_audit"] == 0
+        fused = np.log(torch.cat([E, V], 1).cpu().numpy())
+        Xh = X.cpu().numpy()
+        yn, _, _ = M.forward(p, Xh, norm.astype(np.float64), bf=True, act=M.gelu_poly)
+        ref = yn * norm[81:90] + norm[72:81]
+        yl, _, _ = M.forward(p, Xh, norm.astype(np.float64), bf=True)
+        ref_l = yl * norm[81:90] + norm[72:81]
+        scale = np.abs(ref).max()
+        e_fused, e_layers = np.abs(fused - ref).max() / scale, np.abs(Y - ref_l).max() / scale
+        print("P=%d fused vs CPU %.2e, layers vs CPU %.2e, fused vs layers %.2e"
+              % (P, e_fused, e_layers, np.abs(fused - Y).max() / scale))
+        # no less accurate than the layer-by-layer kernels (same bf16 rounding points)
+        assert e_fused < max(2e-3, 1.5 * e_layers)
+        assert np.abs(fused - Y).max() < 1e-2 * scale
+
+
 def test_gradients_match_cpu(gpu_ctx):
     import torch
     rng = np.random.default_rng(2)
@@ -124,7 +164,7 @@ def test_surrogate_vs_plain_on_field(gpu_ctx):
         assert np.abs(E.cpu().numpy()[fbk] - Ep.cpu().numpy()[fbk]).max() == 0.0
 
 
-def _field(P, steps, seed):
+def _field(P, steps, seed, spread=1e-3):
     import math
     import torch
     from specfab_amd import _lib
@@ -139,9 +179,10 @@ def _field(P, steps, seed):
     fb.L, fb.n, fb.dev, fb.P = L, n, dev, P
     fb.ugrad = S.random_gradients(P, g, dev)
     fb.gam0 = (10.0 ** (torch.rand(P, generator=g, device=dev, dtype=torch.float64) * 2 - 4)).contiguous()
-    x = torch.randn((n, P), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    x = torch.randn((n, P), generator=g, device=dev, dtype=torch.float64) * spread
     x[0] = 1 / math.sqrt(4 * math.pi)
     fb.x, fb.tau, fb.lam, fb.steps = x.contiguous(), None, None, None
     fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
-    fb.evolve(steps, 0.5, scheme="rk4", procs=("LROT", "REG", "DDRX"))
+    if steps:
+        fb.evolve(steps, 0.5, scheme="rk4", procs=("LROT", "REG", "DDRX"))
     return fb
