@@ -45,24 +45,19 @@ def gelu(z):
     return 0.5 * z * (1 + np.vectorize(math.erf)(z / math.sqrt(2)))
 
 
-# Coefficients of the fused inference kernel's GELU (specfab_amd/csrc/
-# sfg_surrogate.hip, gelu_poly): Phi(z) = 1/2 + zc Q(u), zc = clamp(z, -4.25,
-# 4.25), u = 2 zc^2 / 4.25^2 - 1, Q(u) = sum_k C[k] u^k (FP32 constants).
-GELU_POLY = tuple(float.fromhex(h) for h in (
-    "0x1.53d5eep-3", "-0x1.4acc94p-4", "0x1.c84d4ep-5", "-0x1.425d56p-5", "0x1.a34ed4p-6",
-    "-0x1.ce5844p-7", "0x1.193d8ap-7", "-0x1.a3cc1ep-8", "0x1.42cf3ap-9"))
-GELU_POLY_U = float.fromhex("0x1.c5894ep-4")
-
-
-def gelu_poly(z):
-    """The polynomial GELU of the fused inference kernel, in FP64."""
+def gelu_table(z):
+    """GELU of the fused inference kernel (specfab_amd/csrc/sfg_surrogate.hip,
+    gelu_tab): Phi linearly interpolated between nodes z_k = (k - 512) / 102.4,
+    k = 0..1024, index clamped to [0, 1024), in FP64."""
     z = np.asarray(z, dtype=np.float64)
-    zc = np.clip(z, -4.25, 4.25)
-    u = zc * zc * GELU_POLY_U - 1.0
-    q = np.full_like(z, GELU_POLY[-1])
-    for c in GELU_POLY[-2::-1]:
-        q = q * u + c
-    return z * (zc * q + 0.5)
+    sc = float(np.float32(102.4))
+    t = np.clip(z * sc + 512.0, 0.0, float.fromhex("0x1.fffffep+9"))
+    k = np.floor(t)
+    zk = (k - 512.0) / sc
+    zk1 = (k + 1 - 512.0) / sc
+    p0 = 0.5 * (1 + np.vectorize(math.erf)(zk / math.sqrt(2)))
+    p1 = 0.5 * (1 + np.vectorize(math.erf)(zk1 / math.sqrt(2)))
+    return z * (p0 + (p1 - p0) * (t - k))
 
 
 def dgelu(z):
@@ -71,7 +66,7 @@ def dgelu(z):
 
 def forward(params, X, norm, bf=True, act=None):
     """Normalised outputs (B, 9) and the saved intermediates; `act` replaces
-    the exact GELU (e.g. gelu_poly)."""
+    the exact GELU (e.g. gelu_table)."""
     act = act or gelu
     r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     im, isd = norm[0:36], norm[36:72]

@@ -1155,11 +1155,10 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     for (int i = 0; i < 3; i++) { fa.env_lo[i] = net->net.env_lo[i]; fa.env_hi[i] = net->net.env_hi[i]; }
     fa.audit_frac = audit_frac; fa.seed = seed;
     for (int i = 0; i < sfg::MLP_NLAYER; i++) { ia.wf[i] = net->net.wf[i]; ia.bias[i] = net->net.bias_pad[i]; }
-    ia.norm = net->net.norm; ia.Eij = Eij_dev; ia.Vi = Vi_dev;
-    HIPCHK(sfg::launch_surrogate_infer(ia, s));
-    // 2. plain homogenization of the flagged parcels
+    ia.norm = net->net.norm; ia.gtab = net->net.gtab; ia.Eij = Eij_dev; ia.Vi = Vi_dev; ia.idx = net->idx; ia.count = net->count;
     HIPCHK(hipMemsetAsync(net->count, 0, sizeof(int), s));
-    HIPCHK(sfg::launch_select(P, net->flags, net->idx, net->count, s));
+    HIPCHK(sfg::launch_surrogate_infer(ia, s));
+    // 2. plain homogenization of the flagged parcels (work list from the kernel)
     int nsel = 0;
     HIPCHK(hipMemcpyAsync(&nsel, net->count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -244,7 +244,9 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
     return status;
 }
 
-__global__ void eij_kernel(EijArgs a) {
+// one wave per SIMD: the Taylor 6x6 system, a4 and the frame need ~380
+// registers per lane, which then stay in VGPRs/AGPRs instead of scratch
+__global__ void __launch_bounds__(128, 1) eij_kernel(EijArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (a.idx ? a.nidx : a.P)) return;
     const int64_t p = a.idx ? (int64_t)a.idx[t] : t;   // state column

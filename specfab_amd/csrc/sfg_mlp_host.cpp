@@ -12,6 +12,7 @@
 #include "../../include/specfab_gpu.h"
 #include "sfg_mlp.h"
 #include "sfg_mlp_net.h"
+#include "sfg_surrogate.h"
 
 namespace sfg {
 
@@ -55,6 +56,8 @@ hipError_t MlpNet::init(int device_, uint64_t seed) {
     if ((e = alloc((void**)&m, np * 4)) != hipSuccess) return e;
     if ((e = alloc((void**)&v, np * 4)) != hipSuccess) return e;
     if ((e = alloc((void**)&norm, 2 * (MLP_IN + MLP_OUT) * 4)) != hipSuccess) return e;
+    if ((e = alloc((void**)&gtab, 1024 * sizeof(float2))) != hipSuccess) return e;
+    if ((e = launch_gelu_table(gtab, nullptr)) != hipSuccess) return e;
     for (int i = 0; i < MLP_NLAYER; i++) {
         Layer L = layer(i);
         if ((e = alloc((void**)&wb[i], (size_t)L.in_pad * L.out_pad * 2)) != hipSuccess) return e;

@@ -20,6 +20,7 @@ struct MlpNet {
     __bf16* wb[MLP_NLAYER] = {};    // bf16 weights [in_pad][out_pad]
     float* bias_pad[MLP_NLAYER] = {};
     __bf16* wf[MLP_NLAYER] = {};    // bf16 weights in the fused inference kernel's fragment order
+    float2* gtab = nullptr;         // GELU table of the fused inference kernel
     // batch buffers
     int64_t cap = 0;
     float* xin = nullptr;           // normalised input (B x 64)

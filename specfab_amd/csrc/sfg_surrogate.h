@@ -25,14 +25,18 @@ struct SurrogateInferArgs {
     const void* wf[4];         // bf16 weights in MFMA fragment order (MlpNet::wf)
     const float* bias[4];      // padded biases (256, 256, 256, 16)
     const float* norm;         // in_mean (36), in_std (36), out_mean (9), out_std (9)
+    const float2* gtab;        // GELU table (launch_gelu_table)
     double* Eij;
     double* Vi;
+    int* idx;                  // or null: flagged parcels appended here (arbitrary order)
+    int* count;                // their number (zeroed by the caller)
 };
 
 hipError_t launch_features(const SurrogateFeatArgs& a, hipStream_t s);
 hipError_t launch_surrogate_infer(const SurrogateInferArgs& a, hipStream_t s);
+// the 1024-interval table of the normal CDF used by the fused kernel's GELU
+hipError_t launch_gelu_table(float2* tab, hipStream_t s);
 hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float* T, hipStream_t s);
-hipError_t launch_select(int64_t P, const int* flags, int* idx, int* count, hipStream_t s);
 hipError_t launch_merge(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp, double* Eij,
                         double* Vi, int* status, double* errE, double* errV, hipStream_t s);
 

@@ -19,40 +19,31 @@ namespace {
 unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }
 
-// symmetric 3x3 eigenvalues (cyclic Jacobi), sorted descending
-__device__ inline void d_eigvals3(const double* M, double* lam) {
-    double a[3][3];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (M[3 * i + j] + M[3 * j + i]);
-    for (int sweep = 0; sweep < 40; sweep++) {
-        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
-        if (off == 0.0) break;
-#pragma unroll
-        for (int pq = 0; pq < 3; pq++) {
-            const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
-            if (a[p][q] == 0.0) continue;
-            double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
-            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
-            double c = 1 / sqrt(t * t + 1), s = t * c;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double akp = a[k][p], akq = a[k][q];
-                a[k][p] = c * akp - s * akq;
-                a[k][q] = s * akp + c * akq;
-            }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double apk = a[p][k], aqk = a[q][k];
-                a[p][k] = c * apk - s * aqk;
-                a[q][k] = s * apk + c * aqk;
-            }
-        }
+// eigenvalues of a symmetric 3x3 matrix, sorted descending: the
+// trigonometric solution of the characteristic cubic (Smith, Commun. ACM 4,
+// 1961), A = q I + p B with q = tr A / 3, p^2 = |A - q I|^2 / 6 and
+// lambda = q + 2 p cos(acos(det B / 2) / 3 + 2 pi k / 3)
+__device__ inline void d_eigvals3(const double* A, double* lam) {
+    const double a00 = A[0], a11 = A[4], a22 = A[8];
+    const double a01 = 0.5 * (A[1] + A[3]), a02 = 0.5 * (A[2] + A[6]), a12 = 0.5 * (A[5] + A[7]);
+    const double q = (a00 + a11 + a22) / 3.0;
+    const double b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const double p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * (a01 * a01 + a02 * a02 + a12 * a12);
+    const double p = sqrt(p2 / 6.0);
+    if (!(p > 0.0)) {
+        lam[0] = lam[1] = lam[2] = q;
+        return;
     }
-    double d0 = a[0][0], d1 = a[1][1], d2 = a[2][2], t;
-    if (d1 > d0) { t = d0; d0 = d1; d1 = t; }
-    if (d2 > d0) { t = d0; d0 = d2; d2 = t; }
-    if (d2 > d1) { t = d1; d1 = d2; d2 = t; }
-    lam[0] = d0; lam[1] = d1; lam[2] = d2;
+    const double ip = 1.0 / p;
+    const double c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+    double r = 0.5 * (c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02));
+    r = fmin(1.0, fmax(-1.0, r));
+    const double phi = acos(r) / 3.0;
+    const double e0 = q + 2.0 * p * cos(phi);
+    const double e2 = q + 2.0 * p * cos(phi + 2.0943951023931957);
+    lam[0] = e0;
+    lam[1] = 3.0 * q - e0 - e2;
+    lam[2] = e2;
 }
 
 // splitmix-style hash -> [0, 1)
@@ -69,7 +60,7 @@ __device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, flo
     double xq[15];
 #pragma unroll
     for (int q = 0; q < 15; q++) xq[q] = a.x[(int64_t)q * a.P + p];
-    const double x0 = xq[0];
+    const double x0 = xq[0], ix0 = 1.0 / x0;
     // complex l <= 4 coefficients from the real basis [Re n_l0, Re n_l1, Im n_l1, ...]
     int j = 0;
     for (int l = 0; l <= 4; l += 2) {
@@ -83,8 +74,8 @@ __device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, flo
                 re *= s;
                 im *= -s;
             }
-            f[2 * j] = (float)(re / x0);
-            f[2 * j + 1] = (float)(im / x0);
+            f[2 * j] = (float)(re * ix0);
+            f[2 * j + 1] = (float)(im * ix0);
         }
     }
     // unit strain rate (Mandel)
@@ -104,7 +95,7 @@ __device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, flo
             double s = 0;
 #pragma unroll
             for (int q = 0; q < 6; q++) s += a.A2[i * 15 + q] * xq[q];
-            a2v[i] = s / x0 + (i < 3 ? 1.0 / 3.0 : 0.0);
+            a2v[i] = s * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
         }
         double A[9];
         A[0] = a2v[0]; A[4] = a2v[1]; A[8] = a2v[2];
@@ -137,16 +128,6 @@ __global__ void targets_kernel(int64_t P, const double* Eij, const double* Vi, f
     T[p * 9 + 6] = (float)log(v0);
     T[p * 9 + 7] = (float)log(v1);
     T[p * 9 + 8] = (float)log(v2);
-}
-
-// work list of flagged parcels (fallback or audit); order is arbitrary
-__global__ void select_kernel(int64_t P, const int* flags, int* idx, int* count) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    if (flags[p]) {
-        int k = atomicAdd(count, 1);
-        idx[k] = (int)p;
-    }
 }
 
 // merge the plain results of the work list: fallback parcels take them,
@@ -196,136 +177,221 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// GELU(z) = z Phi(z), Phi(z) = 1/2 + zc Q(u) with zc = clamp(z, -4.25, 4.25),
-// u = 2 zc^2 / 4.25^2 - 1 and Q a degree-8 polynomial fitted to the normal CDF
-// (weighted least squares on Chebyshev nodes); |error| <= 3.7e-5 for |z| <= 15
-// in FP32 (about 1 % of a bf16 step at 1), against erf-GELU.
-__device__ inline float gelu_poly(float z) {
-    const float zc = __builtin_amdgcn_fmed3f(z, -4.25f, 4.25f);
-    const float u = fmaf(zc * zc, 0x1.c5894ep-4f, -1.0f);
-    float q = 0x1.42cf3ap-9f;
-    q = fmaf(q, u, -0x1.a3cc1ep-8f);
-    q = fmaf(q, u, 0x1.193d8ap-7f);
-    q = fmaf(q, u, -0x1.ce5844p-7f);
-    q = fmaf(q, u, 0x1.a34ed4p-6f);
-    q = fmaf(q, u, -0x1.425d56p-5f);
-    q = fmaf(q, u, 0x1.c84d4ep-5f);
-    q = fmaf(q, u, -0x1.4acc94p-4f);
-    q = fmaf(q, u, 0x1.53d5eep-3f);
-    return z * fmaf(zc, q, 0.5f);
+// GELU(z) = z Phi(z) with Phi interpolated linearly in a table of 1024
+// intervals on [-5, 5] (LDS, built by each workgroup from erfc): nodes
+// z_k = (k - 512) / 102.4, entry k = (Phi(z_k), Phi(z_k+1) - Phi(z_k)).
+// Interpolation error of Phi <= h^2 / 8 max|Phi''| = 3e-6 (of GELU <= 4e-6 on
+// [-5, 5]); beyond +-5 Phi is held at its end values (1 - Phi(5) = 2.9e-7).
+constexpr int GTAB_N = 1024;
+constexpr float GTAB_SCALE = 102.4f, GTAB_OFF = 512.0f;
+
+__device__ inline float gtab_phi(int k) {
+    const float z = ((float)k - GTAB_OFF) / GTAB_SCALE;
+    return 0.5f * erfcf(-z * 0.70710678118654752f);
 }
 
-// bias + GELU of a 256-row layer output, repacked as the next layer's B
-// fragments (k step s, parcel tile t)
-__device__ inline void gelu_repack(const f32x4 (&acc)[16][4], const float* __restrict__ bias, int g,
-                                   bf16x8 (&B)[8][4]) {
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const f32x4 b0 = *(const f32x4*)(bias + 32 * s + 4 * g);
-        const f32x4 b1 = *(const f32x4*)(bias + 32 * s + 16 + 4 * g);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                B[s][t][i] = (__bf16)gelu_poly(acc[2 * s][t][i] + b0[i]);
-                B[s][t][4 + i] = (__bf16)gelu_poly(acc[2 * s + 1][t][i] + b1[i]);
-            }
-        }
-    }
+__device__ inline float gelu_tab(const float2* __restrict__ tab, float z) {
+    const float t = __builtin_amdgcn_fmed3f(fmaf(z, GTAB_SCALE, GTAB_OFF), 0.0f, 0x1.fffffep+9f);
+    const float2 e = tab[(int)t];
+    return z * fmaf(e.y, __builtin_amdgcn_fractf(t), e.x);
 }
 
-// 256 x 256 hidden layer: acc = W^T H^T with W in fragment order
-__device__ inline void hidden_layer(const bf16x8* __restrict__ wf, const bf16x8 (&B)[8][4], int lane,
-                                    f32x4 (&acc)[16][4]) {
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-#pragma unroll
-        for (int t = 0; t < 4; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+// GELU of a 256-row layer output (bias already in the accumulator), repacked
+// as the next layer's B fragments (k step s, parcel tile t)
+template <int NT>
+__device__ inline void gelu_repack(const float2* __restrict__ tab, const f32x4 (&acc)[16][NT],
+                                   bf16x8 (&B)[8][NT]) {
 #pragma unroll
     for (int s = 0; s < 8; s++)
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-            const bf16x8 A = wf[(m * 8 + s) * 64 + lane];
+        for (int t = 0; t < NT; t++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[s][t], acc[m][t], 0, 0, 0);
+            for (int i = 0; i < 4; i++) {
+                B[s][t][i] = (__bf16)gelu_tab(tab, acc[2 * s][t][i]);
+                B[s][t][4 + i] = (__bf16)gelu_tab(tab, acc[2 * s + 1][t][i]);
+            }
+}
+
+// bias as the initial accumulator: element i of tile (m, t) is row 16 m + 4 g + i
+template <int NT>
+__device__ inline void acc_bias(const float* __restrict__ bias, int g, f32x4 (&acc)[16][NT]) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const f32x4 b = *(const f32x4*)(bias + 16 * m + 4 * g);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = b;
+    }
+}
+
+constexpr int WSLICE = 16 * 64;   // weight fragments of one k step for the 16 M tiles
+
+// workgroup barrier without the memory fence of __syncthreads() (whose
+// vmcnt(0) would also wait for the weight slices still in flight): the
+// caller's s_waitcnt completes the LDS traffic that must be visible, and the
+// empty asm statements keep the compiler from moving memory accesses across
+__device__ inline void wg_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// slice s of a layer's weights -> LDS by direct global-to-LDS loads (no
+// registers): the wave's 64 lanes move 64 consecutive 16-byte fragments, lane
+// i to LDS base + 16 i, four times
+template <int KS>
+__device__ inline void stage_slice(const bf16x8* __restrict__ wf, int s, bf16x8* buf, int tid) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int e = tid + 256 * q;
+        __builtin_amdgcn_global_load_lds((const void*)(wf + ((e >> 6) * KS + s) * 64 + (e & 63)),
+                                         (__attribute__((address_space(3))) void*)(buf + (e & ~63)), 16, 0, 0);
+    }
+}
+
+// acc += W^T H^T for a layer of 256 rows and KS k steps (fragment (m, s, lane)
+// at wf[(m KS + s) 64 + lane]); B fragments from bfrag(s, t).  The weights of
+// a k step are staged once per workgroup in LDS and shared by the four waves;
+// triple buffered: slices s + 1 and s + 2 are in flight during the MFMAs of
+// step s.
+// G0: number of k steps of the earlier layers; step s of this layer uses
+// buffer (G0 + s) % 3, so that across layers too a buffer is refilled only
+// after the barrier that follows every wave's last read of it.
+template <int KS, int G0, int NT, class BF>
+__device__ inline void mfma_layer(const bf16x8* __restrict__ wf, BF&& bfrag, f32x4 (&acc)[16][NT],
+                                  bf16x8* const (&wst)[3], int tid, int lane) {
+    stage_slice<KS>(wf, 0, wst[G0 % 3], tid);
+    stage_slice<KS>(wf, 1, wst[(G0 + 1) % 3], tid);
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        const bf16x8* buf = wst[(G0 + s) % 3];
+        // slice s has landed (own loads: at most the 4 of slice s + 1 still in
+        // flight) and is visible from every wave; every wave is done with the
+        // buffer of step s - 1, which slice s + 2 refills
+        if (s + 1 < KS) __builtin_amdgcn_s_waitcnt(0x0074);   // vmcnt(4) lgkmcnt(0)
+        else __builtin_amdgcn_s_waitcnt(0);
+        wg_barrier();
+        if (s + 2 < KS) stage_slice<KS>(wf, s + 2, wst[(G0 + s + 2) % 3], tid);
+        __builtin_amdgcn_sched_barrier(0);   // issue the refill before this step's MFMAs
+        // A of tile m + 1 is read while the MFMAs of tile m run
+        bf16x8 A = buf[lane];
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const bf16x8 An = buf[((m + 1) & 15) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, bfrag(s, t), acc[m][t], 0, 0, 0);
+            A = An;
         }
+    }
 }
 
 constexpr int FEAT_LD = 72;   // LDS row of one parcel's 64 padded features (+8: bank spread)
+constexpr int INFER_NT = 2;   // parcel tiles of 16 per wave (two waves per SIMD)
+constexpr int INFER_WP = 16 * INFER_NT;   // parcels per wave
 
 }  // namespace
 
-__global__ void __launch_bounds__(256, 1) surrogate_infer_kernel(SurrogateInferArgs a) {
-    __shared__ __bf16 fs[4][64][FEAT_LD];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 64;
+__global__ void build_gelu_table_kernel(float2* tab) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= GTAB_N) return;
+    const float p0 = gtab_phi(k), p1 = gtab_phi(k + 1);
+    tab[k] = make_float2(p0, p1 - p0);
+}
+
+hipError_t launch_gelu_table(float2* tab, hipStream_t s) {
+    hipLaunchKernelGGL(build_gelu_table_kernel, dim3(GTAB_N / 256), dim3(256), 0, s, tab);
+    return hipGetLastError();
+}
+
+// one workgroup = 4 waves x INFER_WP parcels; every wave runs the network on
+// its own parcels, the workgroup shares the staged weights and the GELU table
+__global__ void __launch_bounds__(256, 4 / INFER_NT) surrogate_infer_kernel(SurrogateInferArgs a) {
+    constexpr int NT = INFER_NT, WP = INFER_WP;
+    __shared__ __bf16 fs[4][WP][FEAT_LD];
+    __shared__ bf16x8 wst0[WSLICE], wst1[WSLICE], wst2[WSLICE];
+    bf16x8* const wst[3] = {wst0, wst1, wst2};
+    __shared__ float2 gtab[GTAB_N];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {   // GELU table: 8 KB, two 16-byte loads per thread
+        const f32x4* src = (const f32x4*)a.gtab;
+        f32x4* dst = (f32x4*)gtab;
+        dst[tid] = src[tid];
+        dst[tid + 256] = src[tid + 256];
+    }
+    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * WP;
     const int64_t P = a.fa.P;
-    {   // features, one lane per parcel; lanes past P feed zeros
-        const int64_t p = base + lane;
+    if (tid < 4 * WP) {   // features of the workgroup's parcels, one lane each; past P: zeros
+        const int64_t p = (int64_t)blockIdx.x * 4 * WP + tid;
+        const int fw = tid / WP, fp = tid % WP;
         float f[36];
+        int fl = 0;
         if (p < P) {
-            const int fl = parcel_features(a.fa, p, f);
+            fl = parcel_features(a.fa, p, f);
             if (a.fa.flags) a.fa.flags[p] = fl;
 #pragma unroll
-            for (int k = 0; k < 36; k++) fs[wave][lane][k] = (__bf16)((f[k] - a.norm[k]) / a.norm[36 + k]);
+            for (int k = 0; k < 36; k++)
+                fs[fw][fp][k] = (__bf16)((f[k] - a.norm[k]) * __builtin_amdgcn_rcpf(a.norm[36 + k]));
         } else {
 #pragma unroll
-            for (int k = 0; k < 36; k++) fs[wave][lane][k] = (__bf16)0.0f;
+            for (int k = 0; k < 36; k++) fs[fw][fp][k] = (__bf16)0.0f;
         }
 #pragma unroll
-        for (int k = 36; k < 64; k++) fs[wave][lane][k] = (__bf16)0.0f;
-    }
-    __syncthreads();
-    const int g = lane >> 4, c = lane & 15;
-    f32x4 acc[16][4];
-    bf16x8 B[8][4];
-    {   // input layer, K = 64 (36 features + zero padding), natural K order
-        const bf16x8* wf = (const bf16x8*)a.wf[0];
-#pragma unroll
-        for (int m = 0; m < 16; m++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            bf16x8 b[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) b[t] = *(const bf16x8*)&fs[wave][16 * t + c][32 * s + 8 * g];
-#pragma unroll
-            for (int m = 0; m < 16; m++) {
-                const bf16x8 A = wf[(m * 2 + s) * 64 + lane];
-#pragma unroll
-                for (int t = 0; t < 4; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b[t], acc[m][t], 0, 0, 0);
+        for (int k = 36; k < 64; k++) fs[fw][fp][k] = (__bf16)0.0f;
+        // flagged parcels (fallback / audit) join the work list: one atomic per wave
+        if (a.idx) {
+            const uint64_t m = __ballot(fl != 0);
+            if (m) {
+                const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const int leader = __builtin_ctzll(m);
+                int slot = 0;
+                if (lane == leader) slot = atomicAdd(a.count, __popcll(m));
+                slot = __shfl(slot, leader);
+                if (fl) a.idx[slot + below] = (int)p;
             }
         }
     }
-    gelu_repack(acc, a.bias[0], g, B);
-    hidden_layer((const bf16x8*)a.wf[1], B, lane, acc);
-    gelu_repack(acc, a.bias[1], g, B);
-    hidden_layer((const bf16x8*)a.wf[2], B, lane, acc);
-    gelu_repack(acc, a.bias[2], g, B);
+    // (the first barrier of mfma_layer orders these LDS writes before the reads)
+    const int g = lane >> 4, c = lane & 15;
+    f32x4 acc[16][NT];
+    bf16x8 B[8][NT];
+    // input layer, K = 64 (36 features + zero padding), natural K order
+    acc_bias<NT>(a.bias[0], g, acc);
+    mfma_layer<2, 0, NT>((const bf16x8*)a.wf[0],
+                      [&](int s, int t) { return *(const bf16x8*)&fs[wave][16 * t + c][32 * s + 8 * g]; }, acc,
+                      wst, tid, lane);
+    gelu_repack<NT>(gtab, acc, B);
+    acc_bias<NT>(a.bias[1], g, acc);
+    mfma_layer<8, 2, NT>((const bf16x8*)a.wf[1], [&](int s, int t) { return B[s][t]; }, acc, wst, tid, lane);
+    gelu_repack<NT>(gtab, acc, B);
+    acc_bias<NT>(a.bias[2], g, acc);
+    mfma_layer<8, 10, NT>((const bf16x8*)a.wf[2], [&](int s, int t) { return B[s][t]; }, acc, wst, tid, lane);
+    gelu_repack<NT>(gtab, acc, B);
     // output layer: 16 rows (9 outputs + zero padding)
-    f32x4 o[4];
+    f32x4 o[NT];
+    {
+        const f32x4 b = *(const f32x4*)(a.bias[3] + 4 * g);
 #pragma unroll
-    for (int t = 0; t < 4; t++) o[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NT; t++) o[t] = b;
+    }
     const bf16x8* wf3 = (const bf16x8*)a.wf[3];
 #pragma unroll
     for (int s = 0; s < 8; s++) {
-        const bf16x8 A = wf3[s * 64 + lane];
+        const bf16x8 A3 = wf3[s * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < 4; t++) o[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[s][t], o[t], 0, 0, 0);
+        for (int t = 0; t < NT; t++) o[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3, B[s][t], o[t], 0, 0, 0);
     }
     // outputs 4 g + i of parcel base + 16 t + c: de-normalise, exp
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < NT; t++) {
         const int64_t q = base + 16 * t + c;
         if (q >= P) continue;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int oi = 4 * g + i;
             if (oi >= 9) continue;
-            const float yn = o[t][i] + a.bias[3][oi];
-            const float y = yn * a.norm[81 + oi] + a.norm[72 + oi];
-            const double v = exp((double)y);
+            const float y = o[t][i] * a.norm[81 + oi] + a.norm[72 + oi];
+            const double v = (double)expf(y);
             if (oi < 6) a.Eij[q * 6 + oi] = v;
             else a.Vi[q * 3 + oi - 6] = v;
         }
@@ -334,7 +400,8 @@ __global__ void __launch_bounds__(256, 1) surrogate_infer_kernel(SurrogateInferA
 
 hipError_t launch_surrogate_infer(const SurrogateInferArgs& a, hipStream_t s) {
     if (a.fa.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(surrogate_infer_kernel, dim3(nb(a.fa.P, 256)), dim3(256), 0, s, a);
+    if (!a.gtab) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(surrogate_infer_kernel, dim3(nb(a.fa.P, 4 * INFER_WP)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -347,12 +414,6 @@ hipError_t launch_features(const SurrogateFeatArgs& a, hipStream_t s) {
 hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float* T, hipStream_t s) {
     if (P == 0) return hipSuccess;
     hipLaunchKernelGGL(targets_kernel, dim3(nb(P, 256)), dim3(256), 0, s, P, Eij, Vi, T);
-    return hipGetLastError();
-}
-
-hipError_t launch_select(int64_t P, const int* flags, int* idx, int* count, hipStream_t s) {
-    if (P == 0) return hipSuccess;
-    hipLaunchKernelGGL(select_kernel, dim3(nb(P, 256)), dim3(256), 0, s, P, flags, idx, count);
     return hipGetLastError();
 }
 
