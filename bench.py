@@ -98,7 +98,7 @@ def cpu_baseline(target_s=10.0):
 
 
 def bench_uneven(a):
-    """C4: 2^18 parcels, L ~ U{4, 8, 12, 20}, steps ~ U[50, 500], traceless
+    """Uneven batch: 2^18 parcels, L ~ U{4, 8, 12, 20}, steps ~ U[50, 500], traceless
     N(0, 1e-3) gradients, Gamma0 ~ 10^U(-4,-2), Lambda ~ 10^U(-4,-3),
     LROT+REG+DDRX+CDRX, RK4 dt = 0.5, whole time loop as one graph launch."""
     import numpy as np
@@ -143,7 +143,7 @@ def bench_uneven(a):
 
 
 def bench_surrogate(a):
-    """C5 in-loop homogenization on the 2^20-parcel L = 12 field after 200 RK4
+    """In-loop homogenization of the 2^20-parcel L = 12 field after 200 RK4
     steps: plain FP64 Eij (Cartesian) + Vi (z) vs the trained network with
     envelope fallback and a 1 % audit."""
     import torch
@@ -183,6 +183,23 @@ def bench_surrogate(a):
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(kernel, P):
+    """HBM bytes of one launch of `kernel` on P parcels: rocprofv3 FETCH_SIZE
+    and WRITE_SIZE of the bench command, scaled by a 1 GiB copy (x 2048 and
+    x 1024 bytes per KiB counted), per parcel, times P; from the committed
+    summary profiles/evolve_traffic.json (null for a kernel not measured)."""
+    path = os.path.join(ROOT, "profiles", "evolve_traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    ent = tab.get(kernel)
+    if not ent:
+        return None
+    return ent["bytes_per_parcel"] * P
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,8 +216,8 @@ def main():
                     help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the parcel sharding across GPUs is described in "
-              "DESIGN.md and not run here (--gpus %d requested)" % a.gpus, file=sys.stderr)
+        print("bench.py: measures one GPU; parcels shard across GPUs without communication, but the "
+              "multi-GPU run is not part of this benchmark (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -229,7 +246,9 @@ def main():
     fb.x, fb.ugrad, fb.tau, fb.gam0, fb.lam, fb.steps = x, G, None, gam0, None, None
     fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
     procs = ("LROT", "REG", "DDRX")
-    variant, variant_ms = fb.autotune(a.precision)
+    # autotune on the full field, so that every launch of the chosen kernel
+    # (also in a rocprofv3 trace of this command) has the timed size
+    variant, variant_ms = fb.autotune(a.precision, P)
     for _ in range(a.warmup):
         fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
     torch.cuda.synchronize()
@@ -253,6 +272,8 @@ def main():
     flops = flops_per_parcel_step(L) * P
     achieved = flops / (launch_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[a.precision]
+    kname = "evolve_fast_L%d_%s_v%d" % (L, "f64" if a.precision == "fp64" else "f32", variant)
+    traffic = measured_traffic(kname, P)
     # Eij of the evolved field (eigenframe, ice linear Sachs/Taylor mix)
     E0, E1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     fb.Eij((1.0, 1e3), 0.0125, 1)
@@ -289,13 +310,12 @@ def main():
                                % (P, L, fb.n), "parcels": P, "L": L, "scheme": "rk4", "dt": 0.5,
                    "procs": list(procs), "full_run_steps": 200, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
                      "compute": "%s vector ALU (sparse FMA; MI355X %s peak %.1f TFLOP/s)"
                                 % ("fp64" if a.precision == "fp64" else "fp32", a.precision, peak),
-                     "kernel": "evolve_fast_L%d_%s_v%d (one RK4 step of the field)"
-                               % (L, "f64" if a.precision == "fp64" else "f32", variant),
+                     "kernel": "%s (one RK4 step of the field)" % kname,
                      "flop_per_parcel_step": flops_per_parcel_step(L), "launch_ms": launch_ms},
-        "autotune": {"variant": variant, "ms_per_step_by_variant": variant_ms, "probe_parcels": min(P, 1 << 18)},
+        "autotune": {"variant": variant, "ms_per_step_by_variant": variant_ms, "probe_parcels": P},
         "eij_evals_per_s": eij_rate,
         "vi_elastic_evals_per_s": vi_rate,
         "nonfinite_parcels": nonfinite,

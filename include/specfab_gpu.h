@@ -14,7 +14,7 @@
  *     (reference: src/header.f90:23-33).  Batched nlm arrays are (P, n),
  *     parcel-major, i.e. the layout of specfabpy's *_arr arguments.
  *   - 3x3 tensors are row-major: A[3*i + j] = A_ij.  Batched tensors are
- *     (P, 3, 3).  (Fortran callers pass transpose(A); see INTEGRATION.md.)
+ *     (P, 3, 3).  (Fortran callers pass transpose(A), as the specfab_gpu module does.)
  *   - n x n operators are row-major: M[i*n + j], so d nlm_i/dt = M_ij nlm_j.
  *   - Pointers named *_dev are device (HBM) pointers; all others are host.
  *   - `stream` is a hipStream_t (NULL = HIP's default stream).  Batched calls

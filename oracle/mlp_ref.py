@@ -8,7 +8,7 @@ weights, activations, output gradients), so the remaining difference to the
 GPU is FP32 accumulation order; with `bf16=False` the math is exact FP64 and
 is pinned by central finite differences in tests/test_mlp_oracle.py.
 
-The network has no counterpart in the reference; its judge is the plain
+The network has no counterpart in the reference; its reference is the plain
 homogenization (oracle/sf_oracle.c, checked against the reference).
 """
 import math

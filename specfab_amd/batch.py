@@ -181,7 +181,7 @@ class FabricBatch:
 
 
 class UnevenBatch:
-    """Parcels of different truncations L and step counts (SURVEY config C4),
+    """Parcels of different truncations L and step counts,
     stepped through their whole time loop by one graph launch.
 
     Ls: (P,) truncations; nlm: sequence of P complex arrays (length
