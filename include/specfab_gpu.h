@@ -143,6 +143,11 @@ typedef struct sfg_evolve_desc {
                                  decrement it (remaining steps) */
     double* x_dev;            /* state, real SoA (n, P), FP64 */
     int32_t* status_dev;      /* (P) or NULL */
+    /* regularization law (SFG_REG): reg_expo <= 0 (zero-initialised
+     * descriptors) keeps the calibrated M_REG of L (src/include/regcalib.f90);
+     * reg_expo > 0 uses -reg_nu |D| (l(l+1)/(L(L+1)))^reg_expo instead, the
+     * custom law of src/specfabpy/integrator.py:85-97 (any L) */
+    double reg_nu, reg_expo;
 } sfg_evolve_desc;
 int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* desc, void* stream);
 
@@ -183,6 +188,30 @@ int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], doubl
  * (P, 81). */
 int sfg_a4(sfg_ctx* ctx, int L, const double* nlm, double a4[81]);
 int sfg_a4_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4_dev, void* stream);
+
+/* calibrated regularization of M_REG at L (src/include/regcalib.f90): expo, nu */
+int sfg_reg_calibration(int L, double* expo, double* nu);
+
+/* ---- initial states, DDRX rate factor, power-spectrum bounds ---- */
+/* a2_to_nlm (src/moments.f90:68-74): nlm[12] = the 6 complex coefficients
+ * l <= 2 in specfab order (the reference's REAL(4) literals reproduced) */
+int sfg_a2_to_nlm(sfg_ctx* ctx, const double a2[9], double* nlm);
+/* a4_to_nlm (src/moments.f90:76-84): nlm[30] = the 15 complex coefficients l <= 4 */
+int sfg_a4_to_nlm(sfg_ctx* ctx, const double a4[81], double* nlm);
+/* batched: a2 (P,3,3) resp. a4 (P,3,3,3,3) -> states x_dev (n, P) at
+ * truncation L (rows above l = 2 resp. l = 4 zeroed; a4 needs L >= 4) */
+int sfg_a2_to_state_batch(sfg_ctx* ctx, int L, int64_t P, const double* a2_dev, double* x_dev, void* stream);
+int sfg_a4_to_state_batch(sfg_ctx* ctx, int L, int64_t P, const double* a4_dev, double* x_dev, void* stream);
+/* Gamma0(eps, T, A, Q) = A sqrt(tr(eps.eps)/2) exp(-Q/(R T)) (src/dynamics.f90:451-472) */
+int sfg_Gamma0(sfg_ctx* ctx, const double eps[9], double T, double A, double Q, double* G);
+/* batched: eps_dev (P,3,3), temperatures T_dev (P) or the constant T (T_dev NULL) -> G_dev (P) */
+int sfg_Gamma0_batch(sfg_ctx* ctx, int64_t P, const double* eps_dev, const double* T_dev, double T, double A,
+                     double Q, double* G_dev, void* stream);
+/* apply_bounds (src/dynamics.f90:530-557): rescale the l = 2 and l = 4
+ * blocks whose power S(l) exceeds S(0); single parcel (nlm in, out: n
+ * complex) and in place on batched states */
+int sfg_apply_bounds(sfg_ctx* ctx, int L, const double* nlm, double* out);
+int sfg_apply_bounds_batch(sfg_ctx* ctx, int L, int64_t P, double* x_dev, void* stream);
 
 /* Cij_to_Lame_tranisotropic (src/elasticities.f90:107-120): (C11, C33, C55,
  * C12, C13) -> (lam, mu, Elam, Emu, Egam).  Host only. */

@@ -22,7 +22,7 @@ OPENBLAS_SO  ?= $(firstword $(wildcard $(OPENBLAS_DIR)/libscipy_openblas*.so))
 
 MODS = header tensorproducts mandel reducedform gaunt lambdasolver rheologies \
        elasticities moments frames rotation idealstate dynamics homogenizations \
-       enhancementfactors wavepropagation
+       enhancementfactors wavepropagation deformationmodes
 M77  = amach derv1 dnqsol dnrm2 erfin ermsg ierm1 ierv1
 
 MODOBJS = $(addprefix $(OBJ)/,$(addsuffix .o,$(MODS)))
@@ -62,11 +62,12 @@ $(eval $(call MODRULE,idealstate,dynamics))
 $(eval $(call MODRULE,dynamics,homogenizations))
 $(eval $(call MODRULE,homogenizations,enhancementfactors))
 $(eval $(call MODRULE,enhancementfactors,wavepropagation))
+$(eval $(call MODRULE,wavepropagation,deformationmodes))
 
 $(OBJ)/m77_%.o: $(REFSRC)/include/math77/%.f | $(OBJ)
 	$(FC) $(FFLAGS) -c $< -o $@
 
-$(OBJ)/ref_driver.o: oracle/ref_driver.f90 $(OBJ)/wavepropagation.o
+$(OBJ)/ref_driver.o: oracle/ref_driver.f90 $(OBJ)/deformationmodes.o
 	$(FC) $(FFLAGS) -J$(OBJ) -I$(OBJ) -c $< -o $@
 
 $(OUT)/libsfref.so: $(OBJ)/ref_driver.o $(MODOBJS) $(M77OBJS)

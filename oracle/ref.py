@@ -52,6 +52,19 @@ class RefLib:
         L.sfref_rotate_nlm4.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
         L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                    _c.c_double, _dp, _dp, _dp, _cp]
+        L.sfref_a2_to_nlm.argtypes = [_dp, _cp]
+        L.sfref_a4_to_nlm.argtypes = [_dp, _cp]
+        L.sfref_Gamma0.argtypes = [_dp, _c.c_double, _c.c_double, _c.c_double]
+        L.sfref_Gamma0.restype = _c.c_double
+        L.sfref_apply_bounds.argtypes = [_cp, _cp]
+        L.sfref_pureshear_F.argtypes = [_c.c_int, _c.c_double, _c.c_double, _c.c_double, _dp]
+        L.sfref_pureshear_ugrad.argtypes = [_c.c_int, _c.c_double, _c.c_double, _dp]
+        L.sfref_pureshear_strainii_to_t.argtypes = [_c.c_double, _c.c_double]
+        L.sfref_pureshear_strainii_to_t.restype = _c.c_double
+        L.sfref_simpleshear_F.argtypes = [_c.c_int, _c.c_double, _c.c_double, _dp]
+        L.sfref_simpleshear_ugrad.argtypes = [_c.c_int, _c.c_double, _dp]
+        L.sfref_simpleshear_gamma_to_t.argtypes = [_c.c_double, _c.c_double]
+        L.sfref_simpleshear_gamma_to_t.restype = _c.c_double
         self.L = None
         self.n = 0
 
@@ -181,3 +194,84 @@ class RefLib:
         self.lib.sfref_evolve(P, int(nsteps), 0 if scheme == "euler" else 1, mask, float(dt), float(iota),
                               float(zeta), U, g, lm, X)
         return np.ascontiguousarray(X.T)
+
+    # -- initial states, rate factor, bounds (src/moments.f90:68-84,
+    #    src/dynamics.f90:451-472, 530-557) --
+    def a2_to_nlm(self, a2):
+        out = np.zeros(6, dtype=np.complex128, order="F")
+        self.lib.sfref_a2_to_nlm(self._m3(a2), out)
+        return out.copy()
+
+    def a4_to_nlm(self, a4):
+        out = np.zeros(15, dtype=np.complex128, order="F")
+        self.lib.sfref_a4_to_nlm(np.asfortranarray(np.asarray(a4, dtype=np.float64).reshape(3, 3, 3, 3)), out)
+        return out.copy()
+
+    def Gamma0(self, eps, T, A, Q):
+        return self.lib.sfref_Gamma0(self._m3(eps), float(T), float(A), float(Q))
+
+    def apply_bounds(self, nlm):
+        out = np.zeros(self.n, dtype=np.complex128, order="F")
+        self.lib.sfref_apply_bounds(self._v(nlm, np.complex128), out)
+        return out.copy()
+
+    # -- kinematics (src/deformationmodes.f90) --
+    def pureshear_F(self, ax, q, tau, t):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_pureshear_F(int(ax), float(q), float(tau), float(t), out)
+        return np.ascontiguousarray(out)
+
+    def pureshear_ugrad(self, ax, q, tau):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_pureshear_ugrad(int(ax), float(q), float(tau), out)
+        return np.ascontiguousarray(out)
+
+    def pureshear_strainii_to_t(self, s, tau):
+        return self.lib.sfref_pureshear_strainii_to_t(float(s), float(tau))
+
+    def simpleshear_F(self, plane, tau, t):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_simpleshear_F(int(plane), float(tau), float(t), out)
+        return np.ascontiguousarray(out)
+
+    def simpleshear_ugrad(self, plane, tau):
+        out = np.zeros((3, 3), order="F")
+        self.lib.sfref_simpleshear_ugrad(int(plane), float(tau), out)
+        return np.ascontiguousarray(out)
+
+    def simpleshear_gamma_to_t(self, gam, tau):
+        return self.lib.sfref_simpleshear_gamma_to_t(float(gam), float(tau))
+
+
+def lagrangianparcel(R, ugrad, dt, Nt, nlm0=None, iota=None, zeta=0, Lambda=None, Gamma0=None, nu=None,
+                     regexpo=None):
+    """The reference's Lagrangian-parcel Euler loop (src/specfabpy/integrator.py:
+    40-82 and M_REG_custom :85-97) restated over the reference's own operators
+    (R: a RefLib initialised at the truncation L), with its dtypes: steady
+    operators complex128, trajectory stored in complex64.  Returns nlm
+    (Nt+1, n) complex64."""
+    ugrad = np.asarray(ugrad, dtype=np.float64)
+    D, W = (ugrad + ugrad.T) / 2, (ugrad - ugrad.T) / 2
+    S = D.copy()
+    n, L = R.n, R.L
+    nlm = np.zeros((Nt + 1, n), dtype=np.complex64)
+    if nlm0 is None:
+        nlm[0, 0] = 1 / np.sqrt(4 * np.pi)
+    else:
+        nlm[0, :] = nlm0
+    M_zero = np.zeros((n, n), dtype=np.complex64)
+    if regexpo is None:
+        M_REG = nu * R.M_REG(D) if nu is not None else M_zero
+    else:
+        ls = np.concatenate([np.full(2 * l + 1, l) for l in range(0, L + 1, 2)])
+        Lm = np.diag(-(ls * (ls + 1)).astype(np.float64)) / (L * (L + 1))
+        M_REG = -(nu * np.linalg.norm(D)) * np.power(np.abs(Lm), regexpo)
+    M_CDRX = Lambda * R.M_CDRX() if Lambda is not None else M_zero
+    M_LROT = R.M_LROT(D, W, iota, zeta) if iota is not None else M_zero
+    Gamma0_ = Gamma0(ugrad=ugrad) if callable(Gamma0) else Gamma0
+    for nt in range(1, Nt + 1):
+        nlm_prev = nlm[nt - 1, :]
+        M = M_LROT + M_CDRX + M_REG
+        M = M + (Gamma0_ * R.M_DDRX(nlm_prev, S) if Gamma0_ is not None else M_zero)
+        nlm[nt, :] = nlm_prev + dt * np.matmul(M, nlm_prev)
+    return nlm

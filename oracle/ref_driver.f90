@@ -25,6 +25,7 @@ module sfref_driver
     use elasticities
     use wavepropagation
     use rotation
+    use deformationmodes
 
     implicit none
 
@@ -105,6 +106,73 @@ contains
         real(c_double), intent(out) :: a2v(6), a4v(6,6)
         call f_ev_ck_Mandel(nlm, a2v, a4v)
     end subroutine
+
+    subroutine sfref_a2_to_nlm(a2in, nlm) bind(C, name="sfref_a2_to_nlm")
+        real(c_double), intent(in) :: a2in(3,3)
+        complex(c_double_complex), intent(out) :: nlm(6)
+        nlm = a2_to_nlm(a2in)
+    end subroutine
+
+    subroutine sfref_a4_to_nlm(a4in, nlm) bind(C, name="sfref_a4_to_nlm")
+        real(c_double), intent(in) :: a4in(3,3,3,3)
+        complex(c_double_complex), intent(out) :: nlm(15)
+        nlm = a4_to_nlm(a4in)
+    end subroutine
+
+    ! ---- rate factor, bounds, kinematics ----
+
+    function sfref_Gamma0(eps, T, A, Q) bind(C, name="sfref_Gamma0") result(G)
+        real(c_double), intent(in) :: eps(3,3)
+        real(c_double), value :: T, A, Q
+        real(c_double) :: G
+        G = Gamma0(eps, T, A, Q)
+    end function
+
+    subroutine sfref_apply_bounds(nlm, out) bind(C, name="sfref_apply_bounds")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        complex(c_double_complex), intent(out) :: out(ncur)
+        out = apply_bounds(nlm)
+    end subroutine
+
+    subroutine sfref_pureshear_F(ax, q, tau, t, F) bind(C, name="sfref_pureshear_F")
+        integer(c_int), value :: ax
+        real(c_double), value :: q, tau, t
+        real(c_double), intent(out) :: F(3,3)
+        F = pureshear_F(ax, q, tau, t)
+    end subroutine
+
+    subroutine sfref_pureshear_ugrad(ax, q, tau, ugrad) bind(C, name="sfref_pureshear_ugrad")
+        integer(c_int), value :: ax
+        real(c_double), value :: q, tau
+        real(c_double), intent(out) :: ugrad(3,3)
+        ugrad = pureshear_ugrad(ax, q, tau)
+    end subroutine
+
+    function sfref_pureshear_strainii_to_t(s, tau) bind(C, name="sfref_pureshear_strainii_to_t") result(t)
+        real(c_double), value :: s, tau
+        real(c_double) :: t
+        t = pureshear_strainii_to_t(s, tau)
+    end function
+
+    subroutine sfref_simpleshear_F(plane, tau, t, F) bind(C, name="sfref_simpleshear_F")
+        integer(c_int), value :: plane
+        real(c_double), value :: tau, t
+        real(c_double), intent(out) :: F(3,3)
+        F = simpleshear_F(plane, tau, t)
+    end subroutine
+
+    subroutine sfref_simpleshear_ugrad(plane, tau, ugrad) bind(C, name="sfref_simpleshear_ugrad")
+        integer(c_int), value :: plane
+        real(c_double), value :: tau
+        real(c_double), intent(out) :: ugrad(3,3)
+        ugrad = simpleshear_ugrad(plane, tau)
+    end subroutine
+
+    function sfref_simpleshear_gamma_to_t(gam, tau) bind(C, name="sfref_simpleshear_gamma_to_t") result(t)
+        real(c_double), value :: gam, tau
+        real(c_double) :: t
+        t = simpleshear_gamma_to_t(gam, tau)
+    end function
 
     ! ---- homogenization ----
 

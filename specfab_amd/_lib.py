@@ -26,7 +26,8 @@ class EvolveDesc(_c.Structure):
                 ("precision", _c.c_int), ("nsteps", _c.c_int), ("dt", _c.c_double), ("iota", _c.c_double),
                 ("zeta", _c.c_double), ("ugrad_dev", _c.c_void_p), ("tau_dev", _c.c_void_p),
                 ("gam0_dev", _c.c_void_p), ("lam_dev", _c.c_void_p), ("steps_dev", _c.c_void_p),
-                ("x_dev", _c.c_void_p), ("status_dev", _c.c_void_p)]
+                ("x_dev", _c.c_void_p), ("status_dev", _c.c_void_p), ("reg_nu", _c.c_double),
+                ("reg_expo", _c.c_double)]
 
 
 class HomogParams(_c.Structure):
@@ -70,6 +71,15 @@ SIGNATURES = {
     "sfg_ev_D2": (_i, [_vp, _i, _dp, _dp, _dp]),
     "sfg_a4": (_i, [_vp, _i, _dp, _dp]),
     "sfg_a4_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_reg_calibration": (_i, [_i, _dp, _dp]),
+    "sfg_a2_to_nlm": (_i, [_vp, _dp, _dp]),
+    "sfg_a4_to_nlm": (_i, [_vp, _dp, _dp]),
+    "sfg_a2_to_state_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_a4_to_state_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_Gamma0": (_i, [_vp, _dp, _c.c_double, _c.c_double, _c.c_double, _dp]),
+    "sfg_Gamma0_batch": (_i, [_vp, _i64, _vp, _vp, _c.c_double, _c.c_double, _c.c_double, _vp, _vp]),
+    "sfg_apply_bounds": (_i, [_vp, _i, _dp, _dp]),
+    "sfg_apply_bounds_batch": (_i, [_vp, _i, _i64, _vp, _vp]),
     "sfg_Cij_to_Lame_tranisotropic": (_i, [_dp, _dp]),
     "sfg_Vi_elastic_tranisotropic": (_i, [_vp, _i, _dp, _d, _dp, _d, _i, _dp, _dp, _dp]),
     "sfg_Vi_elastic_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _d, _dp, _d, _i, _dp, _dp, _vp, _vp, _vp]),

@@ -28,6 +28,20 @@ def nlm_len(L):
     return (L + 1) * (L + 2) // 2
 
 
+def gamma0(eps, T, A, Q, ctx=None):
+    """DDRX rate factors (P,) A sqrt(tr(eps.eps)/2) exp(-Q/(R T)) of strain
+    rates eps (P, 3, 3) (device tensor) and temperatures T ((P,) tensor or a
+    number), src/dynamics.f90:451-472."""
+    ctx = ctx or _lib.default_context()
+    E = eps.to(torch.float64).contiguous()
+    P = E.shape[0]
+    G = torch.empty(P, dtype=torch.float64, device=E.device)
+    Tt = T.to(E.device, torch.float64).contiguous() if torch.is_tensor(T) else None
+    check(lib().sfg_Gamma0_batch(ctx.handle, P, _ptr(E), _ptr(Tt), float(0.0 if Tt is not None else T), float(A),
+                                 float(Q), _ptr(G), _stream()))
+    return G
+
+
 class FabricBatch:
     """P parcels at truncation L with per-parcel kinematics.
 
@@ -80,7 +94,11 @@ class FabricBatch:
         return out
 
     # ---- evolution ----
-    def evolve(self, nsteps, dt, scheme="rk4", procs=("LROT", "REG"), iota=1.0, zeta=0.0, precision="fp64"):
+    def evolve(self, nsteps, dt, scheme="rk4", procs=("LROT", "REG"), iota=1.0, zeta=0.0, precision="fp64",
+               reg=None):
+        """Advance every parcel nsteps (or its own step count) steps.  reg =
+        (nu, expo) replaces the calibrated regularization of L by
+        -nu |D| (l(l+1) / (L(L+1)))^expo."""
         if self.ugrad is None:
             raise ValueError("ugrad is required")
         mask = 0
@@ -103,7 +121,30 @@ class FabricBatch:
         d.steps_dev = self.steps.data_ptr() if self.steps is not None else None
         d.x_dev = self.x.data_ptr()
         d.status_dev = self.status.data_ptr()
+        if reg is not None:
+            d.reg_nu, d.reg_expo = float(reg[0]), float(reg[1])
         check(lib().sfg_evolve(self.ctx.handle, _c.byref(d), _stream()))
+        return self
+
+    # ---- initial states and bounds ----
+    def set_a2(self, a2):
+        """States whose l <= 2 part has the structure tensors a2 (P, 3, 3)
+        (a2_to_nlm, src/moments.f90:68-74); higher coefficients zeroed."""
+        A = self._f64(a2, (self.P, 3, 3))
+        check(lib().sfg_a2_to_state_batch(self.ctx.handle, self.L, self.P, _ptr(A), _ptr(self.x), _stream()))
+        return self
+
+    def set_a4(self, a4):
+        """States whose l <= 4 part has the structure tensors a4 (P, 3, 3, 3, 3)
+        (a4_to_nlm, src/moments.f90:76-84); higher coefficients zeroed."""
+        A = self._f64(a4, (self.P, 3, 3, 3, 3))
+        check(lib().sfg_a4_to_state_batch(self.ctx.handle, self.L, self.P, _ptr(A), _ptr(self.x), _stream()))
+        return self
+
+    def apply_bounds(self):
+        """In place: l = 2, 4 power spectra bounded by a delta function's
+        (apply_bounds, src/dynamics.f90:530-557)."""
+        check(lib().sfg_apply_bounds_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _stream()))
         return self
 
     def autotune(self, precision="fp64", P=None):

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "sfg_kernels.h"
 #include "sfg_fast.h"
 #include "sfg_homog.h"
+#include "sfg_init.h"
 #include "sfg_wave.h"
 #include "sfg_mlp_net.h"
 #include "sfg_surrogate.h"
@@ -68,6 +70,7 @@ struct DevOp {
     double* A4 = nullptr;
     double* A4F = nullptr;
     double* reg_shape = nullptr;
+    std::vector<double> reg_shape_h;
     int* r2c = nullptr;
     int* rpart = nullptr;
     int *cre = nullptr, *cim = nullptr, *csgn = nullptr, *cneg = nullptr;
@@ -120,6 +123,7 @@ int get_op(sfg_ctx* ctx, int L, const DevOp** out) {
     d.nblk = L / 2 + 1;
     d.nq = std::min(op.n, 15);
     d.reg_nu = op.reg_nu;
+    d.reg_shape_h = op.reg_shape;
     d.reg_defined = op.reg_defined;
     std::vector<int2> rows;
     std::vector<int> row_blk, gparam, tcol, r2c, rpart;
@@ -373,6 +377,20 @@ int sfg_state_to_nlm(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double
 
 namespace {
 
+// regularization law of a launch: the calibrated M_REG of L
+// (src/include/regcalib.f90), or with reg_expo > 0 the law
+// -reg_nu |D| (l(l+1) / (L(L+1)))^reg_expo (src/specfabpy/integrator.py:85-97)
+void reg_law(const DevOp* op, const sfg_evolve_desc* d, sfg::ParamArgs& pa) {
+    const bool custom = d->reg_expo > 0;
+    pa.reg_nu = custom ? d->reg_nu : op->reg_nu;
+    const int L = op->L;
+    for (int b = 0; b < op->nblk && b < 11; b++) {
+        const int l = 2 * b;
+        const double Ld = -(double)(l * (l + 1));
+        pa.reg_shape[b] = custom ? std::pow(std::fabs(Ld / (L * (L + 1))), d->reg_expo) : op->reg_shape_h[b];
+    }
+}
+
 int variant_of(sfg_ctx* ctx, int L, int prec) {
     const int nv = sfg::fast_variants(L, prec);
     if (ctx->variant_env >= 0 && ctx->variant_env < nv) return ctx->variant_env;
@@ -386,7 +404,7 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
     const DevOp* op;
     int rc = get_op(ctx, d->L, &op);
     if (rc) return rc;
-    if ((d->procs & SFG_REG) && !op->reg_defined)
+    if ((d->procs & SFG_REG) && !op->reg_defined && !(d->reg_expo > 0))
         return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
     if (d->P == 0) return SFG_OK;
     HIPCHK(hipSetDevice(ctx->device));
@@ -404,9 +422,10 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
 
     sfg::ParamArgs pa{};
     pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
-    pa.iota = d->iota; pa.zeta = d->zeta; pa.reg_nu = op->reg_nu;
+    pa.iota = d->iota; pa.zeta = d->zeta;
+    reg_law(op, d, pa);
     pa.ugrad = d->ugrad_dev; pa.tau = d->tau_dev; pa.gam0 = d->gam0_dev; pa.lam = d->lam_dev;
-    pa.A2 = op->A2; pa.A4 = op->A4; pa.reg_shape = op->reg_shape;
+    pa.A2 = op->A2; pa.A4 = op->A4;
     pa.theta = theta; pa.omega = omega; pa.sl = sl;
     HIPCHK(sfg::launch_params(pa, s));
 
@@ -650,6 +669,109 @@ static int single_state_eval(sfg_ctx* ctx, int L, const double* nlm, const doubl
     return SFG_OK;
 }
 
+// ---------------------------------------------------------------- initial states, Gamma0, bounds
+
+int sfg_reg_calibration(int L, double* expo, double* nu) {
+    if (!expo || !nu) return fail(SFG_ERR_ARG, "NULL argument");
+    if (L < 4 || L > sfg::LMAX_TABLE || (L & 1))
+        return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
+    const sfg::RealOperator& op = sfg::real_operator(L);
+    if (!op.reg_defined)
+        return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
+    *expo = op.reg_expo;
+    *nu = op.reg_nu;
+    return SFG_OK;
+}
+
+int sfg_a2_to_state_batch(sfg_ctx* ctx, int L, int64_t P, const double* a2_dev, double* x_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!a2_dev || !x_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_a2_to_state(P, op->n, a2_dev, x_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_a4_to_state_batch(sfg_ctx* ctx, int L, int64_t P, const double* a4_dev, double* x_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!a4_dev || !x_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (L < 4) return fail(SFG_ERR_L, "a4_to_nlm fills l <= 4: needs L >= 4");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_a4_to_state(P, op->n, a4_dev, x_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+namespace {
+// one tensor -> complex nlm (first ncoef coefficients, specfab order) through
+// the batched kernel and the L = 4 state conversion
+int tensor_to_nlm(sfg_ctx* ctx, const double* t, int tlen, int ncoef, double* nlm) {
+    if (!ctx || !t || !nlm) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, 4, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(tlen + n + 2 * n) * sizeof(double)));
+    double* dt = (double*)buf.p;
+    double* dx = dt + tlen;
+    double* dn = dx + n;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dt, t, tlen * sizeof(double), hipMemcpyHostToDevice, s));
+    if (tlen == 9) HIPCHK(sfg::launch_a2_to_state(1, n, dt, dx, s));
+    else HIPCHK(sfg::launch_a4_to_state(1, n, dt, dx, s));
+    HIPCHK(sfg::launch_state_to_nlm(1, n, op->cre, op->cim, op->csgn, op->cneg, dx, dn, s));
+    HIPCHK(hipMemcpyAsync(nlm, dn, 2 * ncoef * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+}  // namespace
+
+int sfg_a2_to_nlm(sfg_ctx* ctx, const double a2[9], double* nlm) { return tensor_to_nlm(ctx, a2, 9, 6, nlm); }
+
+int sfg_a4_to_nlm(sfg_ctx* ctx, const double a4[81], double* nlm) { return tensor_to_nlm(ctx, a4, 81, 15, nlm); }
+
+int sfg_Gamma0_batch(sfg_ctx* ctx, int64_t P, const double* eps_dev, const double* T_dev, double T, double A,
+                     double Q, double* G_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!eps_dev || !G_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_gamma0(P, eps_dev, T_dev, T, A, Q, G_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_Gamma0(sfg_ctx* ctx, const double eps[9], double T, double A, double Q, double* G) {
+    if (!ctx || !eps || !G) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, 10 * sizeof(double)));
+    double* de = (double*)buf.p;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(de, eps, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_gamma0(1, de, nullptr, T, A, Q, de + 9, s));
+    HIPCHK(hipMemcpyAsync(G, de + 9, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_apply_bounds_batch(sfg_ctx* ctx, int L, int64_t P, double* x_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && !x_dev)) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_bounds(P, op->n, x_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
 int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* D) {
     if (!tau || !D) return fail(SFG_ERR_ARG, "NULL argument");
     if (L < 4) return fail(SFG_ERR_L, "<D> needs L >= 4");
@@ -890,6 +1012,23 @@ int single_parcel(sfg_ctx* ctx, int L, const double* nlm, size_t out_doubles, do
 struct ViElUser { double alpha, rho; const double* Lame; int ndir; const double *th, *ph; };
 struct ViEmUser { double eps_m, eps_t, mu; int ndir; const double *th, *ph; };
 }  // namespace
+
+int sfg_apply_bounds(sfg_ctx* ctx, int L, const double* nlm, double* out) {
+    if (!out) return fail(SFG_ERR_ARG, "NULL argument");
+    const int n = sfg::nlm_len(L);
+    return single_parcel(ctx, L, nlm, (size_t)2 * n, out,
+                         [](sfg_ctx* c, int L_, const double* dx, double* dout, void*) {
+                             const DevOp* o = nullptr;
+                             int r = get_op(c, L_, &o);
+                             if (r) return r;
+                             // dx is the converted input state (scratch of this call)
+                             HIPCHK(sfg::launch_bounds(1, o->n, (double*)dx, c->stream));
+                             HIPCHK(sfg::launch_state_to_nlm(1, o->n, o->cre, o->cim, o->csgn, o->cneg, dx, dout,
+                                                             c->stream));
+                             return SFG_OK;
+                         },
+                         nullptr);
+}
 
 int sfg_a4(sfg_ctx* ctx, int L, const double* nlm, double a4[81]) {
     return single_parcel(ctx, L, nlm, 81, a4,
@@ -1273,9 +1412,10 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         double* scr = sl + nsl;
         sfg::ParamArgs pa{};
         pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
-        pa.iota = d->iota; pa.zeta = d->zeta; pa.reg_nu = op->reg_nu;
+        pa.iota = d->iota; pa.zeta = d->zeta;
+        reg_law(op, d, pa);
         pa.ugrad = d->ugrad_dev; pa.tau = d->tau_dev; pa.gam0 = d->gam0_dev; pa.lam = d->lam_dev;
-        pa.A2 = op->A2; pa.A4 = op->A4; pa.reg_shape = op->reg_shape;
+        pa.A2 = op->A2; pa.A4 = op->A4;
         pa.theta = theta; pa.omega = omega; pa.sl = sl;
         e = sfg::launch_params(pa, s);
         if (e != hipSuccess) return bail(fail(SFG_ERR_HIP, hipGetErrorString(e)));

@@ -74,7 +74,7 @@ struct ParamArgs {
     const double* lam;     // (P) or null
     const double* A2;      // (6,15)
     const double* A4;      // (36,15)
-    const double* reg_shape;  // (nblk)
+    double reg_shape[11];  // (nblk <= 11): (l(l+1) / (L(L+1)))^expo per l block
     double* theta;         // (NPAR_K, P)
     double* omega;         // (15, P)
     double* sl;            // (nblk, P)

@@ -31,6 +31,7 @@ module specfab_gpu
         integer(c_int) :: procs, scheme, precision, nsteps
         real(c_double) :: dt, iota, zeta
         type(c_ptr) :: ugrad_dev, tau_dev, gam0_dev, lam_dev, steps_dev, x_dev, status_dev
+        real(c_double) :: reg_nu, reg_expo
     end type
 
     interface
@@ -266,6 +267,7 @@ contains
         d%nsteps = nsteps; d%dt = dt; d%iota = iota; d%zeta = zeta
         d%ugrad_dev = dG; d%tau_dev = c_null_ptr; d%gam0_dev = dg0; d%lam_dev = dlam
         d%steps_dev = c_null_ptr; d%x_dev = dx; d%status_dev = c_null_ptr
+        d%reg_nu = 0; d%reg_expo = 0
         call check(c_evolve(ctx, d, c_null_ptr), 'sfg_evolve')
         call check(c_state_to_nlm(ctx, int(Lcap, c_int), P, dx, dnlm, c_null_ptr), 'state_to_nlm')
         call check(hipDeviceSynchronize(), 'hipDeviceSynchronize')

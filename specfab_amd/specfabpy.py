@@ -229,3 +229,116 @@ def Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi):
     check(lib().sfg_Vi_electromagnetic_tranisotropic(_ctx(), L, _zp(z), float(eps_m), float(eps_t), float(mu),
                                                      th.size, dptr(th), dptr(ph), dptr(out)))
     return out.T.copy()
+
+
+# ---- initial states, rate factor, bounds ----
+
+def a2_to_nlm(a2):
+    """nlm (6,) of the l <= 2 coefficients with structure tensor a2,
+    src/specfabpy.f90:619-627 -> moments.f90:68-74 (on the GPU)."""
+    out = np.zeros(6, dtype=np.complex128)
+    check(lib().sfg_a2_to_nlm(_ctx(), dptr(_m3(a2)), _zp(out)))
+    return out
+
+
+def a4_to_nlm(a4):
+    """nlm (15,) of the l <= 4 coefficients with structure tensor a4,
+    src/specfabpy.f90:629-637 -> moments.f90:76-84 (on the GPU)."""
+    a = np.ascontiguousarray(np.asarray(a4, dtype=np.float64).reshape(81))
+    out = np.zeros(15, dtype=np.complex128)
+    check(lib().sfg_a4_to_nlm(_ctx(), dptr(a), _zp(out)))
+    return out
+
+
+def Gamma0(eps, T, A, Q):
+    """DDRX rate factor A sqrt(tr(eps.eps)/2) exp(-Q/(R T)),
+    src/specfabpy.f90:1114-1122 -> dynamics.f90:451-472 (on the GPU)."""
+    out = np.zeros(1)
+    check(lib().sfg_Gamma0(_ctx(), dptr(_m3(eps)), float(T), float(A), float(Q), dptr(out)))
+    return float(out[0])
+
+
+def apply_bounds(nlm):
+    """nlm with its l = 2 and l = 4 power spectrum bounded by that of a delta
+    function, src/specfabpy.f90:764-771 -> dynamics.f90:530-557 (on the GPU)."""
+    L = _L_of(nlm)
+    z = _z(nlm)
+    out = np.zeros(len(z), dtype=np.complex128)
+    check(lib().sfg_apply_bounds(_ctx(), L, _zp(z), _zp(out)))
+    return out
+
+
+def _l_of_index(n):
+    L = int(round((-3 + np.sqrt(1 + 8 * n)) / 2))
+    return np.concatenate([np.full(2 * l + 1, l) for l in range(0, L + 1, 2)])
+
+
+def Lmat(nlm):
+    """Laplacian diag(-l(l+1)) (n, n), src/specfabpy.f90:247-258."""
+    ls = _l_of_index(len(nlm))
+    return np.diag(-(ls * (ls + 1)).astype(np.float64))
+
+
+def Sl(nlm, l):
+    """Angular power S(l) = sum_m |n_l^m|^2 / (2l + 1), src/idealstate.f90:80-93."""
+    z = _z(nlm)
+    ls = _l_of_index(len(z))
+    return float(np.sum(np.abs(z[ls == l]) ** 2) / (2 * l + 1))
+
+
+# ---- idealized kinematics (src/deformationmodes.f90) ----
+
+def F_to_strain(F):
+    F = np.asarray(F, dtype=np.float64)
+    return 0.5 * (F + F.T) - np.eye(3)
+
+
+def pureshear_r(tau, time):
+    return np.exp(-time / tau)
+
+
+def pureshear_strainii_to_t(strainii, tau):
+    return -tau * np.log(strainii + 1.0)
+
+
+def pureshear_F(ax, q, tau, time):
+    r = pureshear_r(tau, time)
+    qp, qm = -(1 + q) / 2.0, -(1 - q) / 2.0
+    d = {0: (r, r ** qp, r ** qm), 1: (r ** qm, r, r ** qp), 2: (r ** qp, r ** qm, r)}[int(ax)]
+    return np.diag(d)
+
+
+def pureshear_ugrad(ax, q, tau):
+    qp, qm = -(1 + q) / 2.0, -(1 - q) / 2.0
+    d = {0: (1.0, qp, qm), 1: (qm, 1.0, qp), 2: (qp, qm, 1.0)}[int(ax)]
+    return -1 / tau * np.diag(d)
+
+
+def simpleshear_gamma(tau, time):
+    return np.arctan(time / tau)
+
+
+def simpleshear_gamma_to_t(gam, tau):
+    return tau * np.tan(gam)
+
+
+_SS_IJ = {0: (1, 2), 1: (0, 2), 2: (0, 1)}   # yz, xz, xy shear (du_i/dx_j)
+
+
+def simpleshear_F(plane, tau, time):
+    F = np.eye(3)
+    F[_SS_IJ[int(plane)]] = np.tan(simpleshear_gamma(tau, time))
+    return F
+
+
+def simpleshear_ugrad(plane, tau):
+    G = np.zeros((3, 3))
+    G[_SS_IJ[int(plane)]] = 1 / tau
+    return G
+
+
+def __getattr__(name):
+    # Lcap: the active truncation (src/specfabpy.f90:132,156)
+    if name == "Lcap":
+        return _state["L"]
+    raise AttributeError(name)

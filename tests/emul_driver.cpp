@@ -49,7 +49,7 @@ extern "C" int emul_evolve(int L, int64_t P, int procs, int scheme, int nsteps, 
     sfg::ParamArgs pa{};
     pa.P = P; pa.nblk = nblk; pa.nq = nq; pa.procs = procs; pa.iota = iota; pa.zeta = zeta; pa.reg_nu = op.reg_nu;
     pa.ugrad = ugrad; pa.tau = nullptr; pa.gam0 = gam0; pa.lam = lam;
-    pa.A2 = A2.data(); pa.A4 = A4.data(); pa.reg_shape = op.reg_shape.data();
+    pa.A2 = A2.data(); pa.A4 = A4.data(); for (size_t b = 0; b < op.reg_shape.size() && b < 11; b++) pa.reg_shape[b] = op.reg_shape[b];
     pa.theta = theta.data(); pa.omega = omega.data(); pa.sl = sl.data();
     grid(P, 256, [&] { sfg::params_kernel(pa); });
     sfg::EvolveArgs ea{};
