@@ -39,7 +39,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
-FAST_L = {"f64": (4, 6, 8, 10, 12), "f32": (4, 6, 8, 10, 12, 14, 16)}
+FAST_L = {"f64": (4, 6, 8, 10, 12, 14, 16, 18, 20), "f32": (4, 6, 8, 10, 12, 14, 16, 18, 20)}
 
 # Kernel variants (the library's autotuner times them and keeps the fastest):
 #   mode    "lit": coupling values as literal constants in the code;
@@ -62,6 +62,8 @@ V = lambda mode, chunk, waves, bfree, fence, block=64, sync=False: dict(
 #           (else one stage loop that tests the stage kind per row chunk)
 R = lambda chunk, waves, xn, split=True, block=64: dict(mode="reg", chunk=chunk, waves=waves, xn=xn,
                                                       split=split, block=block)
+#   window  ("win" mode) stage-input columns held in registers per row chunk
+Wn = lambda window, chunk, waves, block=64: dict(mode="win", window=window, chunk=chunk, waves=waves, block=block)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
                        V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False)],
@@ -69,7 +71,10 @@ VARIANTS = {
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg")],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
+    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2)],
+    ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2)],
 }
+WIDE_L = {"f64": (14, 16, 18, 20), "f32": (18, 20)}
 TUNED_L = (8, 12)
 
 
@@ -465,6 +470,174 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
     return "\n".join(o) + "\n"
 
 
+def gen_kernel_win(L, prec, n, rows, cfg, vid):
+    """Window stepper ("win" mode) for states too large for a lane's
+    registers (L >= 14 FP64, L >= 18 FP32).
+
+    Rows are taken in order and grouped greedily into chunks whose coupled
+    columns (the union of the columns their terms read, plus the rows
+    themselves) number at most cfg["window"]; per chunk those columns of the
+    stage input are loaded into registers once (the operator is banded:
+    |dl| <= 4, |dm| <= 4), then the chunk's rows are evaluated in sub-chunks
+    of cfg["chunk"] rows with the coupling values as literals.  Stage buffers
+    (YS increment sum, XA / XB stage inputs) go through HBM; launch constants
+    are staged in LDS.  Stage input / output: RK4 X -> XA -> XB -> XA -> X;
+    Euler X -> XB (FP64) -> X.
+    """
+    T = "double" if prec == "f64" else "float"
+    LDT = "bld64" if prec == "f64" else "bld32"
+    STT = "bst64" if prec == "f64" else "bst32"
+    W = 8 if prec == "f64" else 4
+    cv = (lambda e: e) if prec == "f64" else (lambda e: "(float)(%s)" % e)
+    name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
+    WIN, CHUNK, B = cfg["window"], cfg["chunk"], cfg["block"]
+    used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
+    NPU = len(used_p)
+    nq = min(n, 15)
+    nblk = L // 2 + 1
+    OMC, SLC = NPU, NPU + nq
+    pidx = {q: i for i, q in enumerate(used_p)}
+    need = [{col for (_, tl) in rows[r][1] for (col, _, _) in tl} | {r} for r in range(n)]
+    chunks, cur, cols = [], [], set()
+    for r in range(n):
+        if cur and len(cols | need[r]) > WIN:
+            chunks.append((cur, sorted(cols)))
+            cur, cols = [], set()
+        cur.append(r)
+        cols |= need[r]
+    chunks.append((cur, sorted(cols)))
+    o = []
+    w = o.append
+    w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
+    w("// Window stepper, L = %d (n = %d), %s; variant %d: %s; %d row chunks, %d column loads per stage." %
+      (L, n, "FP64" if prec == "f64" else "FP32 stepping, FP64 state", vid,
+       ", ".join("%s=%s" % kv for kv in sorted(cfg.items())), len(chunks), sum(len(c) for _, c in chunks)))
+    w('#include "../sfg_kernels.h"')
+    w("namespace sfg {")
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, cfg["waves"], name))
+    w("    const int64_t P = a.P;")
+    w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
+    w("    if (p >= P) return;")
+    w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
+    w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
+    w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
+    w("    const sfg_rsrc RYS = make_rsrc(a.YS, PT * %du);" % n)
+    w("    const sfg_rsrc RXA = make_rsrc(a.XA, PT * %du);" % n)
+    w("    const sfg_rsrc RXB = make_rsrc(a.XB, PT * %du);" % n)
+    w("    const sfg_rsrc RE = make_rsrc(a.XB, P8 * %du);   // Euler output (FP64 rows)" % n)
+    w("    const int nstage = a.scheme ? 4 : 1;")
+    w("    const bool euler = (nstage == 1);")
+    w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
+    w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
+    w("    const double h = a.dt;")
+    w("    __shared__ %s thl[%d][%d];" % (T, NPU + nq + nblk, B))
+    for q in used_p:
+        w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
+    for q in range(nq):
+        w("    thl[%d][threadIdx.x] = %s;" % (OMC + q, cv("a.omega[(int64_t)%d * P + p]" % q)))
+    for b in range(nblk):
+        w("    thl[%d][threadIdx.x] = %s;" % (SLC + b, cv("a.sl[(int64_t)%d * P + p]" % b)))
+    w("    for (int step = 0; step < ns; step++) {")
+    w("        for (int s = 0; s < nstage; s++) {")
+    w("            const bool first = (s == 0), last = (s == nstage - 1);")
+    w("            const %s cy = %s;" % (T, cv("euler ? h : ((s == 0 || s == 3) ? h / 6 : h / 3)")))
+    w("            const %s cx = %s;" % (T, cv("(s == 2) ? h : h / 2")))
+    w("            // stage input (s > 0) and output buffers")
+    w("            const sfg_rsrc RIN = (s == 2) ? RXB : RXA;")
+    w("            const sfg_rsrc ROUT = (s == 1) ? RXB : RXA;")
+    w("            // a.zero is 0 at run time but unknown to the compiler: it keeps the")
+    w("            // LDS reads of different chunks and stages apart")
+    w("            const int z = (int)a.zero * (step * 4 + s);")
+    zc = [0]
+
+    def zk():
+        zc[0] += 1
+        return zc[0]
+
+    def loads(colset, indent):
+        # column values of the stage input into registers
+        w("%s%s %s;" % (indent, T, ", ".join("c%d" % c for c in colset)))
+        w("%sif (first) {" % indent)
+        for c in colset:
+            w("%s    c%d = %s;" % (indent, c, cv("bld64(RX, v8, %du * P8)" % c)))
+        w("%s} else {" % indent)
+        for c in colset:
+            w("%s    c%d = %s(RIN, vT, %du * PT);" % (indent, c, LDT, c))
+        w("%s}" % indent)
+
+    # <D>: omega . x / x0 over the l <= 4 rows
+    w("            %s gD = (%s)0;" % (T, T))
+    w("            if (a.use_ddrx) {")
+    loads(list(range(nq)), "                ")
+    k = zk()
+    terms = " + ".join("thl[%d][threadIdx.x + z * %d] * c%d" % (OMC + q, k, q) for q in range(nq))
+    w("                gD = (%s) / c0;" % terms)
+    w("            }")
+    for (crow, ccols) in chunks:
+        w("            {   // rows %d..%d: %d columns" % (crow[0], crow[-1], len(ccols)))
+        loads(ccols, "                ")
+        for c0 in range(0, len(crow), CHUNK):
+            sub = crow[c0:c0 + CHUNK]
+            k = zk()
+            w("                {")
+            w("                    %s %s;" % (T, ", ".join("acc%d = (%s)0" % (r, T) for r in sub)))
+            byp = {}
+            for r in sub:
+                for (pp, tl) in rows[r][1]:
+                    byp.setdefault(pp, []).append((r, tl))
+            for pp in sorted(byp):
+                w("                    {")
+                w("                        const %s th = thl[%d][threadIdx.x + z * %d];" % (T, pidx[pp], k))
+                for (r, tl) in byp[pp]:
+                    col, ti, v = tl[0]
+                    w("                        %s u%d = %s * c%d;" % (T, r, lit(v, prec), col))
+                    for (col, ti, v) in tl[1:]:
+                        w("                        u%d = fma(%s, c%d, u%d);" % (r, lit(v, prec), col, r))
+                    w("                        acc%d = fma(th, u%d, acc%d);" % (r, r, r))
+                w("                    }")
+            for r in sub:
+                l = rows[r][0]
+                w("                    {")
+                w("                        const %s f = fma(thl[%d][threadIdx.x + z * %d] - gD, c%d, acc%d);"
+                  % (T, SLC + l // 2, k, r, r))
+                w("                        const double x0a = bld64(RX, v8, %du * P8);" % r)
+                w("                        if (euler) {")
+                w("                            bst64(RE, v8, %du * P8, x0a + (double)(cy * f));" % r)
+                w("                        } else {")
+                w("                            const %s ys = first ? cy * f : fma(cy, f, %s(RYS, vT, %du * PT));"
+                  % (T, LDT, r))
+                w("                            if (last) {")
+                w("                                bst64(RX, v8, %du * P8, x0a + (double)ys);" % r)
+                w("                            } else {")
+                w("                                %s(RYS, vT, %du * PT, ys);" % (STT, r))
+                w("                                %s(ROUT, vT, %du * PT, fma(cx, f, (%s)x0a));" % (STT, r, T))
+                w("                            }")
+                w("                        }")
+                w("                    }")
+            w("                }")
+        w("            }")
+    w("        }")
+    w("        if (euler) {")
+    w("            for (int r = 0; r < %d; r++) bst64(RX, v8, (uint32_t)r * P8, bld64(RE, v8, (uint32_t)r * P8));" % n)
+    w("        }")
+    w("    }")
+    w("    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;")
+    w("    if (a.status) {")
+    w("        int st = 0;")
+    w("        for (int r = 0; r < %d; r++) st |= isfinite(bld64(RX, v8, (uint32_t)r * P8)) ? 0 : 1;" % n)
+    w("        a.status[p] |= st;")
+    w("    }")
+    w("}")
+    w("#ifndef SFG_HOST_EMUL")
+    w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + %d) / %d)), dim3(%d), 0, s, a);" % (name, B - 1, B, B))
+    w("    return hipGetLastError();")
+    w("}")
+    w("#endif")
+    w("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
 def gen_dispatch(kinds):
     """kinds: list of (L, prec, variant configs)."""
     o = ["// Generated by specfab_amd/tables/gen_fast.py -- do not edit.",
@@ -521,9 +694,10 @@ def main():
     for prec, Ls in FAST_L.items():
         for L in Ls:
             n, rows = dump(L, exe)
-            vs = VARIANTS[(prec, "tuned" if L in TUNED_L else "default")]
+            kind = "wide" if L in WIDE_L[prec] else ("tuned" if L in TUNED_L else "default")
+            vs = VARIANTS[(prec, kind)]
             for vid, cfg in enumerate(vs):
-                gen = gen_kernel_reg if cfg["mode"] == "reg" else gen_kernel
+                gen = {"reg": gen_kernel_reg, "win": gen_kernel_win}.get(cfg["mode"], gen_kernel)
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
                                  gen(L, prec, n, rows, cfg, vid))
             kinds.append((L, prec, vs))

@@ -84,8 +84,11 @@ def _variants(L, prec):
                   if f.startswith("sfg_fast_L%d_%s_v" % (L, prec))) if os.path.isdir(GEN) else [0]
 
 
-CASES = [(L, prec, v) for (L, prec) in [(4, "f64"), (8, "f64"), (12, "f64"), (8, "f32"), (12, "f32")]
-         for v in (_variants(L, prec) if L == 8 else [0])]
+# every variant of the tuned L = 8 kernels and of the smallest window-stepper L
+# per precision (14 FP64, 18 FP32); the first variant elsewhere
+CASES = [(L, prec, v) for (L, prec) in [(4, "f64"), (8, "f64"), (12, "f64"), (14, "f64"), (8, "f32"), (12, "f32"),
+                                        (18, "f32")]
+         for v in (_variants(L, prec) if (L, prec) in ((8, "f64"), (8, "f32"), (14, "f64"), (18, "f32")) else [0])]
 
 
 @pytest.mark.parametrize("L,prec,v", CASES)
