@@ -93,6 +93,55 @@ module specfab_gpu
             real(c_double) :: e1(3), e2(3), e3(3), Eg(2), Eij(6)
             real(c_double), value :: alpha
         end function
+        integer(c_int) function c_a4(c, L, nlm, a4) bind(C, name="sfg_a4")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: a4(81)
+        end function
+        integer(c_int) function c_Vi_el(c, L, nlm, alpha, Lame, rho, ndir, th, ph, Vi) &
+                bind(C, name="sfg_Vi_elastic_tranisotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ndir
+            complex(c_double_complex) :: nlm(*)
+            real(c_double), value :: alpha, rho
+            real(c_double) :: Lame(5), th(*), ph(*), Vi(*)
+        end function
+        integer(c_int) function c_Vi_em(c, L, nlm, eps_m, eps_t, mu, ndir, th, ph, Vi) &
+                bind(C, name="sfg_Vi_electromagnetic_tranisotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ndir
+            complex(c_double_complex) :: nlm(*)
+            real(c_double), value :: eps_m, eps_t, mu
+            real(c_double) :: th(*), ph(*), Vi(*)
+        end function
+        integer(c_int) function c_a2_to_nlm(c, a2, nlm) bind(C, name="sfg_a2_to_nlm")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            real(c_double) :: a2(9)
+            complex(c_double_complex) :: nlm(6)
+        end function
+        integer(c_int) function c_a4_to_nlm(c, a4, nlm) bind(C, name="sfg_a4_to_nlm")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            real(c_double) :: a4(81)
+            complex(c_double_complex) :: nlm(15)
+        end function
+        integer(c_int) function c_Gamma0(c, eps, T, A, Q, G) bind(C, name="sfg_Gamma0")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            real(c_double) :: eps(9), G
+            real(c_double), value :: T, A, Q
+        end function
+        integer(c_int) function c_apply_bounds(c, L, nlm, out) bind(C, name="sfg_apply_bounds")
+            import :: c_int, c_ptr, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*), out(*)
+        end function
         integer(c_int) function c_evolve(c, desc, stream) bind(C, name="sfg_evolve")
             import :: c_int, c_ptr, sfg_evolve_desc
             type(c_ptr), value :: c
@@ -223,6 +272,73 @@ contains
         z = nlm(1:nlm_len)
         a1 = e1; a2_ = e2; a3 = e3; Eg = Eij_grain
         call check(c_Eij(ctx, int(Lcap, c_int), z, a1, a2_, a3, Eg, alpha, int(n_grain, c_int), Eij), 'Eij')
+    end function
+
+    ! a4(nlm): src/moments.f90:46-55 (the full tensor, REAL(4) body literals)
+    function a4(nlm) result(A)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp) :: A(3,3,3,3)
+        real(c_double) :: v(81)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len)
+        call check(c_a4(ctx, int(Lcap, c_int), z, v), 'a4')
+        A = reshape(v, [3,3,3,3], order=[4,3,2,1])   ! C row-major -> Fortran
+    end function
+
+    ! Vi_elastic_tranisotropic(nlm, alpha, Lame, rho, theta, phi): src/wavepropagation.f90:121-144
+    function Vi_elastic_tranisotropic(nlm, alpha, Lame, rho, theta, phi) result(Vi)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(in) :: alpha, Lame(5), rho, theta(:), phi(:)
+        real(dp) :: Vi(3, size(theta)), lm(5), th(size(theta)), ph(size(theta))
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len); lm = Lame; th = theta; ph = phi
+        call check(c_Vi_el(ctx, int(Lcap, c_int), z, alpha, lm, rho, int(size(theta), c_int), th, ph, Vi), &
+                   'Vi_elastic_tranisotropic')
+    end function
+
+    ! Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi): src/wavepropagation.f90:217-261
+    function Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi) result(Vi)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(in) :: eps_m, eps_t, mu, theta(:), phi(:)
+        real(dp) :: Vi(2, size(theta)), th(size(theta)), ph(size(theta))
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len); th = theta; ph = phi
+        call check(c_Vi_em(ctx, int(Lcap, c_int), z, eps_m, eps_t, mu, int(size(theta), c_int), th, ph, Vi), &
+                   'Vi_electromagnetic_tranisotropic')
+    end function
+
+    ! a2_to_nlm(a2), a4_to_nlm(a4): src/moments.f90:68-84
+    function a2_to_nlm(a2in) result(nlm)
+        real(dp), intent(in) :: a2in(3,3)
+        complex(dp) :: nlm(6)
+        real(c_double) :: t(9)
+        t = reshape(transpose(a2in), [9])
+        call check(c_a2_to_nlm(ctx, t, nlm), 'a2_to_nlm')
+    end function
+
+    function a4_to_nlm(a4in) result(nlm)
+        real(dp), intent(in) :: a4in(3,3,3,3)
+        complex(dp) :: nlm(15)
+        real(c_double) :: t(81)
+        t = reshape(reshape(a4in, [3,3,3,3], order=[4,3,2,1]), [81])   ! Fortran -> C row-major
+        call check(c_a4_to_nlm(ctx, t, nlm), 'a4_to_nlm')
+    end function
+
+    ! Gamma0(eps, T, A, Q): src/dynamics.f90:451-472
+    function Gamma0(eps, T, A, Q) result(G)
+        real(dp), intent(in) :: eps(3,3), T, A, Q
+        real(dp) :: G
+        real(c_double) :: e(9)
+        e = reshape(transpose(eps), [9])
+        call check(c_Gamma0(ctx, e, T, A, Q, G), 'Gamma0')
+    end function
+
+    ! apply_bounds(nlm): src/dynamics.f90:530-557
+    function apply_bounds(nlm) result(nlm_bounded)
+        complex(dp), intent(in) :: nlm(:)
+        complex(dp) :: nlm_bounded(nlm_len), z(nlm_len)
+        z = nlm(1:nlm_len)
+        call check(c_apply_bounds(ctx, int(Lcap, c_int), z, nlm_bounded), 'apply_bounds')
     end function
 
     ! Batched evolution of P parcels on the GPU (host arrays in, host arrays out):

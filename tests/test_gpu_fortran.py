@@ -26,3 +26,62 @@ def test_fortran_c1_parcel():
     ref = load_golden("c1_traj.npz")["traj"][-1]
     assert relerr(host, ref) < 1e-10
     assert relerr(batched, ref) < 1e-10
+
+
+EXE2 = os.path.join(ROOT, "specfab_amd", "fortran", "_build", "homog_faces")
+
+
+def test_fortran_homog_faces(tmp_path):
+    """Fortran host code calling a4, Vi_elastic_tranisotropic (Hill),
+    Vi_electromagnetic_tranisotropic, a2_to_nlm, a4_to_nlm, Gamma0 and
+    apply_bounds of the specfab_gpu module, against the reference fixtures
+    (wave_L8, init_L8); 1e-10 relative (1e-13 for a4)."""
+    if not os.path.exists(EXE2):
+        pytest.skip("Fortran demo not built (amdflang missing at build time)")
+    w = load_golden("wave_L8.npz")
+    g = load_golden("init_L8.npz")
+    ns, nd = 4, len(w["theta"])
+    kb = next(k for k, x in enumerate(g["b_nlm"]) if not np.array_equal(x, g["b_out"][k]))
+    f = tmp_path / "in.txt"
+    fmt = lambda a: " ".join(repr(float(v)) for v in np.ravel(a))
+    lines = ["%d %d" % (ns, nd)]
+    for k in range(ns):
+        lines += [fmt(w["nlm"][k].real), fmt(w["nlm"][k].imag)]
+    lines += [fmt(w["theta"]), fmt(w["phi"]),
+              fmt(np.concatenate([w["Lame"], [w["rho"], w["eps_m"], w["eps_t"], w["mu"]]]))]
+    lines += [fmt(g["a2"][0]), fmt(np.ravel(g["a4"][0], order="F")), fmt(g["g_eps"][0]),
+              fmt([g["g_T"][0], g["g_A"], g["g_Q"]]), fmt(g["b_nlm"][kb].real), fmt(g["b_nlm"][kb].imag)]
+    f.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([EXE2, str(f)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [l.split() for l in out.stdout.splitlines()]
+    a4 = np.zeros((ns, 3, 3, 3, 3))
+    vi = np.zeros((ns, nd, 3))
+    ve = np.zeros((ns, nd, 2))
+    a2n = np.zeros(6, complex)
+    a4n = np.zeros(15, complex)
+    bnd = np.zeros(45, complex)
+    G = None
+    for r in rows:
+        if r[0] == "a4":
+            k, i, j, m, q = (int(v) - 1 for v in r[1:6])
+            a4[k, i, j, m, q] = float(r[6])
+        elif r[0] == "vi":
+            vi[int(r[1]) - 1, int(r[2]) - 1] = [float(v) for v in r[3:6]]
+        elif r[0] == "ve":
+            ve[int(r[1]) - 1, int(r[2]) - 1] = [float(v) for v in r[3:5]]
+        elif r[0] == "a2n":
+            a2n[int(r[1]) - 1] = float(r[2]) + 1j * float(r[3])
+        elif r[0] == "a4n":
+            a4n[int(r[1]) - 1] = float(r[2]) + 1j * float(r[3])
+        elif r[0] == "bnd":
+            bnd[int(r[1]) - 1] = float(r[2]) + 1j * float(r[3])
+        elif r[0] == "g0":
+            G = float(r[1])
+    assert relerr(a4, w["a4"][:ns]) < 1e-13
+    assert relerr(vi, np.transpose(w["Vi_hill"][:ns], (0, 2, 1))) < 1e-10
+    assert relerr(ve, np.transpose(w["Vi_em"][:ns], (0, 2, 1))) < 1e-10
+    assert relerr(a2n, g["a2_nlm"][0]) < 1e-14
+    assert relerr(a4n, g["a4_nlm"][0]) < 1e-14
+    assert abs(G - g["g_G"][0]) < 1e-14 * g["g_G"][0]
+    assert relerr(bnd, g["b_out"][kb]) < 1e-14
