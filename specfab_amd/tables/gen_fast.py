@@ -55,18 +55,26 @@ FAST_L = {"f64": (4, 6, 8, 10, 12, 14, 16, 18, 20), "f32": (4, 6, 8, 10, 12, 14,
 #           the waves of a workgroup then walk the ~100 KB stage code in
 #           lockstep and share its instruction-cache lines
 #   xn      ("reg" mode) where the next-stage input lives: "reg" (registers,
-#           with an FP32 copy of the step-start state) or "mem" (XN buffer)
+#           with an FP32 copy of the step-start state), "mem" (XN buffer) or
+#           "roll" (registers: a row's next-stage value replaces its stage
+#           input as soon as no later row chunk of the stage reads it, so only
+#           about two l-levels of rows are held twice)
+#   ahead   ("reg" mode) the step-start state rows of row chunk c + 1 loaded
+#           at the top of chunk c, before chunk c's stores: waiting for them
+#           then does not wait for the stores just issued (one memory counter
+#           covers loads and stores)
 V = lambda mode, chunk, waves, bfree, fence, block=64, sync=False: dict(
     mode=mode, chunk=chunk, waves=waves, bfree=bfree, fence=fence, block=block, sync=sync)
 #   split   ("reg" mode) the last stage as a separate copy of the stage code
 #           (else one stage loop that tests the stage kind per row chunk)
-R = lambda chunk, waves, xn, split=True, block=64: dict(mode="reg", chunk=chunk, waves=waves, xn=xn,
-                                                      split=split, block=block)
+R = lambda chunk, waves, xn, split=True, block=64, ahead=False: dict(
+    mode="reg", chunk=chunk, waves=waves, xn=xn, split=split, block=block, ahead=ahead)
 #   window  ("win" mode) stage-input columns held in registers per row chunk
 Wn = lambda window, chunk, waves, block=64: dict(mode="win", window=window, chunk=chunk, waves=waves, block=block)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
-                       V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False)],
+                       V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False),
+                       R(4, 1, "roll", False, ahead=True)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg")],
@@ -297,6 +305,8 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
     name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
     CHUNK, B = cfg["chunk"], cfg["block"]
     XREG = cfg["xn"] == "reg"
+    ROLL = cfg["xn"] == "roll"
+    AHEAD = cfg.get("ahead", False)
     used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
     NPU = len(used_p)
     nq = min(n, 15)
@@ -320,7 +330,7 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
     w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
     w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
     w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
-    if not XREG:
+    if not XREG and not ROLL:
         w("    const sfg_rsrc RXN = make_rsrc(a.XA, PT * %du);" % n)
     w("    const int nstage = a.scheme ? 4 : 1;")
     w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
@@ -342,6 +352,28 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
         w("    %s %s;" % (T, ", ".join("xf%d" % b for b in range(n))))
         for b in range(n):
             w("    x%d = %s; xf%d = x%d;" % (b, cv("bld64(RX, v8, %du * P8)" % b), b, b))
+    chunks = [list(range(c0, min(n, c0 + CHUNK))) for c0 in range(0, n, CHUNK)]
+    if AHEAD:
+        w("    double %s;" % ", ".join("xo%d" % b for b in range(n)))
+    commit = {}
+    if ROLL:
+        w("    %s %s;" % (T, ", ".join("xn%d" % b for b in range(n))))
+        for b in range(n):
+            w("    x%d = %s;" % (b, cv("bld64(RX, v8, %du * P8)" % b)))
+        # chunk after which row b's stage input is dead: the last chunk whose
+        # terms or diagonal read column b (the DDRX moment sum reads rows
+        # 0..14 before the first chunk); its next-stage value is written
+        # into x_b after max(that chunk, b's own chunk)
+        lastread = {b: -1 for b in range(n)}
+        for ci, ch in enumerate(chunks):
+            for r in ch:
+                lastread[r] = max(lastread[r], ci)
+                for (pp, tl) in rows[r][1]:
+                    for (col, ti, v) in tl:
+                        lastread[col] = max(lastread[col], ci)
+        for ci, ch in enumerate(chunks):
+            for r in ch:
+                commit.setdefault(max(ci, lastread[r]), []).append(r)
     zc = [0]
 
     def zk():
@@ -372,11 +404,16 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
         terms = " + ".join("thl[%d][threadIdx.x + z * %d] * x%d" % (OMC + q, k, q) for q in range(nq))
         w("                gD = (%s) / x0;" % terms)
         w("            }")
-        for c0 in range(0, n, CHUNK):
-            chunk = list(range(c0, min(n, c0 + CHUNK)))
+        loadxo = last or dyn or not XREG
+        if loadxo and AHEAD:
+            w("            %s" % " ".join("xo%d = bld64(RX, v8, %du * P8);" % (r, r) for r in chunks[0]))
+        for ci, chunk in enumerate(chunks):
             k = zk()
             w("            {   // rows %d..%d" % (chunk[0], chunk[-1]))
-            if last or dyn or not XREG:
+            if loadxo and AHEAD:
+                if ci + 1 < len(chunks):
+                    w("                %s" % " ".join("xo%d = bld64(RX, v8, %du * P8);" % (r, r) for r in chunks[ci + 1]))
+            elif loadxo:
                 w("                const double %s;" % ", ".join("xo%d = bld64(RX, v8, %du * P8)" % (r, r) for r in chunk))
             w("                %s %s;" % (T, ", ".join("acc%d = (%s)0" % (r, T) for r in chunk)))
             byp = {}
@@ -404,10 +441,14 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
                     w("                    bst64(RX, v8, %du * P8, xs);" % r)
                     if XREG:
                         w("                    xf%d = %s;" % (r, cv("xs")))
+                    elif ROLL:
+                        w("                    xn%d = %s;" % (r, cv("xs")))
                 else:
                     w("                    y%d = yv;" % r)
                     if XREG:
                         w("                    xn%d = fma(cx, f, xf%d);" % (r, r))
+                    elif ROLL:
+                        w("                    xn%d = fma(cx, f, (%s)xo%d);" % (r, T, r))
                     else:
                         w("                    %s(RXN, vT, %du * PT, fma(cx, f, (%s)xo%d));" % (ST, r, T, r))
                 w("                }")
@@ -423,8 +464,12 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
                 for r in chunk:
                     epilogue(last, r)
             w("            }")
+            if ROLL and ci in commit:
+                w("            %s" % " ".join("x%d = xn%d;" % (b, b) for b in sorted(commit[ci])))
         # next stage input
-        if dyn:
+        if ROLL:
+            pass
+        elif dyn:
             assert not XREG
             w("            if (!last) {")
             for b in range(n):
@@ -438,7 +483,7 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
                 w("            x%d = %s(RXN, vT, %du * PT);" % (b, LD, b))
 
     w("    for (int step = 0; step < ns; step++) {")
-    if not XREG:
+    if not XREG and not ROLL:
         for b in range(n):
             w("        x%d = %s;" % (b, cv("bld64(RX, v8, %du * P8)" % b)))
     if cfg.get("split", True):
