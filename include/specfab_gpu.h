@@ -320,7 +320,9 @@ int sfg_surrogate_targets(int64_t P, const double* Eij_dev, const double* Vi_dev
  * network; parcels whose a2 eigenvalues leave the envelope get the plain
  * homogenization (status bit SFG_ST_ENVELOPE), a seeded fraction audit_frac of
  * the others is also computed by the plain method (status bit SFG_ST_AUDIT)
- * and the relative errors are reported in *stats.  Synchronises once. */
+ * and the relative errors are reported in *stats.  The work list stays on the
+ * device: with stats == NULL the call is fully stream-ordered on `stream`
+ * (no synchronisation); otherwise it synchronises once, at the end. */
 int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const double* x_dev,
                              const double* ugrad_dev, const sfg_homog_params* hp, double audit_frac, uint64_t seed,
                              double* Eij_dev, double* Vi_dev, int32_t* status_dev, sfg_surrogate_stats* stats,

@@ -1077,6 +1077,7 @@ struct sfg_mlp {
     int* idx = nullptr;
     int* count = nullptr;
     double *Ep = nullptr, *Vp = nullptr, *errE = nullptr, *errV = nullptr;
+    sfg::AuditSums* sums = nullptr;
     std::vector<void*> wbufs;
 };
 
@@ -1100,6 +1101,7 @@ int mlp_ws(sfg_mlp* m, int64_t P) {
     HIPCHK(A((void**)&m->Vp, Pz * 3 * 8));
     HIPCHK(A((void**)&m->errE, Pz * 8));
     HIPCHK(A((void**)&m->errV, Pz * 8));
+    HIPCHK(A((void**)&m->sums, sizeof(sfg::AuditSums)));
     m->wcap = P;
     return SFG_OK;
 }
@@ -1295,48 +1297,43 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     fa.audit_frac = audit_frac; fa.seed = seed;
     for (int i = 0; i < sfg::MLP_NLAYER; i++) { ia.wf[i] = net->net.wf[i]; ia.bias[i] = net->net.bias_pad[i]; }
     ia.norm = net->net.norm; ia.gtab = net->net.gtab; ia.Eij = Eij_dev; ia.Vi = Vi_dev; ia.idx = net->idx; ia.count = net->count;
-    HIPCHK(hipMemsetAsync(net->count, 0, sizeof(int), s));
+    HIPCHK(sfg::launch_reset_count(net->count, s));
     HIPCHK(sfg::launch_surrogate_infer(ia, s));
-    // 2. plain homogenization of the flagged parcels (work list from the kernel)
-    int nsel = 0;
-    HIPCHK(hipMemcpyAsync(&nsel, net->count, sizeof(int), hipMemcpyDeviceToHost, s));
+    // 2. plain homogenization of the flagged parcels: the work list and its
+    // length stay on the device (grid-stride kernels read the count), so the
+    // sequence is stream-ordered without a host round trip
+    sfg::EijArgs ea{};
+    ea.P = P; ea.nq = op->nq; ea.frame = 2; ea.n_grain = hp->n_grain;
+    ea.Ecc = hp->Eij_grain[0]; ea.Eca = hp->Eij_grain[1]; ea.alpha = hp->alpha;
+    ea.x = x_dev; ea.A2 = op->A2; ea.A4 = op->A4; ea.Eij = net->Ep;
+    ea.idx = net->idx; ea.nidx = P; ea.nidx_dev = net->count;
+    HIPCHK(sfg::launch_eij(ea, s));
+    sfg::ViElasticArgs va{};
+    const double zero = 0.0;
+    rc = get_dirs(ctx, 1, &zero, &zero, &va.R);
+    if (rc) return rc;
+    va.P = P; va.ndir = 1; va.nq = op->nq; va.x = x_dev; va.A2 = op->A2; va.A4 = op->A4; va.A4F = op->A4F;
+    va.alpha = hp->vi_alpha; va.rho = hp->rho;
+    sfg::elas_params(hp->Lame, va.kf, va.kr);
+    va.Vi = net->Vp; va.idx = net->idx; va.nidx = P; va.nidx_dev = net->count;
+    HIPCHK(sfg::launch_vi_elastic(va, s));
+    HIPCHK(sfg::launch_merge(net->count, P, net->idx, net->flags, net->Ep, net->Vp, Eij_dev, Vi_dev, status_dev,
+                             net->errE, net->errV, s));
+    if (!stats) return SFG_OK;
+    // 3. audit statistics, reduced on the device; the one synchronisation
+    HIPCHK(sfg::launch_audit_stats(net->count, P, net->errE, net->errV, net->sums, s));
+    sfg::AuditSums h{};
+    HIPCHK(hipMemcpyAsync(&h, net->sums, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (nsel > 0) {
-        sfg::EijArgs ea{};
-        ea.P = P; ea.nq = op->nq; ea.frame = 2; ea.n_grain = hp->n_grain;
-        ea.Ecc = hp->Eij_grain[0]; ea.Eca = hp->Eij_grain[1]; ea.alpha = hp->alpha;
-        ea.x = x_dev; ea.A2 = op->A2; ea.A4 = op->A4; ea.Eij = net->Ep; ea.idx = net->idx; ea.nidx = nsel;
-        HIPCHK(sfg::launch_eij(ea, s));
-        sfg::ViElasticArgs va{};
-        const double zero = 0.0;
-        rc = get_dirs(ctx, 1, &zero, &zero, &va.R);
-        if (rc) return rc;
-        va.P = P; va.ndir = 1; va.nq = op->nq; va.x = x_dev; va.A2 = op->A2; va.A4 = op->A4; va.A4F = op->A4F;
-        va.alpha = hp->vi_alpha; va.rho = hp->rho;
-        sfg::elas_params(hp->Lame, va.kf, va.kr);
-        va.Vi = net->Vp; va.idx = net->idx; va.nidx = nsel;
-        HIPCHK(sfg::launch_vi_elastic(va, s));
-        HIPCHK(sfg::launch_merge(nsel, net->idx, net->flags, net->Ep, net->Vp, Eij_dev, Vi_dev, status_dev,
-                                 net->errE, net->errV, s));
-        std::vector<double> eE((size_t)nsel), eV((size_t)nsel);
-        HIPCHK(hipMemcpyAsync(eE.data(), net->errE, (size_t)nsel * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(eV.data(), net->errV, (size_t)nsel * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        double sE = 0, sV = 0;
-        for (int t = 0; t < nsel; t++) {
-            if (eE[t] < 0) { st.n_fallback++; continue; }
-            st.n_audit++;
-            st.audit_max_err_Eij = std::max(st.audit_max_err_Eij, eE[t]);
-            st.audit_max_err_Vi = std::max(st.audit_max_err_Vi, eV[t]);
-            sE += eE[t];
-            sV += eV[t];
-        }
-        if (st.n_audit) {
-            st.audit_mean_err_Eij = sE / st.n_audit;
-            st.audit_mean_err_Vi = sV / st.n_audit;
-        }
+    st.n_fallback = h.n_fallback;
+    st.n_audit = h.n_audit;
+    st.audit_max_err_Eij = h.max_E;
+    st.audit_max_err_Vi = h.max_V;
+    if (st.n_audit) {
+        st.audit_mean_err_Eij = h.sum_E / (double)st.n_audit;
+        st.audit_mean_err_Vi = h.sum_V / (double)st.n_audit;
     }
-    if (stats) *stats = st;
+    *stats = st;
     return SFG_OK;
 }
 
@@ -1444,12 +1441,21 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         B.grid = (unsigned)((P + B.block - 1) / B.block);
     }
     // the time loop as one graph: chunk c of every bucket depends on chunk
-    // c - 1 of the same bucket only, so the buckets run concurrently
+    // c - 1 of the same bucket only, so the buckets run concurrently.  The
+    // runtime spreads independent chains over a few streams and queues a
+    // surplus chain behind another one, so the chains are added costliest
+    // first (n^2 P: the coupling terms grow about as n^2): the longest chain
+    // then starts at once and the cheap ones share streams.
+    std::vector<size_t> order(plan->b.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    auto cost = [&](size_t i) { return (double)plan->b[i].ea.n * plan->b[i].ea.n * (double)plan->b[i].ea.P; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t u, size_t v) { return cost(u) > cost(v); });
     hipError_t e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGraphCreate(&plan->graph, 0);
     std::vector<hipGraphNode_t> prev(plan->b.size(), nullptr);
     for (int c = 0; c < nchunks && e == hipSuccess; c++) {
-        for (size_t i = 0; i < plan->b.size() && e == hipSuccess; i++) {
+        for (size_t i : order) {
+            if (e != hipSuccess) break;
             sfg_plan::Bucket& B = plan->b[i];
             void* args[] = {&B.ea};
             hipKernelNodeParams kp{};

@@ -10,7 +10,7 @@
 struct int2 { int x, y; };
 struct int4 { int x, y, z, w; };
 struct sfg_dim3 { unsigned x, y, z; };
-extern thread_local sfg_dim3 threadIdx, blockIdx, blockDim;
+extern thread_local sfg_dim3 threadIdx, blockIdx, blockDim, gridDim;
 typedef int hipError_t;
 typedef void* hipStream_t;
 #define __global__

@@ -20,9 +20,12 @@ struct EijArgs {
     double* ei;           // (P, 3, 3) or null
     int* status;          // (P) or null
     // optional work list: items t < nidx evaluate state column idx[t] and
-    // write Eij/lami/ei at row t (status stays at the parcel's own row)
+    // write Eij/lami/ei at row t (status stays at the parcel's own row);
+    // with nidx_dev the item count is min(*nidx_dev, nidx), read on the
+    // device (nidx is then the list's capacity), so no host round trip
     const int* idx;
     int64_t nidx;
+    const int* nidx_dev;
 };
 
 hipError_t launch_eij(const EijArgs& a, hipStream_t s);

@@ -10,6 +10,8 @@
 // states, src/enhancementfactors.f90:23-45 -> src/homogenizations.f90:145-189).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sfg_device.h"
 #include "sfg_homog.h"
 
@@ -244,11 +246,7 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
     return status;
 }
 
-// one wave per SIMD: the Taylor 6x6 system, a4 and the frame need ~380
-// registers per lane, which then stay in VGPRs/AGPRs instead of scratch
-__global__ void __launch_bounds__(128, 1) eij_kernel(EijArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (a.idx ? a.nidx : a.P)) return;
+__device__ __forceinline__ void eij_item(const EijArgs& a, int64_t t) {
     const int64_t p = a.idx ? (int64_t)a.idx[t] : t;   // state column
     const int64_t o = t;                               // output row
     double a2v[6], a4v[36];
@@ -271,10 +269,30 @@ __global__ void __launch_bounds__(128, 1) eij_kernel(EijArgs a) {
     if (a.status && st) atomicOr(a.status + p, st);
 }
 
+// one wave per SIMD: the Taylor 6x6 system, a4 and the frame need ~380
+// registers per lane, which then stay in VGPRs/AGPRs instead of scratch
+__global__ void __launch_bounds__(128, 1) eij_kernel(EijArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.idx ? a.nidx : a.P)) return;
+    eij_item(a, t);
+}
+
+// work list whose length is on the device: grid-stride loop (a separate
+// kernel, so that the loop does not cost the one-item kernel registers)
+__global__ void __launch_bounds__(128, 1) eij_list_kernel(EijArgs a) {
+    const int64_t n = *a.nidx_dev < a.nidx ? *a.nidx_dev : a.nidx;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) eij_item(a, t);
+}
+
 hipError_t launch_eij(const EijArgs& a, hipStream_t s) {
     const int64_t n = a.idx ? a.nidx : a.P;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(eij_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, a);
+    const int64_t nblk = (n + 127) / 128;
+    if (a.idx && a.nidx_dev)   // a device-side count: at most 2048 workgroups
+        hipLaunchKernelGGL(eij_list_kernel, dim3((unsigned)std::min<int64_t>(nblk, 2048)), dim3(128), 0, s, a);
+    else
+        hipLaunchKernelGGL(eij_kernel, dim3((unsigned)nblk), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

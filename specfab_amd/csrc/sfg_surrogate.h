@@ -37,7 +37,21 @@ hipError_t launch_surrogate_infer(const SurrogateInferArgs& a, hipStream_t s);
 // the 1024-interval table of the normal CDF used by the fused kernel's GELU
 hipError_t launch_gelu_table(float2* tab, hipStream_t s);
 hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float* T, hipStream_t s);
-hipError_t launch_merge(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp, double* Eij,
-                        double* Vi, int* status, double* errE, double* errV, hipStream_t s);
+// merge of the work list (count on the device, at most cap items): fallback
+// parcels take the plain values, audited ones record relative errors
+hipError_t launch_merge(const int* count, int64_t cap, const int* idx, const int* flags, const double* Ep,
+                        const double* Vp, double* Eij, double* Vi, int* status, double* errE, double* errV,
+                        hipStream_t s);
+
+// audit statistics of a merged work list, reduced on the device
+struct AuditSums {
+    long long n_fallback, n_audit;
+    double max_E, sum_E, max_V, sum_V;
+};
+hipError_t launch_audit_stats(const int* count, int64_t cap, const double* errE, const double* errV, AuditSums* out,
+                              hipStream_t s);
+// zeroes the work-list count (a kernel, so that the sequence can be captured
+// in a graph)
+hipError_t launch_reset_count(int* count, hipStream_t s);
 
 }  // namespace sfg

@@ -132,10 +132,9 @@ __global__ void targets_kernel(int64_t P, const double* Eij, const double* Vi, f
 
 // merge the plain results of the work list: fallback parcels take them,
 // audited parcels keep the surrogate values and record relative errors
-__global__ void merge_kernel(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp,
-                             double* Eij, double* Vi, int* status, double* errE, double* errV) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__device__ __forceinline__ void merge_item(int64_t t, const int* idx, const int* flags, const double* Ep,
+                                           const double* Vp, double* Eij, double* Vi, int* status, double* errE,
+                                           double* errV) {
     const int64_t p = idx[t];
     double v0 = Vp[t * 3], v1 = Vp[t * 3 + 1], v2 = Vp[t * 3 + 2], tt;
     if (v1 < v0) { tt = v0; v0 = v1; v1 = tt; }
@@ -157,6 +156,61 @@ __global__ void merge_kernel(int64_t n, const int* idx, const int* flags, const 
         if (status) atomicOr(status + p, 32);
     }
 }
+
+__global__ void merge_kernel(const int* count, int64_t cap, const int* idx, const int* flags, const double* Ep,
+                             const double* Vp, double* Eij, double* Vi, int* status, double* errE, double* errV) {
+    const int64_t n = *count < cap ? *count : cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+        merge_item(t, idx, flags, Ep, Vp, Eij, Vi, status, errE, errV);
+}
+
+// one workgroup of 256 threads: counts, maxima and sums of the merged list
+// (errE < 0 marks a fallback parcel)
+__global__ void __launch_bounds__(256) audit_stats_kernel(const int* count, int64_t cap, const double* errE,
+                                                          const double* errV, AuditSums* out) {
+    const int64_t n = *count < cap ? *count : cap;
+    long long nf = 0, na = 0;
+    double mE = 0, sE = 0, mV = 0, sV = 0;
+    for (int64_t t = threadIdx.x; t < n; t += 256) {
+        const double e = errE[t], v = errV[t];
+        if (e < 0) {
+            nf++;
+        } else {
+            na++;
+            mE = fmax(mE, e);
+            sE += e;
+            mV = fmax(mV, v);
+            sV += v;
+        }
+    }
+    __shared__ long long snf[256], sna[256];
+    __shared__ double smE[256], ssE[256], smV[256], ssV[256];
+    const int i = threadIdx.x;
+    snf[i] = nf; sna[i] = na; smE[i] = mE; ssE[i] = sE; smV[i] = mV; ssV[i] = sV;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (i < h) {
+            snf[i] += snf[i + h];
+            sna[i] += sna[i + h];
+            smE[i] = fmax(smE[i], smE[i + h]);
+            ssE[i] += ssE[i + h];
+            smV[i] = fmax(smV[i], smV[i + h]);
+            ssV[i] += ssV[i + h];
+        }
+        __syncthreads();
+    }
+    if (i == 0) {
+        out->n_fallback = snf[0];
+        out->n_audit = sna[0];
+        out->max_E = smE[0];
+        out->sum_E = ssE[0];
+        out->max_V = smV[0];
+        out->sum_V = ssV[0];
+    }
+}
+
+__global__ void reset_count_kernel(int* count) { *count = 0; }
 
 // ------------------------------------------------------------ fused inference
 //
@@ -417,11 +471,24 @@ hipError_t launch_targets(int64_t P, const double* Eij, const double* Vi, float*
     return hipGetLastError();
 }
 
-hipError_t launch_merge(int64_t n, const int* idx, const int* flags, const double* Ep, const double* Vp, double* Eij,
-                        double* Vi, int* status, double* errE, double* errV, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(merge_kernel, dim3(nb(n, 256)), dim3(256), 0, s, n, idx, flags, Ep, Vp, Eij, Vi, status,
+hipError_t launch_merge(const int* count, int64_t cap, const int* idx, const int* flags, const double* Ep,
+                        const double* Vp, double* Eij, double* Vi, int* status, double* errE, double* errV,
+                        hipStream_t s) {
+    if (cap == 0) return hipSuccess;
+    const unsigned grid = nb(cap, 256) < 1024u ? nb(cap, 256) : 1024u;
+    hipLaunchKernelGGL(merge_kernel, dim3(grid), dim3(256), 0, s, count, cap, idx, flags, Ep, Vp, Eij, Vi, status,
                        errE, errV);
+    return hipGetLastError();
+}
+
+hipError_t launch_audit_stats(const int* count, int64_t cap, const double* errE, const double* errV, AuditSums* out,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(audit_stats_kernel, dim3(1), dim3(256), 0, s, count, cap, errE, errV, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_count(int* count, hipStream_t s) {
+    hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, count);
     return hipGetLastError();
 }
 

@@ -25,9 +25,11 @@ struct ViElasticArgs {
     double* Vi;           // (P, ndir, 3): qS1, qS2, qP
     int* status;          // (P) or null
     // optional work list: items t < nidx evaluate state column idx[t] and
-    // write Vi at row t
+    // write Vi at row t; with nidx_dev the item count is min(*nidx_dev,
+    // nidx), read on the device (nidx is then the list's capacity)
     const int* idx;
     int64_t nidx;
+    const int* nidx_dev;
 };
 
 struct ViEMArgs {

@@ -18,6 +18,8 @@
 //   4. Voigt acoustic tensor (:170-187), mix (1-alpha) R + alpha V;
 //   5. Christoffel roots by the reference's complex Cardano formula with its
 //      REAL(4) exponents (Qnorm_to_vi, :267-324).
+#include <algorithm>
+
 #include "sfg_device.h"
 #include "sfg_wave.h"
 
@@ -207,10 +209,7 @@ __device__ void qnorm_to_vi(const double* Q, double rho, double* vi) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(128) vi_elastic_kernel(ViElasticArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nwork = a.idx ? a.nidx : a.P;
-    if (t >= nwork * a.ndir) return;
+__device__ __forceinline__ void vi_elastic_item(const ViElasticArgs& a, int64_t t, int64_t nwork) {
     const int64_t item = t % nwork;
     const int d = (int)(t / nwork);
     const int64_t p = a.idx ? (int64_t)a.idx[item] : item;   // state column
@@ -251,6 +250,20 @@ __global__ void __launch_bounds__(128) vi_elastic_kernel(ViElasticArgs a) {
     o[0] = vi[0];
     o[1] = vi[1];
     o[2] = vi[2];
+}
+
+__global__ void __launch_bounds__(128) vi_elastic_kernel(ViElasticArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nwork = a.idx ? a.nidx : a.P;
+    if (t < nwork * a.ndir) vi_elastic_item(a, t, nwork);
+}
+
+// work list whose length is on the device: grid-stride loop
+__global__ void __launch_bounds__(128) vi_elastic_list_kernel(ViElasticArgs a) {
+    const int64_t nwork = *a.nidx_dev < a.nidx ? *a.nidx_dev : a.nidx;
+    const int64_t n = nwork * a.ndir;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) vi_elastic_item(a, t, nwork);
 }
 
 __global__ void __launch_bounds__(128) vi_em_kernel(ViEMArgs a) {
@@ -341,7 +354,10 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s) {
     const int64_t n = (a.idx ? a.nidx : a.P) * a.ndir;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(vi_elastic_kernel, dim3(nblk(n, 128)), dim3(128), 0, s, a);
+    if (a.idx && a.nidx_dev)   // a device-side count: at most 2048 workgroups
+        hipLaunchKernelGGL(vi_elastic_list_kernel, dim3(std::min(nblk(n, 128), 2048u)), dim3(128), 0, s, a);
+    else
+        hipLaunchKernelGGL(vi_elastic_kernel, dim3(nblk(n, 128)), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

@@ -7,7 +7,7 @@
 #include "sfg_internal.h"
 #include "sfg_wave.h"
 
-thread_local sfg_dim3 threadIdx, blockIdx, blockDim;
+thread_local sfg_dim3 threadIdx, blockIdx, blockDim, gridDim;
 
 namespace sfg {
 __global__ void vi_elastic_kernel(ViElasticArgs a);
@@ -21,6 +21,7 @@ static void grid(int64_t N, int bs, F f) {
         for (int t = 0; t < bs; t++) {
             blockIdx.x = (unsigned)b;
             blockDim.x = (unsigned)bs;
+            gridDim.x = (unsigned)((N + bs - 1) / bs);
             threadIdx.x = (unsigned)t;
             f();
         }

@@ -164,6 +164,96 @@ def test_surrogate_vs_plain_on_field(gpu_ctx):
         assert np.abs(E.cpu().numpy()[fbk] - Ep.cpu().numpy()[fbk]).max() == 0.0
 
 
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_surrogate_worklist_on_device(gpu_ctx):
+    """Every parcel on the work list (audit_frac = 1) of a field larger than
+    the grid-stride kernels' 2048 x 128 threads, so the device-side count is
+    walked in several passes: the device-reduced audit statistics equal a
+    host recomputation from the returned surrogate values and the plain
+    homogenization, and the stream-ordered call without statistics returns
+    the same Eij and Vi."""
+    import torch
+    from specfab_amd import surrogate as S
+    net = S.Surrogate().load(WEIGHTS)
+    P = 2048 * 128 + 4099
+    fb = _field(P, 40, seed=8)
+    E, V, st = net.homogenize(fb, audit_frac=1.0, seed=3)
+    E2, V2, st2 = net.homogenize(fb, audit_frac=1.0, seed=3, stats=False)
+    torch.cuda.synchronize()
+    assert st2 is None
+    assert torch.equal(E, E2) and torch.equal(V, V2)
+    assert st["n_audit"] + st["n_fallback"] == P
+    _, Ep, Vp = S.plain_targets(fb)
+    Vs = torch.sort(Vp, dim=1).values
+    flags = fb.status.cpu().numpy()
+    aud = torch.from_numpy((flags & 32) != 0).to(E.device)
+    fbk = torch.from_numpy((flags & 16) != 0).to(E.device)
+    assert int(aud.sum()) == st["n_audit"] and int(fbk.sum()) == st["n_fallback"]
+    eE = ((E - Ep).abs() / Ep.abs()).max(1).values[aud]
+    eV = ((V - Vs).abs() / Vs.abs()).max(1).values[aud]
+    assert abs(eE.max().item() - st["audit_max_err_Eij"]) <= 1e-15
+    assert abs(eV.max().item() - st["audit_max_err_Vi"]) <= 1e-15
+    assert abs(eE.mean().item() - st["audit_mean_err_Eij"]) <= 1e-12 * max(1.0, st["audit_mean_err_Eij"])
+    assert abs(eV.mean().item() - st["audit_mean_err_Vi"]) <= 1e-12 * max(1.0, st["audit_mean_err_Vi"])
+    if fbk.any():
+        assert torch.equal(E[fbk], Ep[fbk]) and torch.equal(V[fbk], Vs[fbk])
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_surrogate_gain_in_loop(gpu_ctx):
+    """What the network gains over the method without it, in the loop it is
+    built for: one RK4 step of an L = 12 field followed by its
+    homogenization, plain FP64 stepping + Sachs/Taylor Eij + elastic Vi vs
+    FP32 stepping + the network (envelope fallback and a fresh 1 % audit per
+    step, statistics left on the device).  The learned step is at least 1.3x
+    faster (measured ~2x at 2^20 parcels), and its Eij stay within 1e-3 of
+    the plain ones on average (max 5e-3) on the same states."""
+    import torch
+    from specfab_amd import surrogate as S
+    net = S.Surrogate().load(WEIGHTS)
+    hp = S.ice_params()
+    P = 1 << 18
+    fb = _field(P, 100, seed=12)
+    procs = ("LROT", "REG", "DDRX")
+    fb.autotune("fp64", P)
+    fb.autotune("fp32c", P)
+
+    def plain_homog():
+        E = fb.Eij((hp.Eij_grain[0], hp.Eij_grain[1]), hp.alpha, hp.n_grain, frame="xyz")
+        fb.Vi_elastic(hp.vi_alpha, list(hp.Lame), hp.rho, [0.0], [0.0])
+        return E
+
+    seeds = iter(range(100, 10 ** 6))
+
+    def plain_step():
+        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision="fp64")
+        plain_homog()
+
+    def learned_step():
+        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision="fp32c")
+        net.homogenize(fb, hp, audit_frac=0.01, seed=next(seeds), stats=False)
+
+    E0, E1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = {}
+    for tag, fn in (("plain", plain_step), ("learned", learned_step)):
+        fn()
+        torch.cuda.synchronize()
+        E0.record()
+        for _ in range(5):
+            fn()
+        E1.record()
+        torch.cuda.synchronize()
+        ms[tag] = E0.elapsed_time(E1) / 5
+    E, V, st = net.homogenize(fb, hp, audit_frac=0.01, seed=1)
+    Ep = plain_homog()
+    torch.cuda.synchronize()
+    err = ((E - Ep).abs() / Ep.abs()).max(1).values
+    print("in-loop ms per step at %d parcels: %s, gain %.2fx; Eij rel err mean %.2e max %.2e; audit %s"
+          % (P, ms, ms["plain"] / ms["learned"], err.mean().item(), err.max().item(), st))
+    assert ms["plain"] / ms["learned"] > 1.3
+    assert err.mean().item() < 1e-3 and err.max().item() < 5e-3
+
+
 def _field(P, steps, seed, spread=1e-3):
     import math
     import torch
