@@ -446,7 +446,7 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
     const bool fast = fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0);
     if (!fast && prec == 1) {
         ws_release(ctx, s);
-        return fail(SFG_ERR_ARG, "FP32C stepping needs L <= 16 and n * P * 8 < 2^31 (split the batch)");
+        return fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L and n * P * 8 < 2^31 (split the batch)");
     }
     std::pair<hipEvent_t, hipEvent_t> tv{nullptr, nullptr};
     if (ctx->timing) {
@@ -1428,17 +1428,19 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         ea.zero = 0;
         const int prec = d->precision == SFG_FP64 ? 0 : 1;
         const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
+        int per_block = 64;   // parcels stepped by one workgroup
         if (fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0)) {
             int blk = 64;
-            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec), &blk);
+            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec), &blk, &per_block);
             B.block = (unsigned)blk;
         } else if (prec == 0) {
             B.fn = sfg::generic_kernel();
             B.block = sfg::GENERIC_BLOCK;
+            per_block = sfg::GENERIC_BLOCK;
         } else {
-            return bail(fail(SFG_ERR_ARG, "FP32C stepping needs L <= 16 and n * P * 8 < 2^31"));
+            return bail(fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L and n * P * 8 < 2^31"));
         }
-        B.grid = (unsigned)((P + B.block - 1) / B.block);
+        B.grid = (unsigned)((P + per_block - 1) / per_block);
     }
     // the time loop as one graph: chunk c of every bucket depends on chunk
     // c - 1 of the same bucket only, so the buckets run concurrently.  The

@@ -17,7 +17,11 @@ typedef void* hipStream_t;
 #define __device__
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(m) ((void)0)
-#define __syncthreads() ((void)0)
+// workgroup barrier of the emulated block (the driver runs a block's threads
+// concurrently; see tests/emul_driver.cpp); a no-op where the driver runs
+// threads one after another
+void sfg_emul_syncthreads();
+#define __syncthreads() sfg_emul_syncthreads()
 #define __shared__ static
 #define __forceinline__ inline
 #define __constant__

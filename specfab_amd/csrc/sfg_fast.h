@@ -10,8 +10,10 @@ namespace sfg {
 // generated variants of (L, prec) (0: no register-resident stepper).
 int fast_variants(int L, int prec);
 hipError_t launch_evolve_fast(int L, int prec, int variant, const EvolveArgs& a, hipStream_t s);
-// the variant's __global__ function (for graph kernel nodes) and its block size
-const void* fast_kernel(int L, int prec, int variant, int* block);
+// the variant's __global__ function (for graph kernel nodes), its threads per
+// workgroup and the parcels one workgroup steps (fewer than the threads when
+// several waves share a parcel group): grid = ceil(P / parcels)
+const void* fast_kernel(int L, int prec, int variant, int* block, int* parcels);
 // the table-driven stepper's __global__ function and its block size
 const void* generic_kernel();
 constexpr int GENERIC_BLOCK = 64;

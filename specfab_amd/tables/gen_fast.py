@@ -70,7 +70,12 @@ V = lambda mode, chunk, waves, bfree, fence, block=64, sync=False: dict(
 R = lambda chunk, waves, xn, split=True, block=64, ahead=False: dict(
     mode="reg", chunk=chunk, waves=waves, xn=xn, split=split, block=block, ahead=ahead)
 #   window  ("win" mode) stage-input columns held in registers per row chunk
-Wn = lambda window, chunk, waves, block=64: dict(mode="win", window=window, chunk=chunk, waves=waves, block=block)
+#   split   ("win" mode) waves that share one group of 64 parcels, each
+#           evaluating a contiguous share of the row chunks (balanced by term
+#           count), with a workgroup barrier after every stage: a bucket of
+#           few parcels then still runs several waves per SIMD
+Wn = lambda window, chunk, waves, split=1: dict(mode="win", window=window, chunk=chunk, waves=waves,
+                                                block=64 * split, split=split)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
                        V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False),
@@ -79,8 +84,8 @@ VARIANTS = {
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg")],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
-    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2)],
-    ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2)],
+    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2)],
+    ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2)],
 }
 WIDE_L = {"f64": (14, 16, 18, 20), "f32": (18, 20)}
 TUNED_L = (8, 12)
@@ -528,6 +533,13 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     (YS increment sum, XA / XB stage inputs) go through HBM; launch constants
     are staged in LDS.  Stage input / output: RK4 X -> XA -> XB -> XA -> X;
     Euler X -> XB (FP64) -> X.
+
+    With cfg["split"] = S > 1 a workgroup is S waves on the same 64 parcels;
+    wave w evaluates the w-th share of the row chunks and a workgroup
+    barrier closes every stage (the next stage's columns come from all
+    shares).  Lanes past the batch stay alive for the barriers with a lane
+    offset past every buffer (0x80000000; the buffers are < 2^31 bytes), so
+    their loads return 0 and their stores are dropped.
     """
     T = "double" if prec == "f64" else "float"
     LDT = "bld64" if prec == "f64" else "bld32"
@@ -536,6 +548,9 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     cv = (lambda e: e) if prec == "f64" else (lambda e: "(float)(%s)" % e)
     name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
     WIN, CHUNK, B = cfg["window"], cfg["chunk"], cfg["block"]
+    SPLIT = cfg.get("split", 1)
+    assert B == 64 * SPLIT or SPLIT == 1
+    NL = 64 if SPLIT > 1 else B       # parcels per workgroup
     used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
     NPU = len(used_p)
     nq = min(n, 15)
@@ -551,6 +566,13 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
         cur.append(r)
         cols |= need[r]
     chunks.append((cur, sorted(cols)))
+    # shares of the chunks per wave, balanced by term count
+    nterm = [sum(len(tl) for (_, tl) in rows[r][1]) + 1 for r in range(n)]
+    cw = [sum(nterm[r] for r in crow) for (crow, _) in chunks]
+    tot, acc, share = sum(cw), 0, []
+    for c in cw:
+        share.append(min(SPLIT - 1, int(SPLIT * (acc + c / 2) / tot)))
+        acc += c
     o = []
     w = o.append
     w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
@@ -561,10 +583,19 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     w("namespace sfg {")
     w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, cfg["waves"], name))
     w("    const int64_t P = a.P;")
-    w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
-    w("    if (p >= P) return;")
-    w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
-    w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
+    if SPLIT > 1:
+        w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
+        w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
+        w("    const bool valid = p < P;")
+        w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
+        w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
+        w("    const uint32_t vT = valid ? (uint32_t)p * %du : 0x80000000u;" % W)
+    else:
+        w("    const int lane = threadIdx.x;")
+        w("    const int64_t p = (int64_t)blockIdx.x * %d + threadIdx.x;" % B)
+        w("    if (p >= P) return;")
+        w("    const uint32_t P8 = (uint32_t)P * 8u, PT = (uint32_t)P * %du;" % W)
+        w("    const uint32_t v8 = (uint32_t)p * 8u, vT = (uint32_t)p * %du;" % W)
     w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
     w("    const sfg_rsrc RYS = make_rsrc(a.YS, PT * %du);" % n)
     w("    const sfg_rsrc RXA = make_rsrc(a.XA, PT * %du);" % n)
@@ -572,16 +603,26 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     w("    const sfg_rsrc RE = make_rsrc(a.XB, P8 * %du);   // Euler output (FP64 rows)" % n)
     w("    const int nstage = a.scheme ? 4 : 1;")
     w("    const bool euler = (nstage == 1);")
-    w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
+    if SPLIT > 1:
+        w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
+    else:
+        w("    const int rem = a.steps ? a.steps[p] : a.nsteps;")
     w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
     w("    const double h = a.dt;")
-    w("    __shared__ %s thl[%d][%d];" % (T, NPU + nq + nblk, B))
+    w("    __shared__ %s thl[%d][%d];" % (T, NPU + nq + nblk, NL))
+    ind = "    "
+    if SPLIT > 1:
+        w("    if (wv == 0 && valid) {")
+        ind = "        "
     for q in used_p:
-        w("    thl[%d][threadIdx.x] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
+        w("%sthl[%d][lane] = %s;" % (ind, pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
     for q in range(nq):
-        w("    thl[%d][threadIdx.x] = %s;" % (OMC + q, cv("a.omega[(int64_t)%d * P + p]" % q)))
+        w("%sthl[%d][lane] = %s;" % (ind, OMC + q, cv("a.omega[(int64_t)%d * P + p]" % q)))
     for b in range(nblk):
-        w("    thl[%d][threadIdx.x] = %s;" % (SLC + b, cv("a.sl[(int64_t)%d * P + p]" % b)))
+        w("%sthl[%d][lane] = %s;" % (ind, SLC + b, cv("a.sl[(int64_t)%d * P + p]" % b)))
+    if SPLIT > 1:
+        w("    }")
+        w("    __syncthreads();")
     w("    for (int step = 0; step < ns; step++) {")
     w("        for (int s = 0; s < nstage; s++) {")
     w("            const bool first = (s == 0), last = (s == nstage - 1);")
@@ -615,10 +656,12 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     w("            if (a.use_ddrx) {")
     loads(list(range(nq)), "                ")
     k = zk()
-    terms = " + ".join("thl[%d][threadIdx.x + z * %d] * c%d" % (OMC + q, k, q) for q in range(nq))
+    terms = " + ".join("thl[%d][lane + z * %d] * c%d" % (OMC + q, k, q) for q in range(nq))
     w("                gD = (%s) / c0;" % terms)
     w("            }")
-    for (crow, ccols) in chunks:
+    for ci, (crow, ccols) in enumerate(chunks):
+        if SPLIT > 1 and (ci == 0 or share[ci] != share[ci - 1]):
+            w("            %sif (wv == %d) {" % ("" if ci == 0 else "} else ", share[ci]))
         w("            {   // rows %d..%d: %d columns" % (crow[0], crow[-1], len(ccols)))
         loads(ccols, "                ")
         for c0 in range(0, len(crow), CHUNK):
@@ -632,7 +675,7 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
                     byp.setdefault(pp, []).append((r, tl))
             for pp in sorted(byp):
                 w("                    {")
-                w("                        const %s th = thl[%d][threadIdx.x + z * %d];" % (T, pidx[pp], k))
+                w("                        const %s th = thl[%d][lane + z * %d];" % (T, pidx[pp], k))
                 for (r, tl) in byp[pp]:
                     col, ti, v = tl[0]
                     w("                        %s u%d = %s * c%d;" % (T, r, lit(v, prec), col))
@@ -643,7 +686,7 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
             for r in sub:
                 l = rows[r][0]
                 w("                    {")
-                w("                        const %s f = fma(thl[%d][threadIdx.x + z * %d] - gD, c%d, acc%d);"
+                w("                        const %s f = fma(thl[%d][lane + z * %d] - gD, c%d, acc%d);"
                   % (T, SLC + l // 2, k, r, r))
                 w("                        const double x0a = bld64(RX, v8, %du * P8);" % r)
                 w("                        if (euler) {")
@@ -661,11 +704,23 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
                 w("                    }")
             w("                }")
         w("            }")
+    if SPLIT > 1:
+        w("            }")
+        w("            __syncthreads();   // the next stage reads every wave's rows")
     w("        }")
     w("        if (euler) {")
-    w("            for (int r = 0; r < %d; r++) bst64(RX, v8, (uint32_t)r * P8, bld64(RE, v8, (uint32_t)r * P8));" % n)
+    if SPLIT > 1:
+        for wvi in range(SPLIT):
+            rs = [r for ci, (crow, _) in enumerate(chunks) if share[ci] == wvi for r in crow]
+            if rs:
+                w("            if (wv == %d) for (int r = %d; r <= %d; r++) bst64(RX, v8, (uint32_t)r * P8, bld64(RE, v8, (uint32_t)r * P8));" % (wvi, rs[0], rs[-1]))
+        w("            __syncthreads();")
+    else:
+        w("            for (int r = 0; r < %d; r++) bst64(RX, v8, (uint32_t)r * P8, bld64(RE, v8, (uint32_t)r * P8));" % n)
     w("        }")
     w("    }")
+    if SPLIT > 1:
+        w("    if (wv != 0 || !valid) return;")
     w("    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;")
     w("    if (a.status) {")
     w("        int st = 0;")
@@ -675,7 +730,7 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     w("}")
     w("#ifndef SFG_HOST_EMUL")
     w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
-    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + %d) / %d)), dim3(%d), 0, s, a);" % (name, B - 1, B, B))
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + %d) / %d)), dim3(%d), 0, s, a);" % (name, NL - 1, NL, B))
     w("    return hipGetLastError();")
     w("}")
     w("#endif")
@@ -699,12 +754,13 @@ def gen_dispatch(kinds):
     for (L, prec, cfgs) in kinds:
         for v in range(len(cfgs)):
             o.append("__global__ void evolve_fast_L%d_%s_v%d(EvolveArgs a);" % (L, prec, v))
-    o.append("const void* fast_kernel(int L, int prec, int variant, int* block) {")
+    o.append("const void* fast_kernel(int L, int prec, int variant, int* block, int* parcels) {")
     for (L, prec, cfgs) in kinds:
         for v, cfg in enumerate(cfgs):
-            o.append("    if (L == %d && prec == %d && variant == %d) { *block = %d; "
+            ppb = 64 if cfg.get("split", 1) > 1 else cfg["block"]
+            o.append("    if (L == %d && prec == %d && variant == %d) { *block = %d; *parcels = %d; "
                      "return (const void*)&evolve_fast_L%d_%s_v%d; }"
-                     % (L, 0 if prec == "f64" else 1, v, cfg["block"], L, prec, v))
+                     % (L, 0 if prec == "f64" else 1, v, cfg["block"], ppb, L, prec, v))
     o.append("    return nullptr;")
     o.append("}")
     o.append("#endif")

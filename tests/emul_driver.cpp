@@ -3,13 +3,49 @@
 // (-DSFG_HOST_EMUL, see specfab_amd/csrc/sfg_emul.h) and runs them lane by
 // lane, so the generated code's indexing and arithmetic can be checked against
 // the oracle without a GPU.
+#include <condition_variable>
 #include <cstdint>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "sfg_internal.h"
 #include "sfg_kernels.h"
 
 thread_local sfg_dim3 threadIdx, blockIdx, blockDim;
+
+// A block's threads run concurrently so that workgroup barriers work; a
+// thread that leaves the kernel drops out of the barrier (as a finished lane
+// does on the GPU).
+namespace {
+struct Barrier {
+    std::mutex m;
+    std::condition_variable cv;
+    int expected = 0, arrived = 0;
+    long gen = 0;
+    void release() {
+        arrived = 0;
+        gen++;
+        cv.notify_all();
+    }
+    void wait() {
+        std::unique_lock<std::mutex> lk(m);
+        const long g = gen;
+        if (++arrived == expected) release();
+        else cv.wait(lk, [&] { return gen != g; });
+    }
+    void drop() {
+        std::lock_guard<std::mutex> lk(m);
+        expected--;
+        if (arrived > 0 && arrived == expected) release();
+    }
+};
+thread_local Barrier* cur_barrier = nullptr;
+}  // namespace
+
+void sfg_emul_syncthreads() {
+    if (cur_barrier) cur_barrier->wait();
+}
 
 namespace sfg {
 __global__ void params_kernel(ParamArgs a);
@@ -26,6 +62,28 @@ static void grid(int64_t P, int bs, F f) {
             threadIdx.x = (unsigned)t;
             f();
         }
+}
+
+// nblocks blocks of bs threads, each block's threads concurrent
+template <class F>
+static void grid_threads(int64_t nblocks, int bs, F f) {
+    for (int64_t b = 0; b < nblocks; b++) {
+        Barrier bar;
+        bar.expected = bs;
+        std::vector<std::thread> th;
+        th.reserve(bs);
+        for (int t = 0; t < bs; t++)
+            th.emplace_back([&, t] {
+                blockIdx.x = (unsigned)b;
+                blockDim.x = (unsigned)bs;
+                threadIdx.x = (unsigned)t;
+                cur_barrier = &bar;
+                f();
+                cur_barrier = nullptr;
+                bar.drop();
+            });
+        for (auto& x : th) x.join();
+    }
 }
 
 // x: real SoA state (n, P), advanced in place.  which: 0 = generated kernel,
@@ -60,7 +118,7 @@ extern "C" int emul_evolve(int L, int64_t P, int procs, int scheme, int nsteps, 
     ea.theta = theta.data(); ea.omega = omega.data(); ea.sl = sl.data();
     ea.X = x; ea.YS = scr.data(); ea.XA = scr.data() + (size_t)n * P; ea.XB = scr.data() + (size_t)2 * n * P;
     ea.status = status; ea.zero = 0;
-    if (which == 0) grid(P, KERNEL_BLOCK, [&] { sfg::KERNEL_NAME(ea); });
+    if (which == 0) grid_threads((P + KERNEL_PARCELS - 1) / KERNEL_PARCELS, KERNEL_BLOCK, [&] { sfg::KERNEL_NAME(ea); });
     else grid(P, 64, [&] { sfg::evolve_generic(ea); });
     return 0;
 }

@@ -8,6 +8,7 @@
 #include "sfg_wave.h"
 
 thread_local sfg_dim3 threadIdx, blockIdx, blockDim, gridDim;
+void sfg_emul_syncthreads() {}
 
 namespace sfg {
 __global__ void vi_elastic_kernel(ViElasticArgs a);

@@ -19,8 +19,12 @@ _cache = {}
 
 
 def _block(src):
+    """(threads per workgroup, parcels per workgroup) of a generated kernel."""
     import re
-    return int(re.search(r"__launch_bounds__\((\d+),", open(src).read()).group(1))
+    text = open(src).read()
+    bs = int(re.search(r"__launch_bounds__\((\d+),", text).group(1))
+    m = re.search(r"hipLaunchKernelGGL\(\w+, dim3\(\(unsigned\)\(\(a\.P \+ \d+\) / (\d+)\)\)", text)
+    return bs, int(m.group(1))
 
 
 def emul_lib(L, prec, v=0):
@@ -32,7 +36,8 @@ def emul_lib(L, prec, v=0):
         pytest.skip("generated kernels not present (run the build)")
     out = os.path.join(tempfile.gettempdir(), "sfg_emul_%d_%s_v%d_%d.so" % (L, prec, v, os.getpid()))
     cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DSFG_HOST_EMUL",
-           "-DKERNEL_NAME=evolve_fast_L%d_%s_v%d" % (L, prec, v), "-DKERNEL_BLOCK=%d" % _block(src), "-I", CSRC, "-x", "c++",
+           "-DKERNEL_NAME=evolve_fast_L%d_%s_v%d" % (L, prec, v), "-DKERNEL_BLOCK=%d" % _block(src)[0],
+           "-DKERNEL_PARCELS=%d" % _block(src)[1], "-pthread", "-I", CSRC, "-x", "c++",
            os.path.join(ROOT, "tests", "emul_driver.cpp"), os.path.join(CSRC, "sfg_kernels.hip"),
            os.path.join(CSRC, "sfg_basis.cpp"), src, "-o", out]
     subprocess.check_call(cmd)

@@ -150,10 +150,13 @@ def test_fast_vs_generic_stepper(gpu_ctx):
         assert relerr(outs[0], outs[1]) < 1e-12, L
 
 
-@pytest.mark.parametrize("L", [8, 12])
+@pytest.mark.parametrize("L", [8, 12, 14, 20])
 def test_every_stepper_variant(gpu_ctx, oracle, L):
     """Every generated stepper variant (FP64 and FP32 operator) against the
-    oracle, and the autotuner's choice is one of them."""
+    oracle, and the autotuner's choice is one of them; L = 14 and 20 cover
+    the window steppers, including those that split a parcel group's row
+    chunks over several waves with workgroup barriers (P = 200: three full
+    64-parcel groups and a partial one)."""
     import os
     from specfab_amd import _lib
     from specfab_amd import workloads as W
@@ -184,14 +187,51 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32c", 2e-6)])
+def test_uneven_plan_every_window_variant(gpu_ctx, precision, tol):
+    """The L = 20 parcels of the uneven fixture, each repeated 200 times and
+    shuffled (600 parcels: ten parcel groups of 64, per-parcel step counts,
+    so the lanes of one wave stop at different steps), through a graph plan
+    with every window-stepper variant forced in turn, including those that
+    split a parcel group's rows over several waves with workgroup barriers
+    (64 parcels per 128-thread workgroup): final states against the
+    reference trajectories, every step counter at zero."""
+    import os
+    from specfab_amd import _lib
+    from specfab_amd.batch import FabricBatch, UnevenBatch
+    g = load_golden("uneven.npz")
+    L = 20
+    rep = np.random.default_rng(7).permutation(np.repeat(np.arange(len(g["steps_L20"])), 200))
+    nlm0, ug, g0, lam, steps, ref = (g["%s_L%d" % (k, L)][rep] for k in ("nlm0", "ugrad", "gam0", "lam", "steps", "nlm"))
+    _, times = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0).autotune(precision, P=256)
+    nv = len(times)
+    assert nv >= 3
+    for v in range(nv):
+        os.environ["SFG_VARIANT"] = str(v)
+        try:
+            ctx = _lib.Context(0)
+        finally:
+            del os.environ["SFG_VARIANT"]
+        ub = UnevenBatch([L] * len(steps), list(nlm0), np.array(ug), np.array(g0), np.array(lam), np.array(steps),
+                         ctx=ctx)
+        ub.make_plan(0.5, "rk4", ("LROT", "REG", "DDRX", "CDRX"), precision=precision, chunk=64).run()
+        out = ub.nlm()
+        err = max(relerr(out[k], ref[k]) for k in range(len(steps)))
+        print("L=20 %s variant %d: max rel err %.2e over %d parcels, steps %d..%d"
+              % (precision, v, err, len(steps), steps.min(), steps.max()))
+        assert err < tol, (precision, v, err)
+        assert (ub.remaining() == 0).all(), (precision, v)
+        ub.close()
+        ctx.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32c", 2e-6)])
 def test_uneven_plan_vs_reference(gpu_ctx, precision, tol):
     """Uneven batch (L in {4, 8, 12, 20} mixed, 50..500 steps per parcel,
     LROT+REG+DDRX+CDRX, RK4) through one graph launch, in a shuffled parcel
-    order, against the reference trajectories.  L = 20 has no FP32 stepper:
-    the FP32C plan covers L <= 16."""
+    order, against the reference trajectories."""
     from specfab_amd.batch import UnevenBatch
     g = load_golden("uneven.npz")
-    Ls_all = (4, 8, 12, 20) if precision == "fp64" else (4, 8, 12)
+    Ls_all = (4, 8, 12, 20)
     Ls, nlm0, ug, g0, lam, steps, ref = [], [], [], [], [], [], []
     for L in Ls_all:
         for k in range(g["nlm0_L%d" % L].shape[0]):
