@@ -123,6 +123,59 @@ def test_fp32c_field_error_vs_fp64(gpu_ctx, oracle):
     assert not st.any()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32c", 1e-5)])
+def test_full_size_field_sample(gpu_ctx, oracle, precision, tol):
+    """The headline field at full size (2^20 parcels, L = 12: a 763 MB state,
+    LROT + REG + DDRX, RK4 dt = 0.5, 20 steps) on the autotuned stepper.
+    Parcels are independent, so a sample of 64 of them (first and last,
+    wavefront and 2^16 boundaries, random others) stepped alone must be bit
+    for bit the same as inside the full batch (a check of the SoA indexing
+    over the whole state), and matches the oracle; over all 2^20 parcels n00
+    stays put (the LROT and DDRX source rows vanish up to the float32
+    rounding of the coupling tables) and no status bit is set."""
+    import math
+    import torch
+    from specfab_amd import _lib
+    from specfab_amd.batch import FabricBatch
+    dev = torch.device("cuda:0")
+    P, L, nsteps, procs = 1 << 20, 12, 20, ("LROT", "REG", "DDRX")
+    n = (L + 1) * (L + 2) // 2
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    G = torch.randn((P, 3, 3), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    G -= (G.diagonal(dim1=1, dim2=2).sum(-1) / 3)[:, None, None] * torch.eye(3, device=dev, dtype=torch.float64)
+    gam0 = 10.0 ** (torch.rand(P, generator=g, device=dev, dtype=torch.float64) * 2 - 4)
+    x0 = torch.randn((n, P), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    x0[0] = 1 / math.sqrt(4 * math.pi)
+
+    def batch(x, ug, g0):
+        b = FabricBatch.__new__(FabricBatch)
+        b.ctx = _lib.default_context()
+        b.L, b.n, b.dev, b.P = L, n, dev, x.shape[1]
+        b.x, b.ugrad, b.tau, b.gam0, b.lam, b.steps = x.contiguous(), ug.contiguous(), None, g0.contiguous(), None, None
+        b.status = torch.zeros(b.P, dtype=torch.int32, device=dev)
+        return b
+
+    fb = batch(x0.clone(), G, gam0)
+    fb.autotune(precision, P)
+    fb.evolve(nsteps, 0.5, scheme="rk4", procs=procs, precision=precision)
+    rng = np.random.default_rng(11)
+    edges = [0, 1, 63, 64, 65535, 65536, P // 2, P - 65, P - 64, P - 1]
+    sample = np.unique(np.concatenate([edges, rng.choice(P, 64 - len(edges), replace=False)]))
+    si = torch.from_numpy(sample).to(dev)
+    alone = batch(x0[:, si], G[si], gam0[si])
+    nlm0 = alone.nlm().cpu().numpy()
+    alone.evolve(nsteps, 0.5, scheme="rk4", procs=procs, precision=precision)
+    assert torch.equal(alone.x, fb.x[:, si])
+    ref = oracle.evolve(L, nlm0, G[si].cpu().numpy(), gam0[si].cpu().numpy(), 0.0, nsteps, 0.5, "rk4", procs)
+    err = relerr(alone.nlm().cpu().numpy(), ref)
+    drift = ((fb.x[0] - x0[0]).abs() / x0[0]).max().item()
+    print("full field %s: sample vs oracle %.2e, max n00 drift %.2e" % (precision, err, drift))
+    assert err < tol
+    assert drift < 1e-6
+    assert not fb.status.any().item()
+
+
 def test_fast_vs_generic_stepper(gpu_ctx):
     """The generated register-resident stepper and the table-driven generic
     stepper agree on the GPU (same inputs, FP64)."""
