@@ -340,7 +340,8 @@ void sfo_M_CDRX_diag(int L, double *diag) {
  * applies M_lin - Gamma0*<D>(x) I to the current state (dense matmul).
  * scheme 0 = explicit Euler, 1 = classic RK4.  procs bitmask 1 = LROT,
  * 2 = DDRX, 4 = CDRX, 8 = REG.  ugrad row-major (P,3,3); nlm (P,n) in place.
- * pinned: golden trajectories from oracle/_ref (C1 Euler, RK4 fixtures). */
+ * pinned: golden trajectories from oracle/_ref (the one-parcel L = 8 Euler
+ * uniaxial-compression case and the RK4 fixtures). */
 int sfo_evolve(int L, int P, int nsteps, int scheme, int procs, double dt, double iota, double zeta,
                const double *ugrad, const double *gam0, const double *lam, cdbl *nlm) {
     int n = sfo_nlm_len(L);

@@ -76,13 +76,17 @@ R = lambda chunk, waves, xn, split=True, block=64, ahead=False: dict(
 #           few parcels then still runs several waves per SIMD
 Wn = lambda window, chunk, waves, split=1: dict(mode="win", window=window, chunk=chunk, waves=waves,
                                                 block=64 * split, split=split)
+#   sreg    split register stepper (FP32): `split` waves per 64-parcel group,
+#           each holding the whole stage input and the increment sums of its
+#           share of the rows; stage inputs exchanged through LDS
+Sr = lambda chunk, waves, split=2: dict(mode="sreg", chunk=chunk, waves=waves, block=64 * split, split=split)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
                        V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False),
                        R(4, 1, "roll", False, ahead=True)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
-                       R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg")],
+                       R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg"), Sr(8, 2)],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
     ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2)],
     ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2)],
@@ -520,6 +524,158 @@ def gen_kernel_reg(L, prec, n, rows, cfg, vid):
     return "\n".join(o) + "\n"
 
 
+def gen_kernel_sreg(L, prec, n, rows, cfg, vid):
+    """Split register stepper ("sreg" mode, FP32 operator): S waves share one
+    group of 64 parcels.  Every wave holds the whole stage input x (n FP32
+    registers) and the RK increment sum of its own share of the rows (rows
+    split into contiguous shares by term count); at the end of a stage each
+    wave writes its rows' next-stage input to LDS and, after a workgroup
+    barrier, every wave reads all n rows back.  The whole step loop is
+    written once per share inside `if (wv == share)`, so a wave's registers
+    hold only its own increment sums; both copies run the same barriers (the
+    waves of a group step the same parcels, so their loop trip counts are
+    equal).  The FP64 state is read per row chunk (x0) and written in the
+    last stage by the row's owner.  Lanes past the batch stay alive for the
+    barriers with a lane offset past every buffer (loads return 0, stores
+    are dropped).
+    """
+    assert prec == "f32"
+    T = "float"
+    cv = lambda e: "(float)(%s)" % e
+    name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
+    CHUNK, S = cfg["chunk"], cfg["split"]
+    B = 64 * S
+    used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
+    NPU = len(used_p)
+    nq = min(n, 15)
+    nblk = L // 2 + 1
+    OMC, SLC = NPU, NPU + nq
+    pidx = {q: i for i, q in enumerate(used_p)}
+    nterm = [sum(len(tl) for (_, tl) in rows[r][1]) + 1 for r in range(n)]
+    tot, acc_t, share = sum(nterm), 0, []
+    for r in range(n):
+        share.append(min(S - 1, int(S * (acc_t + nterm[r] / 2) / tot)))
+        acc_t += nterm[r]
+    o = []
+    w = o.append
+    w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
+    w("// Split register stepper, L = %d (n = %d), FP32 stepping, FP64 state; variant %d: %s; row shares %s." %
+      (L, n, vid, ", ".join("%s=%s" % kv for kv in sorted(cfg.items())),
+       ", ".join("%d..%d" % (share.index(k), n - 1 - share[::-1].index(k)) for k in range(S))))
+    w('#include "../sfg_kernels.h"')
+    w("namespace sfg {")
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, cfg["waves"], name))
+    w("    const int64_t P = a.P;")
+    w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
+    w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
+    w("    const bool valid = p < P;")
+    w("    const uint32_t P8 = (uint32_t)P * 8u;")
+    w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
+    w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
+    w("    const int nstage = a.scheme ? 4 : 1;")
+    w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
+    w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
+    w("    const double h = a.dt;")
+    w("    // launch constants ([column][lane]: theta_p, omega_q, s_l) and the")
+    w("    // stage input exchanged between the waves")
+    w("    __shared__ float thl[%d][64];" % (NPU + nq + nblk))
+    w("    __shared__ float xs[%d][64];" % n)
+    w("    if (wv == 0 && valid) {")
+    for q in used_p:
+        w("        thl[%d][lane] = %s;" % (pidx[q], cv("a.theta[(int64_t)%d * P + p]" % q)))
+    for q in range(nq):
+        w("        thl[%d][lane] = %s;" % (OMC + q, cv("a.omega[(int64_t)%d * P + p]" % q)))
+    for b in range(nblk):
+        w("        thl[%d][lane] = %s;" % (SLC + b, cv("a.sl[(int64_t)%d * P + p]" % b)))
+    w("    }")
+    w("    float %s;" % ", ".join("x%d" % b for b in range(n)))
+    for b in range(n):
+        w("    x%d = %s;" % (b, cv("bld64(RX, v8, %du * P8)" % b)))
+    w("    __syncthreads();")
+    zc = [0]
+
+    def zk():
+        zc[0] += 1
+        return zc[0]
+
+    for sh in range(S):
+        mine = [r for r in range(n) if share[r] == sh]
+        chunks = [mine[c0:c0 + CHUNK] for c0 in range(0, len(mine), CHUNK)]
+        w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
+        w("        float %s;" % ", ".join("y%d = 0.0f" % r for r in mine))
+        w("        for (int step = 0; step < ns; step++) {")
+        w("            for (int s = 0; s < nstage; s++) {")
+        w("                const bool first = (s == 0), last = (s == nstage - 1);")
+        w("                const float cy = %s;" % cv("(nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3)"))
+        w("                const float cx = %s;" % cv("(s == 2) ? h : h / 2"))
+        w("                // a.zero is 0 at run time but unknown to the compiler: it keeps the")
+        w("                // LDS reads of different row chunks and stages apart")
+        w("                const int z = (int)a.zero * (step * 4 + s);")
+        w("                float gD = 0.0f;")
+        w("                if (a.use_ddrx) {")
+        k = zk()
+        terms = " + ".join("thl[%d][lane + z * %d] * x%d" % (OMC + q, k, q) for q in range(nq))
+        w("                    gD = (%s) / x0;" % terms)
+        w("                }")
+        for chunk in chunks:
+            k = zk()
+            w("                {   // rows %s" % ", ".join(str(r) for r in chunk))
+            w("                    const double %s;" % ", ".join("xo%d = bld64(RX, v8, %du * P8)" % (r, r) for r in chunk))
+            w("                    float %s;" % ", ".join("acc%d = 0.0f" % r for r in chunk))
+            byp = {}
+            for r in chunk:
+                for (pp, tl) in rows[r][1]:
+                    byp.setdefault(pp, []).append((r, tl))
+            for pp in sorted(byp):
+                w("                    {")
+                w("                        const float th = thl[%d][lane + z * %d];" % (pidx[pp], k))
+                for (r, tl) in byp[pp]:
+                    col, ti, v = tl[0]
+                    w("                        float u%d = %s * x%d;" % (r, lit(v, prec), col))
+                    for (col, ti, v) in tl[1:]:
+                        w("                        u%d = fma(%s, x%d, u%d);" % (r, lit(v, prec), col, r))
+                    w("                        acc%d = fma(th, u%d, acc%d);" % (r, r, r))
+                w("                    }")
+            for r in chunk:
+                l = rows[r][0]
+                w("                    {")
+                w("                        const float f = fma(thl[%d][lane + z * %d] - gD, x%d, acc%d);"
+                  % (SLC + l // 2, k, r, r))
+                w("                        const float yv = fma(cy, f, first ? 0.0f : y%d);" % r)
+                w("                        if (last) {")
+                w("                            const double xn = xo%d + (double)yv;" % r)
+                w("                            bst64(RX, v8, %du * P8, xn);" % r)
+                w("                            xs[%d][lane] = (float)xn;" % r)
+                w("                        } else {")
+                w("                            y%d = yv;" % r)
+                w("                            xs[%d][lane] = fma(cx, f, (float)xo%d);" % (r, r))
+                w("                        }")
+                w("                    }")
+            w("                }")
+        w("                __syncthreads();   // every wave's rows of the next stage input are in xs")
+        w("                %s" % " ".join("x%d = xs[%d][lane];" % (b, b) for b in range(n)))
+        w("                __syncthreads();   // read before the next stage overwrites xs")
+        w("            }")
+        w("        }")
+    w("    }")
+    w("    if (wv != 0 || !valid) return;")
+    w("    if (a.steps && a.steps_dec) a.steps[p] = rem - ns;")
+    w("    if (a.status) {")
+    w("        int st = 0;")
+    w("        for (int r = 0; r < %d; r++) st |= isfinite(bld64(RX, v8, (uint32_t)r * P8)) ? 0 : 1;" % n)
+    w("        a.status[p] |= st;")
+    w("    }")
+    w("}")
+    w("#ifndef SFG_HOST_EMUL")
+    w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + 63) / 64)), dim3(%d), 0, s, a);" % (name, B))
+    w("    return hipGetLastError();")
+    w("}")
+    w("#endif")
+    w("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
 def gen_kernel_win(L, prec, n, rows, cfg, vid):
     """Window stepper ("win" mode) for states too large for a lane's
     registers (L >= 14 FP64, L >= 18 FP32).
@@ -798,7 +954,8 @@ def main():
             kind = "wide" if L in WIDE_L[prec] else ("tuned" if L in TUNED_L else "default")
             vs = VARIANTS[(prec, kind)]
             for vid, cfg in enumerate(vs):
-                gen = {"reg": gen_kernel_reg, "win": gen_kernel_win}.get(cfg["mode"], gen_kernel)
+                gen = {"reg": gen_kernel_reg, "win": gen_kernel_win, "sreg": gen_kernel_sreg}.get(cfg["mode"],
+                                                                                                gen_kernel)
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
                                  gen(L, prec, n, rows, cfg, vid))
             kinds.append((L, prec, vs))
