@@ -226,11 +226,13 @@ def bench_surrogate(a):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(kernel, P):
-    """HBM bytes of one launch of `kernel` on P parcels: rocprofv3 FETCH_SIZE
-    and WRITE_SIZE of the bench command, scaled by a 1 GiB copy (x 2048 and
-    x 1024 bytes per KiB counted), per parcel, times P; from the committed
-    summary profiles/evolve_traffic.json (null for a kernel not measured)."""
+def measured_traffic(kernel, P, steps):
+    """HBM bytes of one launch of `kernel` on P parcels for `steps` steps:
+    rocprofv3 FETCH_SIZE and WRITE_SIZE of the bench command, scaled by a 1 GiB
+    copy (x 2048 and x 1024 bytes per KiB counted), per parcel-step of a launch
+    of that many steps, times P x steps; from the committed summary
+    profiles/evolve_traffic.json (null for a kernel or launch length not
+    measured)."""
     path = os.path.join(ROOT, "profiles", "evolve_traffic.json")
     try:
         with open(path) as f:
@@ -238,9 +240,9 @@ def measured_traffic(kernel, P):
     except (OSError, ValueError, KeyError):
         return None
     ent = tab.get(kernel)
-    if not ent:
+    if not ent or ent.get("steps_per_launch", 1) != steps:
         return None
-    return ent["bytes_per_parcel"] * P
+    return ent["bytes_per_parcel_step"] * P * steps
 
 
 def main():
@@ -292,8 +294,10 @@ def main():
     # autotune on the full field, so that every launch of the chosen kernel
     # (also in a rocprofv3 trace of this command) has the timed size
     variant, variant_ms = fb.autotune(a.precision, P)
-    for _ in range(a.warmup):
-        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
+    # the time loop as a caller runs it: all steps in one call (one launch of
+    # the stepper, whose lanes loop over the steps); warmup likewise
+    if a.warmup:
+        fb.evolve(a.warmup, 0.5, scheme="rk4", procs=procs, precision=a.precision)
     torch.cuda.synchronize()
     # the stepping kernel is bracketed by HIP events inside the library, on the
     # stream it is launched on (torch's current stream)
@@ -301,22 +305,21 @@ def main():
     _lib.check(lib.sfg_ctx_set_timing(fb.ctx.handle, 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(a.steps):
-        fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision=a.precision)
+    fb.evolve(a.steps, 0.5, scheme="rk4", procs=procs, precision=a.precision)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     import ctypes
     tot, cnt = ctypes.c_double(0), ctypes.c_int(0)
     _lib.check(lib.sfg_ctx_read_timing(fb.ctx.handle, ctypes.byref(tot), ctypes.byref(cnt)))
     _lib.check(lib.sfg_ctx_set_timing(fb.ctx.handle, 0))
-    launch_ms = tot.value / max(1, cnt.value)
+    launch_ms = tot.value / max(1, cnt.value)   # one launch of a.steps steps
     value = P * a.steps / wall
     nonfinite = int((fb.status != 0).sum().item())
-    flops = flops_per_parcel_step(L) * P
+    flops = flops_per_parcel_step(L) * P * a.steps
     achieved = flops / (launch_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[a.precision]
     kname = "evolve_fast_L%d_%s_v%d" % (L, "f64" if a.precision == "fp64" else "f32", variant)
-    traffic = measured_traffic(kname, P)
+    traffic = measured_traffic(kname, P, a.steps)
     # Eij of the evolved field (eigenframe, ice linear Sachs/Taylor mix)
     E0, E1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     fb.Eij((1.0, 1e3), 0.0125, 1)
@@ -356,8 +359,9 @@ def main():
                      "frac": achieved / peak, "traffic": traffic,
                      "compute": "%s vector ALU (sparse FMA; MI355X %s peak %.1f TFLOP/s)"
                                 % ("fp64" if a.precision == "fp64" else "fp32", a.precision, peak),
-                     "kernel": "%s (one RK4 step of the field)" % kname,
-                     "flop_per_parcel_step": flops_per_parcel_step(L), "launch_ms": launch_ms},
+                     "kernel": "%s (one launch: %d RK4 steps of the field)" % (kname, a.steps),
+                     "flop_per_parcel_step": flops_per_parcel_step(L), "steps_per_launch": a.steps,
+                     "launch_ms": launch_ms},
         "autotune": {"variant": variant, "ms_per_step_by_variant": variant_ms, "probe_parcels": P},
         "eij_evals_per_s": eij_rate,
         "vi_elastic_evals_per_s": vi_rate,
