@@ -120,7 +120,7 @@ def bench_uneven(a):
         nlm.append(x)
     ub = B.UnevenBatch(Ls, nlm, G, g0, lam, steps)
     for L, idx, fb, rem in ub.buckets:
-        fb.autotune(a.precision)
+        fb.autotune(a.precision, steps=10)   # the plan launches 50 steps at a time
     ub.make_plan(0.5, "rk4", ("LROT", "REG", "DDRX", "CDRX"), precision=a.precision, chunk=50)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -187,8 +187,8 @@ def bench_surrogate(a):
     # in-loop use: one RK4 step of the field followed by its homogenization,
     # plain FP64 (stepping + Eij + Vi) vs FP32 stepping + the network
     procs = ("LROT", "REG", "DDRX")
-    fb.autotune("fp64", P)
-    fb.autotune("fp32c", P)
+    fb.autotune("fp64", P, steps=1)    # the loop below makes one-step calls
+    fb.autotune("fp32c", P, steps=1)
 
     def plain_step():
         fb.evolve(1, 0.5, scheme="rk4", procs=procs, precision="fp64")

@@ -74,11 +74,13 @@ const char* sfg_last_error(void);
 
 /* Autotuner, no counterpart in the reference: times every generated
  * register-resident stepper variant of (L, precision) on a synthetic batch of
- * P parcels (launches of 4 RK4 steps, LROT+REG+DDRX) and keeps the fastest
- * for this context.  *best = chosen variant (-1: no generated stepper for L);
- * ms[v] = time per step with variant v (first ms_len variants).  Environment
- * variable SFG_VARIANT=k forces variant k. */
-int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, void* stream, int* best, double* ms, int ms_len);
+ * P parcels (launches of `steps` RK4 steps, LROT+REG+DDRX: the launch length
+ * the caller will use, since a launch's prologue is paid once) and keeps the
+ * fastest for this context.  *best = chosen variant (-1: no generated stepper
+ * for L); ms[v] = time per step with variant v (first ms_len variants).
+ * Environment variable SFG_VARIANT=k forces variant k. */
+int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, int steps, void* stream, int* best, double* ms,
+                 int ms_len);
 
 /* Profiling hook, no counterpart in the reference: while enabled, every
  * sfg_evolve brackets its stepping kernel with HIP events on the launch

@@ -48,7 +48,7 @@ SIGNATURES = {
     "sfg_ctx_destroy": (_i, [_vp]),
     "sfg_last_error": (_c.c_char_p, []),
     "sfg_ctx_set_timing": (_i, [_vp, _i]),
-    "sfg_autotune": (_i, [_vp, _i, _i, _i64, _vp, _ip, _dp, _i]),
+    "sfg_autotune": (_i, [_vp, _i, _i, _i64, _i, _vp, _ip, _dp, _i]),
     "sfg_ctx_read_timing": (_i, [_vp, _dp, _ip]),
     "sfg_nlm_len": (_i, [_i]),
     "sfg_lm": (_i, [_i, _ip]),

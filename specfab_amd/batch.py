@@ -147,15 +147,17 @@ class FabricBatch:
         check(lib().sfg_apply_bounds_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _stream()))
         return self
 
-    def autotune(self, precision="fp64", P=None):
+    def autotune(self, precision="fp64", P=None, steps=4):
         """Time the generated stepper variants for this L and precision on a
-        synthetic batch (default: min(P, 2^18) parcels) and keep the fastest
-        for the context.  Returns (chosen variant, per-variant ms per step)."""
+        synthetic batch (default: min(P, 2^18) parcels) in launches of
+        `steps` steps (the steps per evolve call the caller will make) and
+        keep the fastest for the context.  Returns (chosen variant,
+        per-variant ms per step)."""
         Pp = int(P or min(self.P, 1 << 18))
         best = _c.c_int(-1)
         ms = (_c.c_double * 16)()
         prec = _lib.SFG_FP64 if precision == "fp64" else _lib.SFG_FP32C
-        check(lib().sfg_autotune(self.ctx.handle, self.L, prec, Pp, _stream(), _c.byref(best), ms, 16))
+        check(lib().sfg_autotune(self.ctx.handle, self.L, prec, Pp, int(steps), _stream(), _c.byref(best), ms, 16))
         nv = max(0, best.value + 1)
         times = [ms[i] for i in range(16) if ms[i] > 0]
         return best.value, times

@@ -486,8 +486,10 @@ int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     return evolve_locked(ctx, d, pick(ctx, stream), -1);
 }
 
-int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, void* stream, int* best, double* ms, int ms_len) {
-    if (!ctx || P <= 0 || (precision != SFG_FP64 && precision != SFG_FP32C)) return fail(SFG_ERR_ARG, "bad argument");
+int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, int steps, void* stream, int* best, double* ms,
+                 int ms_len) {
+    if (!ctx || P <= 0 || steps <= 0 || (precision != SFG_FP64 && precision != SFG_FP32C))
+        return fail(SFG_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     const int prec = precision == SFG_FP64 ? 0 : 1;
     const int nv = sfg::fast_variants(L, prec);
@@ -526,10 +528,7 @@ int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, void* stream, in
     HIPCHK(hipMemset(bst.p, 0, (size_t)P * 4));
     sfg_evolve_desc d{};
     d.L = L; d.P = P; d.procs = SFG_LROT | SFG_REG | SFG_DDRX; d.scheme = SFG_RK4; d.precision = precision;
-    // launches of several steps, as time loops make them (a launch's
-    // prologue, the launch constants and the state coming in, is amortised)
-    constexpr int PROBE_STEPS = 4;
-    d.nsteps = PROBE_STEPS; d.dt = 0.5; d.iota = 1.0; d.zeta = 0.0;
+    d.nsteps = steps; d.dt = 0.5; d.iota = 1.0; d.zeta = 0.0;
     d.ugrad_dev = (double*)bg.p; d.gam0_dev = (double*)bgam.p; d.x_dev = (double*)bx.p; d.status_dev = (int32_t*)bst.p;
     const bool timing = ctx->timing;
     ctx->timing = false;
@@ -548,7 +547,7 @@ int sfg_autotune(sfg_ctx* ctx, int L, int precision, int64_t P, void* stream, in
         HIPCHK(hipEventSynchronize(e1));
         float t = 0;
         HIPCHK(hipEventElapsedTime(&t, e0, e1));
-        if (ms && v < ms_len) ms[v] = t / (3 * PROBE_STEPS);
+        if (ms && v < ms_len) ms[v] = t / (3 * steps);
         if (t < tbest) { tbest = t; vbest = v; }
     }
     (void)hipEventDestroy(e0);
