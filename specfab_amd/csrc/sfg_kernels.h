@@ -7,14 +7,6 @@
 #endif
 #include <stdint.h>
 
-// Constant address space for read-only coupling tables: loads through such a
-// pointer with a uniform address become scalar (SGPR) loads.
-#ifdef SFG_HOST_EMUL
-#define SFG_CONST_AS
-#else
-#define SFG_CONST_AS __attribute__((address_space(4)))
-#endif
-
 namespace sfg {
 
 // Raw buffer access for the generated steppers: a wave-uniform resource

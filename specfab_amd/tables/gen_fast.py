@@ -42,9 +42,11 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 FAST_L = {"f64": (4, 6, 8, 10, 12, 14, 16, 18, 20), "f32": (4, 6, 8, 10, 12, 14, 16, 18, 20)}
 
 # Kernel variants (the library's autotuner times them and keeps the fastest):
-#   mode    "lit": coupling values as literal constants in the code;
-#           "tab": read from the term array through the constant address
-#                  space (scalar loads shared by the wavefront)
+#   mode    "lit": coupling values as literal constants in the code, the RK
+#           increment sum and next-stage input in SoA buffers ("reg", "win",
+#           "sreg" below); coupling values read through the constant address
+#           space (scalar loads) measured 3x slower than literals and were
+#           dropped
 #   chunk   output rows accumulated together
 #   waves   waves per SIMD requested from the register allocator
 #   bfree   branch-free RK update (stage flags folded into buffer offsets)
@@ -139,9 +141,7 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
     cv = (lambda e: e) if prec == "f64" else (lambda e: "(float)(%s)" % e)
     name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
     CHUNK, BRANCHFREE, FENCE = cfg["chunk"], cfg["bfree"], cfg["fence"]
-    tab = cfg["mode"] == "tab"
-    Kt = "a.tval" if prec == "f64" else "a.tval_f"
-    K = (lambda v, ti: "K[%d]" % ti) if tab else (lambda v, ti: lit(v, prec))
+    K = lambda v, ti: lit(v, prec)
     used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
     nq = min(n, 15)
     nblk = L // 2 + 1
@@ -226,10 +226,6 @@ def gen_kernel(L, prec, n, rows, cfg, vid):
         for pp in sorted(byp):
             w("                {")
             w("                    const %s th = thl[%d][threadIdx.x + z * %d];" % (T, pidx[pp], c0 // CHUNK + 1))
-            if tab:
-                # per-chunk opaque offset: the scalar loads stay in this chunk
-                w("                    const SFG_CONST_AS %s* K = (const SFG_CONST_AS %s*)(%s + z * %d);"
-                  % (T, T, Kt, c0 // CHUNK + 1))
             for (r, tl) in byp[pp]:
                 col, ti, v = tl[0]
                 w("                    %s u%d = %s * x%d;" % (T, r, K(v, ti), col))
