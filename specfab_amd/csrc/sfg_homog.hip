@@ -149,7 +149,27 @@ __device__ inline void d_chol6_solve(const double* Lf, double* b) {
 #undef LF
 }
 
-// Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69
+// Taylor matrix P = I6 - A 1 (x) a2v + B a4v + C L(a2) (src/homogenizations.f90:164-172)
+__device__ inline void d_taylorP(const double* a2m, const double* a2v, const double* a4v, double tA, double tB,
+                                 double tC, double Pm[6][6]) {
+    const double s = sqrt(2.0);
+    const double* a = a2m;
+    const double Lm[6][6] = {
+        {2 * a[0], 0, 0, 0, s * a[2], s * a[1]},
+        {0, 2 * a[4], 0, s * a[5], 0, s * a[1]},
+        {0, 0, 2 * a[8], s * a[5], s * a[2], 0},
+        {0, s * a[5], s * a[5], a[4] + a[8], a[1], a[2]},
+        {s * a[2], 0, s * a[2], a[1], a[0] + a[8], a[5]},
+        {s * a[1], s * a[1], 0, a[2], a[5], a[0] + a[4]}};
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++)
+            Pm[i][j] = (i == j ? 1.0 : 0.0) - tA * ((i < 3 ? 1.0 : 0.0) * a2v[j]) + tB * a4v[6 * i + j] + tC * Lm[i][j];
+}
+
+// Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69.
+// P is factored once per parcel; the rare regularized branch rebuilds P
+// where it needs it, so P does not occupy registers through the loop over
+// the six stress states.
 __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, double Ecc, double Eca, double alpha,
                      int ng, double* Eij) {
     int status = 0;
@@ -160,38 +180,26 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
     const double taylor_iso = 1.0 / (1 + 2.0 / 15 * tB + 2.0 / 3 * tC);   // :235
     double a2m[9];
     d_vec_to_mat(a2v, a2m);
-    // Taylor matrix P = I6 - A 1 (x) a2v + B a4v + C L(a2), factored once
-    double Pm[6][6];
-    {
-        const double s = sqrt(2.0);
-        const double* a = a2m;
-        const double Lm[6][6] = {
-            {2 * a[0], 0, 0, 0, s * a[2], s * a[1]},
-            {0, 2 * a[4], 0, s * a[5], 0, s * a[1]},
-            {0, 0, 2 * a[8], s * a[5], s * a[2], 0},
-            {0, s * a[5], s * a[5], a[4] + a[8], a[1], a[2]},
-            {s * a[2], 0, s * a[2], a[1], a[0] + a[8], a[5]},
-            {s * a[1], s * a[1], 0, a[2], a[5], a[0] + a[4]}};
-        for (int i = 0; i < 6; i++)
-            for (int j = 0; j < 6; j++)
-                Pm[i][j] = (i == j ? 1.0 : 0.0) - tA * ((i < 3 ? 1.0 : 0.0) * a2v[j]) + tB * a4v[6 * i + j] + tC * Lm[i][j];
-    }
+    // the Cholesky factor of P, or of the regularized normal equations
+    // (P^T P + 1e-6 I, the REAL(4) 1e-6 of :178) when P is not positive definite
     double Lf[21];
-    bool ok = d_chol6(Pm, Lf);
-    double R[6][6];
-    double Lr[21];
-    bool ok_reg = true;
-    if (!ok) {
-        // regularized normal equations (P^T P + 1e-6 I), the REAL(4) 1e-6 of :178
-        status |= 2;
-        for (int i = 0; i < 6; i++)
-            for (int j = 0; j < 6; j++) {
-                double s = 0;
-                for (int k = 0; k < 6; k++) s += Pm[k][i] * Pm[k][j];
-                R[i][j] = s + (i == j ? (double)1e-6f : 0.0);
-            }
-        ok_reg = d_chol6(R, Lr);
-        if (!ok_reg) status |= 4;
+    bool ok, ok_reg = true;
+    {
+        double Pm[6][6];
+        d_taylorP(a2m, a2v, a4v, tA, tB, tC, Pm);
+        ok = d_chol6(Pm, Lf);
+        if (!ok) {
+            status |= 2;
+            double R[6][6];
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) {
+                    double s = 0;
+                    for (int k = 0; k < 6; k++) s += Pm[k][i] * Pm[k][j];
+                    R[i][j] = s + (i == j ? (double)1e-6f : 0.0);
+                }
+            ok_reg = d_chol6(R, Lf);
+            if (!ok_reg) status |= 4;
+        }
     }
     const int pv[6] = {0, 1, 2, 1, 0, 0}, pw[6] = {0, 1, 2, 2, 2, 1};
     for (int q = 0; q < 6; q++) {
@@ -231,12 +239,13 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
         if (ok) {
             d_chol6_solve(Lf, xs);
         } else {
-            double rhs[6];
+            double Pm[6][6], rhs[6];
+            d_taylorP(a2m, a2v, a4v, tA, tB, tC, Pm);
             for (int i = 0; i < 6; i++) {
                 rhs[i] = 0;
                 for (int k = 0; k < 6; k++) rhs[i] += Pm[k][i] * xs[k];
             }
-            if (ok_reg) d_chol6_solve(Lr, rhs);
+            if (ok_reg) d_chol6_solve(Lf, rhs);
             for (int i = 0; i < 6; i++) xs[i] = rhs[i];
         }
         d_vec_to_mat(xs, et);
