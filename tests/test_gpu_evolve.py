@@ -209,7 +209,9 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
     oracle, and the autotuner's choice is one of them; L = 14 and 20 cover
     the window steppers, including those that split a parcel group's row
     chunks over several waves with workgroup barriers (P = 200: three full
-    64-parcel groups and a partial one)."""
+    64-parcel groups and a partial one); RK4 and explicit Euler (whose
+    window steppers write the new state to a scratch buffer and copy it back
+    per wave)."""
     import os
     from specfab_amd import _lib
     from specfab_amd import workloads as W
@@ -220,7 +222,7 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
     ug = W.traceless_gradients(rng, P)
     g0 = 10.0 ** rng.uniform(-4, -2, P)
     procs = ("LROT", "REG", "DDRX")
-    ref = oracle.evolve(L, nlm0, ug, g0, 0.0, 5, 0.5, "rk4", procs)
+    ref = {sc: oracle.evolve(L, nlm0, ug, g0, 0.0, 5, 0.5, sc, procs) for sc in ("rk4", "euler")}
     for prec, tol in (("fp64", 1e-12), ("fp32c", 2e-6)):
         b0 = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0)
         best, times = b0.autotune(prec, P=4096)
@@ -232,10 +234,11 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
                 ctx = _lib.Context(0)
             finally:
                 del os.environ["SFG_VARIANT"]
-            b = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0, ctx=ctx)
-            b.evolve(5, 0.5, scheme="rk4", procs=procs, precision=prec)
-            err = relerr(b.nlm().cpu().numpy(), ref)
-            assert err < tol, (prec, v, err)
+            for sc in ("rk4", "euler"):
+                b = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0, ctx=ctx)
+                b.evolve(5, 0.5, scheme=sc, procs=procs, precision=prec)
+                err = relerr(b.nlm().cpu().numpy(), ref[sc])
+                assert err < tol, (prec, v, sc, err)
             ctx.close()
 
 
