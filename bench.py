@@ -270,6 +270,81 @@ def measured_traffic(kernel, P, steps):
     return ent["bytes_per_parcel_step"] * P * steps
 
 
+def reduce_M_bytes_per_node(L):
+    """Algorithmic HBM bytes of reduce_M per node: the n complex columns of the
+    (L+2)^2/4 reduced rows read once (16 B each), four (rn, rn) FP64 outputs written."""
+    n, rn = (L + 1) * (L + 2) // 2, (L + 2) ** 2 // 4
+    return rn * n * 16 + 4 * rn * rn * 8
+
+
+def bench_export(a):
+    """Eulerian operator export: M_LROT of every mesh node built on the device and
+    reduced to (Mrr, Mri, Mir, Mii) (src/specfabpy/fenics/CPO.py:200 with
+    src/reducedform.f90:76-120), 2^14 nodes at L = 12."""
+    import numpy as np
+    import torch
+    from specfab_amd import batch as B
+    P, L = 1 << 14, 12
+    rng = np.random.default_rng(9)
+    e = rng.standard_normal((P, 3, 3))
+    w = rng.standard_normal((P, 3, 3))
+    eps = torch.from_numpy(e + e.transpose(0, 2, 1)).cuda()
+    omg = torch.from_numpy(w - w.transpose(0, 2, 1)).cuda()
+    M = B.M_LROT_batch(L, eps, omg, 1.0, 0.0)
+    for _ in range(a.warmup):
+        B.M_LROT_batch(L, eps, omg, 1.0, 0.0)
+        B.reduce_M_batch(L, M)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(a.steps):
+        B.M_LROT_batch(L, eps, omg, 1.0, 0.0)
+    ev[1].record()
+    for _ in range(a.steps):
+        out = B.reduce_M_batch(L, M)
+    ev[2].record()
+    torch.cuda.synchronize()
+    ms_lrot = ev[0].elapsed_time(ev[1]) / a.steps
+    ms_red = ev[1].elapsed_time(ev[2]) / a.steps
+    del out
+    n = (L + 1) * (L + 2) // 2
+    per_dense = n * n * 16 + 2 * 9 * 8          # M written once, eps and omg read
+    per = reduce_M_bytes_per_node(L)
+    ach = per_dense * P / (ms_lrot * 1e-3) / 1e9
+    ach_red = per * P / (ms_red * 1e-3) / 1e9
+    tab = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                      "evolve_traffic.json")))["kernels"]
+    traffic = {k: tab[k]["bytes_per_parcel"] * P if k in tab and tab[k]["parcels"] == P else None
+               for k in ("dense_op_kernel", "reduce_M_kernel")}
+    cpu = None
+    if not a.no_cpu:
+        from oracle import ref
+        if ref.available():
+            r = ref.RefLib()
+            r.init(L)
+            k, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 10.0:
+                r.reduce_M(r.M_LROT(e[k % P] + e[k % P].T, w[k % P] - w[k % P].T, 1.0, 0.0))
+                k += 1
+            cpu = {"value": k / (time.perf_counter() - t0), "unit": "nodes/s", "cores": 1, "kind": "reference",
+                   "sample": "%d nodes, M_LROT + reduce_M of the reference FP64 Fortran (oracle/_ref), 10 s" % k}
+    out = {"metric": "nodes/s, Eulerian operator export (M_LROT + reduce_M per node)",
+           "value": P / ((ms_lrot + ms_red) * 1e-3), "unit": "nodes/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_lrot + ms_red, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
+           "config": {"workload": "Eulerian operator export: 2^14 nodes, L=12 (91 coeffs, 49 reduced), M_LROT "
+                                  "built and reduced to Mrr/Mri/Mir/Mii", "nodes": P},
+           "ms_M_LROT": ms_lrot, "ms_reduce_M": ms_red,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                        "traffic": traffic["dense_op_kernel"], "kernel": "dense_op_kernel",
+                        "bytes_per_node": per_dense},
+           "roofline_reduce_M": {"bound": "hbm", "achieved": ach_red, "peak": 8000.0, "unit": "GB/s",
+                                 "frac": ach_red / 8000.0, "traffic": traffic["reduce_M_kernel"],
+                                 "kernel": "reduce_M_kernel", "bytes_per_node": per},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,7 +354,7 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate"],
+    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate", "export"],
                     help="field: the L=12 Eulerian field (default, the headline); uneven: 2^18 parcels with mixed "
                          "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs learned")
     ap.add_argument("--copy-probe", action="store_true",
@@ -307,6 +382,8 @@ def main():
         return bench_uneven(a)
     if a.workload == "surrogate":
         return bench_surrogate(a)
+    if a.workload == "export":
+        return bench_export(a)
     dev = torch.device("cuda:0")
     P, L = a.parcels, a.L
     G, gam0, x = make_field(P, L, 1, dev)

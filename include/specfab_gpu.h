@@ -119,6 +119,28 @@ int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, 
 int sfg_nlm_to_state(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* x_dev, void* stream);
 int sfg_state_to_nlm(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* nlm_dev, void* stream);
 
+/* ---- reduced real form (src/reducedform.f90), the 2-D x-z Eulerian export
+ * of src/specfabpy/fenics/CPO.py:200-202.  rnlm = (n_0^0, n_2^0, n_2^1,
+ * n_2^2, n_4^0, ...), the m >= 0 coefficients, (L+2)^2/4 complex values; a
+ * real ODF has n_l^{-m} = (-1)^m conj(n_l^m).  Even L, 0 <= L <= 30. ---- */
+/* rnlm_len (src/specfabpy.f90:1039-1044); < 0 = error code */
+int sfg_rnlm_len(int L);
+/* reduce_M (src/reducedform.f90:76-120, bound at src/specfabpy.f90:1066-1074):
+ * complex M (n, n) row-major -> real Mrr, Mri, Mir, Mii (rn, rn) row-major with
+ * d(rnlm)/dt = Mrr Re r + Mri Im r + i (Mir Re r + Mii Im r) */
+int sfg_reduce_M(sfg_ctx* ctx, int L, const double* M, double* Mrr, double* Mri, double* Mir, double* Mii);
+/* nlm_to_rnlm (src/reducedform.f90:176-183, src/specfabpy.f90:1046-1054) */
+int sfg_nlm_to_rnlm(sfg_ctx* ctx, int L, const double* nlm, double* rnlm);
+/* rnlm_to_nlm (src/reducedform.f90:160-174, src/specfabpy.f90:1056-1064) */
+int sfg_rnlm_to_nlm(sfg_ctx* ctx, int L, const double* rnlm, double* nlm);
+/* batched over P nodes (device buffers): M_dev (P, n, n) complex as written
+ * by sfg_M_LROT_batch / sfg_M_DDRX_src_batch -> Mrr..Mii_dev (P, rn, rn);
+ * nlm_dev (P, n) complex <-> rnlm_dev (P, rn) complex */
+int sfg_reduce_M_batch(sfg_ctx* ctx, int L, int64_t P, const double* M_dev, double* Mrr_dev, double* Mri_dev,
+                       double* Mir_dev, double* Mii_dev, void* stream);
+int sfg_nlm_to_rnlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* rnlm_dev, void* stream);
+int sfg_rnlm_to_nlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* rnlm_dev, double* nlm_dev, void* stream);
+
 /* ---- batched evolution ----
  * Steps P parcels of truncation L in place.  Per parcel: velocity gradient
  * ugrad (D = sym, W = skew), optional stress tau (NULL: tau := D, the coaxial

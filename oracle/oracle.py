@@ -193,3 +193,62 @@ class Oracle:
         self.lib.sfo_Vi_electromagnetic_tranisotropic(self._z(nlm), float(eps_m), float(eps_t), float(mu),
                                                       th.size, th, ph, out)
         return out.T.copy()
+
+
+# ---- reduced real form (numpy; index and sign work only) ----------------------
+# Restates src/reducedform.f90: rnlm holds the m >= 0 coefficients of every even
+# l in the order (n_0^0, n_2^0, n_2^1, n_2^2, n_4^0, ...) (:37-44, rlm at :52),
+# and a real ODF has n_l^{-m} = (-1)^m conj(n_l^m).
+
+def rnlm_len(L):
+    """(L+2)^2/4 (src/reducedform.f90:25)."""
+    return (L + 2) ** 2 // 4
+
+
+def rlm_list(L):
+    return [(l, m) for l in range(0, L + 1, 2) for m in range(0, l + 1)]
+
+
+def _full_pos(L):
+    """0-based nlm position of (l, m) (the I_all index list, src/reducedform.f90:37)."""
+    pos = {lm: k for k, lm in enumerate(lm_list(L))}
+    return pos
+
+
+def nlm_to_rnlm(nlm, L):
+    """nlm_to_rnlm (src/reducedform.f90:176-183): rnlm = nlm at the m >= 0 positions."""
+    pos = _full_pos(L)
+    return np.asarray(nlm, dtype=np.complex128)[..., [pos[lm] for lm in rlm_list(L)]]
+
+
+def rnlm_to_nlm(rnlm, L):
+    """rnlm_to_nlm (src/reducedform.f90:160-174): m >= 0 copied, m < 0 from
+    (-1)^m conj(n_l^{|m|}) (the Mdiag factor and the conjugation at I_neg)."""
+    rnlm = np.asarray(rnlm, dtype=np.complex128)
+    ridx = {lm: k for k, lm in enumerate(rlm_list(L))}
+    out = np.zeros(rnlm.shape[:-1] + (nlm_len(L),), dtype=np.complex128)
+    for j, (l, m) in enumerate(lm_list(L)):
+        v = rnlm[..., ridx[(l, abs(m))]]
+        out[..., j] = v if m >= 0 else (-1) ** m * np.conj(v)
+    return out
+
+
+def reduce_M(M, L):
+    """reduce_M (src/reducedform.f90:76-120): for reduced column (l, m) the +m
+    and -m columns of M at the reduced rows, Zp and Zn, combine with
+    s = (-1)^m (s = 0 for m = 0) into Mrr = Re Zp + s Re Zn,
+    Mri = -Im Zp + s Im Zn, Mir = Im Zp + s Im Zn, Mii = Re Zp - s Re Zn."""
+    M = np.asarray(M, dtype=np.complex128)
+    pos = _full_pos(L)
+    rows = [pos[lm] for lm in rlm_list(L)]
+    r = len(rows)
+    out = [np.zeros(M.shape[:-2] + (r, r)) for _ in range(4)]
+    for jj, (l, m) in enumerate(rlm_list(L)):
+        zp = M[..., rows, pos[(l, m)]]
+        zn = M[..., rows, pos[(l, -m)]]
+        s = 0 if m == 0 else (-1) ** m
+        out[0][..., jj] = zp.real + s * zn.real
+        out[1][..., jj] = -zp.imag + s * zn.imag
+        out[2][..., jj] = zp.imag + s * zn.imag
+        out[3][..., jj] = zp.real - s * zn.real
+    return tuple(out)

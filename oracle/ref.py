@@ -36,6 +36,10 @@ class RefLib:
         L.sfref_M_DDRX_src.argtypes = [_dp, _cp]
         L.sfref_M_REG.argtypes = [_dp, _dp]
         L.sfref_M_CDRX.argtypes = [_dp]
+        L.sfref_rnlm_len.restype = _c.c_int
+        L.sfref_reduce_M.argtypes = [_cp, _dp, _dp, _dp, _dp]
+        L.sfref_nlm_to_rnlm.argtypes = [_cp, _cp]
+        L.sfref_rnlm_to_nlm.argtypes = [_cp, _cp]
         L.sfref_ev_D2.argtypes = [_cp, _dp]
         L.sfref_ev_D2.restype = _c.c_double
         L.sfref_a2.argtypes = [_cp, _dp]
@@ -107,6 +111,23 @@ class RefLib:
         M = np.zeros((self.n, self.n), order="F")
         self.lib.sfref_M_CDRX(M)
         return np.ascontiguousarray(M)
+
+    # reduced form (src/reducedform.f90:76-165); Mrr.. in natural [row, col] order
+    def reduce_M(self, M):
+        r = self.lib.sfref_rnlm_len()
+        out = [np.zeros((r, r), order="F") for _ in range(4)]
+        self.lib.sfref_reduce_M(np.asfortranarray(np.asarray(M, dtype=np.complex128)), *out)
+        return tuple(np.ascontiguousarray(o) for o in out)
+
+    def nlm_to_rnlm(self, nlm):
+        r = np.zeros(self.lib.sfref_rnlm_len(), dtype=np.complex128)
+        self.lib.sfref_nlm_to_rnlm(self._v(nlm, np.complex128), r)
+        return r
+
+    def rnlm_to_nlm(self, rnlm):
+        nlm = np.zeros(self.n, dtype=np.complex128)
+        self.lib.sfref_rnlm_to_nlm(self._v(rnlm, np.complex128), nlm)
+        return nlm
 
     def ev_D2(self, nlm, tau):
         return self.lib.sfref_ev_D2(self._v(nlm, np.complex128), self._m3(tau))

@@ -26,6 +26,7 @@ module sfref_driver
     use wavepropagation
     use rotation
     use deformationmodes
+    use reducedform
 
     implicit none
 
@@ -38,6 +39,7 @@ contains
         call initmoments(L)
         call initdynamics(L)
         call inithomogenizations(L)
+        call initreduced(L)
         Lcur = L
         ncur = (L+1)*(L+2)/2
     end subroutine
@@ -310,6 +312,31 @@ contains
             end if
         end subroutine
 
+    end subroutine
+
+    ! ---- reduced form (src/reducedform.f90:76-165) ----
+
+    function sfref_rnlm_len() bind(C, name="sfref_rnlm_len") result(r)
+        integer(c_int) :: r
+        r = rnlm_len
+    end function
+
+    subroutine sfref_reduce_M(M, Mrr, Mri, Mir, Mii) bind(C, name="sfref_reduce_M")
+        complex(c_double_complex), intent(in) :: M(ncur,ncur)
+        real(c_double), intent(out), dimension(rnlm_len,rnlm_len) :: Mrr, Mri, Mir, Mii
+        call reduce_M(M, Mrr, Mri, Mir, Mii)
+    end subroutine
+
+    subroutine sfref_nlm_to_rnlm(nlm, rnlm) bind(C, name="sfref_nlm_to_rnlm")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        complex(c_double_complex), intent(out) :: rnlm(rnlm_len)
+        rnlm = nlm_to_rnlm(nlm)
+    end subroutine
+
+    subroutine sfref_rnlm_to_nlm(rnlm, nlm) bind(C, name="sfref_rnlm_to_nlm")
+        complex(c_double_complex), intent(in) :: rnlm(rnlm_len)
+        complex(c_double_complex), intent(out) :: nlm(ncur)
+        nlm = rnlm_to_nlm(rnlm)
     end subroutine
 
 end module sfref_driver

@@ -59,6 +59,13 @@ SIGNATURES = {
     "sfg_M_REG": (_i, [_vp, _i, _dp, _dp]),
     "sfg_M_LROT_batch": (_i, [_vp, _i, _i64, _vp, _vp, _d, _d, _vp, _vp]),
     "sfg_M_DDRX_src_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_rnlm_len": (_i, [_i]),
+    "sfg_reduce_M": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sfg_nlm_to_rnlm": (_i, [_vp, _i, _vp, _vp]),
+    "sfg_rnlm_to_nlm": (_i, [_vp, _i, _vp, _vp]),
+    "sfg_reduce_M_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sfg_nlm_to_rnlm_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_rnlm_to_nlm_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_nlm_to_state": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_state_to_nlm": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_evolve": (_i, [_vp, _c.POINTER(EvolveDesc), _vp]),

@@ -42,6 +42,75 @@ def gamma0(eps, T, A, Q, ctx=None):
     return G
 
 
+def rnlm_len(L):
+    return (L + 2) ** 2 // 4
+
+
+def _cplx(t):
+    """complex128 device tensor, contiguous (its float64 view is the re/im interleaving the kernels read)"""
+    return t.to(torch.complex128).contiguous()
+
+
+def M_LROT_batch(L, eps, omg, iota, zeta, ctx=None):
+    """Lattice-rotation operators (P, n, n) complex of P Eulerian nodes from
+    strain rates eps and spins omg (P, 3, 3) (device tensors):
+    src/specfabpy/fenics/CPO.py:200 over the whole mesh, src/dynamics.f90:52-97."""
+    ctx = ctx or _lib.default_context()
+    E, W = eps.to(torch.float64).contiguous(), omg.to(torch.float64).contiguous()
+    n = nlm_len(L)
+    M = torch.empty((E.shape[0], n, n), dtype=torch.complex128, device=E.device)
+    check(lib().sfg_M_LROT_batch(ctx.handle, L, E.shape[0], _ptr(E), _ptr(W), float(iota), float(zeta), _ptr(M),
+                                 _stream()))
+    return M
+
+
+def M_DDRX_src_batch(L, tau, ctx=None):
+    """DDRX source operators (P, n, n) complex of P nodes from stresses tau
+    (P, 3, 3): src/specfabpy/fenics/CPO.py:201, src/dynamics.f90:277-298."""
+    ctx = ctx or _lib.default_context()
+    T = tau.to(torch.float64).contiguous()
+    n = nlm_len(L)
+    M = torch.empty((T.shape[0], n, n), dtype=torch.complex128, device=T.device)
+    check(lib().sfg_M_DDRX_src_batch(ctx.handle, L, T.shape[0], _ptr(T), _ptr(M), _stream()))
+    return M
+
+
+def reduce_M_batch(L, M, ctx=None):
+    """(Mrr, Mri, Mir, Mii), each (P, rn, rn) float64, of full operators M
+    (P, n, n) (device tensor): reduce_M (src/reducedform.f90:76-120) per node."""
+    ctx = ctx or _lib.default_context()
+    Mc = _cplx(M)
+    P, rn = Mc.shape[0], rnlm_len(L)
+    if Mc.shape[1:] != (nlm_len(L), nlm_len(L)):
+        raise ValueError("M has shape %s, expected (P, %d, %d)" % (tuple(Mc.shape), nlm_len(L), nlm_len(L)))
+    out = torch.empty((4, P, rn, rn), dtype=torch.float64, device=Mc.device)
+    check(lib().sfg_reduce_M_batch(ctx.handle, L, P, _ptr(Mc), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                   _ptr(out[3]), _stream()))
+    return out[0], out[1], out[2], out[3]
+
+
+def nlm_to_rnlm_batch(L, nlm, ctx=None):
+    """(P, n) complex -> (P, rn) complex, src/reducedform.f90:176-183 per node."""
+    ctx = ctx or _lib.default_context()
+    z = _cplx(nlm)
+    if z.shape[1] != nlm_len(L):
+        raise ValueError("nlm has %d coefficients, expected %d" % (z.shape[1], nlm_len(L)))
+    r = torch.empty((z.shape[0], rnlm_len(L)), dtype=torch.complex128, device=z.device)
+    check(lib().sfg_nlm_to_rnlm_batch(ctx.handle, L, z.shape[0], _ptr(z), _ptr(r), _stream()))
+    return r
+
+
+def rnlm_to_nlm_batch(L, rnlm, ctx=None):
+    """(P, rn) complex -> (P, n) complex, src/reducedform.f90:160-174 per node."""
+    ctx = ctx or _lib.default_context()
+    r = _cplx(rnlm)
+    if r.shape[1] != rnlm_len(L):
+        raise ValueError("rnlm has %d coefficients, expected %d" % (r.shape[1], rnlm_len(L)))
+    z = torch.empty((r.shape[0], nlm_len(L)), dtype=torch.complex128, device=r.device)
+    check(lib().sfg_rnlm_to_nlm_batch(ctx.handle, L, r.shape[0], _ptr(r), _ptr(z), _stream()))
+    return z
+
+
 class FabricBatch:
     """P parcels at truncation L with per-parcel kinematics.
 

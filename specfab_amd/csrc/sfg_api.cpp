@@ -22,6 +22,7 @@
 #include "sfg_fast.h"
 #include "sfg_homog.h"
 #include "sfg_init.h"
+#include "sfg_reduced.h"
 #include "sfg_wave.h"
 #include "sfg_mlp_net.h"
 #include "sfg_surrogate.h"
@@ -93,6 +94,7 @@ struct sfg_ctx {
     int* csr_row_ptr = nullptr;
     int4* csr_ent = nullptr;
     double* csr_val = nullptr;
+    int2* csr_seg = nullptr;
     // workspace, shared by all calls on this context: every user waits for
     // the previous user's stream through ws_event before touching it
     void* ws = nullptr;
@@ -199,6 +201,14 @@ int ensure_csr(sfg_ctx* ctx) {
         va.push_back(e->v);
     }
     for (int i = 0; i < sfg::NMAX; i++) rp[i + 1] += rp[i];
+    // [begin, end) of each (row, column) run; the sort makes every run contiguous
+    std::vector<int2> seg((size_t)sfg::NMAX * sfg::NMAX, make_int2(0, 0));
+    for (int e = 0; e < (int)v.size(); e++) {
+        int2& r = seg[(size_t)v[e]->i * sfg::NMAX + v[e]->j];
+        if (r.y == 0) r.x = e;
+        r.y = e + 1;
+    }
+    HIPCHK(upload(&ctx->csr_seg, seg));
     HIPCHK(upload(&ctx->csr_row_ptr, rp));
     HIPCHK(upload(&ctx->csr_ent, en));
     HIPCHK(upload(&ctx->csr_val, va));
@@ -343,6 +353,7 @@ int sfg_ctx_destroy(sfg_ctx* ctx) {
     (void)hipFree(ctx->csr_row_ptr);
     (void)hipFree(ctx->csr_ent);
     (void)hipFree(ctx->csr_val);
+    (void)hipFree(ctx->csr_seg);
     if (ctx->ws) (void)hipFree(ctx->ws);
     for (auto& ds : ctx->dirs) (void)hipFree(ds.R);
     (void)hipStreamDestroy(ctx->stream);
@@ -592,7 +603,8 @@ int dense_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const double* A_dev, c
     if (rc) return rc;
     sfg::DenseArgs a{};
     a.P = P; a.n = op->n; a.mode = mode; a.iota = iota; a.zeta = zeta;
-    a.A = A_dev; a.B = B_dev; a.row_ptr = ctx->csr_row_ptr; a.ent = ctx->csr_ent; a.val = ctx->csr_val; a.M = M_dev;
+    a.A = A_dev; a.B = B_dev; a.row_ptr = ctx->csr_row_ptr; a.ent = ctx->csr_ent; a.val = ctx->csr_val; a.seg = ctx->csr_seg; a.segw = sfg::NMAX;
+    a.M = M_dev;
     HIPCHK(sfg::launch_dense_op(a, s));
     return SFG_OK;
 }
@@ -629,6 +641,102 @@ int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, 
     if (!ctx || P < 0 || (P > 0 && (!tau_dev || !M_dev))) return fail(SFG_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     return dense_batch(ctx, L, P, 1, tau_dev, nullptr, 0, 0, M_dev, pick(ctx, stream));
+}
+
+// ---------------------------------------------------------------- reduced form (src/reducedform.f90:76-183)
+
+namespace {
+
+int reduced_L(int L) {
+    if (L < 0 || L > 30 || (L % 2)) return fail(SFG_ERR_L, "L must be even and 0 <= L <= 30 (src/reducedform.f90:12)");
+    return SFG_OK;
+}
+
+// single node: stage `in_len` doubles, run `fn` on the device copies, copy `out_len` doubles back
+extern "C++" template <class F>
+int reduced_single(sfg_ctx* ctx, const double* in, size_t in_len, double* out, size_t out_len, F fn) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (in_len + out_len) * sizeof(double)));
+    double* din = (double*)buf.p;
+    double* dout = din + in_len;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(din, in, in_len * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(fn(din, dout, s));
+    HIPCHK(hipMemcpyAsync(out, dout, out_len * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+}  // namespace
+
+int sfg_rnlm_len(int L) {
+    int rc = reduced_L(L);
+    return rc ? rc : sfg::rnlm_len(L);
+}
+
+int sfg_reduce_M_batch(sfg_ctx* ctx, int L, int64_t P, const double* M_dev, double* Mrr_dev, double* Mri_dev,
+                       double* Mir_dev, double* Mii_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!M_dev || !Mrr_dev || !Mri_dev || !Mir_dev || !Mii_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    if (int rc = reduced_L(L)) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_reduce_M(P, L, M_dev, Mrr_dev, Mri_dev, Mir_dev, Mii_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_nlm_to_rnlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* rnlm_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!nlm_dev || !rnlm_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (int rc = reduced_L(L)) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_nlm_to_rnlm(P, L, nlm_dev, rnlm_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_rnlm_to_nlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* rnlm_dev, double* nlm_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!nlm_dev || !rnlm_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (int rc = reduced_L(L)) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_rnlm_to_nlm(P, L, rnlm_dev, nlm_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_reduce_M(sfg_ctx* ctx, int L, const double* M, double* Mrr, double* Mri, double* Mir, double* Mii) {
+    if (!ctx || !M || !Mrr || !Mri || !Mir || !Mii) return fail(SFG_ERR_ARG, "NULL argument");
+    if (int rc = reduced_L(L)) return rc;
+    size_t n = sfg::nlm_len(L), r = sfg::rnlm_len(L), rr = r * r;
+    std::vector<double> out(4 * rr);
+    int rc = reduced_single(ctx, M, 2 * n * n, out.data(), 4 * rr, [&](const double* din, double* dout, hipStream_t s) {
+        return sfg::launch_reduce_M(1, L, din, dout, dout + rr, dout + 2 * rr, dout + 3 * rr, s);
+    });
+    if (rc) return rc;
+    std::copy(out.begin(), out.begin() + rr, Mrr);
+    std::copy(out.begin() + rr, out.begin() + 2 * rr, Mri);
+    std::copy(out.begin() + 2 * rr, out.begin() + 3 * rr, Mir);
+    std::copy(out.begin() + 3 * rr, out.end(), Mii);
+    return SFG_OK;
+}
+
+int sfg_nlm_to_rnlm(sfg_ctx* ctx, int L, const double* nlm, double* rnlm) {
+    if (!ctx || !nlm || !rnlm) return fail(SFG_ERR_ARG, "NULL argument");
+    if (int rc = reduced_L(L)) return rc;
+    return reduced_single(ctx, nlm, 2 * (size_t)sfg::nlm_len(L), rnlm, 2 * (size_t)sfg::rnlm_len(L),
+                          [&](const double* din, double* dout, hipStream_t s) {
+                              return sfg::launch_nlm_to_rnlm(1, L, din, dout, s);
+                          });
+}
+
+int sfg_rnlm_to_nlm(sfg_ctx* ctx, int L, const double* rnlm, double* nlm) {
+    if (!ctx || !nlm || !rnlm) return fail(SFG_ERR_ARG, "NULL argument");
+    if (int rc = reduced_L(L)) return rc;
+    return reduced_single(ctx, rnlm, 2 * (size_t)sfg::rnlm_len(L), nlm, 2 * (size_t)sfg::nlm_len(L),
+                          [&](const double* din, double* dout, hipStream_t s) {
+                              return sfg::launch_rnlm_to_nlm(1, L, din, dout, s);
+                          });
 }
 
 int sfg_M_LROT(sfg_ctx* ctx, int L, const double eps[9], const double omg[9], double iota, double zeta, double* M) {

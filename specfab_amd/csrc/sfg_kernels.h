@@ -106,8 +106,10 @@ struct DenseArgs {
     const double* A;       // eps or tau (P,3,3)
     const double* B;       // omg (P,3,3)
     const int* row_ptr;    // (NMAX+1)
-    const int4* ent;       // (t, j, k, 0)
+    const int4* ent;       // (t, j, k, 0), sorted by (row, j, t, k)
     const double* val;
+    const int2* seg;       // (segw, segw): [begin, end) of the entries of (row, j)
+    int segw;
     double* M;             // (P,n,n) complex
 };
 

@@ -307,34 +307,52 @@ __global__ void diag_op_kernel(int64_t P, int n, int mode, const int* row_l, con
 // CSR (entries of row i sorted by j): one thread per (parcel, row).
 //   mode 0: M_LROT   = -sum GC g0 + GCm gz + GC_m1 gn + GC_p1 gp  (k < 6)
 //   mode 1: M_DDRX_src = sum GC g                                  (k < 15)
-__global__ void dense_op_kernel(DenseArgs a) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= a.P * a.n) return;
-    int i = (int)(idx % a.n);
-    int64_t p = idx / a.n;
-    cplx w[4][15];
-    if (a.mode == 0) {
-        double D[9], W[9];
-        for (int q = 0; q < 9; q++) { D[q] = a.A[p * 9 + q]; W[q] = a.B[p * 9 + q]; }
-        cplx qe[5], qo[3];
-        d_lrot_weights(D, W, a.iota, a.zeta, w[0], w[1], w[2], w[3], qe, qo);
-    } else {
-        double T[9];
-        for (int q = 0; q < 9; q++) T[q] = a.A[p * 9 + q];
-        d_ddrx_weights(T, w[0]);
+// One workgroup of four waves per node: thread 0 evaluates the node's
+// coupling weights once into LDS, then wave q builds rows q, q + 4, ..., lane
+// c owning columns c, c + 64, ...: it sums the entries of (row, column) in
+// table order and stores the element once, so every row leaves as contiguous
+// 1 KiB runs and nothing is read back.
+__global__ __launch_bounds__(256) void dense_op_kernel(DenseArgs a) {
+    __shared__ cplx sw[4][15];
+    const int64_t p = blockIdx.x;
+    if (threadIdx.x == 0) {
+        cplx w[4][15];
+        if (a.mode == 0) {
+            double D[9], W[9];
+            for (int q = 0; q < 9; q++) { D[q] = a.A[p * 9 + q]; W[q] = a.B[p * 9 + q]; }
+            cplx qe[5], qo[3];
+            d_lrot_weights(D, W, a.iota, a.zeta, w[0], w[1], w[2], w[3], qe, qo);
+            for (int t = 0; t < 4; t++)
+                for (int k = 0; k < 6; k++) sw[t][k] = w[t][k];
+        } else {
+            double T[9];
+            for (int q = 0; q < 9; q++) T[q] = a.A[p * 9 + q];
+            d_ddrx_weights(T, w[0]);
+            for (int k = 0; k < 15; k++) sw[0][k] = w[0][k];
+        }
     }
-    double* row = a.M + (p * a.n + i) * a.n * 2;
-    for (int j = 0; j < a.n; j++) { row[2 * j] = 0.0; row[2 * j + 1] = 0.0; }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
     const int kmax = a.mode == 0 ? 6 : 15;
     const double sg = a.mode == 0 ? -1.0 : 1.0;
-    for (int e = a.row_ptr[i]; e < a.row_ptr[i + 1]; e++) {
-        int4 ent = a.ent[e];   // (t, j, k, -)
-        int t = ent.x, j = ent.y, k = ent.z;
-        if (j >= a.n || k >= kmax) continue;
-        if (a.mode == 1 && t != 0) continue;
-        double v = a.val[e];
-        row[2 * j] += sg * v * w[t][k].re;
-        row[2 * j + 1] += sg * v * w[t][k].im;
+    for (int i = threadIdx.x >> 6; i < a.n; i += 4) {
+        double* row = a.M + (p * a.n + i) * a.n * 2;
+        const int2* seg = a.seg + (size_t)i * a.segw;
+        for (int c = lane; c < a.n; c += 64) {
+            int2 r = seg[c];
+            double re = 0.0, im = 0.0;
+            for (int e = r.x; e < r.y; e++) {
+                int4 ent = a.ent[e];   // (t, j = c, k, -)
+                int t = ent.x, k = ent.z;
+                if (k >= kmax) continue;
+                if (a.mode == 1 && t != 0) continue;
+                double v = a.val[e];
+                re += sg * v * sw[t][k].re;
+                im += sg * v * sw[t][k].im;
+            }
+            row[2 * c] = re;
+            row[2 * c + 1] = im;
+        }
     }
 }
 
@@ -401,7 +419,8 @@ hipError_t launch_diag_op(int64_t P, int n, int mode, const int* row_l, const do
 
 hipError_t launch_dense_op(const DenseArgs& a, hipStream_t s) {
     if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(dense_op_kernel, dim3(nblocks(a.P * a.n, 64)), dim3(64), 0, s, a);
+    if (a.P > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(dense_op_kernel, dim3((unsigned)a.P), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -124,6 +124,50 @@ def M_REG(nlm, eps):
     return M
 
 
+def _L_of_rnlm_len(rn):
+    h = int(round(np.sqrt(rn)))
+    if h < 1 or h * h != rn:
+        raise ValueError("rnlm_len=%d is not a reduced-form length" % rn)
+    return 2 * h - 2
+
+
+def get_rnlm_len():
+    """rnlm_len of the L set by init(), src/specfabpy.f90:1039-1044."""
+    return check_n(lib().sfg_rnlm_len(_state["L"]))
+
+
+def nlm_to_rnlm(nlm, rnlm_len):
+    """Full -> reduced form, src/specfabpy.f90:1046-1054 -> reducedform.f90:176-183."""
+    L = _L_of(nlm)
+    if rnlm_len != check_n(lib().sfg_rnlm_len(L)):
+        raise ValueError("rnlm_len=%d does not match len(nlm)=%d" % (rnlm_len, len(nlm)))
+    r = np.zeros(rnlm_len, dtype=np.complex128)
+    check(lib().sfg_nlm_to_rnlm(_ctx(), L, _zp(_z(nlm)), _zp(r)))
+    return r
+
+
+def rnlm_to_nlm(rnlm, nlm_len):
+    """Reduced -> full form, src/specfabpy.f90:1056-1064 -> reducedform.f90:160-174."""
+    L = _L_of_rnlm_len(len(rnlm))
+    if nlm_len != (L + 1) * (L + 2) // 2:
+        raise ValueError("nlm_len=%d does not match len(rnlm)=%d" % (nlm_len, len(rnlm)))
+    nlm = np.zeros(nlm_len, dtype=np.complex128)
+    check(lib().sfg_rnlm_to_nlm(_ctx(), L, _zp(_z(rnlm)), _zp(nlm)))
+    return nlm
+
+
+def reduce_M(M, rnlm_len):
+    """(Mrr, Mri, Mir, Mii) of a full operator M, src/specfabpy.f90:1066-1074 ->
+    reducedform.f90:76-120: d(rnlm)/dt = Mrr Re r + Mri Im r + i (Mir Re r + Mii Im r)."""
+    M = np.ascontiguousarray(np.asarray(M, dtype=np.complex128))
+    L = _L_of(M[0])
+    if M.shape != (len(M[0]), len(M[0])) or rnlm_len != check_n(lib().sfg_rnlm_len(L)):
+        raise ValueError("M of shape %s does not match rnlm_len=%d" % (M.shape, rnlm_len))
+    out = [np.zeros((rnlm_len, rnlm_len)) for _ in range(4)]
+    check(lib().sfg_reduce_M(_ctx(), L, _zp(M), *[dptr(o) for o in out]))
+    return tuple(out)
+
+
 def a2(nlm):
     """a2 = <c^2>, src/moments.f90:37-44."""
     L = _L_of(nlm)

@@ -1,0 +1,100 @@
+"""GPU parity of the reduced real form (src/reducedform.f90:76-183): the
+single-node specfabpy face against the reference's outputs
+(tests/golden/reduced_L8.npz), the batched C-ABI entry points against the
+oracle (bit-exact: the kernels only gather, negate and add), the Eulerian
+export chain M_LROT_batch -> reduce_M_batch, and round trips at field size."""
+import numpy as np
+import pytest
+
+from conftest import load_golden, relerr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("Mrr", "Mri", "Mir", "Mii")
+
+
+def test_single_node_face_vs_reference(gpu_ctx):
+    from specfab_amd import specfabpy as sf
+    g = load_golden("reduced_L8.npz")
+    L = int(g["L"])
+    _, n = sf.init(L)
+    rn = sf.get_rnlm_len()
+    assert rn == O.rnlm_len(L)
+    for k, M in enumerate(g["M"]):
+        red = sf.reduce_M(M, rn)
+        for i, name in enumerate(NAMES):
+            assert np.array_equal(red[i], g[name][k]), (k, name)
+    for k in range(len(g["nlm"])):
+        assert np.array_equal(sf.nlm_to_rnlm(g["nlm"][k], rn), g["rnlm_of_nlm"][k])
+        assert np.array_equal(sf.rnlm_to_nlm(g["rnlm"][k], n), g["nlm_of_rnlm"][k])
+
+
+@pytest.mark.parametrize("L", [0, 2, 8, 12, 20, 30])
+def test_batched_vs_oracle(gpu_ctx, L):
+    import torch
+    from specfab_amd import batch as B
+    print("reduced batch L=%d" % L, flush=True)
+    rng = np.random.default_rng(500 + L)
+    n, rn = O.nlm_len(L), O.rnlm_len(L)
+    P = 37
+    M = rng.standard_normal((P, n, n)) + 1j * rng.standard_normal((P, n, n))
+    red = B.reduce_M_batch(L, torch.from_numpy(M).cuda())
+    want = O.reduce_M(M, L)
+    for i, name in enumerate(NAMES):
+        assert np.array_equal(red[i].cpu().numpy(), want[i]), name
+    v = rng.standard_normal((P, n)) + 1j * rng.standard_normal((P, n))
+    assert np.array_equal(B.nlm_to_rnlm_batch(L, torch.from_numpy(v).cuda()).cpu().numpy(), O.nlm_to_rnlm(v, L))
+    w = rng.standard_normal((P, rn)) + 1j * rng.standard_normal((P, rn))
+    assert np.array_equal(B.rnlm_to_nlm_batch(L, torch.from_numpy(w).cuda()).cpu().numpy(), O.rnlm_to_nlm(w, L))
+
+
+def test_eulerian_export_chain(gpu_ctx, oracle):
+    """per-node M_LROT and M_DDRX_src built and reduced on the device, the
+    assembly of src/specfabpy/fenics/CPO.py:200-201 for a whole mesh"""
+    import torch
+    from specfab_amd import batch as B
+    L, P = 12, 64
+    rng = np.random.default_rng(7)
+    eps = rng.standard_normal((P, 3, 3))
+    eps = eps + eps.transpose(0, 2, 1)
+    omg = rng.standard_normal((P, 3, 3))
+    omg = omg - omg.transpose(0, 2, 1)
+    M = B.M_LROT_batch(L, torch.from_numpy(eps).cuda(), torch.from_numpy(omg).cuda(), 1.0, 0.0)
+    red = [t.cpu().numpy() for t in B.reduce_M_batch(L, M)]
+    Md = B.M_DDRX_src_batch(L, torch.from_numpy(eps).cuda())
+    redd = [t.cpu().numpy() for t in B.reduce_M_batch(L, Md)]
+    for p in range(0, P, 9):
+        want = O.reduce_M(oracle.M_LROT(L, eps[p], omg[p], 1.0, 0.0), L)
+        wantd = O.reduce_M(oracle.M_DDRX_src(L, eps[p]), L)
+        for i in range(4):
+            assert relerr(red[i][p], want[i]) < 1e-13
+            assert relerr(redd[i][p], wantd[i]) < 1e-13
+
+
+def test_round_trip_field_size(gpu_ctx):
+    """rnlm -> nlm -> rnlm is the identity, and the rebuilt nlm satisfies
+    n_l^{-m} = (-1)^m conj(n_l^m), over 2^18 nodes at L = 12"""
+    import torch
+    from specfab_amd import batch as B
+    L, P = 12, 1 << 18
+    g = torch.Generator(device="cuda").manual_seed(3)
+    r = torch.complex(torch.randn(P, O.rnlm_len(L), dtype=torch.float64, device="cuda", generator=g),
+                      torch.randn(P, O.rnlm_len(L), dtype=torch.float64, device="cuda", generator=g))
+    z = B.rnlm_to_nlm_batch(L, r)
+    assert torch.equal(B.nlm_to_rnlm_batch(L, z), r)
+    lm = O.lm_list(L)
+    pos = {x: k for k, x in enumerate(lm)}
+    neg = [k for k, (l, m) in enumerate(lm) if m < 0]
+    mir = [pos[(l, -m)] for (l, m) in (lm[k] for k in neg)]
+    sgn = torch.tensor([(-1.0) ** lm[k][1] for k in neg], dtype=torch.float64, device="cuda")
+    assert torch.equal(z[:, neg], sgn * torch.conj(z[:, mir]))
+
+
+def test_argument_errors(gpu_ctx):
+    import torch
+    from specfab_amd import _lib, batch as B
+    with pytest.raises(Exception):
+        B.nlm_to_rnlm_batch(7, torch.zeros((2, 36), dtype=torch.complex128, device="cuda"))
+    assert _lib.lib().sfg_rnlm_len(32) < 0
+    assert B.reduce_M_batch(4, torch.zeros((0, 15, 15), dtype=torch.complex128, device="cuda"))[0].shape == (0, 9, 9)
