@@ -240,6 +240,28 @@ int ws_release(sfg_ctx* ctx, hipStream_t s) {
     return SFG_OK;
 }
 
+// The workspace held by one call: released (the event recorded on the call's
+// stream) on every path out of the call, also the error returns, since a
+// kernel already queued on that stream may still be using it.
+struct WsLease {
+    sfg_ctx* ctx;
+    hipStream_t s;
+    bool held = false;
+    WsLease(sfg_ctx* c, hipStream_t st) : ctx(c), s(st) {}
+    int acquire(size_t bytes, void** out) {
+        int rc = ws_acquire(ctx, bytes, s, out);
+        held = rc == SFG_OK;
+        return rc;
+    }
+    int release() {
+        held = false;
+        return ws_release(ctx, s);
+    }
+    ~WsLease() {
+        if (held) (void)ws_release(ctx, s);
+    }
+};
+
 // Device rotation blocks (ROT4_STRIDE doubles per direction) of the direction
 // set (theta, phi); each direction rotates by (-theta, -phi) as
 // Vi_*_tranisotropic do (src/wavepropagation.f90:139, 235).  Recent sets are
@@ -424,7 +446,8 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
     size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
     size_t nscr = (size_t)3 * n * P;
     void* ws;
-    rc = ws_acquire(ctx, (nth + nom + nsl + nscr) * sizeof(double), s, &ws);
+    WsLease lease(ctx, s);
+    rc = lease.acquire((nth + nom + nsl + nscr) * sizeof(double), &ws);
     if (rc) return rc;
     double* theta = (double*)ws;
     double* omega = theta + nth;
@@ -455,10 +478,8 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
     const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
     const int prec = d->precision == SFG_FP64 ? 0 : 1;
     const bool fast = fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0);
-    if (!fast && prec == 1) {
-        ws_release(ctx, s);
+    if (!fast && prec == 1)
         return fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L and n * P * 8 < 2^31 (split the batch)");
-    }
     std::pair<hipEvent_t, hipEvent_t> tv{nullptr, nullptr};
     if (ctx->timing) {
         HIPCHK(hipEventCreate(&tv.first));
@@ -475,7 +496,7 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
         HIPCHK(hipEventRecord(tv.second, s));
         ctx->tev.push_back(tv);
     }
-    rc = ws_release(ctx, s);
+    rc = lease.release();
     if (rc) return rc;
     if (ctx->sync_check) {
         HIPCHK(hipStreamSynchronize(s));
@@ -1189,7 +1210,29 @@ struct sfg_mlp {
     double *Ep = nullptr, *Vp = nullptr, *errE = nullptr, *errV = nullptr;
     sfg::AuditSums* sums = nullptr;
     std::vector<void*> wbufs;
+    // the network's buffers (activations, work list, statistics) are shared by
+    // every call on this handle: each call waits for the previous caller's
+    // stream through this event and records it on its own stream when done
+    hipEvent_t ev = nullptr;
+    bool ev_recorded = false;
 };
+
+namespace {
+struct MlpLease {
+    sfg_mlp* m;
+    hipStream_t s;
+    bool held = false;
+    MlpLease(sfg_mlp* net, hipStream_t st) : m(net), s(st) {}
+    int acquire() {
+        if (m->ev_recorded) HIPCHK(hipStreamWaitEvent(s, m->ev, 0));
+        held = true;
+        return SFG_OK;
+    }
+    ~MlpLease() {
+        if (held && hipEventRecord(m->ev, s) == hipSuccess) m->ev_recorded = true;
+    }
+};
+}  // namespace
 
 namespace {
 
@@ -1229,6 +1272,7 @@ int sfg_mlp_create(sfg_ctx* ctx, uint64_t seed, sfg_mlp** net) {
     auto* m = new sfg_mlp();
     m->ctx = ctx;
     hipError_t e = m->net.init(ctx->device, seed);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete m;
         return fail(SFG_ERR_HIP, std::string("sfg_mlp_create: ") + hipGetErrorString(e));
@@ -1242,6 +1286,7 @@ int sfg_mlp_destroy(sfg_mlp* net) {
     (void)hipSetDevice(net->net.device);
     (void)hipDeviceSynchronize();
     for (void* p : net->wbufs) (void)hipFree(p);
+    if (net->ev) (void)hipEventDestroy(net->ev);
     delete net;
     return SFG_OK;
 }
@@ -1316,6 +1361,8 @@ int sfg_mlp_train_step(sfg_mlp* net, int64_t B, const float* X_dev, const float*
     if (B > INT32_MAX) return fail(SFG_ERR_ARG, "batch too large");
     HIPCHK(hipSetDevice(net->net.device));
     hipStream_t s = (hipStream_t)stream;
+    MlpLease lease(net, s);
+    if (int rc = lease.acquire()) return rc;
     HIPCHK(net->net.ensure_batch(B));
     HIPCHK(net->net.forward(B, X_dev, s));
     HIPCHK(net->net.backward(B, T_dev, s));
@@ -1331,6 +1378,8 @@ int sfg_mlp_gradients(sfg_mlp* net, int64_t B, const float* X_dev, const float* 
     if (!net || B <= 0 || !X_dev || !T_dev || !grad) return fail(SFG_ERR_ARG, "bad argument");
     HIPCHK(hipSetDevice(net->net.device));
     hipStream_t s = net->ctx->stream;
+    MlpLease lease(net, s);
+    if (int rc = lease.acquire()) return rc;
     HIPCHK(net->net.ensure_batch(B));
     HIPCHK(net->net.forward(B, X_dev, s));
     HIPCHK(net->net.backward(B, T_dev, s));
@@ -1347,6 +1396,8 @@ int sfg_mlp_forward(sfg_mlp* net, int64_t B, const float* X_dev, float* Y_dev, v
     if (!net || B < 0 || (B > 0 && (!X_dev || !Y_dev))) return fail(SFG_ERR_ARG, "bad argument");
     HIPCHK(hipSetDevice(net->net.device));
     hipStream_t s = (hipStream_t)stream;
+    MlpLease lease(net, s);
+    if (int rc = lease.acquire()) return rc;
     HIPCHK(net->net.ensure_batch(std::min<int64_t>(B, MLP_CHUNK)));
     for (int64_t off = 0; off < B; off += MLP_CHUNK) {
         const int64_t n = std::min<int64_t>(MLP_CHUNK, B - off);
@@ -1399,6 +1450,8 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     hipStream_t s = pick(ctx, stream);
     rc = mlp_ws(net, P);
     if (rc) return rc;
+    MlpLease lease(net, s);
+    if ((rc = lease.acquire())) return rc;
     // 1. features, envelope / audit flags, network and decoding in one kernel
     sfg::SurrogateInferArgs ia{};
     sfg::SurrogateFeatArgs& fa = ia.fa;
