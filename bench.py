@@ -73,15 +73,30 @@ def cpu_worker(args):
     return time.perf_counter() - t0
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(target_s=10.0):
+    """The reference Fortran (oracle/_ref; the C restatement where it is not
+    built) on the host: one process per core (the reference keeps global
+    state and is not thread-safe), at most 16 processes: the CPU share that
+    goes with one GPU of the box, whatever the affinity set shows."""
     import multiprocessing as mp
     from oracle import ref as refmod
     kind = "reference" if refmod.available() else "port"
     try:
-        ncores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(ncores, 16))
+        affinity = os.cpu_count() or 1
+    ncores = max(1, min(affinity, 16))
     # calibrate on one core: 16 parcels x 4 steps (stepping time only)
     t1 = cpu_worker((kind, 16, 4, 99)) / 64
     steps = 8
@@ -92,6 +107,7 @@ def cpu_baseline(target_s=10.0):
     total = ncores * per_proc * steps
     slowest = max(times)
     return {"value": total / slowest, "unit": "parcel-steps/s", "cores": ncores, "kind": kind,
+            "affinity_cores": affinity, "cpu_model": cpu_model(), "single_core_value": 1.0 / t1,
             "sample": "%d processes (one per core) x %d parcels x %d RK4 steps at L=12 (LROT+REG+DDRX), "
                       "stepping time of the slowest process %.1f s; single-core rate %.1f parcel-steps/s"
                       % (ncores, per_proc, steps, slowest, 1.0 / t1)}
@@ -252,12 +268,13 @@ def bench_surrogate(a):
 
 
 def measured_traffic(kernel, P, steps):
-    """HBM bytes of one launch of `kernel` on P parcels for `steps` steps:
-    rocprofv3 FETCH_SIZE and WRITE_SIZE of the bench command, scaled by a 1 GiB
-    copy (x 2048 and x 1024 bytes per KiB counted), per parcel-step of a launch
-    of that many steps, times P x steps; from the committed summary
-    profiles/evolve_traffic.json (null for a kernel or launch length not
-    measured)."""
+    """HBM bytes of one launch of `kernel` on P parcels for `steps` steps, from
+    the committed table profiles/evolve_traffic.json: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE per dispatch of the bench command, scaled by a plain device
+    copy (bytes per KiB counted), fitted per parcel as a launch prologue plus
+    a per-step figure from two launch lengths of the same run (so any
+    --steps is covered); an entry with a single launch length is used only
+    for that length.  None for a kernel not measured."""
     path = os.path.join(ROOT, "profiles", "evolve_traffic.json")
     try:
         with open(path) as f:
@@ -265,7 +282,11 @@ def measured_traffic(kernel, P, steps):
     except (OSError, ValueError, KeyError):
         return None
     ent = tab.get(kernel)
-    if not ent or ent.get("steps_per_launch", 1) != steps:
+    if not ent:
+        return None
+    if "prologue_bytes_per_parcel" in ent:
+        return P * (ent["prologue_bytes_per_parcel"] + ent["bytes_per_parcel_step"] * steps)
+    if ent.get("steps_per_launch", 1) != steps:
         return None
     return ent["bytes_per_parcel_step"] * P * steps
 
@@ -343,6 +364,61 @@ def bench_export(a):
                                  "kernel": "reduce_M_kernel", "bytes_per_node": per},
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def homogenization_block(fb, procs, precision, more_steps):
+    """The second half of the metric, on the field after 200 RK4 steps (the
+    timed steps plus `more_steps` untimed ones): Eij (Cartesian, ice linear
+    Sachs/Taylor mix) + elastic Vi along z per parcel by the plain FP64
+    homogenization and by the learned one (network + closed-form Sachs term,
+    envelope / error-bound fallback and a 1 % audit, stream-ordered), both
+    timed with HIP events; the learned outputs of every parcel are then
+    compared with the plain ones (worst parcel)."""
+    import torch
+    from specfab_amd import surrogate as S
+    try:
+        if more_steps > 0:
+            fb.evolve(more_steps, 0.5, scheme="rk4", procs=procs, precision=precision)
+        net = S.Surrogate().load()
+        hp = S.ice_params()
+        g = (hp.Eij_grain[0], hp.Eij_grain[1])
+
+        def plain():
+            E = fb.Eij(g, hp.alpha, hp.n_grain, frame="xyz")
+            V = fb.Vi_elastic(hp.vi_alpha, list(hp.Lame), hp.rho, [0.0], [0.0])
+            return E, V
+
+        def learned():
+            return net.homogenize(fb, hp, audit_frac=0.01, seed=3, stats=False)
+
+        E0, E1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ms = {}
+        for tag, fn in (("plain", plain), ("learned", learned)):
+            fn()
+            torch.cuda.synchronize()
+            E0.record()
+            for _ in range(5):
+                fn()
+            E1.record()
+            torch.cuda.synchronize()
+            ms[tag] = E0.elapsed_time(E1) / 5
+        El, Vl, st = net.homogenize(fb, hp, audit_frac=1.0, seed=3)
+        Ep, Vp = plain()
+        Vs = torch.sort(Vp[:, 0, :], dim=1).values
+        eE = ((El - Ep).abs() / Ep.abs()).max(1).values
+        eV = ((Vl - Vs).abs() / Vs.abs()).max(1).values
+        P = fb.P
+        return {"field": "the timed field after 200 RK4 steps (%d parcels, L=%d)" % (P, fb.L),
+                "plain_evals_per_s": P / (ms["plain"] * 1e-3), "learned_evals_per_s": P / (ms["learned"] * 1e-3),
+                "learned_vs_plain": ms["plain"] / ms["learned"], "ms_plain": ms["plain"], "ms_learned": ms["learned"],
+                "eval": "Eij (6, Cartesian, ice linear Sachs/Taylor mix alpha=0.0125) + elastic Vi (3, along z, "
+                        "Bennett 1968 ice, Hill average) per parcel",
+                "learned_dtype": "bf16 MFMA network (fp32 accumulate) for the Taylor term and Vi, fp64 Sachs term",
+                "eij_rel_err_max": eE.max().item(), "eij_rel_err_mean": eE.mean().item(),
+                "vi_rel_err_max": eV.max().item(), "tolerance": hp.eij_tol,
+                "n_fallback": st["n_fallback"], "n_bound": st["n_bound"]}
+    except Exception as e:   # the stepping metric above stands on its own
+        return {"error": "%s: %s" % (type(e).__name__, e)}
 
 
 def main():
@@ -441,6 +517,7 @@ def main():
     E1.record()
     torch.cuda.synchronize()
     vi_rate = 3 * P / (E0.elapsed_time(E1) * 1e-3)
+    homog = homogenization_block(fb, procs, a.precision, 200 - a.warmup - a.steps)
     out = {
         "metric": METRIC,
         "value": value,
@@ -457,7 +534,7 @@ def main():
         "config": {"workload": "Eulerian CPO field: %d parcels, L=%d (%d coeffs), LROT+REG+DDRX, RK4 dt=0.5"
                                % (P, L, fb.n), "parcels": P, "L": L, "scheme": "rk4", "dt": 0.5,
                    "procs": list(procs), "full_run_steps": 200, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "compute": "%s vector ALU (sparse FMA; MI355X %s peak %.1f TFLOP/s)"
                                 % ("fp64" if a.precision == "fp64" else "fp32", a.precision, peak),
@@ -468,6 +545,7 @@ def main():
         "eij_evals_per_s": eij_rate,
         "vi_elastic_evals_per_s": vi_rate,
         "nonfinite_parcels": nonfinite,
+        "homogenization": homog,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

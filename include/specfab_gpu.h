@@ -62,6 +62,7 @@ extern "C" {
 #define SFG_ST_REUSS_SING 8   /* singular 9x9 Reuss matrix (reference: stop) */
 #define SFG_ST_ENVELOPE 16    /* outside the surrogate's training envelope */
 #define SFG_ST_AUDIT 32       /* drawn for the surrogate's random audit */
+#define SFG_ST_ERRBOUND 64    /* the surrogate's Eij error bound exceeded the tolerance */
 
 typedef struct sfg_ctx sfg_ctx;
 
@@ -114,6 +115,9 @@ int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M);
 int sfg_M_LROT_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, const double* omg_dev, double iota,
                      double zeta, double* M_dev, void* stream);
 int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* M_dev, void* stream);
+/* M_REG(D) of every node (src/dynamics.f90:494-518; assembled per node at
+ * src/specfabpy/fenics/CPO.py:202): D_dev (P,3,3) -> M_dev (P, n, n) real */
+int sfg_M_REG_batch(sfg_ctx* ctx, int L, int64_t P, const double* D_dev, double* M_dev, void* stream);
 
 /* ---- state conversion (device buffers) ---- */
 int sfg_nlm_to_state(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* x_dev, void* stream);
@@ -237,6 +241,29 @@ int sfg_Gamma0_batch(sfg_ctx* ctx, int64_t P, const double* eps_dev, const doubl
 int sfg_apply_bounds(sfg_ctx* ctx, int L, const double* nlm, double* out);
 int sfg_apply_bounds_batch(sfg_ctx* ctx, int L, int64_t P, double* x_dev, void* stream);
 
+/* ---- rotation, a4 eigentensors, state validity ---- */
+/* rotate_nlm(nlm, theta, phi) (src/rotation.f90:29-55, Wigner-D matrices of
+ * src/include/Dlmn.f90): n'_l^m = sum_n exp(-i n phi) d^l_mn(theta) n_l^n,
+ * L <= 12 (the reference's Lmax_wignerD).  Single parcel: nlm, out (n
+ * complex, host).  Batched: per-parcel angles theta_dev, phi_dev (P) on the
+ * states x_dev (n, P) -> out_dev (n, P), which may be x_dev. */
+int sfg_rotate_nlm(sfg_ctx* ctx, int L, const double* nlm, double theta, double phi, double* out);
+int sfg_rotate_nlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* theta_dev,
+                         const double* phi_dev, double* out_dev, void* stream);
+/* a4_eigentensors(nlm, Q1..Q6, eigvals6) (src/frames.f90:131-152): the
+ * eigenvalues (ascending) and eigentensors Q_i = vec_to_mat(v_i) of the Mandel
+ * matrix of the full a4.  Single: Q[6][3][3] row-major per tensor, ev[6].
+ * Batched: Q_dev (P, 6, 3, 3), ev_dev (P, 6) (either may be NULL).  L >= 4. */
+int sfg_a4_eigentensors(sfg_ctx* ctx, int L, const double* nlm, double Q[54], double ev[6]);
+int sfg_a4_eigentensors_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* Q_dev, double* ev_dev,
+                              void* stream);
+/* nlm_isvalid(nhat20, nhat40) (src/idealstate.f90:50-78): for K states
+ * n_0^0 = 1, n_2^0 = nhat20[k], n_4^0 = nhat40[k] (complex, host arrays),
+ * isvalid[k] = 1 if the a2 and a4 eigenvalues all lie in (0, 1), else 0.
+ * Batched on general states: valid_dev (P) for x_dev (n, P), L >= 4. */
+int sfg_nlm_isvalid(sfg_ctx* ctx, int64_t K, const double* nhat20, const double* nhat40, int32_t* isvalid);
+int sfg_state_isvalid_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int32_t* valid_dev, void* stream);
+
 /* Cij_to_Lame_tranisotropic (src/elasticities.f90:107-120): (C11, C33, C55,
  * C12, C13) -> (lam, mu, Elam, Emu, Egam).  Host only. */
 int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]);
@@ -287,12 +314,22 @@ int sfg_plan_destroy(sfg_plan* plan);
  *
  * A multilayer perceptron 36 -> 256 -> 256 -> 256 -> 9 (GELU), bf16 weights
  * and activations with FP32 accumulation on the matrix cores, replaces the
- * per-parcel Sachs/Taylor homogenization and the elastic-velocity solve:
+ * per-parcel Taylor homogenization (the 6x6 solve of
+ * src/homogenizations.f90:145-189) and the elastic-velocity solve:
  *   inputs  Re, Im of n_l^m / n_0^0 for l <= 4 in specfab order (30) and the
  *           unit strain rate D/|D| in Mandel order (6);
- *   outputs log E_ij in the Cartesian frame (E11, E22, E33, E23, E13, E12)
- *           and log of the elastic phase velocities along z, sorted
- *           ascending (slower S, faster S, P).
+ *   outputs log of the Taylor-homogenized enhancement factors T_ij in the
+ *           Cartesian frame (E11, E22, E33, E23, E13, E12; Eij_tranisotropic
+ *           with alpha = 1) and log of the elastic phase velocities along z,
+ *           sorted ascending (slower S, faster S, P).
+ * sfg_surrogate_homogenize returns Eij = (1 - alpha) S_ij + alpha T_ij, the
+ * linear Sachs/Taylor mix of src/enhancementfactors.f90:23-69, with the Sachs
+ * term S_ij in closed form (a linear function of a2 and twelve a4 entries for
+ * the axis stress states: no solve).  A parcel whose bound
+ * alpha T_ij dT_ij / E_ij on the relative error of any Eij exceeds the
+ * tolerance hp->eij_tol (dT_ij: the network's calibrated relative error of
+ * T_ij, sfg_mlp_set_error_model) takes the plain homogenization instead
+ * (status bit SFG_ST_ERRBOUND).
  * Inputs and outputs are normalised inside the network ((v - mean)/std, set
  * by sfg_mlp_set_normalization).  Parameters are a flat FP32 array of
  * sfg_mlp_nparams() = 143,369 values: W1 (36 x 256, [in][out]), b1, W2
@@ -307,13 +344,15 @@ typedef struct {
     double Lame[5];      /* grain elastic parameters (Cij_to_Lame_tranisotropic) */
     double rho;          /* density */
     double vi_alpha;     /* Reuss-Voigt weight of the velocities */
+    double eij_tol;      /* surrogate: relative error bound of Eij (<= 0: 1e-3) */
 } sfg_homog_params;
 
 typedef struct {
-    int64_t n_fallback;  /* parcels outside the envelope: plain homogenization */
+    int64_t n_fallback;  /* parcels given the plain homogenization (envelope or error bound) */
     int64_t n_audit;     /* audited parcels (surrogate kept, plain computed too) */
     double audit_max_err_Eij, audit_mean_err_Eij;  /* max_ij |E - E_plain| / |E_plain| */
     double audit_max_err_Vi, audit_mean_err_Vi;
+    int64_t n_bound;     /* of n_fallback: inside the envelope, sent by the error bound */
 } sfg_surrogate_stats;
 
 int64_t sfg_mlp_nparams(void);
@@ -326,6 +365,9 @@ int sfg_mlp_set_normalization(sfg_mlp* net, const float in_mean[36], const float
 int sfg_mlp_get_normalization(sfg_mlp* net, float norm[90]);
 /* training envelope: sorted a2 eigenvalues lo[i] <= lambda_i <= hi[i] */
 int sfg_mlp_set_envelope(sfg_mlp* net, const double lo[3], const double hi[3]);
+/* error model: calibrated bound of the relative error of the network's six
+ * Taylor-term outputs (>= 0; all 0, the default, disables the bound) */
+int sfg_mlp_set_error_model(sfg_mlp* net, const double taylor_rel_err[6]);
 /* one Adam step on B rows: X_dev (B x 36) raw features, T_dev (B x 9) raw
  * targets; *loss (if non-NULL, synchronises) = batch MSE before the update */
 int sfg_mlp_train_step(sfg_mlp* net, int64_t B, const float* X_dev, const float* T_dev, float lr, float* loss,

@@ -54,6 +54,9 @@ class RefLib:
         L.sfref_Vi_electromagnetic_tranisotropic.argtypes = [_cp, _c.c_double, _c.c_double, _c.c_double,
                                                              _c.c_int, _dp, _dp, _dp]
         L.sfref_rotate_nlm4.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
+        L.sfref_rotate_nlm.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
+        L.sfref_a4_eigentensors.argtypes = [_cp, _dp, _dp]
+        L.sfref_nlm_isvalid.argtypes = [_c.c_int, _cp, _cp, np.ctypeslib.ndpointer(dtype=np.int32)]
         L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                    _c.c_double, _dp, _dp, _dp, _cp]
         L.sfref_a2_to_nlm.argtypes = [_dp, _cp]
@@ -186,6 +189,25 @@ class RefLib:
         out = np.zeros(15, dtype=np.complex128, order="F")
         self.lib.sfref_rotate_nlm4(self._v(nlm4[:15], np.complex128), float(theta), float(phi), out)
         return out.copy()
+
+    def rotate_nlm(self, nlm, theta, phi):
+        out = np.zeros(self.n, dtype=np.complex128, order="F")
+        self.lib.sfref_rotate_nlm(self._v(nlm, np.complex128), float(theta), float(phi), out)
+        return out.copy()
+
+    def a4_eigentensors(self, nlm):
+        """(Q (6, 3, 3): eigentensors of the Mandel a4, eigvals (6) ascending)"""
+        Q = np.zeros((3, 3, 6), order="F")
+        ev = np.zeros(6, order="F")
+        self.lib.sfref_a4_eigentensors(self._v(nlm, np.complex128), Q, ev)
+        return np.ascontiguousarray(np.transpose(Q, (2, 0, 1))), ev.copy()
+
+    def nlm_isvalid(self, nhat20, nhat40):
+        n20 = np.ascontiguousarray(nhat20, dtype=np.complex128)
+        n40 = np.ascontiguousarray(nhat40, dtype=np.complex128)
+        ok = np.zeros(len(n20), dtype=np.int32)
+        self.lib.sfref_nlm_isvalid(len(n20), n20, n40, ok)
+        return ok.astype(bool)
 
     def Qnorm_tranisotropic(self, nlm, alpha, Lame):
         out = np.zeros((3, 3), order="F")

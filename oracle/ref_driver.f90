@@ -25,6 +25,7 @@ module sfref_driver
     use elasticities
     use wavepropagation
     use rotation
+    use idealstate
     use deformationmodes
     use reducedform
 
@@ -251,6 +252,31 @@ contains
         real(c_double), value :: theta, phi
         complex(c_double_complex), intent(out) :: out(15)
         out = rotate_nlm4(nlm4, theta, phi)
+    end subroutine
+
+    ! rotate_nlm(nlm, theta, phi) of a state of the active truncation (L <= 12)
+    subroutine sfref_rotate_nlm(nlm, theta, phi, out) bind(C, name="sfref_rotate_nlm")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), value :: theta, phi
+        complex(c_double_complex), intent(out) :: out(ncur)
+        out = rotate_nlm(nlm, theta, phi)
+    end subroutine
+
+    ! a4_eigentensors(nlm, Q1..Q6, eigvals): Q(3,3,6) column-major, eigvals(6)
+    subroutine sfref_a4_eigentensors(nlm, Q, ev) bind(C, name="sfref_a4_eigentensors")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: Q(3,3,6), ev(6)
+        call a4_eigentensors(nlm, Q(:,:,1), Q(:,:,2), Q(:,:,3), Q(:,:,4), Q(:,:,5), Q(:,:,6), ev)
+    end subroutine
+
+    ! nlm_isvalid(nhat20, nhat40) of k (nhat20, nhat40) pairs -> isvalid(k) as 0/1
+    subroutine sfref_nlm_isvalid(k, n20, n40, ok) bind(C, name="sfref_nlm_isvalid")
+        integer(c_int), value :: k
+        complex(c_double_complex), intent(in) :: n20(k), n40(k)
+        integer(c_int), intent(out) :: ok(k)
+        logical :: v(k)
+        v = nlm_isvalid(n20, n40)
+        ok = merge(1, 0, v)
     end subroutine
 
     ! ---- time stepping ----

@@ -529,44 +529,59 @@ static void anticomm_mandel(const double *a, double L[6][6]) {
     memcpy(L, r, sizeof(r));
 }
 
-/* Cholesky solve reading only the lower triangle of A (LAPACK dposv('L')
- * semantics); returns 0 on success, j+1 if the leading minor j is not PD. */
-static int chol_solve_lower(const double A[6][6], double *b) {
-    double Lf[6][6] = {{0}};
+/* Cholesky factorization reading only the lower triangle of A, in the
+ * order of LAPACK's unblocked dpotf2 (the path dposv('L') takes for n = 6):
+ * column j: pivot d = A[j][j] - sum_k L[j][k]^2; d <= 0 stops with
+ * A[j][j] := d (the rest of column j and the columns after it keep their
+ * input values); else L[j][j] = sqrt(d) and the column below is updated and
+ * scaled by 1 / L[j][j].  F holds the (partially) factored matrix: its lower
+ * triangle is overwritten as LAPACK overwrites it, its upper triangle is the
+ * input's.  Returns 0, or j + 1 for a failed pivot j. */
+static int chol_factor_lower(const double A[6][6], double F[6][6]) {
+    memcpy(F, A, 36 * sizeof(double));
     for (int j = 0; j < 6; j++) {
-        double d = A[j][j];
-        for (int k = 0; k < j; k++) d -= Lf[j][k] * Lf[j][k];
-        if (!(d > 0)) return j + 1;
-        Lf[j][j] = sqrt(d);
-        for (int i = j + 1; i < 6; i++) {
-            double s = A[i][j];
-            for (int k = 0; k < j; k++) s -= Lf[i][k] * Lf[j][k];
-            Lf[i][j] = s / Lf[j][j];
+        double d = F[j][j];
+        for (int k = 0; k < j; k++) d -= F[j][k] * F[j][k];
+        if (!(d > 0)) {
+            F[j][j] = d;
+            return j + 1;
         }
-    }
-    for (int i = 0; i < 6; i++) {
-        double s = b[i];
-        for (int k = 0; k < i; k++) s -= Lf[i][k] * b[k];
-        b[i] = s / Lf[i][i];
-    }
-    for (int i = 5; i >= 0; i--) {
-        double s = b[i];
-        for (int k = i + 1; k < 6; k++) s -= Lf[k][i] * b[k];
-        b[i] = s / Lf[i][i];
+        d = sqrt(d);
+        F[j][j] = d;
+        const double r = 1.0 / d;
+        for (int i = j + 1; i < 6; i++) {
+            double t = F[i][j];
+            for (int k = 0; k < j; k++) t -= F[i][k] * F[j][k];
+            F[i][j] = t * r;
+        }
     }
     return 0;
 }
 
+/* solve with the factor of chol_factor_lower (L L^T x = b, in place) */
+static void chol_solve_factored(const double F[6][6], double *b) {
+    for (int i = 0; i < 6; i++) {
+        double s = b[i];
+        for (int k = 0; k < i; k++) s -= F[i][k] * b[k];
+        b[i] = s / F[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double s = b[i];
+        for (int k = i + 1; k < 6; k++) s -= F[k][i] * b[k];
+        b[i] = s / F[i][i];
+    }
+}
+
 /* Taylor homogenization, n' = 1: src/homogenizations.f90:145-189.
  * status |= 2 when the regularized normal equations were needed, 4 if they
- * failed too (the reference stops).  The regularized branch uses the
- * unfactored P (the reference passes the partially factored P left behind by
- * the failed dposv; parity unpinned for that branch). */
+ * failed too (the reference stops).  As in the reference, the regularized
+ * system is built from the matrix the failed dposv left behind (its lower
+ * triangle partially factored, :174-179) and the unchanged right-hand side. */
 static void taylor(const double *tau, const double *a2v, const double *a4v, const double *Eg, int ng, double *eps,
                    int *status) {
     double cA, cB, cC;
     rheo_params(Eg, (double)ng, -1, &cA, &cB, &cC);
-    double a2m[9], Lm[6][6], P[6][6], x[6];
+    double a2m[9], Lm[6][6], P[6][6], F[6][6], x[6];
     vec_to_mat(a2v, a2m);
     anticomm_mandel(a2m, Lm);
     const double idv[6] = {1, 1, 1, 0, 0, 0};
@@ -574,20 +589,22 @@ static void taylor(const double *tau, const double *a2v, const double *a4v, cons
         for (int j = 0; j < 6; j++)
             P[i][j] = (i == j ? 1.0 : 0.0) - cA * (idv[i] * a2v[j]) + cB * a4v[6 * i + j] + cC * Lm[i][j];
     mat_to_vec(tau, x);
-    if (chol_solve_lower(P, x) != 0) {
+    if (chol_factor_lower(P, F) == 0) {
+        chol_solve_factored(F, x);
+    } else {
         *status |= 2;
-        double R[6][6], rhs[6], t[6];
-        mat_to_vec(tau, t);
+        double R[6][6], RF[6][6], rhs[6];
         for (int i = 0; i < 6; i++) {
             rhs[i] = 0;
-            for (int k = 0; k < 6; k++) rhs[i] += P[k][i] * t[k];
+            for (int k = 0; k < 6; k++) rhs[i] += F[k][i] * x[k];
             for (int j = 0; j < 6; j++) {
                 double s = 0;
-                for (int k = 0; k < 6; k++) s += P[k][i] * P[k][j];
+                for (int k = 0; k < 6; k++) s += F[k][i] * F[k][j];
                 R[i][j] = s + (i == j ? (double)1e-6f : 0.0);
             }
         }
-        if (chol_solve_lower(R, rhs) != 0) *status |= 4;
+        if (chol_factor_lower(R, RF) == 0) chol_solve_factored(RF, rhs);
+        else *status |= 4;
         memcpy(x, rhs, sizeof(x));
     }
     vec_to_mat(x, eps);

@@ -32,13 +32,14 @@ class EvolveDesc(_c.Structure):
 
 class HomogParams(_c.Structure):
     _fields_ = [("Eij_grain", _c.c_double * 2), ("alpha", _c.c_double), ("n_grain", _c.c_int),
-                ("Lame", _c.c_double * 5), ("rho", _c.c_double), ("vi_alpha", _c.c_double)]
+                ("Lame", _c.c_double * 5), ("rho", _c.c_double), ("vi_alpha", _c.c_double),
+                ("eij_tol", _c.c_double)]
 
 
 class SurrogateStats(_c.Structure):
     _fields_ = [("n_fallback", _c.c_int64), ("n_audit", _c.c_int64), ("audit_max_err_Eij", _c.c_double),
                 ("audit_mean_err_Eij", _c.c_double), ("audit_max_err_Vi", _c.c_double),
-                ("audit_mean_err_Vi", _c.c_double)]
+                ("audit_mean_err_Vi", _c.c_double), ("n_bound", _c.c_int64)]
 
 
 _fp = _c.POINTER(_c.c_float)
@@ -58,6 +59,7 @@ SIGNATURES = {
     "sfg_M_CDRX": (_i, [_vp, _i, _dp]),
     "sfg_M_REG": (_i, [_vp, _i, _dp, _dp]),
     "sfg_M_LROT_batch": (_i, [_vp, _i, _i64, _vp, _vp, _d, _d, _vp, _vp]),
+    "sfg_M_REG_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_M_DDRX_src_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_rnlm_len": (_i, [_i]),
     "sfg_reduce_M": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
@@ -95,6 +97,12 @@ SIGNATURES = {
     "sfg_plan_create": (_i, [_vp, _i, _c.POINTER(EvolveDesc), _i, _i, _vp, _c.POINTER(_vp)]),
     "sfg_plan_run": (_i, [_vp, _vp]),
     "sfg_plan_destroy": (_i, [_vp]),
+    "sfg_rotate_nlm": (_i, [_vp, _i, _dp, _d, _d, _dp]),
+    "sfg_rotate_nlm_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "sfg_a4_eigentensors": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "sfg_a4_eigentensors_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp]),
+    "sfg_nlm_isvalid": (_i, [_vp, _i64, _dp, _dp, _c.POINTER(_c.c_int32)]),
+    "sfg_state_isvalid_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "sfg_mlp_nparams": (_i64, []),
     "sfg_mlp_create": (_i, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
     "sfg_mlp_destroy": (_i, [_vp]),
@@ -103,6 +111,7 @@ SIGNATURES = {
     "sfg_mlp_set_normalization": (_i, [_vp, _fp, _fp, _fp, _fp]),
     "sfg_mlp_get_normalization": (_i, [_vp, _fp]),
     "sfg_mlp_set_envelope": (_i, [_vp, _dp, _dp]),
+    "sfg_mlp_set_error_model": (_i, [_vp, _dp]),
     "sfg_mlp_train_step": (_i, [_vp, _i64, _vp, _vp, _c.c_float, _fp, _vp]),
     "sfg_mlp_gradients": (_i, [_vp, _i64, _vp, _vp, _fp, _fp]),
     "sfg_mlp_forward": (_i, [_vp, _i64, _vp, _vp, _vp]),

@@ -75,6 +75,17 @@ def M_DDRX_src_batch(L, tau, ctx=None):
     return M
 
 
+def M_REG_batch(L, D, ctx=None):
+    """Regularization operators (P, n, n) float64 of P nodes from strain
+    rates D (P, 3, 3): src/specfabpy/fenics/CPO.py:202, src/dynamics.f90:494-518."""
+    ctx = ctx or _lib.default_context()
+    Dd = D.to(torch.float64).contiguous()
+    n = nlm_len(L)
+    M = torch.empty((Dd.shape[0], n, n), dtype=torch.float64, device=Dd.device)
+    check(lib().sfg_M_REG_batch(ctx.handle, L, Dd.shape[0], _ptr(Dd), _ptr(M), _stream()))
+    return M
+
+
 def reduce_M_batch(L, M, ctx=None):
     """(Mrr, Mri, Mir, Mii), each (P, rn, rn) float64, of full operators M
     (P, n, n) (device tensor): reduce_M (src/reducedform.f90:76-120) per node."""
@@ -259,6 +270,31 @@ class FabricBatch:
                                                 float(alpha), int(n_grain), _ptr(E), _ptr(lam), _ptr(ei), _ptr(st),
                                                 _stream()))
         return (E, lam, ei) if mode == 0 else E
+
+    # ---- rotation, a4 eigentensors, validity ----
+    def rotate(self, theta, phi):
+        """Rotate every parcel's state in place by its own (theta, phi) (P)
+        or by scalars: rotate_nlm, src/rotation.f90:29-55 (L <= 12)."""
+        th = torch.as_tensor(theta, dtype=torch.float64, device=self.dev).expand(self.P).contiguous()
+        ph = torch.as_tensor(phi, dtype=torch.float64, device=self.dev).expand(self.P).contiguous()
+        check(lib().sfg_rotate_nlm_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(th), _ptr(ph),
+                                         _ptr(self.x), _stream()))
+
+    def a4_eigentensors(self):
+        """(Q (P, 6, 3, 3), eigenvalues (P, 6) ascending): a4_eigentensors,
+        src/frames.f90:131-152."""
+        Q = torch.empty((self.P, 6, 3, 3), dtype=torch.float64, device=self.dev)
+        ev = torch.empty((self.P, 6), dtype=torch.float64, device=self.dev)
+        check(lib().sfg_a4_eigentensors_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(Q), _ptr(ev),
+                                              _stream()))
+        return Q, ev
+
+    def isvalid(self):
+        """(P) bool: a2 and a4 eigenvalues inside (0, 1), the test of
+        nlm_isvalid (src/idealstate.f90:50-78) on general states."""
+        ok = torch.empty(self.P, dtype=torch.int32, device=self.dev)
+        check(lib().sfg_state_isvalid_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(ok), _stream()))
+        return ok.bool()
 
     # ---- full a4 and wave velocities ----
     def a4(self):

@@ -23,6 +23,7 @@
 #include "sfg_homog.h"
 #include "sfg_init.h"
 #include "sfg_reduced.h"
+#include "sfg_rotate.h"
 #include "sfg_wave.h"
 #include "sfg_mlp_net.h"
 #include "sfg_surrogate.h"
@@ -104,6 +105,8 @@ struct sfg_ctx {
     // rotation blocks of recently used direction sets (wave velocities)
     struct DirSet { std::vector<double> th, ph; double* R = nullptr; };
     std::vector<DirSet> dirs;
+    // Wigner sum-formula coefficients (rotate_nlm), uploaded on first use
+    double* wigner = nullptr;
     // stepping-kernel timing (sfg_ctx_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
@@ -378,6 +381,7 @@ int sfg_ctx_destroy(sfg_ctx* ctx) {
     (void)hipFree(ctx->csr_seg);
     if (ctx->ws) (void)hipFree(ctx->ws);
     for (auto& ds : ctx->dirs) (void)hipFree(ds.R);
+    if (ctx->wigner) (void)hipFree(ctx->wigner);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SFG_OK;
@@ -951,6 +955,19 @@ static int diag_single(sfg_ctx* ctx, int L, int mode, const double* D, double* M
 
 int sfg_M_CDRX(sfg_ctx* ctx, int L, double* M) { return diag_single(ctx, L, 0, nullptr, M); }
 
+int sfg_M_REG_batch(sfg_ctx* ctx, int L, int64_t P, const double* D_dev, double* M_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!D_dev || !M_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    if (!op->reg_defined) return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20");
+    if (P > (int64_t(1) << 40) / ((int64_t)op->n * op->n)) return fail(SFG_ERR_ARG, "P too large for one launch");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_diag_op(P, op->n, 1, op->cl, op->reg_shape, op->reg_nu, D_dev, M_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
 // ---------------------------------------------------------------- enhancement factors
 
 static int eij_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame, const double* frames_dev,
@@ -1195,6 +1212,143 @@ int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm,
                          &u);
 }
 
+// ---------------------------------------------------------------- rotation, a4 eigentensors, validity
+
+namespace {
+
+int ensure_wigner(sfg_ctx* ctx) {
+    if (ctx->wigner) return SFG_OK;
+    std::vector<sfg::WignerTerm> terms;
+    std::vector<int> first;
+    sfg::wigner_terms(sfg::ROT_LMAX, terms, first);
+    if ((int)terms.size() != sfg::wigner_offset(sfg::ROT_LMAX + 2, 0, -(sfg::ROT_LMAX + 2)))
+        return fail(SFG_ERR_ARG, "Wigner term table does not match the kernels' layout");
+    std::vector<double> c;
+    for (auto& t : terms) c.push_back(t.coef);
+    HIPCHK(upload(&ctx->wigner, c));
+    return SFG_OK;
+}
+
+int rotate_locked(sfg_ctx* ctx, int L, int64_t P, const double* x, const double* th, const double* ph, double* out,
+                  hipStream_t s) {
+    if (L < 0 || L > sfg::ROT_LMAX || (L % 2)) return fail(SFG_ERR_L, "rotate_nlm needs even L <= 12 (src/rotation.f90:37)");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_wigner(ctx);
+    if (rc) return rc;
+    if (L == 0) {   // only n_0^0, which rotations leave alone
+        if (out != x && P > 0) HIPCHK(hipMemcpyAsync(out, x, (size_t)P * 8, hipMemcpyDeviceToDevice, s));
+        return SFG_OK;
+    }
+    sfg::RotateArgs a{};
+    a.P = P; a.L = L; a.x = x; a.out = out; a.theta = th; a.phi = ph; a.coef = ctx->wigner;
+    HIPCHK(sfg::launch_rotate(a, s));
+    return SFG_OK;
+}
+
+}  // namespace
+
+int sfg_rotate_nlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* theta_dev,
+                         const double* phi_dev, double* out_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !theta_dev || !phi_dev || !out_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return rotate_locked(ctx, L, P, x_dev, theta_dev, phi_dev, out_dev, pick(ctx, stream));
+}
+
+int sfg_rotate_nlm(sfg_ctx* ctx, int L, const double* nlm, double theta, double phi, double* out) {
+    if (!ctx || !nlm || !out) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(2 * n + 2 * n + 2) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* dang = dx + n;
+    double* dy = dang + 2;
+    const double ang[2] = {theta, phi};
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dang, ang, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    rc = rotate_locked(ctx, L, 1, dx, dang, dang + 1, dy, s);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_state_to_nlm(1, n, op->cre, op->cim, op->csgn, op->cneg, dy, dn, s));
+    HIPCHK(hipMemcpyAsync(out, dn, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+namespace {
+int a4eig_locked(sfg_ctx* ctx, int L, int64_t P, const double* x, double* Q, double* ev, int32_t* valid,
+                 hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "a4 needs the l <= 4 coefficients (L >= 4)");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::A4EigArgs a{};
+    a.P = P; a.x = x; a.A2 = op->A2; a.A4F = op->A4F; a.Q = Q; a.ev = ev; a.valid = valid;
+    HIPCHK(sfg::launch_a4eig(a, s));
+    return SFG_OK;
+}
+}  // namespace
+
+int sfg_a4_eigentensors_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* Q_dev, double* ev_dev,
+                              void* stream) {
+    if (!ctx || P < 0 || (P > 0 && !x_dev)) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return a4eig_locked(ctx, L, P, x_dev, Q_dev, ev_dev, nullptr, pick(ctx, stream));
+}
+
+int sfg_state_isvalid_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int32_t* valid_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !valid_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return a4eig_locked(ctx, L, P, x_dev, nullptr, nullptr, valid_dev, pick(ctx, stream));
+}
+
+int sfg_a4_eigentensors(sfg_ctx* ctx, int L, const double* nlm, double Q[54], double ev[6]) {
+    if (!Q || !ev) return fail(SFG_ERR_ARG, "NULL argument");
+    std::vector<double> o(60);
+    int rc = single_parcel(ctx, L, nlm, 60, o.data(),
+                           [](sfg_ctx* c, int L_, const double* dx, double* dout, void*) {
+                               return a4eig_locked(c, L_, 1, dx, dout, dout + 54, nullptr, c->stream);
+                           },
+                           nullptr);
+    if (rc) return rc;
+    std::copy(o.begin(), o.begin() + 54, Q);
+    std::copy(o.begin() + 54, o.end(), ev);
+    return SFG_OK;
+}
+
+int sfg_nlm_isvalid(sfg_ctx* ctx, int64_t K, const double* nhat20, const double* nhat40, int32_t* isvalid) {
+    if (!ctx || K < 0 || (K > 0 && (!nhat20 || !nhat40 || !isvalid))) return fail(SFG_ERR_ARG, "NULL argument");
+    if (K == 0) return SFG_OK;
+    // the states (n_0^0 = 1, n_2^0, n_4^0) at L = 4 in the real basis: rows 0, 1, 6
+    std::vector<double> hx((size_t)15 * K, 0.0);
+    for (int64_t k = 0; k < K; k++) {
+        hx[k] = 1.0;
+        hx[(size_t)1 * K + k] = nhat20[2 * k];
+        hx[(size_t)6 * K + k] = nhat40[2 * k];
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, hx.size() * sizeof(double) + (size_t)K * 4));
+    double* dx = (double*)buf.p;
+    int32_t* dv = (int32_t*)(dx + hx.size());
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dx, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    int rc = a4eig_locked(ctx, 4, K, dx, nullptr, nullptr, dv, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(isvalid, dv, (size_t)K * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
 // ---------------------------------------------------------------- surrogate network
 
 }  // extern "C"
@@ -1345,6 +1499,15 @@ int sfg_mlp_set_envelope(sfg_mlp* net, const double lo[3], const double hi[3]) {
     return SFG_OK;
 }
 
+int sfg_mlp_set_error_model(sfg_mlp* net, const double taylor_rel_err[6]) {
+    if (!net || !taylor_rel_err) return fail(SFG_ERR_ARG, "NULL argument");
+    for (int i = 0; i < 6; i++)
+        if (!(taylor_rel_err[i] >= 0) || !std::isfinite(taylor_rel_err[i]))
+            return fail(SFG_ERR_ARG, "taylor_rel_err must be finite and >= 0");
+    for (int i = 0; i < 6; i++) net->net.taylor_err[i] = taylor_rel_err[i];
+    return SFG_OK;
+}
+
 static int mlp_loss(sfg_mlp* net, int64_t B, hipStream_t s, float* loss) {
     std::vector<float> rows((size_t)B);
     HIPCHK(hipMemcpyAsync(rows.data(), net->net.loss_rows, (size_t)B * 4, hipMemcpyDeviceToHost, s));
@@ -1460,6 +1623,19 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     fa.audit_frac = audit_frac; fa.seed = seed;
     for (int i = 0; i < sfg::MLP_NLAYER; i++) { ia.wf[i] = net->net.wf[i]; ia.bias[i] = net->net.bias_pad[i]; }
     ia.norm = net->net.norm; ia.gtab = net->net.gtab; ia.Eij = Eij_dev; ia.Vi = Vi_dev; ia.idx = net->idx; ia.count = net->count;
+    {   // Sachs term and error bound (rheo_params_tranisotropic, src/rheologies.f90:123-132, ef = +1)
+        sfg::SurrogateMixArgs& mx = ia.mix;
+        const double ne = 2.0 / (hp->n_grain + 1), pc = std::pow(hp->Eij_grain[0], ne),
+                     pa = std::pow(hp->Eij_grain[1], ne);
+        mx.A4 = op->A4;
+        mx.sA = (pc - 1) / 2;
+        mx.sB = (3 * (pc + 1) - 2) / 2 - 2 * pa;
+        mx.sC = pa - 1;
+        mx.snorm = hp->n_grain == 1 ? 1 + 2.0 / 15 * mx.sB + 2.0 / 3 * mx.sC : 1.0;
+        mx.alpha = hp->alpha;
+        mx.tol = hp->eij_tol > 0 ? hp->eij_tol : 1e-3;
+        for (int i = 0; i < 6; i++) mx.dT[i] = net->net.taylor_err[i];
+    }
     HIPCHK(sfg::launch_reset_count(net->count, s));
     HIPCHK(sfg::launch_surrogate_infer(ia, s));
     // 2. plain homogenization of the flagged parcels: the work list and its
@@ -1490,6 +1666,7 @@ int sfg_surrogate_homogenize(sfg_ctx* ctx, sfg_mlp* net, int L, int64_t P, const
     HIPCHK(hipStreamSynchronize(s));
     st.n_fallback = h.n_fallback;
     st.n_audit = h.n_audit;
+    st.n_bound = h.n_bound;
     st.audit_max_err_Eij = h.max_E;
     st.audit_max_err_Vi = h.max_V;
     if (st.n_audit) {

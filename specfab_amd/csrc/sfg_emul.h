@@ -15,6 +15,7 @@ typedef int hipError_t;
 typedef void* hipStream_t;
 #define __global__
 #define __device__
+#define __host__
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(m) ((void)0)
 // workgroup barrier of the emulated block (the driver runs a block's threads

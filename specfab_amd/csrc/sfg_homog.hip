@@ -118,21 +118,50 @@ __device__ void d_eig3(const double* M, double* ei, double* lami) {
 }
 
 // Cholesky of the symmetric matrix given by the lower triangle of A (LAPACK
-// dposv('L') semantics, src/homogenizations.f90:174).  Lf packed lower (21).
+// dposv('L') semantics, src/homogenizations.f90:174), in the order of the
+// unblocked dpotf2 that dposv runs for n = 6: pivot, then the column below
+// scaled by the pivot's reciprocal.  Lf packed lower (21).
 __device__ inline bool d_chol6(const double A[6][6], double* Lf) {
 #define LF(i, j) Lf[(i) * ((i) + 1) / 2 + (j)]
     for (int j = 0; j < 6; j++) {
         double d = A[j][j];
         for (int k = 0; k < j; k++) d -= LF(j, k) * LF(j, k);
         if (!(d > 0)) return false;
-        LF(j, j) = sqrt(d);
+        d = sqrt(d);
+        LF(j, j) = d;
+        const double r = 1.0 / d;
         for (int i = j + 1; i < 6; i++) {
             double s = A[i][j];
             for (int k = 0; k < j; k++) s -= LF(i, k) * LF(j, k);
-            LF(i, j) = s / LF(j, j);
+            LF(i, j) = s * r;
         }
     }
     return true;
+}
+
+// The matrix a failed dposv('L') leaves behind: F = A with its lower
+// triangle overwritten as far as the factorization got (columns before the
+// failed pivot j hold the factor, F[j][j] the non-positive pivot value; the
+// rest of column j, the later columns and the upper triangle keep A's
+// values).  The reference regularizes with this matrix
+// (src/homogenizations.f90:176-179).
+__device__ inline void d_failed_factor6(double F[6][6]) {
+    for (int j = 0; j < 6; j++) {
+        double d = F[j][j];
+        for (int k = 0; k < j; k++) d -= F[j][k] * F[j][k];
+        if (!(d > 0)) {
+            F[j][j] = d;
+            return;
+        }
+        d = sqrt(d);
+        F[j][j] = d;
+        const double r = 1.0 / d;
+        for (int i = j + 1; i < 6; i++) {
+            double t = F[i][j];
+            for (int k = 0; k < j; k++) t -= F[i][k] * F[j][k];
+            F[i][j] = t * r;
+        }
+    }
 }
 
 __device__ inline void d_chol6_solve(const double* Lf, double* b) {
@@ -167,9 +196,10 @@ __device__ inline void d_taylorP(const double* a2m, const double* a2v, const dou
 }
 
 // Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69.
-// P is factored once per parcel; the rare regularized branch rebuilds P
-// where it needs it, so P does not occupy registers through the loop over
-// the six stress states.
+// P is factored once per parcel; the rare regularized branch rebuilds the
+// failed factorization where it needs it (the same arithmetic gives the same
+// matrix), so it does not occupy registers through the loop over the six
+// stress states.
 __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, double Ecc, double Eca, double alpha,
                      int ng, double* Eij) {
     int status = 0;
@@ -190,6 +220,7 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
         ok = d_chol6(Pm, Lf);
         if (!ok) {
             status |= 2;
+            d_failed_factor6(Pm);
             double R[6][6];
             for (int i = 0; i < 6; i++)
                 for (int j = 0; j < 6; j++) {
@@ -241,6 +272,7 @@ __device__ int d_Eij(const double* a2v, const double* a4v, const double* e, doub
         } else {
             double Pm[6][6], rhs[6];
             d_taylorP(a2m, a2v, a4v, tA, tB, tC, Pm);
+            d_failed_factor6(Pm);
             for (int i = 0; i < 6; i++) {
                 rhs[i] = 0;
                 for (int k = 0; k < 6; k++) rhs[i] += Pm[k][i] * xs[k];

@@ -73,6 +73,14 @@ void real4_index(int q, int* l, int* am, int* c);
 void a4full_map(double A4F[15][15]);
 // rotate_nlm4 (src/rotation.f90:14-27) as real 5x5 / 9x9 blocks.
 void rotation_l4(double theta, double phi, double R2[5][5], double R4[9][9]);
+// Terms of the Wigner small-d entries d^l_{mn}(theta), l = 2, 4, ..., L,
+// m = 0..l, n = -l..l (entry index in that order): d = sum of
+// coef * cos(theta/2)^a * sin(theta/2)^b over the entry's terms (Wigner's sum
+// formula; the normalisation with the reference's REAL(4) square roots where
+// src/include/Dlmn.f90 has them as single factors).  first[e] .. first[e+1]
+// are entry e's terms.
+struct WignerTerm { double coef; int a, b; };
+void wigner_terms(int L, std::vector<WignerTerm>& terms, std::vector<int>& first);
 // Reuss (forward) and Voigt (reverse) elastic coefficients k1..k5.
 void elas_params(const double Lame[5], double kf[5], double kr[5]);
 

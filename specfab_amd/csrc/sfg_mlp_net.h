@@ -33,6 +33,8 @@ struct MlpNet {
     std::vector<void*> owned, batch_bufs;
     // envelope of the training states: sorted a2 eigenvalues lo/hi
     double env_lo[3] = {0, 0, 0}, env_hi[3] = {1, 1, 1};
+    // calibrated relative error of the six Taylor-term outputs (0: no bound)
+    double taylor_err[6] = {0, 0, 0, 0, 0, 0};
     bool trained = false;
 
     ~MlpNet();

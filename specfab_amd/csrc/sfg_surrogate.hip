@@ -55,12 +55,89 @@ __device__ inline double unit_hash(uint64_t v) {
     return (double)(v >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// features of parcel p into f[36]; returns its SUR_* flags (0 without a.flags)
-__device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, float* f) {
+// Sachs enhancement factors of the Cartesian axes (E11, E22, E33, E23, E13,
+// E12): the alpha = 0 part of Eij_tranisotropic with (e1, e2, e3) = (x, y, z)
+// (src/enhancementfactors.f90:23-69 with src/homogenizations.f90:69-116),
+// written out for the axis stress states: tau = I/3 - e_i e_i (normal) and
+// e_i e_j + e_j e_i (shear) reduce a4 : tau to the Mandel a4 entries a4v[i][j],
+// i, j < 3, and a4v[q][q], q = 3..5, and tau.a2 + a2.tau to a2 diagonals
+__device__ inline void sachs_xyz(const SurrogateMixArgs& m, const double* xq, double ix0, const double* a2v,
+                                 double* S) {
+    double a4r[3][3], a4d[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < 15; q++) s += m.A4[(6 * i + j) * 15 + q] * xq[q];
+            a4r[i][j] = s * ix0;
+        }
+#pragma unroll
+    for (int k = 3; k < 6; k++) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < 15; q++) s += m.A4[(7 * k) * 15 + q] * xq[q];
+        a4d[k - 3] = s * ix0;
+    }
+    const double tr3 = (a2v[0] + a2v[1] + a2v[2]) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        // tau_ii = tau : e_i e_i = -2/3; a2 : tau = tr a2 / 3 - a2_ii; (tau.a2 + a2.tau)_ii = -4/3 a2_ii
+        double a4t = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) a4t += a4r[i][j] * ((i == j ? -2.0 : 1.0) / 3.0);
+        const double lin = -2.0 / 3.0 - m.sA * (tr3 - a2v[i]) + m.sB * a4t + m.sC * (-4.0 / 3.0 * a2v[i]);
+        S[i] = lin / (m.snorm * (-2.0 / 3.0));
+    }
+    // shear (i, j) = (y, z), (x, z), (x, y): tau_ij = 1, (a4 : tau)_ij = a4v[q][q],
+    // (tau.a2 + a2.tau)_ij = a2_ii + a2_jj, no isotropic (sA) part off the diagonal
+    const int pi[3] = {1, 0, 0}, pj[3] = {2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        S[3 + k] = (1.0 + m.sB * a4d[k] + m.sC * (a2v[pi[k]] + a2v[pj[k]])) / m.snorm;
+}
+
+// features of parcel p into f[36]; returns its SUR_* flags (0 without
+// a.flags); with `mix`, also the Sachs term of its Cartesian Eij -> sachs[6]
+__device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, float* f,
+                                      const SurrogateMixArgs* mix = nullptr, float* sachs = nullptr) {
     double xq[15];
 #pragma unroll
     for (int q = 0; q < 15; q++) xq[q] = a.x[(int64_t)q * a.P + p];
     const double x0 = xq[0], ix0 = 1.0 / x0;
+    const double r2 = sqrt(2.0);
+    // a2 first: the envelope test and the Sachs term are done (and the Sachs
+    // term stored) before the 36 features are formed, so that they are not
+    // live at the same time
+    int fl = 0;
+    if (a.flags || mix) {
+        double a2v[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) s += a.A2[i * 15 + q] * xq[q];
+            a2v[i] = s * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
+        }
+        if (mix) {
+            double S[6];
+            sachs_xyz(*mix, xq, ix0, a2v, S);
+#pragma unroll
+            for (int i = 0; i < 6; i++) sachs[i] = (float)S[i];
+        }
+        // envelope: sorted a2 eigenvalues inside the training box; audit sample
+        if (a.flags) {
+            double A[9];
+            A[0] = a2v[0]; A[4] = a2v[1]; A[8] = a2v[2];
+            A[5] = A[7] = a2v[3] / r2; A[2] = A[6] = a2v[4] / r2; A[1] = A[3] = a2v[5] / r2;
+            double lam[3];
+            d_eigvals3(A, lam);
+            for (int i = 0; i < 3; i++)
+                if (!(lam[i] >= a.env_lo[i] && lam[i] <= a.env_hi[i])) fl |= SUR_FALLBACK;
+            if (!(fl & SUR_FALLBACK) && unit_hash(a.seed ^ (uint64_t)p) < a.audit_frac) fl |= SUR_AUDIT;
+        }
+    }
     // complex l <= 4 coefficients from the real basis [Re n_l0, Re n_l1, Im n_l1, ...]
     int j = 0;
     for (int l = 0; l <= 4; l += 2) {
@@ -83,29 +160,8 @@ __device__ inline int parcel_features(const SurrogateFeatArgs& a, int64_t p, flo
     double D[6] = {G[0], G[4], G[8], 0.5 * (G[5] + G[7]), 0.5 * (G[2] + G[6]), 0.5 * (G[1] + G[3])};
     double nn = D[0] * D[0] + D[1] * D[1] + D[2] * D[2] + 2 * (D[3] * D[3] + D[4] * D[4] + D[5] * D[5]);
     const double inv = nn > 0 ? 1.0 / sqrt(nn) : 0.0;
-    const double r2 = sqrt(2.0);
     f[30] = (float)(D[0] * inv); f[31] = (float)(D[1] * inv); f[32] = (float)(D[2] * inv);
     f[33] = (float)(r2 * D[3] * inv); f[34] = (float)(r2 * D[4] * inv); f[35] = (float)(r2 * D[5] * inv);
-    // envelope: sorted a2 eigenvalues inside the training box; audit sample
-    int fl = 0;
-    if (a.flags) {
-        double a2v[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            double s = 0;
-#pragma unroll
-            for (int q = 0; q < 6; q++) s += a.A2[i * 15 + q] * xq[q];
-            a2v[i] = s * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
-        }
-        double A[9];
-        A[0] = a2v[0]; A[4] = a2v[1]; A[8] = a2v[2];
-        A[5] = A[7] = a2v[3] / r2; A[2] = A[6] = a2v[4] / r2; A[1] = A[3] = a2v[5] / r2;
-        double lam[3];
-        d_eigvals3(A, lam);
-        for (int i = 0; i < 3; i++)
-            if (!(lam[i] >= a.env_lo[i] && lam[i] <= a.env_hi[i])) fl |= SUR_FALLBACK;
-        if (!(fl & SUR_FALLBACK) && unit_hash(a.seed ^ (uint64_t)p) < a.audit_frac) fl |= SUR_AUDIT;
-    }
     return fl;
 }
 
@@ -141,12 +197,14 @@ __device__ __forceinline__ void merge_item(int64_t t, const int* idx, const int*
     if (v2 < v1) { tt = v1; v1 = v2; v2 = tt; }
     if (v1 < v0) { tt = v0; v0 = v1; v1 = tt; }
     const double vs[3] = {v0, v1, v2};
-    if (flags[p] & SUR_FALLBACK) {
+    const int fl = flags[p];
+    if (fl & (SUR_FALLBACK | SUR_BOUND)) {
         for (int i = 0; i < 6; i++) Eij[p * 6 + i] = Ep[t * 6 + i];
         for (int i = 0; i < 3; i++) Vi[p * 3 + i] = vs[i];
-        errE[t] = -1.0;
+        // -2 marks the parcels that only the error bound sent to the plain path
+        errE[t] = (fl & SUR_FALLBACK) ? -1.0 : -2.0;
         errV[t] = -1.0;
-        if (status) atomicOr(status + p, 16);
+        if (status) atomicOr(status + p, ((fl & SUR_FALLBACK) ? 16 : 0) | ((fl & SUR_BOUND) ? 64 : 0));
     } else {
         double eE = 0, eV = 0;
         for (int i = 0; i < 6; i++) eE = fmax(eE, fabs(Eij[p * 6 + i] - Ep[t * 6 + i]) / fabs(Ep[t * 6 + i]));
@@ -170,12 +228,13 @@ __global__ void merge_kernel(const int* count, int64_t cap, const int* idx, cons
 __global__ void __launch_bounds__(256) audit_stats_kernel(const int* count, int64_t cap, const double* errE,
                                                           const double* errV, AuditSums* out) {
     const int64_t n = *count < cap ? *count : cap;
-    long long nf = 0, na = 0;
+    long long nf = 0, na = 0, nb = 0;
     double mE = 0, sE = 0, mV = 0, sV = 0;
     for (int64_t t = threadIdx.x; t < n; t += 256) {
         const double e = errE[t], v = errV[t];
         if (e < 0) {
             nf++;
+            if (e < -1.5) nb++;
         } else {
             na++;
             mE = fmax(mE, e);
@@ -184,15 +243,16 @@ __global__ void __launch_bounds__(256) audit_stats_kernel(const int* count, int6
             sV += v;
         }
     }
-    __shared__ long long snf[256], sna[256];
+    __shared__ long long snf[256], sna[256], snb[256];
     __shared__ double smE[256], ssE[256], smV[256], ssV[256];
     const int i = threadIdx.x;
-    snf[i] = nf; sna[i] = na; smE[i] = mE; ssE[i] = sE; smV[i] = mV; ssV[i] = sV;
+    snf[i] = nf; sna[i] = na; snb[i] = nb; smE[i] = mE; ssE[i] = sE; smV[i] = mV; ssV[i] = sV;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (i < h) {
             snf[i] += snf[i + h];
             sna[i] += sna[i + h];
+            snb[i] += snb[i + h];
             smE[i] = fmax(smE[i], smE[i + h]);
             ssE[i] += ssE[i + h];
             smV[i] = fmax(smV[i], smV[i + h]);
@@ -203,6 +263,7 @@ __global__ void __launch_bounds__(256) audit_stats_kernel(const int* count, int6
     if (i == 0) {
         out->n_fallback = snf[0];
         out->n_audit = sna[0];
+        out->n_bound = snb[0];
         out->max_E = smE[0];
         out->sum_E = ssE[0];
         out->max_V = smV[0];
@@ -365,6 +426,8 @@ __global__ void __launch_bounds__(256, 4 / INFER_NT) surrogate_infer_kernel(Surr
     __shared__ bf16x8 wst0[WSLICE], wst1[WSLICE], wst2[WSLICE];
     bf16x8* const wst[3] = {wst0, wst1, wst2};
     __shared__ float2 gtab[GTAB_N];
+    __shared__ float sS[4 * WP][6];   // Sachs term of each parcel's Cartesian Eij
+    __shared__ int sfl[4 * WP];       // feature-phase flags (envelope, audit)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     {   // GELU table: 8 KB, two 16-byte loads per thread
         const f32x4* src = (const f32x4*)a.gtab;
@@ -380,8 +443,7 @@ __global__ void __launch_bounds__(256, 4 / INFER_NT) surrogate_infer_kernel(Surr
         float f[36];
         int fl = 0;
         if (p < P) {
-            fl = parcel_features(a.fa, p, f);
-            if (a.fa.flags) a.fa.flags[p] = fl;
+            fl = parcel_features(a.fa, p, f, &a.mix, sS[tid]);
 #pragma unroll
             for (int k = 0; k < 36; k++)
                 fs[fw][fp][k] = (__bf16)((f[k] - a.norm[k]) * __builtin_amdgcn_rcpf(a.norm[36 + k]));
@@ -391,6 +453,7 @@ __global__ void __launch_bounds__(256, 4 / INFER_NT) surrogate_infer_kernel(Surr
         }
 #pragma unroll
         for (int k = 36; k < 64; k++) fs[fw][fp][k] = (__bf16)0.0f;
+        sfl[tid] = fl;
         // flagged parcels (fallback / audit) join the work list: one atomic per wave
         if (a.idx) {
             const uint64_t m = __ballot(fl != 0);
@@ -435,20 +498,53 @@ __global__ void __launch_bounds__(256, 4 / INFER_NT) surrogate_infer_kernel(Surr
 #pragma unroll
         for (int t = 0; t < NT; t++) o[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3, B[s][t], o[t], 0, 0, 0);
     }
-    // outputs 4 g + i of parcel base + 16 t + c: de-normalise, exp
+    // outputs 4 g + i of parcel base + 16 t + c: de-normalise, exp.  The six
+    // Eij outputs are the Taylor term T_ij; Eij = (1 - alpha) S_ij + alpha T_ij
+    // with the Sachs term S from the feature phase, and the parcel leaves the
+    // surrogate when alpha T_ij dT_ij > tol E_ij for any ij (or E is not a
+    // positive finite number).  Lanes g = 0 and g = 1 hold the six Eij of a
+    // parcel, g = 0 decides (the feature lanes' flags are in LDS: the parcel's
+    // flags are written once, here).
 #pragma unroll
     for (int t = 0; t < NT; t++) {
+        const int loc = wave * WP + 16 * t + c;
         const int64_t q = base + 16 * t + c;
-        if (q >= P) continue;
+        int bad = 0;
+        if (q < P) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int oi = 4 * g + i;
-            if (oi >= 9) continue;
-            const float y = o[t][i] * a.norm[81 + oi] + a.norm[72 + oi];
-            const double v = (double)expf(y);
-            if (oi < 6) a.Eij[q * 6 + oi] = v;
-            else a.Vi[q * 3 + oi - 6] = v;
+            for (int i = 0; i < 4; i++) {
+                const int oi = 4 * g + i;
+                if (oi >= 9) continue;
+                const float y = o[t][i] * a.norm[81 + oi] + a.norm[72 + oi];
+                const double v = (double)expf(y);
+                if (oi < 6) {
+                    const double E = (1.0 - a.mix.alpha) * (double)sS[loc][oi] + a.mix.alpha * v;
+                    a.Eij[q * 6 + oi] = E;
+                    if (!(a.mix.alpha * v * a.mix.dT[oi] <= a.mix.tol * E) || !(E > 0.0) || !isfinite(E)) bad = 1;
+                } else {
+                    a.Vi[q * 3 + oi - 6] = v;
+                }
+            }
         }
+        bad |= __shfl_xor(bad, 16);
+        const bool own = g == 0 && q < P;
+        int fl = own ? sfl[loc] : 0;
+        const bool over = own && bad;
+        const bool join = over && fl == 0;   // not on the work list yet
+        if (over) fl |= SUR_BOUND;
+        if (a.idx) {
+            const uint64_t m = __ballot(join);
+            if (m) {
+                const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const int leader = __builtin_ctzll(m);
+                int slot = 0;
+                if (lane == leader) slot = atomicAdd(a.count, __popcll(m));
+                slot = __shfl(slot, leader);
+                if (join) a.idx[slot + below] = (int)q;
+            }
+        }
+        if (own && a.fa.flags) a.fa.flags[q] = fl;
     }
 }
 

@@ -146,6 +146,54 @@ void a4full_map(double A4F[15][15]) {
     }
 }
 
+namespace {
+// exponent of the prime p in k! (Legendre's formula)
+int fact_exp(int k, int p) {
+    int e = 0;
+    for (long q = p; q <= k; q *= p) e += (int)(k / q);
+    return e;
+}
+
+// dnorm for any l <= 15: sqrt((l+m)!(l-m)!(l+n)!(l-n)!) = root * sqrt(sq)
+// from the prime exponents of the factorials (the product itself exceeds 64
+// bits from l = 6 on), the square-free part sq as a float32 square root as
+// in dnorm
+long double dnorm_big(int l, int m, int n) {
+    static const int primes[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29};
+    long double root = 1;
+    long long sq = 1;
+    for (int p : primes) {
+        const int e = fact_exp(l + m, p) + fact_exp(l - m, p) + fact_exp(l + n, p) + fact_exp(l - n, p);
+        for (int i = 0; i < e / 2; i++) root *= p;
+        if (e & 1) sq *= p;
+    }
+    if (sq == 1) return root;
+    const int am = std::abs(m), an = std::abs(n);
+    if (l == 4 && am + an == 3 && am * an == 2) return root * (2.0L / (long double)std::sqrt(2.0f));
+    return root * (long double)std::sqrt((float)sq);
+}
+}  // namespace
+
+void wigner_terms(int L, std::vector<WignerTerm>& terms, std::vector<int>& first) {
+    terms.clear();
+    first.assign(1, 0);
+    for (int l = 2; l <= L; l += 2)
+        for (int m = 0; m <= l; m++)
+            for (int n = -l; n <= l; n++) {
+                const long double nrm = dnorm_big(l, m, n);
+                for (int k = std::max(0, n - m); k <= std::min(l + n, l - m); k++) {
+                    long double den = 1;
+                    for (int i = 2; i <= l + n - k; i++) den *= i;
+                    for (int i = 2; i <= k; i++) den *= i;
+                    for (int i = 2; i <= m - n + k; i++) den *= i;
+                    for (int i = 2; i <= l - m - k; i++) den *= i;
+                    const double c = (double)(nrm / den) * (((m - n + k) & 1) ? -1.0 : 1.0);
+                    terms.push_back({c, 2 * l + n - m - 2 * k, m - n + 2 * k});
+                }
+                first.push_back((int)terms.size());
+            }
+}
+
 // Real-basis rotation of the l = 2 and l = 4 blocks by rotate_nlm4(., theta,
 // phi) (src/rotation.f90:14-27): n'_l^m = sum_n D^l(m,n) n_l^n with
 // D^l(m,n) = exp(-i n phi) d^l_{mn}(theta).  Row/column order inside a block

@@ -59,6 +59,31 @@ module specfab_gpu
             complex(c_double_complex) :: nlm(*), M(*)
             real(c_double) :: tau(9)
         end function
+        integer(c_int) function c_M_DDRX_src(c, L, tau, M) bind(C, name="sfg_M_DDRX_src")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            real(c_double) :: tau(9)
+            complex(c_double_complex) :: M(*)
+        end function
+        integer(c_int) function c_Eij_batch(c, L, P, x, frame, frames, Eg, alpha, ng, Eij, lami, ei, st, stream) &
+                bind(C, name="sfg_Eij_tranisotropic_batch")
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: c, x, frames, Eij, lami, ei, st, stream
+            integer(c_int), value :: L, frame, ng
+            integer(c_int64_t), value :: P
+            real(c_double) :: Eg(2)
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_Vi_el_batch(c, L, P, x, alpha, Lame, rho, ndir, th, ph, Vi, st, stream) &
+                bind(C, name="sfg_Vi_elastic_tranisotropic_batch")
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: c, x, Vi, st, stream
+            integer(c_int), value :: L, ndir
+            integer(c_int64_t), value :: P
+            real(c_double), value :: alpha, rho
+            real(c_double) :: Lame(5), th(*), ph(*)
+        end function
         integer(c_int) function c_M_REG(c, L, D, M) bind(C, name="sfg_M_REG")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: c
@@ -223,6 +248,16 @@ contains
         M = transpose(Mc)
     end function
 
+    ! M_DDRX_src(tau): src/dynamics.f90:277-298
+    function M_DDRX_src(tau) result(M)
+        real(dp), intent(in) :: tau(3,3)
+        complex(dp) :: M(nlm_len, nlm_len), Mc(nlm_len, nlm_len)
+        real(c_double) :: t(9)
+        t = reshape(transpose(tau), [9])
+        call check(c_M_DDRX_src(ctx, int(Lcap, c_int), t, Mc), 'M_DDRX_src')
+        M = transpose(Mc)
+    end function
+
     ! M_REG(D): src/dynamics.f90:494-518
     function M_REG(D) result(M)
         real(dp), intent(in) :: D(3,3)
@@ -339,6 +374,74 @@ contains
         complex(dp) :: nlm_bounded(nlm_len), z(nlm_len)
         z = nlm(1:nlm_len)
         call check(c_apply_bounds(ctx, int(Lcap, c_int), z, nlm_bounded), 'apply_bounds')
+    end function
+
+    ! Device state (n, P) of host coefficient vectors nlm(:, p), p = 1..P
+    function states_on_device(nlm) result(dx)
+        complex(dp), intent(in), target :: nlm(:,:)
+        type(c_ptr) :: dx, dnlm
+        integer(c_int64_t) :: P
+        integer :: n
+        n = size(nlm, 1)
+        P = size(nlm, 2)
+        call check(hipMalloc(dx, int(n*P*8, c_size_t)), 'hipMalloc')
+        call check(hipMalloc(dnlm, int(2*n*P*8, c_size_t)), 'hipMalloc')
+        call check(hipMemcpy(dnlm, c_loc(nlm), int(2*n*P*8, c_size_t), 1_c_int), 'hipMemcpy')
+        call check(c_nlm_to_state(ctx, int(Lcap, c_int), P, dnlm, dx, c_null_ptr), 'nlm_to_state')
+        call check(hipDeviceSynchronize(), 'hipDeviceSynchronize')
+        call check(hipFree(dnlm), 'hipFree')
+    end function
+
+    ! Eij_tranisotropic of many parcels (the loop of specfabpy's
+    ! Eij_tranisotropic_arr, src/specfabpy.f90:474-486) in one batched call:
+    ! Eij(:, p) of nlm(:, p) in the a2 eigenframe of each parcel (frame = 0,
+    ! as eig gives it) or the Cartesian frame (frame = 2)
+    function Eij_tranisotropic_batch(nlm, Eij_grain, alpha, n_grain, frame) result(Eij)
+        complex(dp), intent(in) :: nlm(:,:)
+        real(dp), intent(in) :: Eij_grain(2), alpha
+        integer, intent(in) :: n_grain, frame
+        real(dp) :: Eij(6, size(nlm, 2))
+        real(dp), allocatable, target :: buf(:,:)
+        real(c_double) :: Eg(2)
+        type(c_ptr) :: dx, dE
+        integer(c_int64_t) :: P
+        P = size(nlm, 2)
+        Eg = Eij_grain
+        dx = states_on_device(nlm)
+        call check(hipMalloc(dE, int(6*P*8, c_size_t)), 'hipMalloc')
+        call check(c_Eij_batch(ctx, int(Lcap, c_int), P, dx, int(frame, c_int), c_null_ptr, Eg, alpha, &
+                               int(n_grain, c_int), dE, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), &
+                   'Eij_tranisotropic_batch')
+        call check(hipDeviceSynchronize(), 'hipDeviceSynchronize')
+        allocate(buf(6, P))
+        call check(hipMemcpy(c_loc(buf), dE, int(6*P*8, c_size_t), 2_c_int), 'hipMemcpy')
+        Eij = buf
+        call check(hipFree(dE), 'hipFree'); call check(hipFree(dx), 'hipFree')
+    end function
+
+    ! Vi_elastic_tranisotropic of many parcels: Vi(:, d, p) = (qS1, qS2, qP)
+    ! of nlm(:, p) along (theta(d), phi(d))
+    function Vi_elastic_tranisotropic_batch(nlm, alpha, Lame, rho, theta, phi) result(Vi)
+        complex(dp), intent(in) :: nlm(:,:)
+        real(dp), intent(in) :: alpha, Lame(5), rho, theta(:), phi(:)
+        real(dp) :: Vi(3, size(theta), size(nlm, 2))
+        real(dp), allocatable, target :: buf(:,:,:)
+        real(dp) :: lm(5), th(size(theta)), ph(size(theta))
+        type(c_ptr) :: dx, dV
+        integer(c_int64_t) :: P
+        integer(c_size_t) :: nb
+        P = size(nlm, 2)
+        lm = Lame; th = theta; ph = phi
+        nb = int(3*size(theta)*P*8, c_size_t)
+        dx = states_on_device(nlm)
+        call check(hipMalloc(dV, nb), 'hipMalloc')
+        call check(c_Vi_el_batch(ctx, int(Lcap, c_int), P, dx, alpha, lm, rho, int(size(theta), c_int), th, ph, &
+                                 dV, c_null_ptr, c_null_ptr), 'Vi_elastic_tranisotropic_batch')
+        call check(hipDeviceSynchronize(), 'hipDeviceSynchronize')
+        allocate(buf(3, size(theta), P))
+        call check(hipMemcpy(c_loc(buf), dV, nb, 2_c_int), 'hipMemcpy')
+        Vi = buf
+        call check(hipFree(dV), 'hipFree'); call check(hipFree(dx), 'hipFree')
     end function
 
     ! Batched evolution of P parcels on the GPU (host arrays in, host arrays out):

@@ -234,6 +234,42 @@ def a4(nlm):
     return out.reshape(3, 3, 3, 3)
 
 
+def rotate_nlm(nlm, theta, phi):
+    """rotate_nlm(nlm, theta, phi) (src/rotation.f90:29-55, specfabpy
+    src/specfabpy.f90): n'_l^m = sum_n exp(-i n phi) d^l_mn(theta) n_l^n, L <= 12."""
+    L = _L_of(nlm)
+    z = _z(nlm)
+    out = np.zeros_like(z)
+    check(lib().sfg_rotate_nlm(_ctx(), L, _zp(z), float(theta), float(phi), _zp(out)))
+    return out
+
+
+def a4_eigentensors(nlm):
+    """(Q1, ..., Q6, eigvals) of a4 (src/frames.f90:131-152): the Mandel
+    matrix's eigenvalues (ascending) and eigentensors (3, 3)."""
+    L = _L_of(nlm)
+    z = _z(nlm)
+    Q = np.zeros(54)
+    ev = np.zeros(6)
+    check(lib().sfg_a4_eigentensors(_ctx(), L, _zp(z), dptr(Q), dptr(ev)))
+    Q = Q.reshape(6, 3, 3)
+    return (*Q, ev)
+
+
+def nlm_isvalid(nhat20, nhat40):
+    """nlm_isvalid(nhat20, nhat40) (src/idealstate.f90:50-78): whether the
+    states n_0^0 = 1, n_2^0 = nhat20, n_4^0 = nhat40 have a2 and a4
+    eigenvalues inside (0, 1); array-valued like the reference."""
+    n20 = np.ascontiguousarray(np.atleast_1d(nhat20), dtype=np.complex128)
+    n40 = np.ascontiguousarray(np.atleast_1d(nhat40), dtype=np.complex128)
+    if n20.shape != n40.shape:
+        raise ValueError("nhat20 and nhat40 must have the same shape")
+    ok = np.zeros(n20.size, dtype=np.int32)
+    check(lib().sfg_nlm_isvalid(_ctx(), n20.size, _zp(n20.ravel()), _zp(n40.ravel()),
+                                ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return ok.astype(bool).reshape(n20.shape)
+
+
 def Cij_to_Lame_tranisotropic(Cij):
     """(C11, C33, C55, C12, C13) -> (lam, mu, Elam, Emu, Egam), src/elasticities.f90:107-120."""
     C = np.ascontiguousarray(np.asarray(Cij, dtype=np.float64).reshape(5))

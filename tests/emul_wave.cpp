@@ -69,6 +69,20 @@ extern "C" int emul_vi_elastic(int L, int64_t P, const double* x, double alpha, 
     return 0;
 }
 
+// the same with the compliance and stiffness coefficients given directly
+// (kf, kr of elas_params), to reach branches no grain parameters reach
+extern "C" int emul_vi_elastic_k(int L, int64_t P, const double* x, double alpha, const double* kf,
+                                 const double* kr, double rho, int ndir, const double* theta, const double* phi,
+                                 double* Vi, int* status) {
+    Tables t = tables(L, ndir, theta, phi);
+    sfg::ViElasticArgs a{};
+    a.P = P; a.ndir = ndir; a.nq = t.nq; a.x = x; a.A2 = t.A2.data(); a.A4 = t.A4.data(); a.A4F = t.A4F.data();
+    a.R = t.R.data(); a.alpha = alpha; a.rho = rho; a.Vi = Vi; a.status = status;
+    for (int i = 0; i < 5; i++) { a.kf[i] = kf[i]; a.kr[i] = kr[i]; }
+    grid(P * ndir, 128, [&] { sfg::vi_elastic_kernel(a); });
+    return 0;
+}
+
 extern "C" int emul_vi_em(int L, int64_t P, const double* x, double eps_m, double eps_t, double mu, int ndir,
                           const double* theta, const double* phi, double* Vi) {
     Tables t = tables(L, ndir, theta, phi);

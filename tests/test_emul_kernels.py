@@ -141,6 +141,7 @@ def emul_wave_lib():
     c = ctypes
     lib.emul_vi_elastic.argtypes = [c.c_int, c.c_int64, d, c.c_double, d, c.c_double, c.c_int, d, d, d, i]
     lib.emul_vi_em.argtypes = [c.c_int, c.c_int64, d, c.c_double, c.c_double, c.c_double, c.c_int, d, d, d]
+    lib.emul_vi_elastic_k.argtypes = [c.c_int, c.c_int64, d, c.c_double, d, d, c.c_double, c.c_int, d, d, d, i]
     lib.emul_a4.argtypes = [c.c_int, c.c_int64, d, d]
     _wave["lib"] = lib
     return lib
@@ -170,3 +171,161 @@ def test_wave_kernels_vs_reference():
     A = np.zeros((P, 81))
     lib.emul_a4(8, P, X, A)
     assert relerr(A.reshape(P, 3, 3, 3, 3), g["a4"]) < 1e-14
+
+
+def test_reuss_singular_branch():
+    """The Reuss solve flags an exactly singular compliance (a zero LU pivot,
+    the condition under which the reference's dgetrf returns info > 0 and it
+    stops, src/wavepropagation.f90:347-352) with SFG_ST_REUSS_SING and NaN
+    velocities.  Grain parameters from Cij_to_Lame never give a zero pivot
+    (the xy-shear diagonal is at least 1 / (2 mu)), so the coefficients are
+    given directly: with kf = (1, 0, 1/2, 0, 0) the xy-shear row of the
+    Reuss matrix vanishes wherever a2_xy = 0 (no a4 or anticommutator term
+    is left): for a state with only m = 0 coefficients also after its
+    rotation to a direction (theta, phi), which leaves it mirror-symmetric
+    about a plane through z.  The same states with the ice coefficients are regular, and
+    the Voigt average (alpha = 1) does not use the Reuss solve."""
+    lib = emul_wave_lib()
+    L = 8
+    n = 45
+    X = np.zeros((n, 3))
+    X[0] = 1 / np.sqrt(4 * np.pi)
+    X[1, 1], X[6, 1] = 0.3, 0.2     # n_2^0, n_4^0: real basis rows of m = 0
+    X[1, 2], X[6, 2] = -0.4, 0.1
+    th, ph = np.array([0.0, 0.8]), np.array([0.0, 1.1])
+    kf = np.array([1.0, 0.0, 0.5, 0.0, 0.0])
+    kr = np.array([1.0, 2.0, 0.1, 0.2, 0.3])
+    for alpha, sing in ((0.0, True), (0.5, True), (1.0, False)):
+        V = np.zeros((3, 2, 3))
+        st = np.zeros(3, np.int32)
+        lib.emul_vi_elastic_k(L, 3, X, alpha, kf, kr, 917.0, 2, th, ph, V, st)
+        if sing:
+            assert (st == 8).all() and np.isnan(V).all(), (alpha, st)
+        else:
+            assert not st.any() and np.isfinite(V).all()
+    from conftest import load_golden
+    g = load_golden("wave_L8.npz")
+    V = np.zeros((3, 2, 3))
+    st = np.zeros(3, np.int32)
+    lib.emul_vi_elastic(L, 3, X, 0.0, g["Lame"], 917.0, 2, th, ph, V, st)
+    assert not st.any() and np.isfinite(V).all()
+
+
+_rot = {}
+
+
+def emul_rotate_lib():
+    if "lib" in _rot:
+        return _rot["lib"]
+    out = os.path.join(tempfile.gettempdir(), "sfg_emul_rotate_%d.so" % os.getpid())
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DSFG_HOST_EMUL", "-I", CSRC, "-x", "c++",
+           os.path.join(ROOT, "tests", "emul_rotate.cpp"), os.path.join(CSRC, "sfg_rotate.hip"),
+           os.path.join(CSRC, "sfg_wave_host.cpp"), os.path.join(CSRC, "sfg_basis.cpp"), "-o", out]
+    if not os.path.exists(os.path.join(GEN, "..", "generated", "gaunt_L20.inc")):
+        pytest.skip("generated tables not present (run the build)")
+    subprocess.check_call(cmd)
+    lib = ctypes.CDLL(out)
+    d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    c = ctypes
+    lib.emul_rotate.argtypes = [c.c_int, c.c_int64, d, d, d, d]
+    lib.emul_a4eig.argtypes = [c.c_int, c.c_int64, d, d, d, i]
+    _rot["lib"] = lib
+    return lib
+
+
+def _rot_blocks(a, ref, L):
+    """largest relative error per degree block of rotated states (P, n)"""
+    out = {}
+    for l in range(2, L + 1, 2):
+        s = slice(l * (l - 1) // 2, (l + 1) * (l + 2) // 2)
+        out[l] = float(np.abs(a[:, s] - ref[:, s]).max() / np.abs(ref[:, s]).max())
+    return out
+
+
+def test_rotate_kernel_vs_reference():
+    """rotate_nlm (src/rotation.f90:29-55) on the CPU emulation of the
+    rotation kernel, per-parcel angles, against the reference's outputs
+    (tests/golden/rotate_L12.npz) for L = 4, 8, 12.  The l = 2 and l = 4
+    blocks carry the reference's REAL(4) square roots of
+    src/include/Dlmn.f90 and agree to 1e-12.  From l = 6 on, 1473 of 1630
+    Wigner-D entries have the square-free part of their normalisation as a
+    single REAL(4) factor (reproduced); the rest are sums of terms with
+    different REAL(4) literals, so these blocks agree to 1e-7 (the float32
+    rounding of the literals, measured <= 6.4e-8).  D(m, n) = exp(-i n phi)
+    d_mn(theta): rotating by (0, phi) then (theta, 0) equals rotating by
+    (theta, phi); a rotation preserves each block's power S(l) to 1e-7 (the
+    REAL(4) normalisations make the reference's D^l unitary only to float32
+    precision, and this reproduces them)."""
+    from conftest import load_golden
+    g = load_golden("rotate_L12.npz")
+    lib = emul_rotate_lib()
+    ang = g["angles"]
+    for L in (4, 8, 12):
+        X = g["nlm%d" % L]
+        R = g["rot%d" % L]
+        K, A = X.shape[0], len(ang)
+        x = to_state(np.repeat(X, A, axis=0), L)
+        th = np.tile(ang[:, 0], K).copy()
+        ph = np.tile(ang[:, 1], K).copy()
+        out = np.zeros_like(x)
+        assert lib.emul_rotate(L, K * A, x, th, ph, out) == 0
+        got = to_nlm(out, L)
+        err = _rot_blocks(got, R.reshape(K * A, -1), L)
+        print("L=%d rotate_nlm vs reference per degree:" % L, {l: "%.1e" % e for l, e in err.items()})
+        for l, e in err.items():
+            assert e < (1e-12 if l <= 4 else 1e-7), (L, l, e)
+        # composition and power
+        x2 = np.zeros_like(x)
+        lib.emul_rotate(L, K * A, x, np.zeros_like(th), ph, x2)
+        x3 = np.zeros_like(x)
+        lib.emul_rotate(L, K * A, x2, th, np.zeros_like(ph), x3)
+        assert np.abs(x3 - out).max() < 1e-13 * np.abs(out).max()
+        Xr = np.repeat(X, A, axis=0)
+        for l in range(2, L + 1, 2):
+            s = slice(l * (l - 1) // 2, (l + 1) * (l + 2) // 2)
+            p0 = (np.abs(Xr[:, s]) ** 2).sum(1)
+            p1 = (np.abs(got[:, s]) ** 2).sum(1)
+            assert np.abs(p1 - p0).max() < 1e-7 * p0.max()
+
+
+def test_a4_eigentensors_and_isvalid_vs_reference():
+    """a4_eigentensors (src/frames.f90:131-152) and nlm_isvalid
+    (src/idealstate.f90:50-78) on the CPU emulation of their kernel against
+    the reference (tests/golden/rotate_L12.npz): eigenvalues (ascending) to
+    1e-12, eigentensors up to sign where the eigen-gap exceeds 1e-8, the
+    eigentensors orthonormal in the Mandel inner product and
+    sum_i ev_i Q_i (x) Q_i = a4; validity of the (nhat20, nhat40) grid equal to
+    the reference's."""
+    from conftest import load_golden
+    g = load_golden("rotate_L12.npz")
+    lib = emul_rotate_lib()
+    X = g["a4eig_nlm"]
+    P = X.shape[0]
+    x = to_state(X, 12)
+    Q = np.zeros((P, 6, 3, 3))
+    ev = np.zeros((P, 6))
+    ok = np.zeros(P, np.int32)
+    lib.emul_a4eig(12, P, x, Q, ev, ok)
+    assert np.abs(ev - g["a4eig_ev"]).max() < 1e-12
+    gap = np.diff(g["a4eig_ev"], axis=1)
+    for p in range(P):
+        for i in range(6):
+            left = gap[p, i - 1] if i > 0 else 1.0
+            right = gap[p, i] if i < 5 else 1.0
+            if min(left, right) > 1e-8:
+                d = min(np.abs(Q[p, i] - g["a4eig_Q"][p, i]).max(), np.abs(Q[p, i] + g["a4eig_Q"][p, i]).max())
+                assert d < 1e-8 / min(left, right) * 1e-2 + 1e-12, (p, i, d)
+        gram = np.einsum("iab,jab->ij", Q[p], Q[p])
+        assert np.abs(gram - np.eye(6)).max() < 1e-12
+    assert ok.all()   # evolved fabrics are valid states
+    # validity on the reference's grid: n00 = 1, n20, n40 (L = 4 states)
+    K = len(g["n20"])
+    xs = np.zeros((15, K))
+    xs[0] = 1.0
+    xs[1] = g["n20"].real
+    xs[6] = g["n40"].real
+    ok = np.zeros(K, np.int32)
+    lib.emul_a4eig(4, K, xs, np.zeros((K, 6, 3, 3)), np.zeros((K, 6)), ok)
+    assert (ok.astype(bool) == g["isvalid"]).all(), int((ok.astype(bool) != g["isvalid"]).sum())
+    assert 0.2 < ok.mean() < 0.8

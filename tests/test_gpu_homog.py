@@ -65,3 +65,40 @@ def test_Eij_closed_forms(gpu_ctx):
         E = b.Eij(Eij_grain=Eg, alpha=alpha, frame="xyz")
         torch.cuda.synchronize()
         assert np.abs(E.cpu().numpy() - 1).max() < 1e-13
+
+
+def test_taylor_regularized_branch_vs_reference(gpu_ctx):
+    """Invalid L = 4 states (a2 eigenvalues outside [0, 1]) for which the
+    Taylor matrix is not positive definite: the GPU takes the reference's
+    regularized branch (the failed factorization's matrix, P^T P + 1e-6 I,
+    src/homogenizations.f90:174-185), flags it with SFG_ST_TAYLOR_REG on the
+    same states as the oracle, and matches the reference's Eij
+    (tests/golden/homog_invalid_L4.npz) at 1e-10; a non-finite state sets
+    SFG_ST_NAN and SFG_ST_TAYLOR_FAIL (where the reference stops) instead of
+    failing the call."""
+    import torch
+    from oracle.oracle import Oracle
+    from specfab_amd.batch import FabricBatch
+    g = load_golden("homog_invalid_L4.npz")
+    o = Oracle()
+    e = np.eye(3)
+    for tag, alpha in (("taylor", 1.0), ("lin", 0.0125)):
+        b = FabricBatch(4, nlm=g["nlm"])
+        E = b.Eij(Eij_grain=tuple(g["Eij_grain"]), alpha=alpha, frame="xyz")
+        torch.cuda.synchronize()
+        st = b.status.cpu().numpy()
+        assert relerr(E.cpu().numpy(), g["Eij_xyz_" + tag]) < TOL
+        for k, x in enumerate(g["nlm"]):
+            so = o.lib.sfo_Eij_tranisotropic(o._z(x), e[0].copy(), e[1].copy(), e[2].copy(),
+                                             np.asarray(g["Eij_grain"], float), alpha, 1, np.zeros(6))
+            assert (st[k] & 2) == (so & 2), k
+        assert (st & 2).sum() >= 40 and not (st & 4).any()
+    bad = np.zeros((1, 15), complex)
+    bad[0, 0] = 1 / np.sqrt(4 * np.pi)
+    bad[0, 3] = np.nan
+    b = FabricBatch(4, nlm=bad)
+    E = b.Eij(Eij_grain=(1.0, 1e3), alpha=1.0, frame="xyz")
+    torch.cuda.synchronize()
+    st = int(b.status.cpu().numpy()[0])
+    assert st & 1 and st & 2 and st & 4
+    assert np.isnan(E.cpu().numpy()).all()

@@ -98,3 +98,24 @@ def test_argument_errors(gpu_ctx):
         B.nlm_to_rnlm_batch(7, torch.zeros((2, 36), dtype=torch.complex128, device="cuda"))
     assert _lib.lib().sfg_rnlm_len(32) < 0
     assert B.reduce_M_batch(4, torch.zeros((0, 15, 15), dtype=torch.complex128, device="cuda"))[0].shape == (0, 9, 9)
+
+
+def test_M_REG_batch_vs_reference(gpu_ctx):
+    """M_REG(D) of many nodes in one call (the per-node assembly of
+    src/specfabpy/fenics/CPO.py:202) against the reference's M_REG of the
+    ops fixtures (tests/golden/ops_L{4,8,12}.npz) at 1e-14, each node its own
+    D, and reduced to the 2-D form like M_LROT and M_DDRX_src."""
+    import torch
+    from specfab_amd import batch as B
+    for L in (4, 8, 12):
+        g = load_golden("ops_L%d.npz" % L)
+        D = torch.from_numpy(np.concatenate([g["eps"], 2.0 * g["eps"]])).cuda()
+        M = B.M_REG_batch(L, D).cpu().numpy()
+        K = len(g["eps"])
+        for k in range(K):
+            assert relerr(M[k], g["M_REG"][k]) < 1e-14
+            assert relerr(M[K + k], 2.0 * g["M_REG"][k]) < 1e-14
+        red = B.reduce_M_batch(L, torch.from_numpy(M.astype(np.complex128)).cuda())
+        want = O.reduce_M(M.astype(np.complex128), L)
+        for i, name in enumerate(NAMES):
+            assert np.array_equal(red[i].cpu().numpy(), want[i]), name

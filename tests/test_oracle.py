@@ -266,3 +266,25 @@ def test_a4_full_closed_forms(oracle):
         A = oracle.a4(x)
         assert A[2, 1, 0, 1] == A[0, 1, 2, 2]
         assert abs(np.einsum("iikk->", A) - 1) < 1e-7
+
+
+def test_taylor_regularized_branch_matches_reference(oracle):
+    """Invalid L = 4 states whose Taylor matrix is not positive definite: the
+    reference regularizes with the matrix its failed dposv('L') left behind
+    (lower triangle partially factored, src/homogenizations.f90:174-179).
+    The oracle's restatement of that branch matches the reference's Eij
+    (tests/golden/homog_invalid_L4.npz, written by oracle/_ref) at 1e-10 and
+    flags it (status bit 2) on nearly every state; regularizing with the
+    unfactored matrix instead is off by O(1) on the same fixture."""
+    g = load_golden("homog_invalid_L4.npz")
+    e = np.eye(3)
+    nreg = 0
+    for tag, alpha in (("taylor", 1.0), ("lin", 0.0125)):
+        for x, ref in zip(g["nlm"], g["Eij_xyz_" + tag]):
+            out = np.zeros(6)
+            st = oracle.lib.sfo_Eij_tranisotropic(oracle._z(x), e[0].copy(), e[1].copy(), e[2].copy(),
+                                                  np.asarray(g["Eij_grain"], float), alpha, 1, out)
+            assert relerr(out, ref) < 1e-10, (tag, relerr(out, ref))
+            assert not st & 4
+            nreg += bool(st & 2)
+    assert nreg >= 80   # of 2 x 48
