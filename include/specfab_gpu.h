@@ -29,6 +29,7 @@
 #ifndef SPECFAB_GPU_H
 #define SPECFAB_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

@@ -284,37 +284,3 @@ class RefLib:
 
     def simpleshear_gamma_to_t(self, gam, tau):
         return self.lib.sfref_simpleshear_gamma_to_t(float(gam), float(tau))
-
-
-def lagrangianparcel(R, ugrad, dt, Nt, nlm0=None, iota=None, zeta=0, Lambda=None, Gamma0=None, nu=None,
-                     regexpo=None):
-    """The reference's Lagrangian-parcel Euler loop (src/specfabpy/integrator.py:
-    40-82 and M_REG_custom :85-97) restated over the reference's own operators
-    (R: a RefLib initialised at the truncation L), with its dtypes: steady
-    operators complex128, trajectory stored in complex64.  Returns nlm
-    (Nt+1, n) complex64."""
-    ugrad = np.asarray(ugrad, dtype=np.float64)
-    D, W = (ugrad + ugrad.T) / 2, (ugrad - ugrad.T) / 2
-    S = D.copy()
-    n, L = R.n, R.L
-    nlm = np.zeros((Nt + 1, n), dtype=np.complex64)
-    if nlm0 is None:
-        nlm[0, 0] = 1 / np.sqrt(4 * np.pi)
-    else:
-        nlm[0, :] = nlm0
-    M_zero = np.zeros((n, n), dtype=np.complex64)
-    if regexpo is None:
-        M_REG = nu * R.M_REG(D) if nu is not None else M_zero
-    else:
-        ls = np.concatenate([np.full(2 * l + 1, l) for l in range(0, L + 1, 2)])
-        Lm = np.diag(-(ls * (ls + 1)).astype(np.float64)) / (L * (L + 1))
-        M_REG = -(nu * np.linalg.norm(D)) * np.power(np.abs(Lm), regexpo)
-    M_CDRX = Lambda * R.M_CDRX() if Lambda is not None else M_zero
-    M_LROT = R.M_LROT(D, W, iota, zeta) if iota is not None else M_zero
-    Gamma0_ = Gamma0(ugrad=ugrad) if callable(Gamma0) else Gamma0
-    for nt in range(1, Nt + 1):
-        nlm_prev = nlm[nt - 1, :]
-        M = M_LROT + M_CDRX + M_REG
-        M = M + (Gamma0_ * R.M_DDRX(nlm_prev, S) if Gamma0_ is not None else M_zero)
-        nlm[nt, :] = nlm_prev + dt * np.matmul(M, nlm_prev)
-    return nlm

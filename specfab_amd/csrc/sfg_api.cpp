@@ -435,78 +435,138 @@ int variant_of(sfg_ctx* ctx, int L, int prec) {
     return (it != ctx->variant.end() && it->second < nv) ? it->second : 0;
 }
 
-// body of sfg_evolve; ctx->mu held.  force_variant >= 0 selects a generated
-// stepper variant (autotuner).
-int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int force_variant) {
-    const DevOp* op;
-    int rc = get_op(ctx, d->L, &op);
-    if (rc) return rc;
+// How one evolve call steps: the generated stepper variant (or the
+// table-driven one), the stage buffers that variant uses, the parcels per
+// launch, and the workspace: per-parcel launch constants, the used stage
+// buffers (n x slice doubles each) and, for a batch in slices, the slice's
+// state.  The register-resident steppers address SoA rows with 32-bit buffer
+// offsets (and use offsets >= 2^31 as "out of range"), so one launch steps at
+// most (2^31 - 1) / (8 n) parcels; larger batches are stepped slice by slice,
+// each slice's state copied into the workspace and back.
+struct StepPlan {
+    const DevOp* op = nullptr;
+    bool fast = false;
+    int prec = 0, variant = 0, mask = 0;
+    int64_t slice = 0;
+    bool sliced = false;
+    size_t ws_doubles = 0;
+};
+
+// resident_only: fail instead of uploading the tables of L (no allocation)
+int plan_step(sfg_ctx* ctx, const sfg_evolve_desc* d, int force_variant, bool resident_only, StepPlan& sp) {
+    if (resident_only) {
+        auto it = ctx->ops.find(d->L);
+        if (it == ctx->ops.end())
+            return fail(SFG_ERR_ARG, "the tables of this L are not on the device yet: make one allocating call at "
+                                     "this L first (e.g. sfg_evolve with P = 0)");
+        sp.op = &it->second;
+    } else {
+        int rc = get_op(ctx, d->L, &sp.op);
+        if (rc) return rc;
+    }
+    const DevOp* op = sp.op;
     if ((d->procs & SFG_REG) && !op->reg_defined && !(d->reg_expo > 0))
         return fail(SFG_ERR_L, "M_REG is calibrated only for L = 4, 6, ..., 20 (src/include/regcalib.f90)");
-    if (d->P == 0) return SFG_OK;
-    HIPCHK(hipSetDevice(ctx->device));
     const int64_t P = d->P;
     const int n = op->n;
-    size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
-    size_t nscr = (size_t)3 * n * P;
-    void* ws;
-    WsLease lease(ctx, s);
-    rc = lease.acquire((nth + nom + nsl + nscr) * sizeof(double), &ws);
-    if (rc) return rc;
-    double* theta = (double*)ws;
-    double* omega = theta + nth;
-    double* sl = omega + nom;
-    double* scr = sl + nsl;
+    sp.prec = d->precision == SFG_FP64 ? 0 : 1;
+    sp.fast = sfg::fast_available(d->L, sp.prec) && (sp.prec == 1 || ctx->stepper == 0);
+    if (!sp.fast && sp.prec == 1) return fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L");
+    const int64_t maxp = (((int64_t(1) << 31) - 1) / (8 * (int64_t)n)) / 256 * 256;
+    sp.slice = (sp.fast && P > maxp) ? maxp : std::max<int64_t>(P, 1);
+    sp.sliced = sp.slice < P;
+    if (sp.fast) {
+        sp.variant = force_variant >= 0 ? force_variant : variant_of(ctx, d->L, sp.prec);
+        sp.mask = sfg::fast_buffers(d->L, sp.prec, sp.variant);
+    } else {
+        sp.mask = sfg::BUF_YS | sfg::BUF_XA | sfg::BUF_XB;
+    }
+    const int planes = __builtin_popcount(sp.mask) + (sp.sliced ? 1 : 0);
+    sp.ws_doubles = (size_t)(sfg::NPAR_K + 15 + op->nblk) * sp.slice + (size_t)planes * n * sp.slice;
+    return SFG_OK;
+}
 
-    sfg::ParamArgs pa{};
-    pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
-    pa.iota = d->iota; pa.zeta = d->zeta;
-    reg_law(op, d, pa);
-    pa.ugrad = d->ugrad_dev; pa.tau = d->tau_dev; pa.gam0 = d->gam0_dev; pa.lam = d->lam_dev;
-    pa.A2 = op->A2; pa.A4 = op->A4;
-    pa.theta = theta; pa.omega = omega; pa.sl = sl;
-    HIPCHK(sfg::launch_params(pa, s));
-
-    sfg::EvolveArgs ea{};
-    ea.P = P; ea.n = n; ea.nq = op->nq; ea.scheme = d->scheme; ea.nsteps = d->nsteps;
-    ea.use_ddrx = (d->procs & SFG_DDRX) ? 1 : 0;
-    ea.steps = d->steps_dev; ea.dt = d->dt;
-    ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
-    ea.tcol = op->tcol; ea.tval = op->tval; ea.tval_f = op->tval_f;
-    ea.theta = theta; ea.omega = omega; ea.sl = sl;
-    ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
-    ea.status = d->status_dev;
-    ea.zero = 0;
-    // the register-resident steppers address SoA rows with 32-bit buffer
-    // offsets (and use offsets >= 2^31 as "out of range"): n * P * 8 < 2^31
-    const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
-    const int prec = d->precision == SFG_FP64 ? 0 : 1;
-    const bool fast = fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0);
-    if (!fast && prec == 1)
-        return fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L and n * P * 8 < 2^31 (split the batch)");
+// enqueue the stepping of d on s with workspace ws (sp.ws_doubles doubles)
+int run_step(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, const StepPlan& sp, double* ws, bool timing) {
+    const DevOp* op = sp.op;
+    const int64_t P = d->P;
+    const int n = op->n;
     std::pair<hipEvent_t, hipEvent_t> tv{nullptr, nullptr};
-    if (ctx->timing) {
+    if (timing) {
         HIPCHK(hipEventCreate(&tv.first));
         HIPCHK(hipEventCreate(&tv.second));
+        ctx->tev.push_back(tv);
         HIPCHK(hipEventRecord(tv.first, s));
     }
-    if (fast) {
-        const int v = force_variant >= 0 ? force_variant : variant_of(ctx, d->L, prec);
-        HIPCHK(sfg::launch_evolve_fast(d->L, prec, v, ea, s));
-    } else {
-        HIPCHK(sfg::launch_evolve_generic(ea, s));
+    for (int64_t p0 = 0; p0 < P; p0 += sp.slice) {
+        const int64_t Ps = std::min(sp.slice, P - p0);
+        double* theta = ws;
+        double* omega = theta + (size_t)sfg::NPAR_K * Ps;
+        double* sl = omega + (size_t)15 * Ps;
+        double* plane = sl + (size_t)op->nblk * Ps;
+        double* bufs[3] = {nullptr, nullptr, nullptr};
+        for (int b = 0; b < 3; b++)
+            if (sp.mask & (1 << b)) {
+                bufs[b] = plane;
+                plane += (size_t)n * Ps;
+            }
+        double* X = d->x_dev;
+        if (sp.sliced) {
+            X = plane;
+            HIPCHK(hipMemcpy2DAsync(X, (size_t)Ps * 8, d->x_dev + p0, (size_t)P * 8, (size_t)Ps * 8, n,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+        sfg::ParamArgs pa{};
+        pa.P = Ps; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
+        pa.iota = d->iota; pa.zeta = d->zeta;
+        reg_law(op, d, pa);
+        pa.ugrad = d->ugrad_dev + 9 * p0;
+        pa.tau = d->tau_dev ? d->tau_dev + 9 * p0 : nullptr;
+        pa.gam0 = d->gam0_dev ? d->gam0_dev + p0 : nullptr;
+        pa.lam = d->lam_dev ? d->lam_dev + p0 : nullptr;
+        pa.A2 = op->A2; pa.A4 = op->A4;
+        pa.theta = theta; pa.omega = omega; pa.sl = sl;
+        HIPCHK(sfg::launch_params(pa, s));
+        sfg::EvolveArgs ea{};
+        ea.P = Ps; ea.n = n; ea.nq = op->nq; ea.scheme = d->scheme; ea.nsteps = d->nsteps;
+        ea.use_ddrx = (d->procs & SFG_DDRX) ? 1 : 0;
+        ea.steps = d->steps_dev ? d->steps_dev + p0 : nullptr;
+        ea.dt = d->dt;
+        ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
+        ea.tcol = op->tcol; ea.tval = op->tval; ea.tval_f = op->tval_f;
+        ea.theta = theta; ea.omega = omega; ea.sl = sl;
+        ea.X = X; ea.YS = bufs[0]; ea.XA = bufs[1]; ea.XB = bufs[2];
+        ea.status = d->status_dev ? d->status_dev + p0 : nullptr;
+        ea.zero = 0;
+        if (sp.fast) HIPCHK(sfg::launch_evolve_fast(d->L, sp.prec, sp.variant, ea, s));
+        else HIPCHK(sfg::launch_evolve_generic(ea, s));
+        if (sp.sliced)
+            HIPCHK(hipMemcpy2DAsync(d->x_dev + p0, (size_t)P * 8, X, (size_t)Ps * 8, (size_t)Ps * 8, n,
+                                    hipMemcpyDeviceToDevice, s));
     }
-    if (ctx->timing) {
-        HIPCHK(hipEventRecord(tv.second, s));
-        ctx->tev.push_back(tv);
-    }
-    rc = lease.release();
-    if (rc) return rc;
+    if (timing) HIPCHK(hipEventRecord(tv.second, s));
     if (ctx->sync_check) {
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(hipGetLastError());
     }
     return SFG_OK;
+}
+
+// body of sfg_evolve; ctx->mu held.  force_variant >= 0 selects a generated
+// stepper variant (autotuner).
+int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int force_variant) {
+    StepPlan sp;
+    int rc = plan_step(ctx, d, force_variant, false, sp);
+    if (rc) return rc;
+    if (d->P == 0) return SFG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    void* ws;
+    WsLease lease(ctx, s);
+    rc = lease.acquire(sp.ws_doubles * sizeof(double), &ws);
+    if (rc) return rc;
+    rc = run_step(ctx, d, s, sp, (double*)ws, ctx->timing);
+    if (rc) return rc;
+    return lease.release();
 }
 
 }  // namespace
@@ -984,8 +1044,14 @@ static int eij_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev,
     a.Ecc = Eg[0]; a.Eca = Eg[1]; a.alpha = alpha;
     a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4; a.frames = frames_dev;
     a.Eij = Eij_dev; a.lami = lami_dev; a.ei = ei_dev; a.status = status_dev;
+    // one flag per wavefront of 64 rows for the regularized second pass
+    void* ws;
+    WsLease lease(ctx, s);
+    rc = lease.acquire((size_t)((P + 63) / 64) * sizeof(int), &ws);
+    if (rc) return rc;
+    a.wave_flag = (int*)ws;
     HIPCHK(sfg::launch_eij(a, s));
-    return SFG_OK;
+    return lease.release();
 }
 
 int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame,
@@ -1737,8 +1803,27 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         sfg_plan::Bucket bk;
         const int64_t P = d->P;
         const int n = op->n;
+        // the stepper of the bucket first: its stage buffers size the workspace
+        const int prec = d->precision == SFG_FP64 ? 0 : 1;
+        const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
+        int per_block = 64;   // parcels stepped by one workgroup
+        int mask = sfg::BUF_YS | sfg::BUF_XA | sfg::BUF_XB;
+        if (fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0)) {
+            int blk = 64;
+            const int v = variant_of(ctx, d->L, prec);
+            bk.fn = sfg::fast_kernel(d->L, prec, v, &blk, &per_block);
+            bk.block = (unsigned)blk;
+            mask = sfg::fast_buffers(d->L, prec, v);
+        } else if (prec == 0) {
+            bk.fn = sfg::generic_kernel();
+            bk.block = sfg::GENERIC_BLOCK;
+            per_block = sfg::GENERIC_BLOCK;
+        } else {
+            return bail(fail(SFG_ERR_ARG, "FP32C stepping in a plan needs a generated stepper for L and n * P * 8 < 2^31"));
+        }
+        bk.grid = (unsigned)((P + per_block - 1) / per_block);
         const size_t nth = (size_t)sfg::NPAR_K * P, nom = (size_t)15 * P, nsl = (size_t)op->nblk * P;
-        const size_t nscr = (size_t)3 * n * P;
+        const size_t nscr = (size_t)__builtin_popcount(mask) * n * P;
         hipError_t e = hipMalloc(&bk.ws, (nth + nom + nsl + nscr) * sizeof(double));
         if (e != hipSuccess) return bail(fail(SFG_ERR_HIP, hipGetErrorString(e)));
         plan->b.push_back(bk);
@@ -1746,7 +1831,13 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         double* theta = (double*)B.ws;
         double* omega = theta + nth;
         double* sl = omega + nom;
-        double* scr = sl + nsl;
+        double* plane = sl + nsl;
+        double* bufs[3] = {nullptr, nullptr, nullptr};
+        for (int k = 0; k < 3; k++)
+            if (mask & (1 << k)) {
+                bufs[k] = plane;
+                plane += (size_t)n * P;
+            }
         sfg::ParamArgs pa{};
         pa.P = P; pa.nblk = op->nblk; pa.nq = op->nq; pa.procs = d->procs;
         pa.iota = d->iota; pa.zeta = d->zeta;
@@ -1763,24 +1854,9 @@ int sfg_plan_create(sfg_ctx* ctx, int nbuckets, const sfg_evolve_desc* buckets, 
         ea.rows = op->rows; ea.row_blk = op->row_blk; ea.grp_param = op->grp_param; ea.grp_rng = op->grp_rng;
         ea.tcol = op->tcol; ea.tval = op->tval; ea.tval_f = op->tval_f;
         ea.theta = theta; ea.omega = omega; ea.sl = sl;
-        ea.X = d->x_dev; ea.YS = scr; ea.XA = scr + (size_t)n * P; ea.XB = scr + (size_t)2 * n * P;
+        ea.X = d->x_dev; ea.YS = bufs[0]; ea.XA = bufs[1]; ea.XB = bufs[2];
         ea.status = d->status_dev;
         ea.zero = 0;
-        const int prec = d->precision == SFG_FP64 ? 0 : 1;
-        const bool fits = (int64_t)n * P * 8 < (int64_t(1) << 31);
-        int per_block = 64;   // parcels stepped by one workgroup
-        if (fits && sfg::fast_available(d->L, prec) && (prec == 1 || ctx->stepper == 0)) {
-            int blk = 64;
-            B.fn = sfg::fast_kernel(d->L, prec, variant_of(ctx, d->L, prec), &blk, &per_block);
-            B.block = (unsigned)blk;
-        } else if (prec == 0) {
-            B.fn = sfg::generic_kernel();
-            B.block = sfg::GENERIC_BLOCK;
-            per_block = sfg::GENERIC_BLOCK;
-        } else {
-            return bail(fail(SFG_ERR_ARG, "FP32C stepping needs a generated stepper for L and n * P * 8 < 2^31"));
-        }
-        B.grid = (unsigned)((P + per_block - 1) / per_block);
     }
     // the time loop as one graph: chunk c of every bucket depends on chunk
     // c - 1 of the same bucket only, so the buckets run concurrently.  The

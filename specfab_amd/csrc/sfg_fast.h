@@ -14,6 +14,10 @@ hipError_t launch_evolve_fast(int L, int prec, int variant, const EvolveArgs& a,
 // workgroup and the parcels one workgroup steps (fewer than the threads when
 // several waves share a parcel group): grid = ceil(P / parcels)
 const void* fast_kernel(int L, int prec, int variant, int* block, int* parcels);
+// stage buffers of EvolveArgs that a variant uses (the others may be null):
+// the workspace holds only those, each n x P doubles
+enum { BUF_YS = 1, BUF_XA = 2, BUF_XB = 4 };
+int fast_buffers(int L, int prec, int variant);
 // the table-driven stepper's __global__ function and its block size
 const void* generic_kernel();
 constexpr int GENERIC_BLOCK = 64;

@@ -26,6 +26,14 @@ struct EijArgs {
     const int* idx;
     int64_t nidx;
     const int* nidx_dev;
+    // batch form (idx null): one flag per wavefront of 64 items, written by
+    // the first pass (1: one of its rows needs the regularized second pass),
+    // or null (the second pass then scans the rows)
+    int* wave_flag;
+    // filled by launch_eij from (Ecc, Eca, n_grain): rheo_params_tranisotropic
+    // (src/rheologies.f90:123-132) of the Sachs (ef = +1) and Taylor (ef = -1)
+    // branches and the isotropic normalisations (src/homogenizations.f90:207, 235)
+    double sA, sB, sC, tA, tB, tC, sachs_iso, taylor_iso;
 };
 
 hipError_t launch_eij(const EijArgs& a, hipStream_t s);

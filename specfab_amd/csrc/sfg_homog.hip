@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
+#include "generated/sfg_moment_maps.h"
 #include "sfg_device.h"
 #include "sfg_homog.h"
 
@@ -103,17 +105,34 @@ __device__ void d_eig3(const double* M, double* ei, double* lami) {
             }
         }
     }
+    // descending order (the order of the swaps of a three-element sort);
+    // selects instead of indexed reads keep everything in registers
     double d[3] = {a[0][0], a[1][1], a[2][2]};
-    int i0 = 0, i1 = 1, i2 = 2, t;
-    if (d[i1] > d[i0]) { t = i0; i0 = i1; i1 = t; }
-    if (d[i2] > d[i0]) { t = i0; i0 = i2; i2 = t; }
-    if (d[i2] > d[i1]) { t = i1; i1 = i2; i2 = t; }
-    const int idx[3] = {i0, i1, i2};
+    double w[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[i][k] = v[k][i];   // w[i] = eigenvector of d[i]
+    auto swap_if = [&](int i, int j) {
+        const bool sw = d[j] > d[i];
+        const double di = d[i];
+        d[i] = sw ? d[j] : di;
+        d[j] = sw ? di : d[j];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double wi = w[i][k];
+            w[i][k] = sw ? w[j][k] : wi;
+            w[j][k] = sw ? wi : w[j][k];
+        }
+    };
+    swap_if(0, 1);
+    swap_if(0, 2);
+    swap_if(1, 2);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        lami[i] = d[idx[i]];
+        lami[i] = d[i];
 #pragma unroll
-        for (int k = 0; k < 3; k++) ei[3 * i + k] = v[k][idx[i]];
+        for (int k = 0; k < 3; k++) ei[3 * i + k] = w[i][k];
     }
 }
 
@@ -179,8 +198,8 @@ __device__ inline void d_chol6_solve(const double* Lf, double* b) {
 }
 
 // Taylor matrix P = I6 - A 1 (x) a2v + B a4v + C L(a2) (src/homogenizations.f90:164-172)
-__device__ inline void d_taylorP(const double* a2m, const double* a2v, const double* a4v, double tA, double tB,
-                                 double tC, double Pm[6][6]) {
+__device__ inline void d_taylorP(const double* a2m, const double* a2v, const double (&a4v)[6][6], double tA,
+                                 double tB, double tC, double Pm[6][6]) {
     const double s = sqrt(2.0);
     const double* a = a2m;
     const double Lm[6][6] = {
@@ -190,154 +209,365 @@ __device__ inline void d_taylorP(const double* a2m, const double* a2v, const dou
         {0, s * a[5], s * a[5], a[4] + a[8], a[1], a[2]},
         {s * a[2], 0, s * a[2], a[1], a[0] + a[8], a[5]},
         {s * a[1], s * a[1], 0, a[2], a[5], a[0] + a[4]}};
+#pragma unroll
     for (int i = 0; i < 6; i++)
+#pragma unroll
         for (int j = 0; j < 6; j++)
-            Pm[i][j] = (i == j ? 1.0 : 0.0) - tA * ((i < 3 ? 1.0 : 0.0) * a2v[j]) + tB * a4v[6 * i + j] + tC * Lm[i][j];
+            Pm[i][j] = (i == j ? 1.0 : 0.0) - tA * ((i < 3 ? 1.0 : 0.0) * a2v[j]) + tB * a4v[i][j] + tC * Lm[i][j];
 }
 
-// Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69.
-// P is factored once per parcel; the rare regularized branch rebuilds the
-// failed factorization where it needs it (the same arithmetic gives the same
-// matrix), so it does not occupy registers through the loop over the six
-// stress states.
-__device__ int d_Eij(const double* a2v, const double* a4v, const double* e, double Ecc, double Eca, double alpha,
-                     int ng, double* Eij) {
-    int status = 0;
-    double sA, sB, sC, tA, tB, tC;
-    d_rheo_params(Ecc, Eca, (double)ng, 1, &sA, &sB, &sC);
-    d_rheo_params(Ecc, Eca, (double)ng, -1, &tA, &tB, &tC);
-    const double sachs_iso = 1 + 2.0 / 15 * sB + 2.0 / 3 * sC;            // src/homogenizations.f90:207
-    const double taylor_iso = 1.0 / (1 + 2.0 / 15 * tB + 2.0 / 3 * tC);   // :235
-    double a2m[9];
-    d_vec_to_mat(a2v, a2m);
-    // the Cholesky factor of P, or of the regularized normal equations
-    // (P^T P + 1e-6 I, the REAL(4) 1e-6 of :178) when P is not positive definite
-    double Lf[21];
-    bool ok, ok_reg = true;
-    {
-        double Pm[6][6];
-        d_taylorP(a2m, a2v, a4v, tA, tB, tC, Pm);
-        ok = d_chol6(Pm, Lf);
-        if (!ok) {
-            status |= 2;
-            d_failed_factor6(Pm);
-            double R[6][6];
-            for (int i = 0; i < 6; i++)
-                for (int j = 0; j < 6; j++) {
-                    double s = 0;
-                    for (int k = 0; k < 6; k++) s += Pm[k][i] * Pm[k][j];
-                    R[i][j] = s + (i == j ? (double)1e-6f : 0.0);
-                }
-            ok_reg = d_chol6(R, Lf);
-            if (!ok_reg) status |= 4;
+// L L^T x = b with the packed factor of d_chol6 and the reciprocals of its
+// diagonal (the forward and back substitutions multiply by them)
+__device__ inline void d_chol6_solve_rcp(const double* Lf, const double* rd, double* b) {
+#define LF(i, j) Lf[(i) * ((i) + 1) / 2 + (j)]
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) s -= LF(i, k) * b[k];
+        b[i] = s * rd[i];
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; k++) s -= LF(k, i) * b[k];
+        b[i] = s * rd[i];
+    }
+#undef LF
+}
+
+// a2 (Mandel, with the isotropic part) and a4 (Mandel 6x6) of parcel p from the
+// l <= 4 rows of the state (src/moments.f90:164-178)
+__device__ __forceinline__ void load_moments(const EijArgs& a, int64_t p, double (&a2v)[6], double (&a4v)[6][6]) {
+    double xq[15];
+#pragma unroll
+    for (int q = 0; q < 15; q++) xq[q] = a.x[(int64_t)q * a.P + p];
+    moment_maps(xq, a2v, a4v);
+    const double ix0 = 1.0 / xq[0];
+#pragma unroll
+    for (int i = 0; i < 6; i++) a2v[i] = a2v[i] * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) a4v[i][j] *= ix0;
+}
+
+// Mandel vectors of one stress state q of the frame e (src/enhancementfactors.f90:
+// 398-413): tv = mandel(tau) with tau = I/3 - v v (q < 3) or v w + w v, and
+// uv = mandel(sym(v w)); returns v . w.  (v, w) = (e_q, e_q) for q < 3, else
+// (e2, e3), (e1, e3), (e1, e2), picked by selects rather than indexed
+// register reads.
+__device__ __forceinline__ double stress_state(const double (&e)[9], int q, double (&tv)[6], double (&uv)[6]) {
+    const double r2 = sqrt(2.0);
+    const int iv = q < 3 ? q : (q == 3 ? 1 : 0), iw = q < 3 ? q : (q == 5 ? 1 : 2);
+    double v[3], w[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        v[i] = iv == 0 ? e[i] : (iv == 1 ? e[3 + i] : e[6 + i]);
+        w[i] = iw == 0 ? e[i] : (iw == 1 ? e[3 + i] : e[6 + i]);
+    }
+    const bool nrm = q < 3;
+    tv[0] = nrm ? 1.0 / 3.0 - v[0] * v[0] : 2 * v[0] * w[0];
+    tv[1] = nrm ? 1.0 / 3.0 - v[1] * v[1] : 2 * v[1] * w[1];
+    tv[2] = nrm ? 1.0 / 3.0 - v[2] * v[2] : 2 * v[2] * w[2];
+    tv[3] = r2 * (nrm ? -v[1] * v[2] : v[1] * w[2] + w[1] * v[2]);
+    tv[4] = r2 * (nrm ? -v[0] * v[2] : v[0] * w[2] + w[0] * v[2]);
+    tv[5] = r2 * (nrm ? -v[0] * v[1] : v[0] * w[1] + w[0] * v[1]);
+    uv[0] = v[0] * w[0];
+    uv[1] = v[1] * w[1];
+    uv[2] = v[2] * w[2];
+    uv[3] = r2 * (0.5 * (v[1] * w[2] + w[1] * v[2]));
+    uv[4] = r2 * (0.5 * (v[0] * w[2] + w[0] * v[2]));
+    uv[5] = r2 * (0.5 * (v[0] * w[1] + w[0] * v[1]));
+    return v[0] * w[0] + v[1] * w[1] + v[2] * w[2];
+}
+
+// lower triangles (packed, (i, j) at i (i + 1) / 2 + j, j <= i) of the Taylor
+// matrix P = I - tA 1 (x) a2v + tB a4v + tC L(a2) (src/homogenizations.f90:
+// 164-172; dposv('L') reads only this triangle) and of the symmetric Sachs
+// matrix Q = sB a4v + sC L(a2), with L the Mandel anticommutator
+// (src/homogenizations.f90:238-258)
+__device__ __forceinline__ void taylor_sachs_lower(const EijArgs& c, const double (&a2v)[6],
+                                                   const double (&a4v)[6][6], double (&Pl)[21], double (&Ql)[21]) {
+    const double s = sqrt(2.0), r2i = 0.70710678118654752440;
+    const double m[9] = {a2v[0], a2v[5] * r2i, a2v[4] * r2i, a2v[5] * r2i, a2v[1], a2v[3] * r2i,
+                         a2v[4] * r2i, a2v[3] * r2i, a2v[2]};
+    const double Lm[6][6] = {
+        {2 * m[0], 0, 0, 0, s * m[2], s * m[1]},
+        {0, 2 * m[4], 0, s * m[5], 0, s * m[1]},
+        {0, 0, 2 * m[8], s * m[5], s * m[2], 0},
+        {0, s * m[5], s * m[5], m[4] + m[8], m[1], m[2]},
+        {s * m[2], 0, s * m[2], m[1], m[0] + m[8], m[5]},
+        {s * m[1], s * m[1], 0, m[2], m[5], m[0] + m[4]}};
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) {
+            Pl[i * (i + 1) / 2 + j] =
+                (i == j ? 1.0 : 0.0) - c.tA * ((i < 3 ? 1.0 : 0.0) * a2v[j]) + c.tB * a4v[i][j] + c.tC * Lm[i][j];
+            Ql[i * (i + 1) / 2 + j] = c.sB * a4v[i][j] + c.sC * Lm[i][j];
+        }
+}
+
+// in-place Cholesky of a packed lower triangle, the arithmetic of d_chol6
+__device__ inline bool d_chol6_packed(double (&F)[21]) {
+#define PF(i, j) F[(i) * ((i) + 1) / 2 + (j)]
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double d = PF(j, j);
+#pragma unroll
+        for (int k = 0; k < j; k++) d -= PF(j, k) * PF(j, k);
+        if (!(d > 0)) return false;
+        d = sqrt(d);
+        PF(j, j) = d;
+        const double r = 1.0 / d;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double t = PF(i, j);
+#pragma unroll
+            for (int k = 0; k < j; k++) t -= PF(i, k) * PF(j, k);
+            PF(i, j) = t * r;
         }
     }
-    const int pv[6] = {0, 1, 2, 1, 0, 0}, pw[6] = {0, 1, 2, 2, 2, 1};
+    return true;
+#undef PF
+}
+
+// the Sachs ratio es : vw / (siso tau : vw) of one stress state
+__device__ __forceinline__ double sachs_ratio(const EijArgs& c, const double (&a2v)[6], const double (&Ql)[21],
+                                              const double (&tv)[6], const double (&uv)[6], double vdw,
+                                              double* tvw_out) {
+    double tvw = 0, I2 = 0, a2t = 0, qt = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        tvw += tv[i] * uv[i];
+        I2 += tv[i] * tv[i];
+        a2t += a2v[i] * tv[i];
+        double r = 0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const int hi = i > j ? i : j, lo = i > j ? j : i;
+            r += Ql[hi * (hi + 1) / 2 + lo] * tv[j];
+        }
+        qt += uv[i] * r;
+    }
+    const bool lin = c.n_grain == 1;
+    const double esvw = (lin ? 1.0 : I2) * (tvw - c.sA * a2t * vdw + qt);
+    *tvw_out = tvw;
+    return esvw / ((lin ? c.sachs_iso : I2) * tvw);
+}
+
+// Eij of a parcel whose Taylor matrix P is not positive definite: the
+// reference's regularized branch (src/homogenizations.f90:174-185), solving
+// (F^T F + 1e-6 I) x = F^T tau with F the matrix its failed dposv('L') left
+// (the REAL(4) 1e-6), for every stress state.  Rare: run by a second pass
+// (eij_regularized_kernel) over the rows the first one marked, so that its
+// registers do not weigh on the common path.
+__device__ __noinline__ int d_Eij_regularized(const EijArgs& c, int64_t p, const double (&e)[9], double* Eij) {
+    int status = 2;
+    double a2v[6], a4v[6][6];
+    load_moments(c, p, a2v, a4v);
+    const double r2i = 0.70710678118654752440;
+    const double a2m[9] = {a2v[0], a2v[5] * r2i, a2v[4] * r2i, a2v[5] * r2i, a2v[1], a2v[3] * r2i,
+                           a2v[4] * r2i, a2v[3] * r2i, a2v[2]};
+    double Pl[21], Ql[21], F[6][6], R[6][6], Lf[21];
+    taylor_sachs_lower(c, a2v, a4v, Pl, Ql);
+    d_taylorP(a2m, a2v, a4v, c.tA, c.tB, c.tC, F);
+    d_failed_factor6(F);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int k = 0; k < 6; k++) t += F[k][i] * F[k][j];
+            R[i][j] = t + (i == j ? (double)1e-6f : 0.0);
+        }
+    const bool ok_reg = d_chol6(R, Lf);
+    if (!ok_reg) status |= 4;
     for (int q = 0; q < 6; q++) {
-        const double* v = e + 3 * pv[q];
-        const double* w = e + 3 * pw[q];
-        double tau[9], vw[9];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                vw[3 * i + j] = v[i] * w[j];
-                tau[3 * i + j] = (q < 3) ? ((i == j) / 3.0 - v[i] * v[j]) : (v[i] * w[j] + w[i] * v[j]);
-            }
-        const double I2 = d_dinner22(tau, tau);
-        // Sachs (src/homogenizations.f90:69-116), n' = 1 or -3
-        double tv[6], a4t[6], a4tau[9], et2[9], es[9];
-        d_mat_to_vec(tau, tv);
+        double tv[6], uv[6], tvw;
+        const double vdw = stress_state(e, q, tv, uv);
+        const double ts = sachs_ratio(c, a2v, Ql, tv, uv, vdw, &tvw);
+        double x[6];
         for (int i = 0; i < 6; i++) {
-            double s = 0;
-            for (int j = 0; j < 6; j++) s += a4v[6 * i + j] * tv[j];
-            a4t[i] = s;
+            x[i] = 0;
+            for (int k = 0; k < 6; k++) x[i] += F[k][i] * tv[k];
         }
-        d_vec_to_mat(a4t, a4tau);
-        const double e0 = (ng == 1) ? 1.0 : I2;
-        for (int i = 0; i < 9; i++) {
-            et2[i] = (ng == 1) ? a2m[i] : I2 * a2m[i];
-            if (ng != 1) a4tau[i] *= I2;
-        }
-        const double tr = d_dinner22(et2, tau);
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                double ac = 0;
-                for (int k = 0; k < 3; k++) ac += tau[3 * i + k] * et2[3 * k + j] + et2[3 * i + k] * tau[3 * k + j];
-                es[3 * i + j] = e0 * tau[3 * i + j] - sA * tr * (i == j) + sB * a4tau[3 * i + j] + sC * ac;
-            }
-        // Taylor (src/homogenizations.f90:145-189)
-        double xs[6], et[9];
-        d_mat_to_vec(tau, xs);
-        if (ok) {
-            d_chol6_solve(Lf, xs);
-        } else {
-            double Pm[6][6], rhs[6];
-            d_taylorP(a2m, a2v, a4v, tA, tB, tC, Pm);
-            d_failed_factor6(Pm);
-            for (int i = 0; i < 6; i++) {
-                rhs[i] = 0;
-                for (int k = 0; k < 6; k++) rhs[i] += Pm[k][i] * xs[k];
-            }
-            if (ok_reg) d_chol6_solve(Lf, rhs);
-            for (int i = 0; i < 6; i++) xs[i] = rhs[i];
-        }
-        d_vec_to_mat(xs, et);
-        const double siso = (ng == 1) ? sachs_iso : I2;
-        const double tvw = d_dinner22(tau, vw);
-        const double ts = d_dinner22(es, vw) / (siso * tvw);
-        const double tt = d_dinner22(et, vw) / (taylor_iso * tvw);
-        Eij[q] = (1 - alpha) * ts + alpha * tt;
-        if (!isfinite(Eij[q])) status |= 1;
+        if (ok_reg) d_chol6_solve(Lf, x);
+        double etvw = 0;
+        for (int i = 0; i < 6; i++) etvw += x[i] * uv[i];
+        const double tt = etvw / (c.taylor_iso * tvw);
+        const double E = (1 - c.alpha) * ts + c.alpha * tt;
+        Eij[q] = E;
+        if (!isfinite(E)) status |= 1;
     }
     return status;
 }
 
-__device__ __forceinline__ void eij_item(const EijArgs& a, int64_t t) {
-    const int64_t p = a.idx ? (int64_t)a.idx[t] : t;   // state column
-    const int64_t o = t;                               // output row
-    double a2v[6], a4v[36];
-    d_moments(a.x, a.P, p, a.nq, a.A2, a.A4, a2v, a4v);
-    double e[9], lam[3] = {0, 0, 0};
-    if (a.frame == 0) {
-        double a2m[9];
-        d_vec_to_mat(a2v, a2m);
-        d_eig3(a2m, e, lam);
-    } else if (a.frame == 1) {
-        for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
-    } else {
-        for (int i = 0; i < 9; i++) e[i] = (i % 4 == 0) ? 1.0 : 0.0;
+// Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69,
+// with every contraction written in Mandel 6-vectors.  For a stress state tau
+// (tv = mandel(tau)) and the projector vw = v (x) w (uv = mandel(sym(vw))):
+//   tau : vw = tv . uv,   I : vw = v . w,   (a4 : tau) : vw = uv . a4v tv,
+//   (tau a2 + a2 tau) : vw = uv . L(a2) tv,
+// so the Sachs term es : vw = f [tv.uv - sA (a2v.tv)(v.w) + uv.Q tv] with
+// Q = sB a4v + sC L(a2) formed once per parcel and f = 1 (n' = 1) or I2 =
+// tau : tau (n' = -3) (src/homogenizations.f90:69-116), and the Taylor term
+// et : vw = (P^-1 tv) . uv (:145-189).  Pl: the packed lower triangle of P,
+// factored here in place.
+// marks a row whose Taylor matrix is not positive definite for the second
+// pass (a NaN payload that arithmetic does not produce)
+constexpr unsigned long long kRegularizeMark = 0x7ff8f00dfeedc0deull;
+
+__device__ int d_Eij(const EijArgs& c, int64_t p, const double (&a2v)[6], double (&Pl)[21], const double (&Ql)[21],
+                     const double (&e)[9], double* Eij) {
+    if (!d_chol6_packed(Pl)) {
+        Eij[0] = __longlong_as_double((long long)kRegularizeMark);
+        return 2;
     }
-    double E[6];
-    int st = d_Eij(a2v, a4v, e, a.Ecc, a.Eca, a.alpha, a.n_grain, E);
-    for (int i = 0; i < 6; i++) a.Eij[o * 6 + i] = E[i];
-    if (a.lami) for (int i = 0; i < 3; i++) a.lami[o * 3 + i] = lam[i];
-    if (a.ei) for (int i = 0; i < 9; i++) a.ei[o * 9 + i] = e[i];
-    if (a.status && st) atomicOr(a.status + p, st);
+    double rd[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) rd[i] = 1.0 / Pl[i * (i + 1) / 2 + i];
+    int status = 0;
+#pragma unroll 1
+    for (int q = 0; q < 6; q++) {
+        double tv[6], uv[6], tvw;
+        const double vdw = stress_state(e, q, tv, uv);
+        const double ts = sachs_ratio(c, a2v, Ql, tv, uv, vdw, &tvw);
+        double xs[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) xs[i] = tv[i];
+        d_chol6_solve_rcp(Pl, rd, xs);
+        double etvw = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) etvw += xs[i] * uv[i];
+        const double tt = etvw / (c.taylor_iso * tvw);
+        const double E = (1 - c.alpha) * ts + c.alpha * tt;
+        Eij[q] = E;
+        if (!isfinite(E)) status |= 1;
+    }
+    return status;
 }
 
-// one wave per SIMD: the Taylor 6x6 system, a4 and the frame need ~380
-// registers per lane, which then stay in VGPRs/AGPRs instead of scratch
-__global__ void __launch_bounds__(128, 1) eij_kernel(EijArgs a) {
+// the frame (e1, e2, e3) of parcel p: its a2 eigenframe (eigenvalues to lam),
+// the caller's frame or the Cartesian axes
+__device__ __forceinline__ void eij_frame(const EijArgs& a, int64_t p, const double (&a2v)[6], double (&e)[9],
+                                          double (&lam)[3]) {
+    lam[0] = lam[1] = lam[2] = 0.0;
+    if (a.frame == 0) {
+        const double r2i = 0.70710678118654752440;
+        const double a2m[9] = {a2v[0], a2v[5] * r2i, a2v[4] * r2i, a2v[5] * r2i, a2v[1], a2v[3] * r2i,
+                               a2v[4] * r2i, a2v[3] * r2i, a2v[2]};
+        d_eig3(a2m, e, lam);
+    } else if (a.frame == 1) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    }
+}
+
+// returns the row's status bits (2: marked for the regularized second pass)
+__device__ __forceinline__ int eij_item(const EijArgs& a, int64_t t) {
+    const int64_t p = a.idx ? (int64_t)a.idx[t] : t;   // state column
+    const int64_t o = t;                               // output row
+    double a2v[6], Pl[21], Ql[21];
+    {
+        double a4v[6][6];
+        load_moments(a, p, a2v, a4v);
+        taylor_sachs_lower(a, a2v, a4v, Pl, Ql);   // a4 is not needed after this
+    }
+    double e[9], lam[3];
+    eij_frame(a, p, a2v, e, lam);
+    const int st = d_Eij(a, p, a2v, Pl, Ql, e, a.Eij + o * 6);
+    if (a.lami)
+#pragma unroll
+        for (int i = 0; i < 3; i++) a.lami[o * 3 + i] = lam[i];
+    if (a.ei)
+#pragma unroll
+        for (int i = 0; i < 9; i++) a.ei[o * 9 + i] = e[i];
+    if (a.status && st) atomicOr(a.status + p, st);
+    return st;
+}
+
+// two waves per SIMD (at most 256 registers per lane); with wave flags every
+// wavefront writes its own flag, so they need no clearing
+__global__ void __launch_bounds__(128, 2) eij_kernel(EijArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (a.idx ? a.nidx : a.P)) return;
-    eij_item(a, t);
+    const int64_t n = a.idx ? a.nidx : a.P;
+    int st = 0;
+    if (t < n) st = eij_item(a, t);
+    if (a.wave_flag) {
+        const unsigned long long m = __ballot((st & 2) != 0);
+        if ((threadIdx.x & 63) == 0 && t < n) a.wave_flag[t >> 6] = m != 0;
+    }
 }
 
 // work list whose length is on the device: grid-stride loop (a separate
 // kernel, so that the loop does not cost the one-item kernel registers)
-__global__ void __launch_bounds__(128, 1) eij_list_kernel(EijArgs a) {
+__global__ void __launch_bounds__(128, 2) eij_list_kernel(EijArgs a) {
     const int64_t n = *a.nidx_dev < a.nidx ? *a.nidx_dev : a.nidx;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) eij_item(a, t);
 }
 
-hipError_t launch_eij(const EijArgs& a, hipStream_t s) {
-    const int64_t n = a.idx ? a.nidx : a.P;
+// second pass: the rows the first pass marked (Taylor matrix not positive
+// definite), with the reference's regularized branch
+__device__ __forceinline__ void eij_regularized_item(const EijArgs& a, int64_t t) {
+    double* out = a.Eij + t * 6;
+    if ((unsigned long long)__double_as_longlong(out[0]) != kRegularizeMark) return;
+    const int64_t p = a.idx ? (int64_t)a.idx[t] : t;
+    double a2v[6], a4v[6][6], e[9], lam[3];
+    load_moments(a, p, a2v, a4v);
+    eij_frame(a, p, a2v, e, lam);
+    const int st = d_Eij_regularized(a, p, e, out);
+    if (a.status) atomicOr(a.status + p, st);
+}
+
+// the 64 rows of each flagged wavefront (batch form with wave flags)
+__global__ void __launch_bounds__(256) eij_regularized_waves_kernel(EijArgs a) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (a.P + 63) / 64 || !a.wave_flag[w]) return;
+    const int64_t end = (w + 1) * 64 < a.P ? (w + 1) * 64 : a.P;
+    for (int64_t t = w * 64; t < end; t++) eij_regularized_item(a, t);
+}
+
+// every row (lists, or no wave flags)
+__global__ void __launch_bounds__(256) eij_regularized_kernel(EijArgs a) {
+    const int64_t n = (a.idx && a.nidx_dev) ? (*a.nidx_dev < a.nidx ? *a.nidx_dev : a.nidx) : (a.idx ? a.nidx : a.P);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) eij_regularized_item(a, t);
+}
+
+hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
+    const int64_t n = args.idx ? args.nidx : args.P;
     if (n == 0) return hipSuccess;
+    // rheo_params_tranisotropic (src/rheologies.f90:123-132), d = 3: the same
+    // for every parcel, so on the host
+    EijArgs a = args;
+    auto rheo = [&](int ef, double* cI, double* cM, double* cL) {
+        const double ne = (double)(ef * 2) / ((double)a.n_grain + 1);
+        const double pc = std::pow(a.Ecc, ne), pa = std::pow(a.Eca, ne);
+        *cI = (pc - 1) / 2;
+        *cM = (3 * (pc + 1) - 2) / 2 - 2 * pa;
+        *cL = pa - 1;
+    };
+    rheo(1, &a.sA, &a.sB, &a.sC);
+    rheo(-1, &a.tA, &a.tB, &a.tC);
+    a.sachs_iso = 1 + 2.0 / 15 * a.sB + 2.0 / 3 * a.sC;            // src/homogenizations.f90:207
+    a.taylor_iso = 1.0 / (1 + 2.0 / 15 * a.tB + 2.0 / 3 * a.tC);   // :235
     const int64_t nblk = (n + 127) / 128;
     if (a.idx && a.nidx_dev)   // a device-side count: at most 2048 workgroups
         hipLaunchKernelGGL(eij_list_kernel, dim3((unsigned)std::min<int64_t>(nblk, 2048)), dim3(128), 0, s, a);
     else
         hipLaunchKernelGGL(eij_kernel, dim3((unsigned)nblk), dim3(128), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.wave_flag && !a.idx)
+        hipLaunchKernelGGL(eij_regularized_waves_kernel, dim3((unsigned)(((a.P + 63) / 64 + 255) / 256)), dim3(256),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(eij_regularized_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 2048)),
+                           dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -890,6 +890,20 @@ def gen_kernel_win(L, prec, n, rows, cfg, vid):
     return "\n".join(o) + "\n"
 
 
+def scratch_mask(cfg):
+    """stage buffers (n x P each) a variant reads or writes: 1 = YS (increment
+    sum), 2 = XA, 4 = XB (stage inputs); see the make_rsrc lines of each
+    generator"""
+    mode = cfg["mode"]
+    if mode == "lit":
+        return 1 | 2
+    if mode == "reg":
+        return 2 if cfg["xn"] == "mem" else 0
+    if mode == "sreg":
+        return 0
+    return 1 | 2 | 4   # win
+
+
 def gen_dispatch(kinds):
     """kinds: list of (L, prec, variant configs)."""
     o = ["// Generated by specfab_amd/tables/gen_fast.py -- do not edit.",
@@ -916,6 +930,13 @@ def gen_dispatch(kinds):
     o.append("    return nullptr;")
     o.append("}")
     o.append("#endif")
+    o.append("int fast_buffers(int L, int prec, int variant) {")
+    for (L, prec, cfgs) in kinds:
+        for v, cfg in enumerate(cfgs):
+            o.append("    if (L == %d && prec == %d && variant == %d) return %d;"
+                     % (L, 0 if prec == "f64" else 1, v, scratch_mask(cfg)))
+    o.append("    return BUF_YS | BUF_XA | BUF_XB;")
+    o.append("}")
     o.append("hipError_t launch_evolve_fast(int L, int prec, int variant, const EvolveArgs& a, hipStream_t s) {")
     o.append("    if (a.P == 0) return hipSuccess;")
     for (L, prec, cfgs) in kinds:
@@ -923,6 +944,37 @@ def gen_dispatch(kinds):
             o.append("    if (L == %d && prec == %d && variant == %d) return launch_evolve_fast_L%d_%s_v%d(a, s);"
                      % (L, 0 if prec == "f64" else 1, v, L, prec, v))
     o.append("    return hipErrorInvalidValue;")
+    o.append("}")
+    o.append("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
+def gen_moment_maps(exe):
+    """Mandel a2 / a4 of the l <= 4 block of a state as literal FMAs over the
+    maps' nonzero coefficients (in increasing row order, so the sums equal
+    those of a loop over all 15 rows)."""
+    a2, a4 = {}, {}
+    for line in subprocess.check_output([exe, "maps"]).decode().split("\n"):
+        if not line:
+            continue
+        tag, i, q, v = line.split()
+        (a2 if tag == "A2" else a4).setdefault(int(i), []).append((int(q), float.fromhex(v)))
+    o = ["// Generated by specfab_amd/tables/gen_fast.py -- do not edit.",
+         "// Mandel a2 and a4 of the l <= 4 block of a real-basis state x (xq[q] = row q):",
+         "// a2v[i] = a2s[i] / x0 + (i < 3 ? 1/3 : 0), a4v[i][j] = a4s[i][j] / x0, with the",
+         "// nonzero coefficients of the maps of src/moments.f90:164-178 as literals.",
+         "#pragma once", "namespace sfg {",
+         "__device__ __forceinline__ void moment_maps(const double (&xq)[15], double (&a2s)[6], double (&a4s)[6][6]) {"]
+    for i in range(6):
+        terms = sorted(a2.get(i, []))
+        o.append("    { double s = 0.0;" + "".join(" s = fma(%s, xq[%d], s);" % (v.hex(), q) for q, v in terms)
+                 + " a2s[%d] = s; }" % i)
+    for i in range(6):
+        for j in range(i, 6):
+            terms = sorted(a4.get(6 * i + j, []))
+            assert terms == sorted(a4.get(6 * j + i, [])), "a4 map not symmetric"
+            o.append("    { double s = 0.0;" + "".join(" s = fma(%s, xq[%d], s);" % (v.hex(), q) for q, v in terms)
+                     + " a4s[%d][%d] = s;%s }" % (i, j, "" if i == j else " a4s[%d][%d] = s;" % (j, i)))
     o.append("}")
     o.append("}  // namespace sfg")
     return "\n".join(o) + "\n"
@@ -961,6 +1013,7 @@ def main():
         if f.startswith("sfg_fast_L") and f.endswith(".hip") and f not in keep:
             os.remove(os.path.join(outdir, f))
     write_if_changed(os.path.join(outdir, "sfg_fast_dispatch.hip"), gen_dispatch(kinds))
+    write_if_changed(os.path.join(outdir, "sfg_moment_maps.h"), gen_moment_maps(exe))
     print("generated %d fast kernels" % sum(len(c) for (_, _, c) in kinds))
 
 

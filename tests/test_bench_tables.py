@@ -1,7 +1,7 @@
 """CPU checks of bench.py's measurement plumbing: the committed HBM-traffic
 table (profiles/evolve_traffic.json, from rocprofv3 FETCH_SIZE / WRITE_SIZE)
-is self-consistent and feeds `roofline.traffic` only for a launch length it
-was measured at; the algorithmic flop count of a stepping launch matches the
+is self-consistent and feeds `roofline.traffic` for the launch lengths it
+covers (fitted entries: any); the algorithmic flop count of a stepping launch matches the
 operator's term count; `--gpus N` with N > 1 is refused."""
 import json
 import os
@@ -20,7 +20,17 @@ def test_traffic_table_consistent():
     assert tab
     for name, e in tab.items():
         fb, wb = e.get("fetch_bytes"), e.get("write_bytes")
-        if name.startswith("evolve_fast"):
+        if "launches" in e:
+            # fitted entry: every launch's bytes (copy-probe scale) within 2 % of
+            # prologue + per-step x steps, and above the state in and out per step
+            for la in e["launches"]:
+                b = (la["fetch_KiB"] * 2048 + la["write_KiB"] * 1024) / e["parcels"]
+                assert la["bytes_per_parcel"] == pytest.approx(b, abs=0.2)
+                fit = e["prologue_bytes_per_parcel"] + e["bytes_per_parcel_step"] * la["steps"]
+                assert fit == pytest.approx(b, rel=0.02)
+            assert len({la["steps"] for la in e["launches"]}) >= 2
+            assert e["bytes_per_parcel_step"] > 2 * 91 * 8
+        elif name.startswith("evolve_fast"):
             # KiB counted -> bytes with the copy-probe scale (x 2048 read, x 1024 written)
             assert fb == pytest.approx(e["fetch_KiB"] * 2048, rel=1e-6)
             assert wb == pytest.approx(e["write_KiB"] * 1024, rel=1e-6)
@@ -33,12 +43,19 @@ def test_traffic_table_consistent():
 
 
 def test_measured_traffic_lookup():
+    """single-length entries serve their own launch length only; fitted
+    entries any length (the driver's --steps 20 included)"""
     import bench
     tab = json.load(open(os.path.join(ROOT, "profiles", "evolve_traffic.json")))["kernels"]
-    name, e = next((k, v) for k, v in tab.items() if k.startswith("evolve_fast") and v["steps_per_launch"] == 10)
     P = 1 << 20
-    assert bench.measured_traffic(name, P, 10) == pytest.approx(e["bytes_per_parcel_step"] * P * 10)
+    name, e = next((k, v) for k, v in tab.items()
+                   if k.startswith("evolve_fast") and v.get("steps_per_launch") == 4)
+    assert bench.measured_traffic(name, P, 4) == pytest.approx(e["bytes_per_parcel_step"] * P * 4)
     assert bench.measured_traffic(name, P, 3) is None          # launch length not measured
+    f = tab["evolve_fast_L12_f64_v5"]
+    for steps in (1, 10, 20, 200):
+        assert bench.measured_traffic("evolve_fast_L12_f64_v5", P, steps) == pytest.approx(
+            P * (f["prologue_bytes_per_parcel"] + f["bytes_per_parcel_step"] * steps))
     assert bench.measured_traffic("no_such_kernel", P, 10) is None
 
 

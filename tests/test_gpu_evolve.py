@@ -176,6 +176,61 @@ def test_full_size_field_sample(gpu_ctx, oracle, precision, tol):
     assert not fb.status.any().item()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-7)])
+def test_large_batch_in_slices(gpu_ctx, oracle, precision, tol):
+    """2^22 parcels at L = 12 (a 3.05 GB state, more than the 2^31 bytes the
+    steppers address with 32-bit buffer offsets) on the generated steppers:
+    the call steps the batch in slices (2,949,632 parcels: each slice's state
+    staged in the workspace and copied back).  64 sampled parcels (first,
+    last, both sides of the slice boundary, random others) stepped alone are
+    bit for bit the same as inside the batch, and match the oracle; no status
+    bit is set."""
+    import math
+    import torch
+    from specfab_amd import _lib
+    from specfab_amd.batch import FabricBatch
+    dev = torch.device("cuda:0")
+    P, L, nsteps, procs = 1 << 22, 12, 3, ("LROT", "REG", "DDRX")
+    n = (L + 1) * (L + 2) // 2
+    slice_ = ((2 ** 31 - 1) // (8 * n)) // 256 * 256
+    assert n * P * 8 >= 2 ** 31 and slice_ < P
+    print("large batch %s: P=%d, slice %d" % (precision, P, slice_), flush=True)
+    g = torch.Generator(device=dev)
+    g.manual_seed(23)
+    G = torch.randn((P, 3, 3), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    G -= (G.diagonal(dim1=1, dim2=2).sum(-1) / 3)[:, None, None] * torch.eye(3, device=dev, dtype=torch.float64)
+    gam0 = 10.0 ** (torch.rand(P, generator=g, device=dev, dtype=torch.float64) * 2 - 4)
+    x0 = torch.randn((n, P), generator=g, device=dev, dtype=torch.float64) * 1e-3
+    x0[0] = 1 / math.sqrt(4 * math.pi)
+
+    def batch(x, ug, g0):
+        b = FabricBatch.__new__(FabricBatch)
+        b.ctx = _lib.default_context()
+        b.L, b.n, b.dev, b.P = L, n, dev, x.shape[1]
+        b.x, b.ugrad, b.tau, b.gam0, b.lam, b.steps = x.contiguous(), ug.contiguous(), None, g0.contiguous(), None, None
+        b.status = torch.zeros(b.P, dtype=torch.int32, device=dev)
+        return b
+
+    fb = batch(x0.clone(), G, gam0)
+    fb.autotune(precision, 1 << 16, steps=nsteps)
+    fb.evolve(nsteps, 0.5, scheme="rk4", procs=procs, precision=precision)
+    rng = np.random.default_rng(12)
+    edges = [0, 1, 63, slice_ - 2, slice_ - 1, slice_, slice_ + 1, P - 64, P - 1]
+    sample = np.unique(np.concatenate([edges, rng.choice(P, 64 - len(edges), replace=False)]))
+    si = torch.from_numpy(sample).to(dev)
+    alone = batch(x0[:, si], G[si], gam0[si])
+    nlm0 = alone.nlm().cpu().numpy()
+    alone.evolve(nsteps, 0.5, scheme="rk4", procs=procs, precision=precision)
+    assert torch.equal(alone.x, fb.x[:, si])
+    ref = oracle.evolve(L, nlm0, G[si].cpu().numpy(), gam0[si].cpu().numpy(), 0.0, nsteps, 0.5, "rk4", procs)
+    err = relerr(alone.nlm().cpu().numpy(), ref)
+    print("large batch %s: sample vs oracle %.2e" % (precision, err))
+    assert err < tol
+    assert not fb.status.any().item()
+    del fb, x0, G, gam0
+    torch.cuda.empty_cache()
+
+
 def test_fast_vs_generic_stepper(gpu_ctx):
     """The generated register-resident stepper and the table-driven generic
     stepper agree on the GPU (same inputs, FP64)."""
