@@ -180,6 +180,30 @@ typedef struct sfg_evolve_desc {
 } sfg_evolve_desc;
 int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* desc, void* stream);
 
+/* sfg_evolve without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as sfg_evolve; the workspace (per-parcel launch constants and
+ * the stage buffers of the chosen stepper) is a static device array of
+ * SFG_NOALLOC_BYTES that each GPU the library runs on holds until the process
+ * ends.  Limits: the batch must fit that workspace: 45 doubles of launch
+ * constants per parcel plus the chosen stepper's stage buffers (n doubles
+ * each), at L = 12 about 370,000 parcels with the FP64 register stepper
+ * (no stage buffers) and 73,000 with variants that keep two stage buffers;
+ * the call fails with SFG_ERR_ARG and a message otherwise.  The coupling
+ * tables of desc->L must already
+ * be on the device (any earlier call at that L puts them there, e.g.
+ * sfg_evolve with P = 0); the call fails rather than upload them.  The
+ * array's address is taken when a context is created on the device; the
+ * first launch of each kernel in a process still loads its code object (the
+ * HIP runtime's one-time work, as for every kernel), so make one call before
+ * capturing or timing.  Every call
+ * on one GPU uses the same array: calls that overlap on one GPU (from
+ * several streams or threads) race.  Nothing in the array needs to start at
+ * zero (every launch writes what it reads). */
+#define SFG_NOALLOC_BYTES ((size_t)128 << 20)
+int sfg_evolve_noalloc(sfg_ctx* ctx, const sfg_evolve_desc* desc, void* stream);
+
 /* ---- moments and homogenization on batched states (device buffers) ---- */
 /* a2 (P,3,3), src/moments.f90:37-44 */
 int sfg_a2_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a2_dev, void* stream);

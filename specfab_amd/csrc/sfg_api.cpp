@@ -24,6 +24,7 @@
 #include "sfg_init.h"
 #include "sfg_reduced.h"
 #include "sfg_rotate.h"
+#include "sfg_noalloc.h"
 #include "sfg_wave.h"
 #include "sfg_mlp_net.h"
 #include "sfg_surrogate.h"
@@ -363,6 +364,13 @@ int sfg_ctx_create(int device, sfg_ctx** ctx) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ws_event, hipEventDisableTiming);
     if (e != hipSuccess) { delete c; return fail(SFG_ERR_HIP, hipGetErrorString(e)); }
+    // the no-allocation workspace of this device (sfg_evolve_noalloc)
+    if (int rc = sfg::noalloc_setup(device, nullptr)) {
+        (void)hipEventDestroy(c->ws_event);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *ctx = c;
     return SFG_OK;
 }
@@ -570,6 +578,39 @@ int evolve_locked(sfg_ctx* ctx, const sfg_evolve_desc* d, hipStream_t s, int for
 }
 
 }  // namespace
+
+extern "C++" {
+namespace sfg {
+
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
+int ctx_device(const sfg_ctx* ctx) { return ctx->device; }
+
+int evolve_in_workspace(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream, void* ws, size_t ws_bytes) {
+    if (!ctx || !d) return fail(SFG_ERR_ARG, "ctx/desc is NULL");
+    if (d->P < 0 || (d->P > 0 && (!d->ugrad_dev || !d->x_dev))) return fail(SFG_ERR_ARG, "ugrad_dev/x_dev missing");
+    if (d->scheme != SFG_EULER && d->scheme != SFG_RK4) return fail(SFG_ERR_ARG, "unknown scheme");
+    if (d->precision != SFG_FP64 && d->precision != SFG_FP32C) return fail(SFG_ERR_ARG, "unknown precision");
+    if (d->nsteps < 0) return fail(SFG_ERR_ARG, "nsteps < 0");
+    if (d->procs & ~(SFG_LROT | SFG_DDRX | SFG_CDRX | SFG_REG)) return fail(SFG_ERR_ARG, "unknown process bit");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    StepPlan sp;
+    int rc = plan_step(ctx, d, -1, true, sp);
+    if (rc) return rc;
+    if (d->P == 0) return SFG_OK;
+    const size_t need = sp.ws_doubles * sizeof(double);
+    if (need > ws_bytes) {
+        char msg[256];
+        snprintf(msg, sizeof(msg), "the batch needs %zu bytes of workspace, more than the %zu available: split it",
+                 need, ws_bytes);
+        return fail(SFG_ERR_ARG, msg);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    return run_step(ctx, d, pick(ctx, stream), sp, (double*)ws, false);
+}
+
+}  // namespace sfg
+}  // extern "C++"
 
 int sfg_evolve(sfg_ctx* ctx, const sfg_evolve_desc* d, void* stream) {
     if (!ctx || !d) return fail(SFG_ERR_ARG, "ctx/desc is NULL");
