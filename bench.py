@@ -274,7 +274,9 @@ def measured_traffic(kernel, P, steps):
     copy (bytes per KiB counted), fitted per parcel as a launch prologue plus
     a per-step figure from two launch lengths of the same run (so any
     --steps is covered); an entry with a single launch length is used only
-    for that length.  None for a kernel not measured."""
+    for that length; a resident stepper (state read once per launch, written
+    once at its end) has one figure per launch, whatever the step count.
+    None for a kernel not measured."""
     path = os.path.join(ROOT, "profiles", "evolve_traffic.json")
     try:
         with open(path) as f:
@@ -284,6 +286,8 @@ def measured_traffic(kernel, P, steps):
     ent = tab.get(kernel)
     if not ent:
         return None
+    if "bytes_per_parcel_launch" in ent:
+        return P * ent["bytes_per_parcel_launch"]
     if "prologue_bytes_per_parcel" in ent:
         return P * (ent["prologue_bytes_per_parcel"] + ent["bytes_per_parcel_step"] * steps)
     if ent.get("steps_per_launch", 1) != steps:

@@ -82,10 +82,17 @@ Wn = lambda window, chunk, waves, split=1: dict(mode="win", window=window, chunk
 #           each holding the whole stage input and the increment sums of its
 #           share of the rows; stage inputs exchanged through LDS
 Sr = lambda chunk, waves, split=2: dict(mode="sreg", chunk=chunk, waves=waves, block=64 * split, split=split)
+#   dreg    split resident stepper (FP64): `split` waves per 64-parcel group,
+#           each holding the whole stage input and the increment sums and
+#           step-start state of its share of the rows (no HBM traffic per
+#           step); stage inputs exchanged through LDS
+#   gb      ("dreg" mode) (row chunk, parameter) groups per basic block
+Dr = lambda chunk, split=2, gb=2: dict(mode="dreg", chunk=chunk, waves=1, block=64 * split, split=split, gb=gb)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
                        V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False),
-                       R(4, 1, "roll", False, ahead=True)],
+                       R(4, 1, "roll", False, ahead=True), Dr(4), Dr(4, 4), Dr(8, 4), Dr(8, 2),
+                       Dr(8, 4, gb=1), Dr(16, 4, gb=4)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg"), Sr(8, 2)],
@@ -672,6 +679,161 @@ def gen_kernel_sreg(L, prec, n, rows, cfg, vid):
     return "\n".join(o) + "\n"
 
 
+def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
+    """Split resident stepper ("dreg" mode, FP64): S waves share one group of
+    64 parcels, each on its own SIMD.  Every wave holds the whole stage input
+    x (n FP64 values) and, for its own share of the rows (contiguous, split
+    by term count), the RK increment sums y and the step-start state x0, so
+    the state is read from HBM once per launch and written once at its end:
+    no per-stage or per-step memory traffic besides LDS.  At the end of a
+    stage each wave writes its rows' next-stage input (or, in the last
+    stage, the new state) to LDS; after a workgroup barrier every wave reads
+    all n rows back, and a second barrier keeps the next stage's writes
+    behind those reads.  The step loop is written once per share inside
+    `if (wv == share)`; the waves of a group step the same parcels, so both
+    copies run the same number of barriers.  Lanes past the batch stay alive
+    for the barriers with a lane offset past every buffer (loads return 0,
+    stores are dropped).  The arithmetic of every row is that of the
+    register steppers ("reg" mode), term for term.
+    """
+    assert prec == "f64"
+    name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
+    CHUNK, S = cfg["chunk"], cfg["split"]
+    B = 64 * S
+    used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
+    NPU = len(used_p)
+    nq = min(n, 15)
+    nblk = L // 2 + 1
+    OMC, SLC = NPU, NPU + nq
+    pidx = {q: i for i, q in enumerate(used_p)}
+    nterm = [sum(len(tl) for (_, tl) in rows[r][1]) + 1 for r in range(n)]
+    tot, acc_t, share = sum(nterm), 0, []
+    for r in range(n):
+        share.append(min(S - 1, int(S * (acc_t + nterm[r] / 2) / tot)))
+        acc_t += nterm[r]
+    o = []
+    w = o.append
+    w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
+    w("// Split resident stepper, L = %d (n = %d), FP64; variant %d: %s; row shares %s." %
+      (L, n, vid, ", ".join("%s=%s" % kv for kv in sorted(cfg.items())),
+       ", ".join("%d..%d" % (share.index(k), n - 1 - share[::-1].index(k)) for k in range(S))))
+    w('#include "../sfg_kernels.h"')
+    w("namespace sfg {")
+    w("__global__ void __launch_bounds__(%d, 1) %s(EvolveArgs a) {" % (B, name))
+    w("    const int64_t P = a.P;")
+    w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
+    w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
+    w("    const bool valid = p < P;")
+    w("    const uint32_t P8 = (uint32_t)P * 8u;")
+    w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
+    w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
+    w("    const int nstage = a.scheme ? 4 : 1;")
+    w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
+    w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
+    w("    const double h = a.dt;")
+    w("    // launch constants ([column][lane]: theta_p, omega_q, s_l) and the")
+    w("    // stage input exchanged between the waves")
+    w("    __shared__ double thl[%d][64];" % (NPU + nq + nblk))
+    w("    __shared__ double xs[%d][64];" % n)
+    cols = [("a.theta", q, pidx[q]) for q in used_p] + [("a.omega", q, OMC + q) for q in range(nq)] + \
+           [("a.sl", b, SLC + b) for b in range(nblk)]
+    for k in range(S):
+        part = cols[k::S]
+        w("    if (wv == %d && valid) {" % k)
+        for (arr, q, c) in part:
+            w("        thl[%d][lane] = %s[(int64_t)%d * P + p];" % (c, arr, q))
+        w("    }")
+    w("    double %s;" % ", ".join("x%d" % b for b in range(n)))
+    for b in range(n):
+        w("    x%d = bld64(RX, v8, %du * P8);" % (b, b))
+    w("    __syncthreads();")
+    zc = [0]
+
+    def zk():
+        zc[0] += 1
+        return zc[0]
+
+    for sh in range(S):
+        mine = [r for r in range(n) if share[r] == sh]
+        chunks = [mine[c0:c0 + CHUNK] for c0 in range(0, len(mine), CHUNK)]
+        w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
+        w("        double %s;" % ", ".join("y%d = 0.0" % r for r in mine))
+        w("        double %s;" % ", ".join("s%d = x%d" % (r, r) for r in mine))
+        w("        for (int step = 0; step < ns; step++) {")
+        w("            for (int s = 0; s < nstage; s++) {")
+        w("                const bool last = (s == nstage - 1);")
+        w("                const double cy = (nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3);")
+        w("                const double cx = (s == 2) ? h : h / 2;")
+        w("                // a.zero is 0 at run time but unknown to the compiler: it keeps the")
+        w("                // LDS reads of different row chunks and stages apart")
+        w("                const int z = (int)a.zero * (step * 4 + s);")
+        w("                double gD = 0.0;")
+        w("                if (a.use_ddrx) {")
+        k = zk()
+        terms = " + ".join("thl[%d][lane + z * %d] * x%d" % (OMC + q, k, q) for q in range(nq))
+        w("                    gD = (%s) / x0;" % terms)
+        w("                }")
+        ngroup = 0
+        for chunk in chunks:
+            k = zk()
+            w("                {   // rows %s" % ", ".join(str(r) for r in chunk))
+            w("                    double %s;" % ", ".join("acc%d = 0.0" % r for r in chunk))
+            byp = {}
+            for r in chunk:
+                for (pp, tl) in rows[r][1]:
+                    byp.setdefault(pp, []).append((r, tl))
+            for pp in sorted(byp):
+                if ngroup % cfg["gb"] == 0:
+                    # a basic-block boundary every gb (row chunk, parameter)
+                    # groups: the code generator materialises a constant once
+                    # per block for all its uses there, and over a whole stage
+                    # those shared constants outgrow the scalar registers
+                    w("                    if (z) asm volatile(\"\");")
+                ngroup += 1
+                w("                    {")
+                w("                        const double th = thl[%d][lane + z * %d];" % (pidx[pp], k))
+                for (r, tl) in byp[pp]:
+                    col, ti, v = tl[0]
+                    w("                        double u%d = %s * x%d;" % (r, lit(v, prec), col))
+                    for (col, ti, v) in tl[1:]:
+                        w("                        u%d = fma(%s, x%d, u%d);" % (r, lit(v, prec), col, r))
+                    w("                        acc%d = fma(th, u%d, acc%d);" % (r, r, r))
+                w("                    }")
+            for r in chunk:
+                l = rows[r][0]
+                w("                    {")
+                w("                        const double f = fma(thl[%d][lane + z * %d] - gD, x%d, acc%d);"
+                  % (SLC + l // 2, k, r, r))
+                # y is zero at every step's first stage (reset in the last one)
+                w("                        const double yv = fma(cy, f, y%d);" % r)
+                w("                        const double sn = s%d + yv, xn = fma(cx, f, s%d);" % (r, r))
+                w("                        y%d = last ? 0.0 : yv;" % r)
+                w("                        s%d = last ? sn : s%d;" % (r, r))
+                w("                        xs[%d][lane] = last ? sn : xn;" % r)
+                w("                    }")
+            w("                }")
+        w("                __syncthreads();   // every wave's rows of the next stage input are in xs")
+        w("                %s" % " ".join("x%d = xs[%d][lane];" % (b, b) for b in range(n)))
+        w("                __syncthreads();   // read before the next stage overwrites xs")
+        w("            }")
+        w("        }")
+        w("        %s" % " ".join("bst64(RX, v8, %du * P8, s%d);" % (r, r) for r in mine))
+        w("        int st = 0;")
+        w("        %s" % " ".join("st |= isfinite(s%d) ? 0 : 1;" % r for r in mine))
+        w("        if (a.status && valid && st) atomicOr(a.status + p, st);")
+    w("    }")
+    w("    if (wv == 0 && valid && a.steps && a.steps_dec) a.steps[p] = rem - ns;")
+    w("}")
+    w("#ifndef SFG_HOST_EMUL")
+    w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + 63) / 64)), dim3(%d), 0, s, a);" % (name, B))
+    w("    return hipGetLastError();")
+    w("}")
+    w("#endif")
+    w("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
 def gen_kernel_win(L, prec, n, rows, cfg, vid):
     """Window stepper ("win" mode) for states too large for a lane's
     registers (L >= 14 FP64, L >= 18 FP32).
@@ -899,7 +1061,7 @@ def scratch_mask(cfg):
         return 1 | 2
     if mode == "reg":
         return 2 if cfg["xn"] == "mem" else 0
-    if mode == "sreg":
+    if mode in ("sreg", "dreg"):
         return 0
     return 1 | 2 | 4   # win
 
@@ -1002,8 +1164,8 @@ def main():
             kind = "wide" if L in WIDE_L[prec] else ("tuned" if L in TUNED_L else "default")
             vs = VARIANTS[(prec, kind)]
             for vid, cfg in enumerate(vs):
-                gen = {"reg": gen_kernel_reg, "win": gen_kernel_win, "sreg": gen_kernel_sreg}.get(cfg["mode"],
-                                                                                                gen_kernel)
+                gen = {"reg": gen_kernel_reg, "win": gen_kernel_win, "sreg": gen_kernel_sreg,
+                       "dreg": gen_kernel_dreg}.get(cfg["mode"], gen_kernel)
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
                                  gen(L, prec, n, rows, cfg, vid))
             kinds.append((L, prec, vs))

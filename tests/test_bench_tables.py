@@ -20,7 +20,15 @@ def test_traffic_table_consistent():
     assert tab
     for name, e in tab.items():
         fb, wb = e.get("fetch_bytes"), e.get("write_bytes")
-        if "launches" in e:
+        if e.get("resident"):
+            # resident stepper: one figure per launch whatever its length (every
+            # measured launch within 5 % of it), at least the state in and out once
+            for la in e["launches"]:
+                b = (la["fetch_KiB"] * 2048 + la["write_KiB"] * 1024) / e["parcels"]
+                assert la["bytes_per_parcel"] == pytest.approx(b, abs=0.2)
+                assert la["bytes_per_parcel"] == pytest.approx(e["bytes_per_parcel_launch"], rel=0.05)
+            assert e["bytes_per_parcel_launch"] > 2 * 91 * 8
+        elif "launches" in e:
             # fitted entry: every launch's bytes (copy-probe scale) within 2 % of
             # prologue + per-step x steps, and above the state in and out per step
             for la in e["launches"]:
@@ -57,6 +65,10 @@ def test_measured_traffic_lookup():
         assert bench.measured_traffic("evolve_fast_L12_f64_v5", P, steps) == pytest.approx(
             P * (f["prologue_bytes_per_parcel"] + f["bytes_per_parcel_step"] * steps))
     assert bench.measured_traffic("no_such_kernel", P, 10) is None
+    r = next(v for k, v in tab.items() if v.get("resident"))
+    rname = next(k for k, v in tab.items() if v is r)
+    for steps in (1, 10, 200):
+        assert bench.measured_traffic(rname, P, steps) == pytest.approx(P * r["bytes_per_parcel_launch"])
 
 
 def test_flop_count_matches_operator():

@@ -122,6 +122,37 @@ def test_generated_kernel_vs_oracle(oracle, L, prec, v, scheme):
         assert not st.any()
 
 
+# the stepper families: literal-constant stage buffers (v0), register state
+# with the rolled next-stage input (v5), split resident steppers with two and
+# four waves per parcel group and workgroup barriers (v6, v7)
+@pytest.mark.parametrize("L,v", [(8, 0), (8, 5), (8, 6), (8, 7)])
+def test_emulation_under_sanitizers(L, v):
+    """The parameter kernel, the generic stepper and a generated stepper,
+    compiled for the host with AddressSanitizer and UndefinedBehaviorSanitizer
+    (any report is fatal), step one full and one partial wavefront with RK4
+    and Euler; the generated stepper agrees with the generic one."""
+    src = os.path.join(GEN, "sfg_fast_L%d_f64_v%d.hip" % (L, v))
+    if not os.path.exists(src):
+        pytest.skip("generated kernels not present (run the build)")
+    exe = os.path.join(tempfile.gettempdir(), "sfg_emul_san_%d_v%d_%d" % (L, v, os.getpid()))
+    bs, parcels = _block(src)
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", "-DSFG_HOST_EMUL", "-DKERNEL_NAME=evolve_fast_L%d_f64_v%d" % (L, v),
+           "-DKERNEL_BLOCK=%d" % bs, "-DKERNEL_PARCELS=%d" % parcels, "-pthread", "-I", CSRC, "-x", "c++",
+           os.path.join(ROOT, "tests", "emul_driver.cpp"), os.path.join(ROOT, "tests", "emul_sanitize_main.cpp"),
+           os.path.join(CSRC, "sfg_kernels.hip"), os.path.join(CSRC, "sfg_basis.cpp"), src, "-o", exe]
+    subprocess.check_call(cmd)
+    try:
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
+        for scheme in (1, 0):
+            r = subprocess.run([exe, str(L), str(scheme)], capture_output=True, text=True, timeout=600, env=env)
+            print(r.stdout.strip())
+            assert r.returncode == 0, r.stdout + r.stderr[-4000:]
+            assert "runtime error" not in r.stderr and "Sanitizer" not in r.stderr, r.stderr[-4000:]
+    finally:
+        os.remove(exe)
+
+
 _wave = {}
 
 
