@@ -87,12 +87,18 @@ Sr = lambda chunk, waves, split=2: dict(mode="sreg", chunk=chunk, waves=waves, b
 #           step-start state of its share of the rows (no HBM traffic per
 #           step); stage inputs exchanged through LDS
 #   gb      ("dreg" mode) (row chunk, parameter) groups per basic block
-Dr = lambda chunk, split=2, gb=2: dict(mode="dreg", chunk=chunk, waves=1, block=64 * split, split=split, gb=gb)
+#   cols    ("dreg" mode) a wave holds only the stage-input columns its rows
+#           read, so that eight waves (two per SIMD, 256 registers each) share
+#           a parcel group and one wave's scalar constant moves issue beside
+#           the other's vector FMAs
+Dr = lambda chunk, split=2, gb=2, cols=False: dict(mode="dreg", chunk=chunk, waves=2 if cols else 1,
+                                                  block=64 * split, split=split, gb=gb, cols=cols)
 VARIANTS = {
     ("f64", "tuned"): [V("lit", 8, 2, False, False), V("lit", 8, 1, False, False),
                        V("lit", 8, 2, False, False, 256, True), R(8, 1, "mem", False), R(4, 1, "mem", False),
                        R(4, 1, "roll", False, ahead=True), Dr(4), Dr(4, 4), Dr(8, 4), Dr(8, 2),
-                       Dr(8, 4, gb=1), Dr(16, 4, gb=4)],
+                       Dr(8, 4, gb=1), Dr(16, 4, gb=4), Dr(4, 8, cols=True), Dr(8, 8, cols=True),
+                       Dr(16, 8, gb=4, cols=True), Dr(16, 8, gb=8, cols=True)],
     ("f64", "default"): [V("lit", 8, 2, False, False)],
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg"), Sr(8, 2)],
@@ -719,7 +725,8 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
        ", ".join("%d..%d" % (share.index(k), n - 1 - share[::-1].index(k)) for k in range(S))))
     w('#include "../sfg_kernels.h"')
     w("namespace sfg {")
-    w("__global__ void __launch_bounds__(%d, 1) %s(EvolveArgs a) {" % (B, name))
+    COLS = cfg.get("cols", False)
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, 2 if COLS else 1, name))
     w("    const int64_t P = a.P;")
     w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
     w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
@@ -735,6 +742,8 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
     w("    // stage input exchanged between the waves")
     w("    __shared__ double thl[%d][64];" % (NPU + nq + nblk))
     w("    __shared__ double xs[%d][64];" % n)
+    if COLS:
+        w("    __shared__ double ys[%d][64];   // RK increment sums" % n)
     cols = [("a.theta", q, pidx[q]) for q in used_p] + [("a.omega", q, OMC + q) for q in range(nq)] + \
            [("a.sl", b, SLC + b) for b in range(nblk)]
     for k in range(S):
@@ -743,10 +752,11 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
         for (arr, q, c) in part:
             w("        thl[%d][lane] = %s[(int64_t)%d * P + p];" % (c, arr, q))
         w("    }")
-    w("    double %s;" % ", ".join("x%d" % b for b in range(n)))
-    for b in range(n):
-        w("    x%d = bld64(RX, v8, %du * P8);" % (b, b))
-    w("    __syncthreads();")
+    if not COLS:
+        w("    double %s;" % ", ".join("x%d" % b for b in range(n)))
+        for b in range(n):
+            w("    x%d = bld64(RX, v8, %du * P8);" % (b, b))
+        w("    __syncthreads();")
     zc = [0]
 
     def zk():
@@ -757,22 +767,47 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
         mine = [r for r in range(n) if share[r] == sh]
         chunks = [mine[c0:c0 + CHUNK] for c0 in range(0, len(mine), CHUNK)]
         w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
-        w("        double %s;" % ", ".join("y%d = 0.0" % r for r in mine))
-        w("        double %s;" % ", ".join("s%d = x%d" % (r, r) for r in mine))
+        if not COLS:
+            w("        double %s;" % ", ".join("y%d = 0.0" % r for r in mine))
+        if COLS:
+            # only the columns this share's rows read (and its own rows) are
+            # held; the stage input arrives through xs, also the first one
+            need = set(mine)
+            for r in mine:
+                for (pp, tl) in rows[r][1]:
+                    for (col, ti, v) in tl:
+                        need.add(col)
+            need = sorted(need)
+            w("        double %s;" % ", ".join("s%d = bld64(RX, v8, %du * P8)" % (r, r) for r in mine))
+            w("        %s" % " ".join("xs[%d][lane] = s%d; ys[%d][lane] = 0.0;" % (r, r, r) for r in mine))
+            w("        double %s;" % ", ".join("x%d" % b for b in need))
+            w("        __syncthreads();")
+            X = lambda q: ("x%d" % q) if q in need else ("xs[%d][lane]" % q)
+        else:
+            w("        double %s;" % ", ".join("s%d = x%d" % (r, r) for r in mine))
+            X = lambda q: "x%d" % q
         w("        for (int step = 0; step < ns; step++) {")
         w("            for (int s = 0; s < nstage; s++) {")
+        if COLS:
+            w("                %s" % " ".join("x%d = xs[%d][lane];" % (b, b) for b in need))
         w("                const bool last = (s == nstage - 1);")
         w("                const double cy = (nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3);")
         w("                const double cx = (s == 2) ? h : h / 2;")
+        if COLS:
+            w("                // stage kind as factors: the state takes the increment sum in the")
+            w("                // last stage only, which also clears the sum and hands on the state")
+            w("                const double lastf = last ? 1.0 : 0.0, keepf = last ? 0.0 : 1.0, cxe = last ? 0.0 : cx;")
         w("                // a.zero is 0 at run time but unknown to the compiler: it keeps the")
         w("                // LDS reads of different row chunks and stages apart")
         w("                const int z = (int)a.zero * (step * 4 + s);")
         w("                double gD = 0.0;")
         w("                if (a.use_ddrx) {")
         k = zk()
-        terms = " + ".join("thl[%d][lane + z * %d] * x%d" % (OMC + q, k, q) for q in range(nq))
-        w("                    gD = (%s) / x0;" % terms)
+        terms = " + ".join("thl[%d][lane + z * %d] * %s" % (OMC + q, k, X(q)) for q in range(nq))
+        w("                    gD = (%s) / %s;" % (terms, X(0)))
         w("                }")
+        if COLS:
+            w("                __syncthreads();   // every wave has read xs before any overwrites it")
         ngroup = 0
         for chunk in chunks:
             k = zk()
@@ -805,16 +840,26 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
                 w("                        const double f = fma(thl[%d][lane + z * %d] - gD, x%d, acc%d);"
                   % (SLC + l // 2, k, r, r))
                 # y is zero at every step's first stage (reset in the last one)
-                w("                        const double yv = fma(cy, f, y%d);" % r)
-                w("                        const double sn = s%d + yv, xn = fma(cx, f, s%d);" % (r, r))
-                w("                        y%d = last ? 0.0 : yv;" % r)
-                w("                        s%d = last ? sn : s%d;" % (r, r))
-                w("                        xs[%d][lane] = last ? sn : xn;" % r)
+                if COLS:
+                    # s + yv in the last stage (fma with 1: one rounding, as s + yv),
+                    # s unchanged before it; the next stage input s + cx f before
+                    # the last stage, the new state in it
+                    w("                        const double yv = fma(cy, f, ys[%d][lane]);" % r)
+                    w("                        s%d = fma(yv, lastf, s%d);" % (r, r))
+                    w("                        ys[%d][lane] = yv * keepf;" % r)
+                    w("                        xs[%d][lane] = fma(cxe, f, s%d);" % (r, r))
+                else:
+                    w("                        const double yv = fma(cy, f, y%d);" % r)
+                    w("                        const double sn = s%d + yv, xn = fma(cx, f, s%d);" % (r, r))
+                    w("                        y%d = last ? 0.0 : yv;" % r)
+                    w("                        s%d = last ? sn : s%d;" % (r, r))
+                    w("                        xs[%d][lane] = last ? sn : xn;" % r)
                 w("                    }")
             w("                }")
         w("                __syncthreads();   // every wave's rows of the next stage input are in xs")
-        w("                %s" % " ".join("x%d = xs[%d][lane];" % (b, b) for b in range(n)))
-        w("                __syncthreads();   // read before the next stage overwrites xs")
+        if not COLS:
+            w("                %s" % " ".join("x%d = xs[%d][lane];" % (b, b) for b in range(n)))
+            w("                __syncthreads();   // read before the next stage overwrites xs")
         w("            }")
         w("        }")
         w("        %s" % " ".join("bst64(RX, v8, %du * P8, s%d);" % (r, r) for r in mine))

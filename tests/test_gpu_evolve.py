@@ -297,21 +297,22 @@ def test_every_stepper_variant(gpu_ctx, oracle, L):
             ctx.close()
 
 
+@pytest.mark.parametrize("L", [12, 20])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32c", 2e-6)])
-def test_uneven_plan_every_window_variant(gpu_ctx, precision, tol):
-    """The L = 20 parcels of the uneven fixture, each repeated 200 times and
-    shuffled (600 parcels: ten parcel groups of 64, per-parcel step counts,
-    so the lanes of one wave stop at different steps), through a graph plan
-    with every window-stepper variant forced in turn, including those that
-    split a parcel group's rows over several waves with workgroup barriers
-    (64 parcels per 128-thread workgroup): final states against the
-    reference trajectories, every step counter at zero."""
+def test_uneven_plan_every_variant(gpu_ctx, precision, tol, L):
+    """The L = 12 and L = 20 parcels of the uneven fixture, each repeated 200
+    times and shuffled (600 parcels: ten parcel groups of 64, per-parcel step
+    counts, so the lanes of one wave stop at different steps), through a
+    graph plan with every stepper variant of that truncation forced in turn,
+    including those that split a parcel group's rows over 2, 4 or 8 waves
+    with workgroup barriers in the step loop (L = 12 resident steppers, L =
+    20 window steppers): final states against the reference trajectories,
+    every step counter at zero."""
     import os
     from specfab_amd import _lib
     from specfab_amd.batch import FabricBatch, UnevenBatch
     g = load_golden("uneven.npz")
-    L = 20
-    rep = np.random.default_rng(7).permutation(np.repeat(np.arange(len(g["steps_L20"])), 200))
+    rep = np.random.default_rng(7).permutation(np.repeat(np.arange(len(g["steps_L%d" % L])), 200))
     nlm0, ug, g0, lam, steps, ref = (g["%s_L%d" % (k, L)][rep] for k in ("nlm0", "ugrad", "gam0", "lam", "steps", "nlm"))
     _, times = FabricBatch(L, nlm=nlm0, ugrad=ug, gam0=g0).autotune(precision, P=256)
     nv = len(times)
@@ -327,8 +328,8 @@ def test_uneven_plan_every_window_variant(gpu_ctx, precision, tol):
         ub.make_plan(0.5, "rk4", ("LROT", "REG", "DDRX", "CDRX"), precision=precision, chunk=64).run()
         out = ub.nlm()
         err = max(relerr(out[k], ref[k]) for k in range(len(steps)))
-        print("L=20 %s variant %d: max rel err %.2e over %d parcels, steps %d..%d"
-              % (precision, v, err, len(steps), steps.min(), steps.max()))
+        print("L=%d %s variant %d: max rel err %.2e over %d parcels, steps %d..%d"
+              % (L, precision, v, err, len(steps), steps.min(), steps.max()))
         assert err < tol, (precision, v, err)
         assert (ub.remaining() == 0).all(), (precision, v)
         ub.close()
