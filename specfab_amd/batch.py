@@ -235,11 +235,10 @@ class FabricBatch:
         per-variant ms per step)."""
         Pp = int(P or min(self.P, 1 << 18))
         best = _c.c_int(-1)
-        ms = (_c.c_double * 16)()
+        ms = (_c.c_double * 64)()
         prec = _lib.SFG_FP64 if precision == "fp64" else _lib.SFG_FP32C
-        check(lib().sfg_autotune(self.ctx.handle, self.L, prec, Pp, int(steps), _stream(), _c.byref(best), ms, 16))
-        nv = max(0, best.value + 1)
-        times = [ms[i] for i in range(16) if ms[i] > 0]
+        check(lib().sfg_autotune(self.ctx.handle, self.L, prec, Pp, int(steps), _stream(), _c.byref(best), ms, 64))
+        times = [ms[i] for i in range(64) if ms[i] > 0]
         return best.value, times
 
     # ---- moments ----

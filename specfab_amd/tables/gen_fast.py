@@ -91,6 +91,11 @@ Sr = lambda chunk, waves, split=2: dict(mode="sreg", chunk=chunk, waves=waves, b
 #           read, so that eight waves (two per SIMD, 256 registers each) share
 #           a parcel group and one wave's scalar constant moves issue beside
 #           the other's vector FMAs
+#   lwin    LDS window stepper (FP64, L >= 14): `split` waves per 64-parcel
+#           group, the group's stage input in LDS, each wave's rows evaluated
+#           in column windows loaded from LDS
+Lw = lambda window, chunk, split=4, gb=2: dict(mode="lwin", window=window, chunk=chunk, waves=1, block=64 * split,
+                                              split=split, gb=gb)
 Dr = lambda chunk, split=2, gb=2, cols=False: dict(mode="dreg", chunk=chunk, waves=2 if cols else 1,
                                                   block=64 * split, split=split, gb=gb, cols=cols)
 VARIANTS = {
@@ -103,7 +108,7 @@ VARIANTS = {
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg"), Sr(8, 2)],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
-    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2)],
+    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2), Lw(96, 8), Lw(64, 8), Lw(128, 8)],
     ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2)],
 }
 WIDE_L = {"f64": (14, 16, 18, 20), "f32": (18, 20)}
@@ -772,12 +777,23 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
         if COLS:
             # only the columns this share's rows read (and its own rows) are
             # held; the stage input arrives through xs, also the first one
-            need = set(mine)
-            for r in mine:
-                for (pp, tl) in rows[r][1]:
-                    for (col, ti, v) in tl:
-                        need.add(col)
-            need = sorted(need)
+            # in the order of first use by the stage code (terms grouped by
+            # parameter over the chunk's rows, then the rows' own diagonals),
+            # so that the first groups can start while later columns arrive
+            need = []
+            for c0 in range(0, len(mine), CHUNK):
+                byp0 = {}
+                for r in mine[c0:c0 + CHUNK]:
+                    for (pp, tl) in rows[r][1]:
+                        byp0.setdefault(pp, []).append(tl)
+                for pp in sorted(byp0):
+                    for tl in byp0[pp]:
+                        for (col, ti, v) in tl:
+                            if col not in need:
+                                need.append(col)
+                for r in mine[c0:c0 + CHUNK]:
+                    if r not in need:
+                        need.append(r)
             w("        double %s;" % ", ".join("s%d = bld64(RX, v8, %du * P8)" % (r, r) for r in mine))
             w("        %s" % " ".join("xs[%d][lane] = s%d; ys[%d][lane] = 0.0;" % (r, r, r) for r in mine))
             w("        double %s;" % ", ".join("x%d" % b for b in need))
@@ -865,6 +881,186 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
         w("        %s" % " ".join("bst64(RX, v8, %du * P8, s%d);" % (r, r) for r in mine))
         w("        int st = 0;")
         w("        %s" % " ".join("st |= isfinite(s%d) ? 0 : 1;" % r for r in mine))
+        w("        if (a.status && valid && st) atomicOr(a.status + p, st);")
+    w("    }")
+    w("    if (wv == 0 && valid && a.steps && a.steps_dec) a.steps[p] = rem - ns;")
+    w("}")
+    w("#ifndef SFG_HOST_EMUL")
+    w("hipError_t launch_%s(const EvolveArgs& a, hipStream_t s) {" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)((a.P + 63) / 64)), dim3(%d), 0, s, a);" % (name, B))
+    w("    return hipGetLastError();")
+    w("}")
+    w("#endif")
+    w("}  // namespace sfg")
+    return "\n".join(o) + "\n"
+
+
+def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
+    """LDS window stepper ("lwin" mode, FP64) for states too large for the
+    registers of the waves of a parcel group (L >= 14).  S waves (one per
+    SIMD) share 64 parcels; the stage input of the group lives in LDS (n
+    FP64 rows), wave w owns a contiguous share of the rows (balanced by term
+    count) and keeps their RK increment sums and next-stage inputs in
+    registers (the increment sums in the YS buffer).  Per row chunk of its share (rows grouped greedily so that
+    their coupled columns number at most cfg["window"]) a wave loads those
+    columns of the stage input from LDS into registers and the chunk's
+    step-start state rows from HBM (each row's owner is its only reader and
+    writer), then evaluates the rows in sub-chunks of cfg["chunk"] with the
+    coupling values as literals; the chunk's RK increment sums come from and
+    go to the YS buffer (HBM / L2), prefetched with the state rows.  A stage
+    ends with a workgroup barrier (every wave has read the stage input), the
+    owners' stores of their next-stage rows into LDS and a second barrier.
+    The state is read once per row and stage and written in the last stage.  Lanes
+    past the batch stay alive for the barriers with a lane offset past every
+    buffer (loads return 0, stores are dropped).  The arithmetic of every row
+    is that of the other steppers, term for term.
+    """
+    assert prec == "f64"
+    name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
+    WIN, CHUNK, S = cfg["window"], cfg["chunk"], cfg["split"]
+    B = 64 * S
+    used_p = sorted({p for (_, gs) in rows for (p, _) in gs})
+    NPU = len(used_p)
+    nq = min(n, 15)
+    nblk = L // 2 + 1
+    OMC, SLC = NPU, NPU + nq
+    pidx = {q: i for i, q in enumerate(used_p)}
+    need = [{col for (_, tl) in rows[r][1] for (col, _, _) in tl} | {r} for r in range(n)]
+    nterm = [sum(len(tl) for (_, tl) in rows[r][1]) + 1 for r in range(n)]
+    tot, acc_t, share = sum(nterm), 0, []
+    for r in range(n):
+        share.append(min(S - 1, int(S * (acc_t + nterm[r] / 2) / tot)))
+        acc_t += nterm[r]
+    chunks = []   # (share, rows, columns)
+    for sh in range(S):
+        cur, cols = [], set()
+        for r in (r for r in range(n) if share[r] == sh):
+            if cur and len(cols | need[r]) > WIN:
+                chunks.append((sh, cur, sorted(cols)))
+                cur, cols = [], set()
+            cur.append(r)
+            cols |= need[r]
+        if cur:
+            chunks.append((sh, cur, sorted(cols)))
+    o = []
+    w = o.append
+    w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
+    w("// LDS window stepper, L = %d (n = %d), FP64; variant %d: %s; %d row chunks, %d column loads per stage." %
+      (L, n, vid, ", ".join("%s=%s" % kv for kv in sorted(cfg.items())), len(chunks),
+       sum(len(c) for (_, _, c) in chunks)))
+    w('#include "../sfg_kernels.h"')
+    w("namespace sfg {")
+    w("__global__ void __launch_bounds__(%d, 1) %s(EvolveArgs a) {" % (B, name))
+    w("    const int64_t P = a.P;")
+    w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
+    w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
+    w("    const bool valid = p < P;")
+    w("    const uint32_t P8 = (uint32_t)P * 8u;")
+    w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
+    w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
+    w("    const sfg_rsrc RYS = make_rsrc(a.YS, P8 * %du);" % n)
+    w("    const int nstage = a.scheme ? 4 : 1;")
+    w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
+    w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
+    w("    const double h = a.dt;")
+    w("    // launch constants ([column][lane]: theta_p, omega_q, s_l) and the")
+    w("    // group's stage input")
+    w("    __shared__ double thl[%d][64];" % (NPU + nq + nblk))
+    w("    __shared__ double xs[%d][64];" % n)
+    consts = [("a.theta", q, pidx[q]) for q in used_p] + [("a.omega", q, OMC + q) for q in range(nq)] + \
+             [("a.sl", b, SLC + b) for b in range(nblk)]
+    for k in range(S):
+        w("    if (wv == %d && valid) {" % k)
+        for (arr, q, c) in consts[k::S]:
+            w("        thl[%d][lane] = %s[(int64_t)%d * P + p];" % (c, arr, q))
+        w("    }")
+    zc = [0]
+
+    def zk():
+        zc[0] += 1
+        return zc[0]
+
+    for sh in range(S):
+        mine = [r for r in range(n) if share[r] == sh]
+        w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
+        w("        %s" % " ".join("xs[%d][lane] = bld64(RX, v8, %du * P8);" % (r, r) for r in mine))
+        w("        double %s;" % ", ".join("xn%d" % r for r in mine))
+        w("        __syncthreads();")
+        w("        for (int step = 0; step < ns; step++) {")
+        w("            for (int s = 0; s < nstage; s++) {")
+        w("                const bool first = (s == 0), last = (s == nstage - 1);")
+        w("                const double cy = (nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3);")
+        w("                const double cx = (s == 2) ? h : h / 2;")
+        w("                // a.zero is 0 at run time but unknown to the compiler: it keeps the")
+        w("                // LDS reads of different row chunks and stages apart")
+        w("                const int z = (int)a.zero * (step * 4 + s);")
+        w("                // row offsets (r + zu) P8 vary with the stage for the compiler, so")
+        w("                // they are formed where used instead of held in scalar registers")
+        w("                const uint32_t zu = (uint32_t)z;")
+        w("                double gD = 0.0;")
+        w("                if (a.use_ddrx) {")
+        k = zk()
+        terms = " + ".join("thl[%d][lane + z * %d] * xs[%d][lane]" % (OMC + q, k, q) for q in range(nq))
+        w("                    gD = (%s) / xs[0][lane];" % terms)
+        w("                }")
+        ngroup = 0
+        for (csh, crow, ccols) in chunks:
+            if csh != sh:
+                continue
+            w("                {   // rows %d..%d: %d columns" % (crow[0], crow[-1], len(ccols)))
+            w("                    const double %s;" % ", ".join("c%d = xs[%d][lane]" % (c, c) for c in ccols))
+            for c0 in range(0, len(crow), CHUNK):
+                sub = crow[c0:c0 + CHUNK]
+                k = zk()
+                w("                    {")
+                w("                        const double %s;" % ", ".join("xo%d = bld64(RX, v8, (%du + zu) * P8)" % (r, r)
+                                                                     for r in sub))
+                w("                        const double %s;" % ", ".join(
+                    "yo%d = first ? 0.0 : bld64(RYS, v8, (%du + zu) * P8)" % (r, r) for r in sub))
+                w("                        double %s;" % ", ".join("acc%d = 0.0" % r for r in sub))
+                byp = {}
+                for r in sub:
+                    for (pp, tl) in rows[r][1]:
+                        byp.setdefault(pp, []).append((r, tl))
+                for pp in sorted(byp):
+                    if ngroup % cfg["gb"] == 0:
+                        # a basic-block boundary every gb (row chunk, parameter)
+                        # groups keeps the constants shared within a block
+                        # inside the scalar registers
+                        w("                        if (z) asm volatile(\"\");")
+                    ngroup += 1
+                    w("                        {")
+                    w("                            const double th = thl[%d][lane + z * %d];" % (pidx[pp], k))
+                    for (r, tl) in byp[pp]:
+                        col, ti, v = tl[0]
+                        w("                            double u%d = %s * c%d;" % (r, lit(v, prec), col))
+                        for (col, ti, v) in tl[1:]:
+                            w("                            u%d = fma(%s, c%d, u%d);" % (r, lit(v, prec), col, r))
+                        w("                            acc%d = fma(th, u%d, acc%d);" % (r, r, r))
+                    w("                        }")
+                for r in sub:
+                    l = rows[r][0]
+                    w("                        {")
+                    w("                            const double f = fma(thl[%d][lane + z * %d] - gD, c%d, acc%d);"
+                      % (SLC + l // 2, k, r, r))
+                    w("                            const double yv = fma(cy, f, yo%d);" % r)
+                    w("                            if (last) {")
+                    w("                                xn%d = xo%d + yv;" % (r, r))
+                    w("                                bst64(RX, v8, (%du + zu) * P8, xn%d);" % (r, r))
+                    w("                            } else {")
+                    w("                                xn%d = fma(cx, f, xo%d);" % (r, r))
+                    w("                                bst64(RYS, v8, (%du + zu) * P8, yv);" % r)
+                    w("                            }")
+                    w("                        }")
+                w("                    }")
+            w("                }")
+        w("                __syncthreads();   // every wave has read this stage's input")
+        w("                %s" % " ".join("xs[%d][lane] = xn%d;" % (r, r) for r in mine))
+        w("                __syncthreads();   // the next stage input is complete")
+        w("            }")
+        w("        }")
+        w("        int st = 0;")
+        w("        %s" % " ".join("st |= isfinite(xs[%d][lane]) ? 0 : 1;" % r for r in mine))
         w("        if (a.status && valid && st) atomicOr(a.status + p, st);")
     w("    }")
     w("    if (wv == 0 && valid && a.steps && a.steps_dec) a.steps[p] = rem - ns;")
@@ -1108,6 +1304,8 @@ def scratch_mask(cfg):
         return 2 if cfg["xn"] == "mem" else 0
     if mode in ("sreg", "dreg"):
         return 0
+    if mode == "lwin":
+        return 1
     return 1 | 2 | 4   # win
 
 
@@ -1210,7 +1408,7 @@ def main():
             vs = VARIANTS[(prec, kind)]
             for vid, cfg in enumerate(vs):
                 gen = {"reg": gen_kernel_reg, "win": gen_kernel_win, "sreg": gen_kernel_sreg,
-                       "dreg": gen_kernel_dreg}.get(cfg["mode"], gen_kernel)
+                       "dreg": gen_kernel_dreg, "lwin": gen_kernel_lwin}.get(cfg["mode"], gen_kernel)
                 write_if_changed(os.path.join(outdir, "sfg_fast_L%d_%s_v%d.hip" % (L, prec, vid)),
                                  gen(L, prec, n, rows, cfg, vid))
             kinds.append((L, prec, vs))
