@@ -108,7 +108,8 @@ VARIANTS = {
     ("f32", "tuned"): [V("lit", 8, 2, True, False), V("lit", 8, 3, True, False), R(8, 1, "reg"),
                        R(8, 2, "mem"), R(4, 1, "reg"), R(16, 1, "reg"), Sr(8, 2)],
     ("f32", "default"): [V("lit", 8, 2, True, False)],
-    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2), Lw(96, 8), Lw(64, 8), Lw(128, 8)],
+    ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2), Lw(96, 8), Lw(64, 8), Lw(128, 8),
+                      Lw(64, 4, split=8), Lw(48, 4, split=8)],
     ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2)],
 }
 WIDE_L = {"f64": (14, 16, 18, 20), "f32": (18, 20)}
@@ -900,8 +901,8 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     registers of the waves of a parcel group (L >= 14).  S waves (one per
     SIMD) share 64 parcels; the stage input of the group lives in LDS (n
     FP64 rows), wave w owns a contiguous share of the rows (balanced by term
-    count) and keeps their RK increment sums and next-stage inputs in
-    registers (the increment sums in the YS buffer).  Per row chunk of its share (rows grouped greedily so that
+    count); their RK increment sums and next-stage inputs go through the YS
+    and XA buffers (HBM / L2).  Per row chunk of its share (rows grouped greedily so that
     their coupled columns number at most cfg["window"]) a wave loads those
     columns of the stage input from LDS into registers and the chunk's
     step-start state rows from HBM (each row's owner is its only reader and
@@ -909,7 +910,8 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     coupling values as literals; the chunk's RK increment sums come from and
     go to the YS buffer (HBM / L2), prefetched with the state rows.  A stage
     ends with a workgroup barrier (every wave has read the stage input), the
-    owners' stores of their next-stage rows into LDS and a second barrier.
+    owners' copies of their next-stage rows from XA (the new state from X in
+    the last stage) into LDS and a second barrier.
     The state is read once per row and stage and written in the last stage.  Lanes
     past the batch stay alive for the barriers with a lane offset past every
     buffer (loads return 0, stores are dropped).  The arithmetic of every row
@@ -950,7 +952,8 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
        sum(len(c) for (_, _, c) in chunks)))
     w('#include "../sfg_kernels.h"')
     w("namespace sfg {")
-    w("__global__ void __launch_bounds__(%d, 1) %s(EvolveArgs a) {" % (B, name))
+    # four waves: one per SIMD; eight: two per SIMD (256 registers each)
+    w("__global__ void __launch_bounds__(%d, %d) %s(EvolveArgs a) {" % (B, max(1, S // 4), name))
     w("    const int64_t P = a.P;")
     w("    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;   // wv is uniform per wave")
     w("    const int64_t p = (int64_t)blockIdx.x * 64 + lane;")
@@ -959,6 +962,7 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
     w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
     w("    const sfg_rsrc RYS = make_rsrc(a.YS, P8 * %du);" % n)
+    w("    const sfg_rsrc RXA = make_rsrc(a.XA, P8 * %du);" % n)
     w("    const int nstage = a.scheme ? 4 : 1;")
     w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
     w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
@@ -984,7 +988,6 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
         mine = [r for r in range(n) if share[r] == sh]
         w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
         w("        %s" % " ".join("xs[%d][lane] = bld64(RX, v8, %du * P8);" % (r, r) for r in mine))
-        w("        double %s;" % ", ".join("xn%d" % r for r in mine))
         w("        __syncthreads();")
         w("        for (int step = 0; step < ns; step++) {")
         w("            for (int s = 0; s < nstage; s++) {")
@@ -1007,8 +1010,11 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
         for (csh, crow, ccols) in chunks:
             if csh != sh:
                 continue
+            kc = zk()
             w("                {   // rows %d..%d: %d columns" % (crow[0], crow[-1], len(ccols)))
-            w("                    const double %s;" % ", ".join("c%d = xs[%d][lane]" % (c, c) for c in ccols))
+            # the z term keeps the compiler from reusing one chunk's column
+            # values in another (which would keep them all live)
+            w("                    const double %s;" % ", ".join("c%d = xs[%d][lane + z * %d]" % (c, c, kc) for c in ccols))
             for c0 in range(0, len(crow), CHUNK):
                 sub = crow[c0:c0 + CHUNK]
                 k = zk()
@@ -1045,17 +1051,19 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
                       % (SLC + l // 2, k, r, r))
                     w("                            const double yv = fma(cy, f, yo%d);" % r)
                     w("                            if (last) {")
-                    w("                                xn%d = xo%d + yv;" % (r, r))
-                    w("                                bst64(RX, v8, (%du + zu) * P8, xn%d);" % (r, r))
+                    w("                                bst64(RX, v8, (%du + zu) * P8, xo%d + yv);" % (r, r))
                     w("                            } else {")
-                    w("                                xn%d = fma(cx, f, xo%d);" % (r, r))
+                    w("                                bst64(RXA, v8, (%du + zu) * P8, fma(cx, f, xo%d));" % (r, r))
                     w("                                bst64(RYS, v8, (%du + zu) * P8, yv);" % r)
                     w("                            }")
                     w("                        }")
                 w("                    }")
             w("                }")
         w("                __syncthreads();   // every wave has read this stage's input")
-        w("                %s" % " ".join("xs[%d][lane] = xn%d;" % (r, r) for r in mine))
+        w("                {")
+        w("                    const sfg_rsrc RN = last ? RX : RXA;   // this lane's own stores, read back")
+        w("                    %s" % " ".join("xs[%d][lane] = bld64(RN, v8, (%du + zu) * P8);" % (r, r) for r in mine))
+        w("                }")
         w("                __syncthreads();   // the next stage input is complete")
         w("            }")
         w("        }")
@@ -1305,7 +1313,7 @@ def scratch_mask(cfg):
     if mode in ("sreg", "dreg"):
         return 0
     if mode == "lwin":
-        return 1
+        return 1 | 2
     return 1 | 2 | 4   # win
 
 
