@@ -110,7 +110,8 @@ VARIANTS = {
     ("f32", "default"): [V("lit", 8, 2, True, False)],
     ("f64", "wide"): [Wn(96, 8, 1), Wn(64, 8, 2), Wn(64, 8, 2, split=2), Lw(96, 8), Lw(64, 8), Lw(128, 8),
                       Lw(64, 4, split=8), Lw(48, 4, split=8)],
-    ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2)],
+    ("f32", "wide"): [Wn(128, 8, 1), Wn(96, 8, 2), Wn(96, 8, 2, split=2), Lw(128, 8), Lw(96, 8, split=8),
+                      Lw(64, 8, split=8)],
 }
 WIDE_L = {"f64": (14, 16, 18, 20), "f32": (18, 20)}
 TUNED_L = (8, 12)
@@ -897,10 +898,10 @@ def gen_kernel_dreg(L, prec, n, rows, cfg, vid):
 
 
 def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
-    """LDS window stepper ("lwin" mode, FP64) for states too large for the
-    registers of the waves of a parcel group (L >= 14).  S waves (one per
-    SIMD) share 64 parcels; the stage input of the group lives in LDS (n
-    FP64 rows), wave w owns a contiguous share of the rows (balanced by term
+    """LDS window stepper ("lwin" mode) for states too large for the
+    registers of the waves of a parcel group (L >= 14).  S waves share 64
+    parcels; the stage input of the group lives in LDS (n rows of the
+    operator precision: FP64, or FP32 with the FP64 state), wave w owns a contiguous share of the rows (balanced by term
     count); their RK increment sums and next-stage inputs go through the YS
     and XA buffers (HBM / L2).  Per row chunk of its share (rows grouped greedily so that
     their coupled columns number at most cfg["window"]) a wave loads those
@@ -915,9 +916,15 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     The state is read once per row and stage and written in the last stage.  Lanes
     past the batch stay alive for the barriers with a lane offset past every
     buffer (loads return 0, stores are dropped).  The arithmetic of every row
-    is that of the other steppers, term for term.
+    is that of the other steppers, term for term (FP32: the operator, the
+    increment sums and the next-stage input in FP32, the state and its update
+    x0 + sum in FP64).
     """
-    assert prec == "f64"
+    F64 = prec == "f64"
+    T = "double" if F64 else "float"
+    LDT, STT = ("bld64", "bst64") if F64 else ("bld32", "bst32")
+    cv = (lambda e: e) if F64 else (lambda e: "(float)(%s)" % e)
+    zero = "0.0" if F64 else "0.0f"
     name = "evolve_fast_L%d_%s_v%d" % (L, prec, vid)
     WIN, CHUNK, S = cfg["window"], cfg["chunk"], cfg["split"]
     B = 64 * S
@@ -947,8 +954,8 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     o = []
     w = o.append
     w("// Generated by specfab_amd/tables/gen_fast.py -- do not edit.")
-    w("// LDS window stepper, L = %d (n = %d), FP64; variant %d: %s; %d row chunks, %d column loads per stage." %
-      (L, n, vid, ", ".join("%s=%s" % kv for kv in sorted(cfg.items())), len(chunks),
+    w("// LDS window stepper, L = %d (n = %d), %s; variant %d: %s; %d row chunks, %d column loads per stage." %
+      (L, n, "FP64" if F64 else "FP32 stepping, FP64 state", vid, ", ".join("%s=%s" % kv for kv in sorted(cfg.items())), len(chunks),
        sum(len(c) for (_, _, c) in chunks)))
     w('#include "../sfg_kernels.h"')
     w("namespace sfg {")
@@ -961,22 +968,28 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     w("    const uint32_t P8 = (uint32_t)P * 8u;")
     w("    const uint32_t v8 = valid ? (uint32_t)p * 8u : 0x80000000u;")
     w("    const sfg_rsrc RX = make_rsrc(a.X, P8 * %du);" % n)
-    w("    const sfg_rsrc RYS = make_rsrc(a.YS, P8 * %du);" % n)
-    w("    const sfg_rsrc RXA = make_rsrc(a.XA, P8 * %du);" % n)
+    if F64:
+        PT, vT = "P8", "v8"
+    else:
+        PT, vT = "P4", "v4"
+        w("    const uint32_t P4 = (uint32_t)P * 4u;")
+        w("    const uint32_t v4 = valid ? (uint32_t)p * 4u : 0x80000000u;")
+    w("    const sfg_rsrc RYS = make_rsrc(a.YS, %s * %du);" % (PT, n))
+    w("    const sfg_rsrc RXA = make_rsrc(a.XA, %s * %du);" % (PT, n))
     w("    const int nstage = a.scheme ? 4 : 1;")
     w("    const int rem = valid ? (a.steps ? a.steps[p] : a.nsteps) : 0;")
     w("    const int ns = (a.steps && a.steps_dec) ? (rem < a.nsteps ? rem : a.nsteps) : rem;")
     w("    const double h = a.dt;")
     w("    // launch constants ([column][lane]: theta_p, omega_q, s_l) and the")
     w("    // group's stage input")
-    w("    __shared__ double thl[%d][64];" % (NPU + nq + nblk))
-    w("    __shared__ double xs[%d][64];" % n)
+    w("    __shared__ %s thl[%d][64];" % (T, NPU + nq + nblk))
+    w("    __shared__ %s xs[%d][64];" % (T, n))
     consts = [("a.theta", q, pidx[q]) for q in used_p] + [("a.omega", q, OMC + q) for q in range(nq)] + \
              [("a.sl", b, SLC + b) for b in range(nblk)]
     for k in range(S):
         w("    if (wv == %d && valid) {" % k)
         for (arr, q, c) in consts[k::S]:
-            w("        thl[%d][lane] = %s[(int64_t)%d * P + p];" % (c, arr, q))
+            w("        thl[%d][lane] = %s;" % (c, cv("%s[(int64_t)%d * P + p]" % (arr, q))))
         w("    }")
     zc = [0]
 
@@ -987,20 +1000,20 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
     for sh in range(S):
         mine = [r for r in range(n) if share[r] == sh]
         w("    %sif (wv == %d) {" % ("" if sh == 0 else "} else ", sh))
-        w("        %s" % " ".join("xs[%d][lane] = bld64(RX, v8, %du * P8);" % (r, r) for r in mine))
+        w("        %s" % " ".join("xs[%d][lane] = %s;" % (r, cv("bld64(RX, v8, %du * P8)" % r)) for r in mine))
         w("        __syncthreads();")
         w("        for (int step = 0; step < ns; step++) {")
         w("            for (int s = 0; s < nstage; s++) {")
         w("                const bool first = (s == 0), last = (s == nstage - 1);")
-        w("                const double cy = (nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3);")
-        w("                const double cx = (s == 2) ? h : h / 2;")
+        w("                const %s cy = %s;" % (T, cv("(nstage == 1) ? h : ((s == 0 || s == 3) ? h / 6 : h / 3)")))
+        w("                const %s cx = %s;" % (T, cv("(s == 2) ? h : h / 2")))
         w("                // a.zero is 0 at run time but unknown to the compiler: it keeps the")
         w("                // LDS reads of different row chunks and stages apart")
         w("                const int z = (int)a.zero * (step * 4 + s);")
         w("                // row offsets (r + zu) P8 vary with the stage for the compiler, so")
         w("                // they are formed where used instead of held in scalar registers")
         w("                const uint32_t zu = (uint32_t)z;")
-        w("                double gD = 0.0;")
+        w("                %s gD = %s;" % (T, zero))
         w("                if (a.use_ddrx) {")
         k = zk()
         terms = " + ".join("thl[%d][lane + z * %d] * xs[%d][lane]" % (OMC + q, k, q) for q in range(nq))
@@ -1014,16 +1027,16 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
             w("                {   // rows %d..%d: %d columns" % (crow[0], crow[-1], len(ccols)))
             # the z term keeps the compiler from reusing one chunk's column
             # values in another (which would keep them all live)
-            w("                    const double %s;" % ", ".join("c%d = xs[%d][lane + z * %d]" % (c, c, kc) for c in ccols))
+            w("                    const %s %s;" % (T, ", ".join("c%d = xs[%d][lane + z * %d]" % (c, c, kc) for c in ccols)))
             for c0 in range(0, len(crow), CHUNK):
                 sub = crow[c0:c0 + CHUNK]
                 k = zk()
                 w("                    {")
                 w("                        const double %s;" % ", ".join("xo%d = bld64(RX, v8, (%du + zu) * P8)" % (r, r)
                                                                      for r in sub))
-                w("                        const double %s;" % ", ".join(
-                    "yo%d = first ? 0.0 : bld64(RYS, v8, (%du + zu) * P8)" % (r, r) for r in sub))
-                w("                        double %s;" % ", ".join("acc%d = 0.0" % r for r in sub))
+                w("                        const %s %s;" % (T, ", ".join(
+                    "yo%d = first ? %s : %s(RYS, %s, (%du + zu) * %s)" % (r, zero, LDT, vT, r, PT) for r in sub)))
+                w("                        %s %s;" % (T, ", ".join("acc%d = %s" % (r, zero) for r in sub)))
                 byp = {}
                 for r in sub:
                     for (pp, tl) in rows[r][1]:
@@ -1036,10 +1049,10 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
                         w("                        if (z) asm volatile(\"\");")
                     ngroup += 1
                     w("                        {")
-                    w("                            const double th = thl[%d][lane + z * %d];" % (pidx[pp], k))
+                    w("                            const %s th = thl[%d][lane + z * %d];" % (T, pidx[pp], k))
                     for (r, tl) in byp[pp]:
                         col, ti, v = tl[0]
-                        w("                            double u%d = %s * c%d;" % (r, lit(v, prec), col))
+                        w("                            %s u%d = %s * c%d;" % (T, r, lit(v, prec), col))
                         for (col, ti, v) in tl[1:]:
                             w("                            u%d = fma(%s, c%d, u%d);" % (r, lit(v, prec), col, r))
                         w("                            acc%d = fma(th, u%d, acc%d);" % (r, r, r))
@@ -1047,28 +1060,41 @@ def gen_kernel_lwin(L, prec, n, rows, cfg, vid):
                 for r in sub:
                     l = rows[r][0]
                     w("                        {")
-                    w("                            const double f = fma(thl[%d][lane + z * %d] - gD, c%d, acc%d);"
-                      % (SLC + l // 2, k, r, r))
-                    w("                            const double yv = fma(cy, f, yo%d);" % r)
+                    w("                            const %s f = fma(thl[%d][lane + z * %d] - gD, c%d, acc%d);"
+                      % (T, SLC + l // 2, k, r, r))
+                    w("                            const %s yv = fma(cy, f, yo%d);" % (T, r))
                     w("                            if (last) {")
-                    w("                                bst64(RX, v8, (%du + zu) * P8, xo%d + yv);" % (r, r))
+                    w("                                bst64(RX, v8, (%du + zu) * P8, xo%d + %s);"
+                      % (r, r, "yv" if F64 else "(double)yv"))
                     w("                            } else {")
-                    w("                                bst64(RXA, v8, (%du + zu) * P8, fma(cx, f, xo%d));" % (r, r))
-                    w("                                bst64(RYS, v8, (%du + zu) * P8, yv);" % r)
+                    w("                                %s(RXA, %s, (%du + zu) * %s, fma(cx, f, %s));"
+                      % (STT, vT, r, PT, ("xo%d" % r) if F64 else ("(float)xo%d" % r)))
+                    w("                                %s(RYS, %s, (%du + zu) * %s, yv);" % (STT, vT, r, PT))
                     w("                            }")
                     w("                        }")
                 w("                    }")
             w("                }")
         w("                __syncthreads();   // every wave has read this stage's input")
-        w("                {")
-        w("                    const sfg_rsrc RN = last ? RX : RXA;   // this lane's own stores, read back")
-        w("                    %s" % " ".join("xs[%d][lane] = bld64(RN, v8, (%du + zu) * P8);" % (r, r) for r in mine))
-        w("                }")
+        if F64:
+            w("                {")
+            w("                    const sfg_rsrc RN = last ? RX : RXA;   // this lane's own stores, read back")
+            w("                    %s" % " ".join("xs[%d][lane] = bld64(RN, v8, (%du + zu) * P8);" % (r, r) for r in mine))
+            w("                }")
+        else:
+            w("                if (last) {   // this lane's own stores, read back")
+            w("                    %s" % " ".join("xs[%d][lane] = (float)bld64(RX, v8, (%du + zu) * P8);" % (r, r)
+                                                for r in mine))
+            w("                } else {")
+            w("                    %s" % " ".join("xs[%d][lane] = bld32(RXA, v4, (%du + zu) * P4);" % (r, r) for r in mine))
+            w("                }")
         w("                __syncthreads();   // the next stage input is complete")
         w("            }")
         w("        }")
         w("        int st = 0;")
-        w("        %s" % " ".join("st |= isfinite(xs[%d][lane]) ? 0 : 1;" % r for r in mine))
+        if F64:
+            w("        %s" % " ".join("st |= isfinite(xs[%d][lane]) ? 0 : 1;" % r for r in mine))
+        else:
+            w("        %s" % " ".join("st |= isfinite(bld64(RX, v8, %du * P8)) ? 0 : 1;" % r for r in mine))
         w("        if (a.status && valid && st) atomicOr(a.status + p, st);")
     w("    }")
     w("    if (wv == 0 && valid && a.steps && a.steps_dec) a.steps[p] = rem - ns;")
