@@ -425,6 +425,90 @@ def homogenization_block(fb, procs, precision, more_steps):
         return {"error": "%s: %s" % (type(e).__name__, e)}
 
 
+def bench_discrete(a):
+    """Discrete-grain (DDM) path of the reference's SDM-vs-DDM harness
+    (tests/SDM-DDM-performance/SDM-DDM-performance.py:24-73): ri_LROT
+    (src/dynamics.f90:112-137) of 2^20 grain c-axes over Nt = 2000 Euler steps
+    of pure shear along z (T = 1 / ln 2, t = 4, dt = t / Nt), the whole
+    trajectory (Nt, N, 3) written to HBM; then ri_to_nlm (src/moments.f90:98-131)
+    of the end state grouped as 13,500 ensembles of 64 grains at L = 6."""
+    import numpy as np
+    import torch
+    from specfab_amd import batch as B
+    from specfab_amd import specfabpy as sf
+    N, Nt, E, G = 1 << 20, 2000, 13500, 64
+    T = 1 / math.log(2)
+    ugrad = sf.pureshear_ugrad(2, 0.0, T)
+    D, W = (ugrad + ugrad.T) / 2, (ugrad - ugrad.T) / 2
+    dt = 4.0 / Nt
+    rng = np.random.default_rng(12)
+    r0 = rng.standard_normal((N, 3))
+    r0 /= np.linalg.norm(r0, axis=1, keepdims=True)
+    dev = torch.device("cuda:0")
+    r0d = torch.from_numpy(r0).to(dev)
+    Dn = torch.from_numpy(np.repeat(D[None], Nt, axis=0)).to(dev)
+    Wn = torch.from_numpy(np.repeat(W[None], Nt, axis=0)).to(dev)
+    wi = torch.full((G,), 1.0 / G, dtype=torch.float64, device=dev)
+    fb = B.FabricBatch(6, P=E)
+    for _ in range(a.warmup):
+        ri = B.ri_LROT_batch(r0d, dt, Nt, Dn, Wn, 1.0)
+        fb.set_grains(ri[-1, :E * G].reshape(E, G, 3), wi)
+    del ri
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ms_lrot, ms_r2n = 0.0, 0.0
+    for _ in range(a.steps):
+        ev[0].record()
+        ri = B.ri_LROT_batch(r0d, dt, Nt, Dn, Wn, 1.0)
+        ev[1].record()
+        fb.set_grains(ri[-1, :E * G].reshape(E, G, 3), wi)
+        ev[2].record()
+        torch.cuda.synchronize()
+        ms_lrot += ev[0].elapsed_time(ev[1])
+        ms_r2n += ev[1].elapsed_time(ev[2])
+        del ri
+    ms_lrot /= a.steps
+    ms_r2n /= a.steps
+    gsteps = N * (Nt - 1)
+    per_gs = 24                                   # one (x, y, z) FP64 axis written per grain-step
+    ach = (per_gs * gsteps + 24 * N) / (ms_lrot * 1e-3) / 1e9
+    per_grain = 24                                # one axis read per grain (weights shared, cached)
+    ach_r2n = (per_grain * E * G + E * 28 * 8) / (ms_r2n * 1e-3) / 1e9
+    tab = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                      "evolve_traffic.json")))["kernels"]
+    tr = tab.get("ri_lrot_kernel")
+    traffic = tr["bytes_per_parcel"] * gsteps if tr and tr.get("parcels") == N else None
+    cpu = None
+    if not a.no_cpu:
+        from oracle import ref
+        if ref.available():
+            r = ref.RefLib()
+            Ns, Ds, Ws = 1000, np.repeat(D[None], Nt, axis=0), np.repeat(W[None], Nt, axis=0)
+            k, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 10.0:
+                r.ri_LROT(r0[:Ns], dt, Nt, Ds, Ws, 1.0)
+                k += 1
+            el = time.perf_counter() - t0
+            cpu = {"value": k * Ns * (Nt - 1) / el, "unit": "grain-steps/s", "cores": 1, "kind": "reference",
+                   "sample": "%d calls of ri_LROT on 1000 grains x 2000 steps, reference FP64 Fortran "
+                             "(oracle/_ref), %.1f s" % (k, el)}
+    out = {"metric": "grain-steps/s, discrete lattice rotation (ri_LROT, whole trajectory written)",
+           "value": gsteps / (ms_lrot * 1e-3), "unit": "grain-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_lrot, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded random unit axes)",
+           "config": {"workload": "DDM: ri_LROT of 2^20 grains, Nt=2000 Euler steps of pure shear along z "
+                                  "(T=1/ln2, t=4), trajectory (2000, 2^20, 3) in HBM; ri_to_nlm L=6 of "
+                                  "13500 ensembles x 64 grains", "grains": N, "Nt": Nt},
+           "ms_ri_to_nlm": ms_r2n, "ensembles_per_s": E / (ms_r2n * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                        "traffic": traffic, "kernel": "ri_lrot_kernel", "bytes_per_grain_step": per_gs},
+           "roofline_ri_to_nlm": {"bound": "hbm", "achieved": ach_r2n, "peak": 8000.0, "unit": "GB/s",
+                                  "frac": ach_r2n / 8000.0, "kernel": "ri_moments_kernel",
+                                  "bytes_per_grain": per_grain},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,9 +518,10 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate", "export"],
+    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate", "export", "discrete"],
                     help="field: the L=12 Eulerian field (default, the headline); uneven: 2^18 parcels with mixed "
-                         "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs learned")
+                         "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs "
+                         "learned; export: operator export; discrete: ri_LROT / ri_to_nlm of 2^20 grains")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -464,6 +549,8 @@ def main():
         return bench_surrogate(a)
     if a.workload == "export":
         return bench_export(a)
+    if a.workload == "discrete":
+        return bench_discrete(a)
     dev = torch.device("cuda:0")
     P, L = a.parcels, a.L
     G, gam0, x = make_field(P, L, 1, dev)

@@ -289,6 +289,27 @@ int sfg_a4_eigentensors_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_de
 int sfg_nlm_isvalid(sfg_ctx* ctx, int64_t K, const double* nhat20, const double* nhat40, int32_t* isvalid);
 int sfg_state_isvalid_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int32_t* valid_dev, void* stream);
 
+/* ---- discrete grains ---- */
+/* ri_LROT(ri0, dt, Nt, D, W, iota) (src/dynamics.f90:112-137; specfabpy
+ * src/specfabpy.f90:309-325): Nt - 1 Euler steps of N grain axes under
+ * r' = (W + iota (r r^T D - D r r^T)) r, each step renormalised to |r| = 1.
+ * ri0 (N, 3), D and W (Nt, 3, 3) row-major per step (step Nt - 1 unused),
+ * ri (Nt, N, 3) with ri[0] = ri0 -- the reference's ri(Nt, N, 3) in C
+ * order.  Host arrays, or device arrays on the stream (_batch). */
+int sfg_ri_LROT(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0, double dt, const double* D, const double* W,
+                double iota, double* ri);
+int sfg_ri_LROT_batch(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0_dev, double dt, const double* D_dev,
+                      const double* W_dev, double iota, double* ri_dev, void* stream);
+/* ri_to_nlm(ri, wi, L) (src/moments.f90:98-131): the harmonic expansion of a
+ * weighted ensemble of N grain axes, from its a2 (L = 2), a4 (L = 4) or a6
+ * (L >= 6; rows l > 6 zero) moment.  n00 = 1 / (2 sqrt(pi)) whatever the
+ * weights.  Single: ri (N, 3), wi (N) host -> nlm (n complex, host).
+ * Batched: E ensembles ri_dev (E, N, 3) sharing the weights wi_dev (N) ->
+ * real states x_dev (n, E).  L >= 2. */
+int sfg_ri_to_nlm(sfg_ctx* ctx, int L, int64_t N, const double* ri, const double* wi, double* nlm);
+int sfg_ri_to_state_batch(sfg_ctx* ctx, int L, int64_t E, int64_t N, const double* ri_dev, const double* wi_dev,
+                          double* x_dev, void* stream);
+
 /* Cij_to_Lame_tranisotropic (src/elasticities.f90:107-120): (C11, C33, C55,
  * C12, C13) -> (lam, mu, Elam, Emu, Egam).  Host only. */
 int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]);

@@ -57,6 +57,8 @@ class RefLib:
         L.sfref_rotate_nlm.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
         L.sfref_a4_eigentensors.argtypes = [_cp, _dp, _dp]
         L.sfref_nlm_isvalid.argtypes = [_c.c_int, _cp, _cp, np.ctypeslib.ndpointer(dtype=np.int32)]
+        L.sfref_ri_LROT.argtypes = [_c.c_int, _c.c_int, _dp, _c.c_double, _dp, _dp, _c.c_double, _dp]
+        L.sfref_ri_to_nlm.argtypes = [_c.c_int, _c.c_int, _dp, _dp, _cp]
         L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                    _c.c_double, _dp, _dp, _dp, _cp]
         L.sfref_a2_to_nlm.argtypes = [_dp, _cp]
@@ -208,6 +210,22 @@ class RefLib:
         ok = np.zeros(len(n20), dtype=np.int32)
         self.lib.sfref_nlm_isvalid(len(n20), n20, n40, ok)
         return ok.astype(bool)
+
+    def ri_LROT(self, ri0, dt, Nt, D, W, iota):
+        """ri0 (N, 3), D and W (Nt, 3, 3) -> ri (Nt, N, 3)"""
+        r0 = np.asfortranarray(ri0, dtype=np.float64)
+        N = r0.shape[0]
+        out = np.zeros((Nt, N, 3), order="F")
+        self.lib.sfref_ri_LROT(N, int(Nt), r0, float(dt), np.asfortranarray(D, dtype=np.float64),
+                               np.asfortranarray(W, dtype=np.float64), float(iota), out)
+        return np.ascontiguousarray(out)
+
+    def ri_to_nlm(self, ri, wi, L):
+        """ri (N, 3), wi (N) -> nlm ((L+1)(L+2)/2) complex"""
+        r = np.asfortranarray(ri, dtype=np.float64)
+        out = np.zeros((L + 1) * (L + 2) // 2, dtype=np.complex128, order="F")
+        self.lib.sfref_ri_to_nlm(r.shape[0], int(L), r, self._v(wi), out)
+        return out.copy()
 
     def Qnorm_tranisotropic(self, nlm, alpha, Lame):
         out = np.zeros((3, 3), order="F")

@@ -279,6 +279,25 @@ contains
         ok = merge(1, 0, v)
     end subroutine
 
+    ! ---- discrete grains ----
+
+    ! ri_LROT(ri0, dt, Nt, D, W, iota): ri0(N,3), D/W(Nt,3,3) -> ri(Nt,N,3)
+    subroutine sfref_ri_LROT(N, Nt, ri0, dt, D, W, iota, ri) bind(C, name="sfref_ri_LROT")
+        integer(c_int), value :: N, Nt
+        real(c_double), intent(in) :: ri0(N,3), D(Nt,3,3), W(Nt,3,3)
+        real(c_double), value :: dt, iota
+        real(c_double), intent(out) :: ri(Nt,N,3)
+        ri = ri_LROT(ri0, dt, Nt, D, W, iota)
+    end subroutine
+
+    ! ri_to_nlm(ri, wi, L): ri(N,3), wi(N) -> nlm((L+1)(L+2)/2)
+    subroutine sfref_ri_to_nlm(N, L, ri, wi, nlm) bind(C, name="sfref_ri_to_nlm")
+        integer(c_int), value :: N, L
+        real(c_double), intent(in) :: ri(N,3), wi(N)
+        complex(c_double_complex), intent(out) :: nlm((L+1)*(L+2)/2)
+        nlm = ri_to_nlm(ri, wi, L)
+    end subroutine
+
     ! ---- time stepping ----
     !
     ! procs bitmask: 1 = LROT, 2 = DDRX, 4 = CDRX, 8 = REG.  scheme: 0 = Euler,

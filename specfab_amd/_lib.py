@@ -103,6 +103,10 @@ SIGNATURES = {
     "sfg_a4_eigentensors_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp]),
     "sfg_nlm_isvalid": (_i, [_vp, _i64, _dp, _dp, _c.POINTER(_c.c_int32)]),
     "sfg_state_isvalid_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_ri_LROT": (_i, [_vp, _i64, _i, _dp, _d, _dp, _dp, _d, _dp]),
+    "sfg_ri_LROT_batch": (_i, [_vp, _i64, _i, _vp, _d, _vp, _vp, _d, _vp, _vp]),
+    "sfg_ri_to_nlm": (_i, [_vp, _i, _i64, _dp, _dp, _dp]),
+    "sfg_ri_to_state_batch": (_i, [_vp, _i, _i64, _i64, _vp, _vp, _vp, _vp]),
     "sfg_evolve_noalloc": (_i, [_vp, _c.POINTER(EvolveDesc), _vp]),
     "sfg_mlp_nparams": (_i64, []),
     "sfg_mlp_create": (_i, [_vp, _c.c_uint64, _c.POINTER(_vp)]),

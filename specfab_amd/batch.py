@@ -122,6 +122,27 @@ def rnlm_to_nlm_batch(L, rnlm, ctx=None):
     return z
 
 
+def ri_LROT_batch(ri0, dt, Nt, eps, omg, iota, ctx=None):
+    """Trajectories ri (Nt, N, 3) of N grain axes ri0 (N, 3) (device tensor)
+    under the discrete lattice rotation with per-step strain rates eps and
+    spins omg (Nt, 3, 3): ri_LROT, src/dynamics.f90:112-137."""
+    ctx = ctx or _lib.default_context()
+    r0 = ri0.to(torch.float64).contiguous()
+    if r0.dim() != 2 or r0.shape[1] != 3:
+        raise ValueError("ri0 must be (N, 3)")
+    Nt = int(Nt)
+    if Nt < 1:
+        raise ValueError("Nt must be >= 1")
+    D = eps.to(r0.device, torch.float64).contiguous()
+    W = omg.to(r0.device, torch.float64).contiguous()
+    if D.shape != (Nt, 3, 3) or W.shape != (Nt, 3, 3):
+        raise ValueError("eps and omg must be (Nt, 3, 3)")
+    ri = torch.empty((Nt, r0.shape[0], 3), dtype=torch.float64, device=r0.device)
+    check(lib().sfg_ri_LROT_batch(ctx.handle, r0.shape[0], Nt, _ptr(r0), float(dt), _ptr(D), _ptr(W), float(iota),
+                                  _ptr(ri), _stream()))
+    return ri
+
+
 class FabricBatch:
     """P parcels at truncation L with per-parcel kinematics.
 
@@ -219,6 +240,18 @@ class FabricBatch:
         (a4_to_nlm, src/moments.f90:76-84); higher coefficients zeroed."""
         A = self._f64(a4, (self.P, 3, 3, 3, 3))
         check(lib().sfg_a4_to_state_batch(self.ctx.handle, self.L, self.P, _ptr(A), _ptr(self.x), _stream()))
+        return self
+
+    def set_grains(self, ri, wi):
+        """States of P grain ensembles ri (P, N, 3) with the shared weights
+        wi (N,) (ri_to_nlm, src/moments.f90:98-131): from the a2 (L = 2), a4
+        (L = 4) or a6 moment (L >= 6; coefficients above l = 6 zeroed)."""
+        R = self._f64(ri, (self.P, -1, 3))
+        w = torch.as_tensor(wi, dtype=torch.float64, device=self.dev).contiguous()
+        if w.dim() != 1 or w.shape[0] != R.shape[1]:
+            raise ValueError("wi must be (N,) with N = ri.shape[1]")
+        check(lib().sfg_ri_to_state_batch(self.ctx.handle, self.L, self.P, R.shape[1], _ptr(R), _ptr(w),
+                                          _ptr(self.x), _stream()))
         return self
 
     def apply_bounds(self):

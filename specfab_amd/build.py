@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libspecfab_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SFG_ARCH", "gfx950")
 
-SOURCES = ["sfg_basis.cpp", "sfg_wave_host.cpp", "sfg_api.cpp", "sfg_kernels.hip", "sfg_homog.hip", "sfg_wave.hip", "sfg_mlp_host.cpp", "sfg_mlp.hip", "sfg_surrogate.hip", "sfg_init.hip", "sfg_reduced.hip", "sfg_rotate.hip", "sfg_noalloc.hip"]
+SOURCES = ["sfg_basis.cpp", "sfg_wave_host.cpp", "sfg_api.cpp", "sfg_kernels.hip", "sfg_homog.hip", "sfg_wave.hip", "sfg_mlp_host.cpp", "sfg_mlp.hip", "sfg_surrogate.hip", "sfg_init.hip", "sfg_reduced.hip", "sfg_rotate.hip", "sfg_noalloc.hip", "sfg_discrete.hip"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", "-Wno-unused-result"]
@@ -35,6 +35,7 @@ def gen_tables(force=False):
     inc = os.path.join(GEN, "gaunt_L20.inc")
     if force or not os.path.exists(inc):
         run([sys.executable, os.path.join(HERE, "tables", "make_gaunt.py"), inc])
+    run([sys.executable, os.path.join(HERE, "tables", "make_a6_map.py"), os.path.join(GEN, "sfg_a6_map.h")])
     gen_fast = os.path.join(HERE, "tables", "gen_fast.py")
     if os.path.exists(gen_fast):
         run([sys.executable, gen_fast, GEN])

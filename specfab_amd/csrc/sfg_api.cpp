@@ -24,6 +24,7 @@
 #include "sfg_init.h"
 #include "sfg_reduced.h"
 #include "sfg_rotate.h"
+#include "sfg_discrete.h"
 #include "sfg_noalloc.h"
 #include "sfg_wave.h"
 #include "sfg_mlp_net.h"
@@ -1385,6 +1386,114 @@ int sfg_rotate_nlm(sfg_ctx* ctx, int L, const double* nlm, double theta, double 
     if (rc) return rc;
     HIPCHK(sfg::launch_state_to_nlm(1, n, op->cre, op->cim, op->csgn, op->cneg, dy, dn, s));
     HIPCHK(hipMemcpyAsync(out, dn, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+/* ---- discrete grains ---- */
+
+int sfg_ri_LROT_batch(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0_dev, double dt, const double* D_dev,
+                      const double* W_dev, double iota, double* ri_dev, void* stream) {
+    if (!ctx || N < 0 || Nt < 1 || (N > 0 && (!ri0_dev || !ri_dev || (Nt > 1 && (!D_dev || !W_dev)))))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::RiLrotArgs a{};
+    a.N = N; a.Nt = Nt; a.dt = dt; a.iota = iota; a.ri0 = ri0_dev; a.D = D_dev; a.W = W_dev; a.ri = ri_dev;
+    HIPCHK(sfg::launch_ri_lrot(a, pick(ctx, stream)));
+    return SFG_OK;
+}
+
+int sfg_ri_LROT(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0, double dt, const double* D, const double* W,
+                double iota, double* ri) {
+    if (!ctx || N < 0 || Nt < 1 || (N > 0 && (!ri0 || !ri || (Nt > 1 && (!D || !W)))))
+        return fail(SFG_ERR_ARG, "bad argument");
+    if (N == 0) return SFG_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t nr = (size_t)N * 3, nd = (size_t)Nt * 9, nt = (size_t)Nt * N * 3;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (nr + 2 * nd + nt) * sizeof(double)));
+    double* d0 = (double*)buf.p;
+    double* dD = d0 + nr;
+    double* dW = dD + nd;
+    double* dri = dW + nd;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(d0, ri0, nr * sizeof(double), hipMemcpyHostToDevice, s));
+    if (Nt > 1) {
+        HIPCHK(hipMemcpyAsync(dD, D, nd * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dW, W, nd * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    sfg::RiLrotArgs a{};
+    a.N = N; a.Nt = Nt; a.dt = dt; a.iota = iota; a.ri0 = d0; a.D = dD; a.W = dW; a.ri = dri;
+    HIPCHK(sfg::launch_ri_lrot(a, s));
+    HIPCHK(hipMemcpyAsync(ri, dri, nt * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+namespace {
+// ri_to_nlm of E ensembles into the real state x (n(L), E) on stream s
+int ri_to_state_locked(sfg_ctx* ctx, int L, int64_t E, int64_t N, const double* ri, const double* wi, double* x,
+                       hipStream_t s) {
+    if (L < 2) return fail(SFG_ERR_L, "ri_to_nlm needs L >= 2 (src/moments.f90:111-130)");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::RiMomentArgs a{};
+    a.E = E; a.N = N; a.ri = ri; a.wi = wi; a.n = op->n;
+    a.order = L == 2 ? 2 : (L == 4 ? 4 : 6);
+    if (a.order == 6) {
+        a.out = x;
+        HIPCHK(sfg::launch_ri_moments(a, s));
+        return SFG_OK;
+    }
+    // the a2 / a4 tensors, then the reference's tensor-to-harmonics bodies
+    void* ws;
+    WsLease lease(ctx, s);
+    rc = lease.acquire((size_t)E * (a.order == 2 ? 9 : 81) * sizeof(double), &ws);
+    if (rc) return rc;
+    a.out = (double*)ws;
+    HIPCHK(sfg::launch_ri_moments(a, s));
+    if (a.order == 2) HIPCHK(sfg::launch_a2_to_state(E, op->n, a.out, x, s));
+    else HIPCHK(sfg::launch_a4_to_state(E, op->n, a.out, x, s));
+    return lease.release();
+}
+}  // namespace
+
+int sfg_ri_to_state_batch(sfg_ctx* ctx, int L, int64_t E, int64_t N, const double* ri_dev, const double* wi_dev,
+                          double* x_dev, void* stream) {
+    if (!ctx || E < 0 || N < 0 || (E > 0 && (!x_dev || (N > 0 && (!ri_dev || !wi_dev)))))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (E == 0) return SFG_OK;
+    return ri_to_state_locked(ctx, L, E, N, ri_dev, wi_dev, x_dev, pick(ctx, stream));
+}
+
+int sfg_ri_to_nlm(sfg_ctx* ctx, int L, int64_t N, const double* ri, const double* wi, double* nlm) {
+    if (!ctx || N < 0 || !nlm || (N > 0 && (!ri || !wi))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, ((size_t)N * 4 + 3 * n + 1) * sizeof(double)));
+    double* dr = (double*)buf.p;
+    double* dw = dr + (size_t)N * 3;
+    double* dx = dw + N;
+    double* dn = dx + n;
+    hipStream_t s = ctx->stream;
+    if (N > 0) {
+        HIPCHK(hipMemcpyAsync(dr, ri, (size_t)N * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dw, wi, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    rc = ri_to_state_locked(ctx, L, 1, N, dr, dw, dx, s);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_state_to_nlm(1, n, op->cre, op->cim, op->csgn, op->cneg, dx, dn, s));
+    HIPCHK(hipMemcpyAsync(nlm, dn, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return SFG_OK;
 }

@@ -270,6 +270,45 @@ def nlm_isvalid(nhat20, nhat40):
     return ok.astype(bool).reshape(n20.shape)
 
 
+# ---- discrete grains ----
+
+def ri_LROT(ri0, dt, Nt, eps, omg, iota):
+    """ri (Nt, N, 3): Euler steps of the grain axes ri0 (N, 3) under the
+    discrete lattice rotation with strain rates eps (Nt, 3, 3) and spins
+    omg (Nt, 3, 3), src/specfabpy.f90:271-280 -> dynamics.f90:112-137 (on
+    the GPU)."""
+    r0 = np.ascontiguousarray(np.asarray(ri0, dtype=np.float64))
+    if r0.ndim != 2 or r0.shape[1] != 3:
+        raise ValueError("ri0 must be (N, 3)")
+    Nt = int(Nt)
+    if Nt < 1:
+        raise ValueError("Nt must be >= 1")
+    D = np.ascontiguousarray(np.asarray(eps, dtype=np.float64))
+    W = np.ascontiguousarray(np.asarray(omg, dtype=np.float64))
+    if D.ndim != 3 or D.shape[1:] != (3, 3) or D.shape[0] < Nt - 1 or W.shape != D.shape:
+        raise ValueError("eps and omg must be (>= Nt - 1, 3, 3) and alike")
+    D = np.ascontiguousarray(D[:Nt]) if D.shape[0] >= Nt else np.vstack([D, np.zeros((1, 3, 3))])
+    W = np.ascontiguousarray(W[:Nt]) if W.shape[0] >= Nt else np.vstack([W, np.zeros((1, 3, 3))])
+    out = np.zeros((Nt, r0.shape[0], 3))
+    check(lib().sfg_ri_LROT(_ctx(), r0.shape[0], Nt, dptr(r0), float(dt), dptr(D), dptr(W), float(iota),
+                            dptr(out)))
+    return out
+
+
+def ri_to_nlm(ri, wi, L):
+    """nlm ((L+1)(L+2)/2,) of the weighted grain-axis ensemble ri (N, 3),
+    wi (N,), from its a2 (L = 2), a4 (L = 4) or a6 (L >= 6) moment,
+    src/specfabpy.f90:676-684 -> moments.f90:98-131 (on the GPU)."""
+    r = np.ascontiguousarray(np.asarray(ri, dtype=np.float64))
+    w = np.ascontiguousarray(np.asarray(wi, dtype=np.float64).ravel())
+    if r.ndim != 2 or r.shape[1] != 3 or w.size != r.shape[0]:
+        raise ValueError("ri must be (N, 3) and wi (N,)")
+    n = check_n(lib().sfg_nlm_len(int(L)))
+    out = np.zeros(n, dtype=np.complex128)
+    check(lib().sfg_ri_to_nlm(_ctx(), int(L), r.shape[0], dptr(r), dptr(w), _zp(out)))
+    return out
+
+
 def Cij_to_Lame_tranisotropic(Cij):
     """(C11, C33, C55, C12, C13) -> (lam, mu, Elam, Emu, Egam), src/elasticities.f90:107-120."""
     C = np.ascontiguousarray(np.asarray(Cij, dtype=np.float64).reshape(5))
