@@ -295,7 +295,8 @@ int sfg_state_isvalid_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev,
  * r' = (W + iota (r r^T D - D r r^T)) r, each step renormalised to |r| = 1.
  * ri0 (N, 3), D and W (Nt, 3, 3) row-major per step (step Nt - 1 unused),
  * ri (Nt, N, 3) with ri[0] = ri0 -- the reference's ri(Nt, N, 3) in C
- * order.  Host arrays, or device arrays on the stream (_batch). */
+ * order.  Host arrays, or device arrays on the stream (_batch); ri_dev must
+ * not overlap D_dev or W_dev. */
 int sfg_ri_LROT(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0, double dt, const double* D, const double* W,
                 double iota, double* ri);
 int sfg_ri_LROT_batch(sfg_ctx* ctx, int64_t N, int Nt, const double* ri0_dev, double dt, const double* D_dev,

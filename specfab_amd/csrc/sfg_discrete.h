@@ -20,7 +20,8 @@ struct RiLrotArgs {
     double* ri;             // (Nt, N, 3) trajectory, ri[0] = ri0
 };
 hipError_t launch_ri_lrot(const RiLrotArgs& a, hipStream_t s);
-__global__ void ri_lrot_kernel(RiLrotArgs a);
+__global__ void ri_lrot_kernel(int64_t N, int Nt, double dt, double iota, const double* __restrict__ ri0,
+                               const double* __restrict__ D, const double* __restrict__ W, double* __restrict__ ri);
 
 struct RiMomentArgs {
     int64_t E, N;           // ensembles, grains per ensemble
@@ -31,6 +32,5 @@ struct RiMomentArgs {
     int n;                  // rows of x (order 6)
 };
 hipError_t launch_ri_moments(const RiMomentArgs& a, hipStream_t s);
-__global__ void ri_moments_kernel(RiMomentArgs a);
 
 }  // namespace sfg

@@ -186,6 +186,23 @@ module specfab_gpu
             integer(c_int64_t), value :: P
         end function
         ! HIP runtime (device buffers for the batched path)
+        integer(c_int) function c_ri_LROT(c, N, Nt, ri0, dt, D, W, iota, ri) bind(C, name="sfg_ri_LROT")
+            import :: c_ptr, c_int, c_int64_t, c_double
+            type(c_ptr), value :: c
+            integer(c_int64_t), value :: N
+            integer(c_int), value :: Nt
+            real(c_double), intent(in) :: ri0(*), D(*), W(*)
+            real(c_double), value :: dt, iota
+            real(c_double), intent(out) :: ri(*)
+        end function
+        integer(c_int) function c_ri_to_nlm(c, L, N, ri, wi, nlm) bind(C, name="sfg_ri_to_nlm")
+            import :: c_ptr, c_int, c_int64_t, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            integer(c_int64_t), value :: N
+            real(c_double), intent(in) :: ri(*), wi(*)
+            complex(c_double_complex), intent(out) :: nlm(*)
+        end function
         integer(c_int) function hipMalloc(p, bytes) bind(C, name="hipMalloc")
             import :: c_int, c_ptr, c_size_t
             type(c_ptr) :: p
@@ -357,6 +374,40 @@ contains
         real(c_double) :: t(81)
         t = reshape(reshape(a4in, [3,3,3,3], order=[4,3,2,1]), [81])   ! Fortran -> C row-major
         call check(c_a4_to_nlm(ctx, t, nlm), 'a4_to_nlm')
+    end function
+
+    ! ri_LROT(ri0, dt, Nt, D, W, iota): src/dynamics.f90:125-137 -- ri0(N,3),
+    ! D(Nt,3,3) and W(Nt,3,3) as the reference takes them, ri(Nt,N,3)
+    function ri_LROT(ri0, dt, Nt, D, W, iota) result(ri)
+        real(dp), intent(in) :: ri0(:,:), dt, D(:,:,:), W(:,:,:), iota
+        integer, intent(in) :: Nt
+        real(dp) :: ri(Nt, size(ri0, 1), 3)
+        real(c_double), allocatable :: r0(:,:), Dc(:,:,:), Wc(:,:,:), out(:,:,:)
+        integer :: N, t, i, j
+        N = size(ri0, 1)
+        allocate(r0(3, N), Dc(3, 3, Nt), Wc(3, 3, Nt), out(3, N, Nt))
+        r0 = transpose(ri0)                       ! C row-major (N, 3)
+        Dc = 0; Wc = 0
+        do t = 1, min(Nt, size(D, 1)); do i = 1, 3; do j = 1, 3
+            Dc(j, i, t) = D(t, i, j)               ! C row-major (Nt, 3, 3)
+            Wc(j, i, t) = W(t, i, j)
+        end do; end do; end do
+        call check(c_ri_LROT(ctx, int(N, c_int64_t), int(Nt, c_int), r0, dt, Dc, Wc, iota, out), 'ri_LROT')
+        do t = 1, Nt; do i = 1, N; do j = 1, 3
+            ri(t, i, j) = out(j, i, t)
+        end do; end do; end do
+    end function
+
+    ! ri_to_nlm(ri, wi, L): src/moments.f90:98-131 -- ri(N,3), wi(N)
+    function ri_to_nlm(ri, wi, L) result(nlm)
+        real(dp), intent(in) :: ri(:,:), wi(:)
+        integer, intent(in) :: L
+        complex(dp) :: nlm((L+1)*(L+2)/2)
+        real(c_double), allocatable :: r(:,:), w(:)
+        allocate(r(3, size(ri, 1)), w(size(wi)))
+        r = transpose(ri)
+        w = wi
+        call check(c_ri_to_nlm(ctx, int(L, c_int), int(size(ri, 1), c_int64_t), r, w, nlm), 'ri_to_nlm')
     end function
 
     ! Gamma0(eps, T, A, Q): src/dynamics.f90:451-472

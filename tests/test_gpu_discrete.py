@@ -9,9 +9,14 @@ restatement (oracle/discrete.py):
   * ri_to_nlm (src/moments.f90:98-131): L = 2, 4 (a2 / a4 branches), 6, 8,
     10 (a6 branch, rows above l = 6 zero) equal the reference to 1e-13
     relative to the largest coefficient; single grains equal conj(Y_lm);
-    FabricBatch.set_grains on 4096 ensembles of 1000 grains equals the
-    restatement (a6 branch) and the library's own a2_to_nlm / a4_to_nlm of
-    the restated moments (a2 / a4 branches); an empty ensemble gives the
+    FabricBatch.set_grains on 4096 x 1000, 8192 x 100 and 2048 x 1024
+    ensembles x grains (32, 4 and 64 lanes per ensemble), 1024 x 1024 and
+    16 x 20000 (a workgroup per ensemble) equals the restatement (a6 branch)
+    and the
+    library's own a2_to_nlm /
+    a4_to_nlm of the restated moments (a2 / a4 branches); odd N, partial
+    waves and an unaligned output take the per-lane store path of ri_LROT
+    and equal the restatement; an empty ensemble gives the
     reference's output for zero moments; L < 2 and odd L are rejected.
 """
 import numpy as np
@@ -62,6 +67,37 @@ def test_ri_LROT_many_grains(gpu_ctx):
     assert np.abs(ri[:, torch.from_numpy(sample).cuda()].cpu().numpy() - ref).max() < 1e-13
 
 
+def test_ri_LROT_ragged(gpu_ctx):
+    """N odd (every other trajectory row starts 8 bytes off a 16-byte
+    boundary), a partial last wave, and an output slice that starts 8 bytes
+    off: the per-lane store path, against the restatement."""
+    import torch
+    from specfab_amd import specfabpy as sf
+    rng = np.random.default_rng(23)
+    g = load_golden("discrete.npz")
+    for N in (1, 63, 65, 1001):
+        r0 = rng.normal(size=(N, 3))
+        r0 /= np.linalg.norm(r0, axis=1, keepdims=True)
+        ref = O.ri_LROT(r0, 0.05, 9, g["lrot_D"][:9], g["lrot_W"][:9], 1.0)
+        got = sf.ri_LROT(r0, 0.05, 9, g["lrot_D"][:9], g["lrot_W"][:9], 1.0)
+        assert np.abs(got - ref).max() < 1e-14, N
+    # device output 8 bytes past a 16-byte boundary (through the C-ABI)
+    from specfab_amd._lib import check, lib
+    from specfab_amd.batch import _ptr, _stream
+    N, Nt = 256, 5
+    r0 = rng.normal(size=(N, 3))
+    r0 /= np.linalg.norm(r0, axis=1, keepdims=True)
+    buf = torch.zeros(Nt * N * 3 + 1, dtype=torch.float64, device="cuda")
+    D = torch.from_numpy(np.ascontiguousarray(g["lrot_D"][:Nt])).cuda()
+    Wt = torch.from_numpy(np.ascontiguousarray(g["lrot_W"][:Nt])).cuda()
+    r0d = torch.from_numpy(r0).cuda()
+    check(lib().sfg_ri_LROT_batch(gpu_ctx.handle, N, Nt, _ptr(r0d), 0.05, _ptr(D), _ptr(Wt), 1.0, _ptr(buf[1:]),
+                                  _stream()))
+    ref = O.ri_LROT(r0, 0.05, Nt, g["lrot_D"][:Nt], g["lrot_W"][:Nt], 1.0)
+    assert float(buf[0]) == 0.0
+    assert np.abs(buf[1:].cpu().numpy().reshape(Nt, N, 3) - ref).max() < 1e-14
+
+
 def test_ri_to_nlm_matches_reference(gpu_ctx):
     from specfab_amd import specfabpy as sf
     g = load_golden("discrete.npz")
@@ -87,7 +123,19 @@ def test_set_grains_batched(gpu_ctx):
     wi /= wi.sum()
     Rd = torch.from_numpy(ri).cuda()
     sample = [0, 1, 777, E - 1]
-    for L in (2, 4, 6, 12):
+    # lanes per ensemble: 32 (4096 x 1000), 4 (8192 x 100), 64 (2048 x 1024);
+    # a workgroup per ensemble (1024 x 1024, 16 x 20000)
+    cases = ([(L, E, N) for L in (2, 4, 6, 12)] + [(L, 8192, 100) for L in (2, 4, 6, 8)] + [(6, 2048, 1024)]
+             + [(6, 1024, 1024)] + [(L, 16, 20000) for L in (2, 4, 6)])
+    for L, E_, N_ in cases:
+        if (E_, N_) != (E, N):
+            ri = rng.normal(size=(E_, N_, 3))
+            ri /= np.linalg.norm(ri, axis=-1, keepdims=True)
+            wi = rng.uniform(0.5, 1.0, N_)
+            wi /= wi.sum()
+            Rd = torch.from_numpy(ri).cuda()
+            E, N = E_, N_
+            sample = sorted({0, 1, min(777, E - 2), E - 1})
         b = FabricBatch(L, P=E)
         b.set_grains(Rd, wi)
         got = b.nlm().cpu().numpy()

@@ -115,3 +115,30 @@ def test_fortran_homog_faces(tmp_path):
     assert relerr(a4n, g["a4_nlm"][0]) < 1e-14
     assert abs(G - g["g_G"][0]) < 1e-14 * g["g_G"][0]
     assert relerr(bnd, g["b_out"][kb]) < 1e-14
+
+
+EXE3 = os.path.join(ROOT, "specfab_amd", "fortran", "_build", "discrete_faces")
+
+
+def test_fortran_discrete_faces(tmp_path):
+    """Fortran host code calling ri_LROT and ri_to_nlm of the specfab_gpu
+    module with the reference's argument shapes (ri0(N,3), D(Nt,3,3),
+    ri(Nt,N,3)), against the reference fixture discrete.npz: 1e-13."""
+    _need(EXE3)
+    g = load_golden("discrete.npz")
+    fmt = lambda a: " ".join(repr(float(v)) for v in np.ravel(a))
+    N, Nt = g["lrot_ri0"].shape[0], g["lrot_ri"].shape[1]
+    lines = ["%d %d %r %r" % (N, Nt, float(g["lrot_dt"]), float(g["lrot_iota"][0]))]
+    lines += [fmt(r) for r in g["lrot_ri0"]]
+    lines += [fmt(d) for d in g["lrot_D"]] + [fmt(w) for w in g["lrot_W"]]
+    e, L = 1, 8
+    lines += ["%d %d" % (len(g["r2n_wi"][e]), L)] + [fmt(r) for r in g["r2n_ri"][e]] + [fmt(g["r2n_wi"][e])]
+    f = tmp_path / "in.txt"
+    f.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([EXE3, str(f)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [l.split() for l in out.stdout.splitlines()]
+    ri = np.array([[float(x) for x in r[2:5]] for r in rows if r[0] == "ri"])
+    nlm = np.array([float(r[2]) + 1j * float(r[3]) for r in rows if r[0] == "nlm"])
+    assert np.abs(ri - g["lrot_ri"][0, -1]).max() < 1e-13
+    assert relerr(nlm, g["r2n_nlm%d" % L][e]) < 1e-13
