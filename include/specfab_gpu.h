@@ -217,7 +217,11 @@ int sfg_a4mandel_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, doub
  * eigenvalues -> lami_dev (P,3), eigenvectors -> ei_dev (P,3,3) rows e1,e2,e3,
  * largest eigenvalue first), 1 = caller frames frames_dev (P,3,3),
  * 2 = Cartesian (x, y, z).  Eij_grain = (Ecc, Eca), alpha = Taylor weight,
- * n_grain = 1 or -3.  status_dev bits: SFG_ST_TAYLOR_REG / _FAIL / _NAN. */
+ * n_grain = 1, -3 or 3 (n' = 3 Sachs, src/homogenizations.f90:93-103: needs
+ * L >= 8; its a6 / a8 terms use the exact sixth and eighth moments, the
+ * reference's REAL(4)-literal bodies agree with them to ~1e-7; the Taylor
+ * part keeps the n' = 1 form with n' = 3 parameters, as the reference's).
+ * status_dev bits: SFG_ST_TAYLOR_REG / _FAIL / _NAN. */
 int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame,
                                 const double* frames_dev, const double Eij_grain[2], double alpha, int n_grain,
                                 double* Eij_dev, double* lami_dev, double* ei_dev, int32_t* status_dev, void* stream);
@@ -232,6 +236,25 @@ int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e
 int sfg_a2(sfg_ctx* ctx, int L, const double* nlm, double a2[9]);
 /* <D> = ev_D2(nlm, tau), src/dynamics.f90:402-422 */
 int sfg_ev_D2(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], double* D);
+/* <D> = ev_D(nlm, tau, n_RSS) (src/dynamics.f90:385-400; specfabpy
+ * deformability_tranisotropic(_arr), src/specfabpy.f90:557-578): the average
+ * basal resolved shear stress to the power n_RSS = 2 (ev_D2) or 4 (ev_D4,
+ * src/dynamics.f90:424-449, from a4 and the a6 / a8 moments; L >= 4, the rows
+ * a state lacks count as zero), normalised by its isotropic value.  Single:
+ * host nlm, tau.  Batched: states x_dev (n, P), per-parcel tau_dev (P, 3, 3)
+ * -> D_dev (P). */
+int sfg_ev_D(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], int n_RSS, double* D);
+int sfg_ev_D_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* tau_dev, int n_RSS,
+                   double* D_dev, void* stream);
+/* E_CAFFE(nlm, eps, Emin, Emax, n_grain) (src/enhancementfactors.f90:302-332;
+ * specfabpy E_CAFFE(_arr), src/specfabpy.f90:461-470,543-555): the CAFFE
+ * enhancement factor of the deformability <D> of the strain rate eps,
+ * n_RSS = 4 for n_grain = 3 and 2 otherwise.  Batched: eps_dev (P, 3, 3)
+ * -> E_dev (P). */
+int sfg_E_CAFFE(sfg_ctx* ctx, int L, const double* nlm, const double eps[9], double Emin, double Emax, int n_grain,
+                double* E);
+int sfg_E_CAFFE_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* eps_dev, double Emin,
+                      double Emax, int n_grain, double* E_dev, void* stream);
 
 /* ---- full fourth-order structure tensor and wave velocities ---- */
 

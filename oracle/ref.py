@@ -57,6 +57,9 @@ class RefLib:
         L.sfref_rotate_nlm.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
         L.sfref_a4_eigentensors.argtypes = [_cp, _dp, _dp]
         L.sfref_nlm_isvalid.argtypes = [_c.c_int, _cp, _cp, np.ctypeslib.ndpointer(dtype=np.int32)]
+        L.sfref_ev_ck.argtypes = [_cp, _dp, _dp, _dp, _dp]
+        L.sfref_ev_D.argtypes = [_cp, _dp, _c.c_int, _dp]
+        L.sfref_E_CAFFE.argtypes = [_cp, _dp, _c.c_double, _c.c_double, _c.c_int, _dp]
         L.sfref_ri_LROT.argtypes = [_c.c_int, _c.c_int, _dp, _c.c_double, _dp, _dp, _c.c_double, _dp]
         L.sfref_ri_to_nlm.argtypes = [_c.c_int, _c.c_int, _dp, _dp, _cp]
         L.sfref_evolve.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
@@ -210,6 +213,23 @@ class RefLib:
         ok = np.zeros(len(n20), dtype=np.int32)
         self.lib.sfref_nlm_isvalid(len(n20), n20, n40, ok)
         return ok.astype(bool)
+
+    def ev_ck(self, nlm):
+        """(a2, a4, a6, a8) full tensors of f_ev_ck(nlm, 'f') (L >= 8)"""
+        out = [np.zeros((3,) * k, order="F") for k in (2, 4, 6, 8)]
+        self.lib.sfref_ev_ck(self._v(nlm, np.complex128), *out)
+        return tuple(np.ascontiguousarray(o) for o in out)
+
+    def ev_D(self, nlm, tau, n_rss):
+        out = np.zeros(1, order="F")
+        self.lib.sfref_ev_D(self._v(nlm, np.complex128), self._m3(tau), int(n_rss), out)
+        return float(out[0])
+
+    def E_CAFFE(self, nlm, eps, Emin, Emax, n_grain):
+        out = np.zeros(1, order="F")
+        self.lib.sfref_E_CAFFE(self._v(nlm, np.complex128), self._m3(eps), float(Emin), float(Emax), int(n_grain),
+                               out)
+        return float(out[0])
 
     def ri_LROT(self, ri0, dt, Nt, D, W, iota):
         """ri0 (N, 3), D and W (Nt, 3, 3) -> ri (Nt, N, 3)"""

@@ -279,6 +279,34 @@ contains
         ok = merge(1, 0, v)
     end subroutine
 
+    ! ---- n' = 3 structure tensors, deformability, CAFFE ----
+
+    ! f_ev_ck(nlm, 'f', ...): a2(3,3), a4(3,3,3,3), a6(3^6), a8(3^8) (column-major)
+    subroutine sfref_ev_ck(nlm, a2, a4, a6, a8) bind(C, name="sfref_ev_ck")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(out) :: a2(3,3), a4(3,3,3,3), a6(3,3,3,3,3,3), a8(3,3,3,3,3,3,3,3)
+        call f_ev_ck(nlm, 'f', a2, a4, a6, a8)
+    end subroutine
+
+    ! ev_D(nlm, tau, n_RSS)
+    subroutine sfref_ev_D(nlm, tau, n, D) bind(C, name="sfref_ev_D")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: tau(3,3)
+        integer(c_int), value :: n
+        real(c_double), intent(out) :: D
+        D = ev_D(nlm, tau, n)
+    end subroutine
+
+    ! E_CAFFE(nlm, eps, Emin, Emax, n_grain)
+    subroutine sfref_E_CAFFE(nlm, eps, Emin, Emax, ng, E) bind(C, name="sfref_E_CAFFE")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: eps(3,3)
+        real(c_double), value :: Emin, Emax
+        integer(c_int), value :: ng
+        real(c_double), intent(out) :: E
+        E = E_CAFFE(nlm, eps, Emin, Emax, ng)
+    end subroutine
+
     ! ---- discrete grains ----
 
     ! ri_LROT(ri0, dt, Nt, D, W, iota): ri0(N,3), D/W(Nt,3,3) -> ri(Nt,N,3)

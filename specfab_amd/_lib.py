@@ -103,6 +103,10 @@ SIGNATURES = {
     "sfg_a4_eigentensors_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp]),
     "sfg_nlm_isvalid": (_i, [_vp, _i64, _dp, _dp, _c.POINTER(_c.c_int32)]),
     "sfg_state_isvalid_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
+    "sfg_ev_D": (_i, [_vp, _i, _dp, _dp, _i, _dp]),
+    "sfg_ev_D_batch": (_i, [_vp, _i, _i64, _vp, _vp, _i, _vp, _vp]),
+    "sfg_E_CAFFE": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _dp]),
+    "sfg_E_CAFFE_batch": (_i, [_vp, _i, _i64, _vp, _vp, _d, _d, _i, _vp, _vp]),
     "sfg_ri_LROT": (_i, [_vp, _i64, _i, _dp, _d, _dp, _dp, _d, _dp]),
     "sfg_ri_LROT_batch": (_i, [_vp, _i64, _i, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "sfg_ri_to_nlm": (_i, [_vp, _i, _i64, _dp, _dp, _dp]),

@@ -285,10 +285,30 @@ class FabricBatch:
         check(lib().sfg_a4mandel_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(out), _stream()))
         return out
 
+    # ---- deformability, CAFFE ----
+    def deformability(self, tau, n_RSS=2):
+        """<D> (P,) of per-parcel stresses tau (P, 3, 3), n_RSS = 2 or 4:
+        ev_D, src/dynamics.f90:385-449."""
+        T = self._f64(tau, (self.P, 3, 3))
+        D = torch.empty(self.P, dtype=torch.float64, device=self.dev)
+        check(lib().sfg_ev_D_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(T), int(n_RSS), _ptr(D),
+                                   _stream()))
+        return D
+
+    def E_CAFFE(self, eps, Emin, Emax, n_grain=1):
+        """CAFFE enhancement factors (P,) of per-parcel strain rates eps
+        (P, 3, 3): E_CAFFE, src/enhancementfactors.f90:302-332."""
+        T = self._f64(eps, (self.P, 3, 3))
+        E = torch.empty(self.P, dtype=torch.float64, device=self.dev)
+        check(lib().sfg_E_CAFFE_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), _ptr(T), float(Emin),
+                                      float(Emax), int(n_grain), _ptr(E), _stream()))
+        return E
+
     # ---- homogenization ----
     def Eij(self, Eij_grain=(1.0, 1e3), alpha=0.0125, n_grain=1, frame="eig", frames=None, status=None):
         """Enhancement factors (P, 6) of transversely isotropic grains
-        (Eij_tranisotropic, src/enhancementfactors.f90:23-45) in the a2
+        (Eij_tranisotropic, src/enhancementfactors.f90:23-45; n_grain = 1, -3
+        or 3, the last with L >= 8) in the a2
         eigenframe ("eig", returns also eigenvalues (P,3) and eigenvectors
         (P,3,3)), the Cartesian frame ("xyz"), or caller frames (P,3,3)."""
         mode = {"eig": 0, "frames": 1, "xyz": 2}[frame if frames is None else "frames"]

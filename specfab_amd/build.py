@@ -36,6 +36,7 @@ def gen_tables(force=False):
     if force or not os.path.exists(inc):
         run([sys.executable, os.path.join(HERE, "tables", "make_gaunt.py"), inc])
     run([sys.executable, os.path.join(HERE, "tables", "make_a6_map.py"), os.path.join(GEN, "sfg_a6_map.h")])
+    run([sys.executable, os.path.join(HERE, "tables", "make_moment_maps.py"), os.path.join(GEN, "sfg_moments68.h")])
     gen_fast = os.path.join(HERE, "tables", "gen_fast.py")
     if os.path.exists(gen_fast):
         run([sys.executable, gen_fast, GEN])

@@ -1076,7 +1076,9 @@ static int eij_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev,
                             const double Eg[2], double alpha, int n_grain, double* Eij_dev, double* lami_dev,
                             double* ei_dev, int32_t* status_dev, hipStream_t s) {
     if (L < 4) return fail(SFG_ERR_L, "Eij needs the l <= 4 coefficients (L >= 4)");
-    if (n_grain != 1 && n_grain != -3) return fail(SFG_ERR_ARG, "n_grain must be 1 or -3 (n'=3 needs a6/a8)");
+    if (n_grain != 1 && n_grain != -3 && n_grain != 3) return fail(SFG_ERR_ARG, "n_grain must be 1, -3 or 3");
+    if (n_grain == 3 && L < 8)   // src/homogenizations.f90:94-96
+        return fail(SFG_ERR_L, "Sachs homogenization with n'=3 requires L >= 8");
     if (frame < 0 || frame > 2 || (frame == 1 && !frames_dev)) return fail(SFG_ERR_ARG, "bad frame selection");
     const DevOp* op;
     int rc = get_op(ctx, L, &op);
@@ -1086,14 +1088,31 @@ static int eij_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev,
     a.Ecc = Eg[0]; a.Eca = Eg[1]; a.alpha = alpha;
     a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4; a.frames = frames_dev;
     a.Eij = Eij_dev; a.lami = lami_dev; a.ei = ei_dev; a.status = status_dev;
-    // one flag per wavefront of 64 rows for the regularized second pass
-    void* ws;
-    WsLease lease(ctx, s);
-    rc = lease.acquire((size_t)((P + 63) / 64) * sizeof(int), &ws);
-    if (rc) return rc;
-    a.wave_flag = (int*)ws;
-    HIPCHK(sfg::launch_eij(a, s));
-    return lease.release();
+    if (n_grain != 3 || alpha > 0) {
+        // one flag per wavefront of 64 rows for the regularized second pass
+        void* ws;
+        WsLease lease(ctx, s);
+        rc = lease.acquire((size_t)((P + 63) / 64) * sizeof(int), &ws);
+        if (rc) return rc;
+        a.wave_flag = (int*)ws;
+        if (n_grain == 3) a.alpha = 1.0;   // the Taylor values (n' = 3 parameters), mixed in below
+        HIPCHK(sfg::launch_eij(a, s));
+        rc = lease.release();
+        if (rc) return rc;
+        a.wave_flag = nullptr;
+        a.alpha = alpha;
+    }
+    if (n_grain == 3) {
+        // n' = 3 Sachs (src/homogenizations.f90:93-103) and the mix with the Taylor values
+        sfg::Sachs3Args b{};
+        b.A4F = op->A4F;
+        b.nrows = op->n;
+        double m[15][15];
+        sfg::a4full_map(m);
+        for (int e = 0; e < 15; e++) b.u_iso[e] = m[e][0];   // the isotropic state x = (x0, 0, ...)
+        HIPCHK(sfg::launch_eij_sachs3(a, b, s));
+    }
+    return SFG_OK;
 }
 
 int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int frame,
@@ -1153,6 +1172,82 @@ int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e
 }
 
 int sfg_M_REG(sfg_ctx* ctx, int L, const double D[9], double* M) { return diag_single(ctx, L, 1, D, M); }
+
+// ---------------------------------------------------------------- deformability, CAFFE
+
+namespace {
+// <D> of P states with per-parcel tau_dev (P, 3, 3): n_RSS = 2 (ev_D2) or 4 (ev_D4)
+int ev_D_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* tau_dev, int n_rss,
+                double* D_dev, hipStream_t s) {
+    if (n_rss != 2 && n_rss != 4) return fail(SFG_ERR_ARG, "n_RSS must be 2 or 4 (src/dynamics.f90:385-400)");
+    if (L < 4) return fail(SFG_ERR_L, "<D> needs the l <= 4 coefficients (L >= 4)");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (n_rss == 2) HIPCHK(sfg::launch_evD2(P, op->nq, op->A2, op->A4, x_dev, tau_dev, D_dev, s));
+    else HIPCHK(sfg::launch_ev_D4(P, op->n, op->A4F, x_dev, tau_dev, D_dev, s));
+    return SFG_OK;
+}
+
+// one host nlm and host tau through the batched path; caffe: E_CAFFE of <D>
+int ev_D_single(sfg_ctx* ctx, int L, const double* nlm, const double* tau, int n_rss, bool caffe, double Emin,
+                double Emax, double* out) {
+    if (!ctx || !nlm || !tau || !out) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(3 * n + 9 + 1) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* dt = dx + n;
+    double* dD = dt + 9;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dt, tau, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    rc = ev_D_locked(ctx, L, 1, dx, dt, n_rss, dD, s);
+    if (rc) return rc;
+    if (caffe) HIPCHK(sfg::launch_caffe(1, n_rss, Emin, Emax, dD, s));
+    HIPCHK(hipMemcpyAsync(out, dD, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+}  // namespace
+
+int sfg_ev_D(sfg_ctx* ctx, int L, const double* nlm, const double tau[9], int n_RSS, double* D) {
+    return ev_D_single(ctx, L, nlm, tau, n_RSS, false, 0.0, 0.0, D);
+}
+
+int sfg_ev_D_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* tau_dev, int n_RSS,
+                   double* D_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !tau_dev || !D_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    return ev_D_locked(ctx, L, P, x_dev, tau_dev, n_RSS, D_dev, pick(ctx, stream));
+}
+
+int sfg_E_CAFFE(sfg_ctx* ctx, int L, const double* nlm, const double eps[9], double Emin, double Emax, int n_grain,
+                double* E) {
+    return ev_D_single(ctx, L, nlm, eps, n_grain == 3 ? 4 : 2, true, Emin, Emax, E);
+}
+
+int sfg_E_CAFFE_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, const double* eps_dev, double Emin,
+                      double Emax, int n_grain, double* E_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !eps_dev || !E_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    hipStream_t s = pick(ctx, stream);
+    const int n_rss = n_grain == 3 ? 4 : 2;   // src/enhancementfactors.f90:317-321
+    int rc = ev_D_locked(ctx, L, P, x_dev, eps_dev, n_rss, E_dev, s);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_caffe(P, n_rss, Emin, Emax, E_dev, s));
+    return SFG_OK;
+}
 
 // ---------------------------------------------------------------- full a4, wave velocities
 

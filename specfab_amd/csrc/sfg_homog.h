@@ -9,7 +9,7 @@ struct EijArgs {
     int64_t P;
     int nq;
     int frame;            // 0 = a2 eigenframe, 1 = given frames, 2 = Cartesian (x, y, z)
-    int n_grain;          // 1 or -3
+    int n_grain;          // 1 or -3 (3: Taylor part only, the Sachs part by eij_sachs3_kernel)
     double Ecc, Eca, alpha;
     const double* x;      // real SoA state (n, P)
     const double* A2;     // (6, 15)
@@ -37,5 +37,22 @@ struct EijArgs {
 };
 
 hipError_t launch_eij(const EijArgs& a, hipStream_t s);
+
+// n' = 3 Sachs (src/homogenizations.f90:93-103): the full-a4 map (15 x 15,
+// the reference body) and its isotropic entries, the state's row count
+struct Sachs3Args {
+    const double* A4F;
+    double u_iso[15];
+    int nrows;
+};
+// Eij with n' = 3 grains: a.alpha > 0 mixes with the Taylor values that
+// launch_eij (n_grain = 3, alpha = 1) left in a.Eij; the Sachs parameters are
+// filled here from (Ecc, Eca)
+hipError_t launch_eij_sachs3(const EijArgs& a, const Sachs3Args& b, hipStream_t s);
+// <D> with n_RSS = 4 (src/dynamics.f90:424-449): x (nrows, P), tau (P, 3, 3) -> out (P)
+hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x, const double* tau, double* out,
+                        hipStream_t s);
+// E_CAFFE (src/enhancementfactors.f90:302-332) of <D> (P), in place
+hipError_t launch_caffe(int64_t P, int n_rss, double Emin, double Emax, double* DE, hipStream_t s);
 
 }  // namespace sfg

@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "generated/sfg_moment_maps.h"
+#include "generated/sfg_moments68.h"
 #include "sfg_device.h"
 #include "sfg_homog.h"
 
@@ -538,6 +539,368 @@ __global__ void __launch_bounds__(256) eij_regularized_kernel(EijArgs a) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) eij_regularized_item(a, t);
 }
 
+
+// ---------------------------------------------------------------------------
+// n' = 3 Sachs homogenization (src/homogenizations.f90:93-103) and the
+// basal-RSS deformabilities <D> (n_RSS = 2, 4; src/dynamics.f90:385-449).
+//
+// The fluidity of an n' = 3 grain makes the grain average an expectation
+// of polynomials in the c-axis of degree <= 8.  a2 and a4 enter as the
+// reference's structure-tensor bodies (the full-a4 body with its REAL(4)
+// literals and its a4(3,2,1,2) entry); a6 and a8 only through products with
+// tau, evaluated as <c_i c_j P(c)> with P a polynomial in c built from the
+// quadratic forms c.tau.c and c.tau^2.c, against the exact sixth and eighth
+// moments (tables/make_moment_maps.py; the reference's a6 / a8 bodies carry
+// REAL(4) literal coefficients, so those terms agree with it to ~1e-7).
+
+__constant__ double kMOM6[28][28] = SFG_MOM6_INIT;
+__constant__ double kMOM8[45][45] = SFG_MOM8_INIT;
+__constant__ double kMOM6_ISO[28] = SFG_MOM6_ISO_INIT;
+__constant__ double kMOM8_ISO[45] = SFG_MOM8_ISO_INIT;
+
+namespace {
+
+// element k of a lane's column in LDS (stride 64: one column per lane of the
+// wavefront) or of a plain array (stride 1)
+template <int S>
+struct Col {
+    const double* p;
+    __device__ __forceinline__ double operator[](int k) const { return p[k * S]; }
+};
+
+// index of x^a y^b z^(d-a-b) among the degree-d monomials (a = d..0, b = d-a..0)
+__host__ __device__ constexpr int midx(int d, int a, int b) { return (d - a) * (d - a + 1) / 2 + (d - a - b); }
+
+// c.S.c for symmetric S (row-major) as a degree-2 polynomial
+__device__ __forceinline__ void quad_form(const double (&S)[9], double (&P)[6]) {
+    P[0] = S[0], P[1] = 2 * S[1], P[2] = 2 * S[2], P[3] = S[4], P[4] = 2 * S[5], P[5] = S[8];
+}
+
+// C = A B for homogeneous polynomials of degrees DA and DB
+template <int DA, int DB>
+__device__ __forceinline__ void poly_mul(const double* A, const double* B, double* C) {
+    constexpr int D = DA + DB, NC = (D + 1) * (D + 2) / 2;
+#pragma unroll
+    for (int k = 0; k < NC; k++) C[k] = 0.0;
+#pragma unroll
+    for (int a1 = DA; a1 >= 0; a1--)
+#pragma unroll
+        for (int b1 = DA - a1; b1 >= 0; b1--)
+#pragma unroll
+            for (int a2 = DB; a2 >= 0; a2--)
+#pragma unroll
+                for (int b2 = DB - a2; b2 >= 0; b2--)
+                    C[midx(D, a1 + a2, b1 + b2)] += A[midx(DA, a1, b1)] * B[midx(DB, a2, b2)];
+}
+
+// sorted-entry index (0..14) of a4(i,j,k,l); the reference's body sets
+// a4(3,2,1,2) to its a4(1,2,3,3) entry (src/include/ev_c4__body.f90)
+__host__ __device__ constexpr int a4_slot(int i, int j, int k, int l) {
+    if (i == 2 && j == 1 && k == 0 && l == 1) return 8;
+    int v[4] = {i, j, k, l};
+    for (int p = 0; p < 4; p++)
+        for (int q = p + 1; q < 4; q++)
+            if (v[q] < v[p]) { int t = v[p]; v[p] = v[q]; v[q] = t; }
+    int idx = 0;
+    for (int a0 = 0; a0 < 3; a0++)
+        for (int a1 = a0; a1 < 3; a1++)
+            for (int a2 = a1; a2 < 3; a2++)
+                for (int a3 = a2; a3 < 3; a3++) {
+                    if (a0 == v[0] && a1 == v[1] && a2 == v[2] && a3 == v[3]) return idx;
+                    idx++;
+                }
+    return -1;
+}
+
+// sum_kl a4(i,j,k,l) S_kl
+template <class U>
+__device__ __forceinline__ double a4_dot(const U& u, int i, int j, const double (&S)[9]) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) s += u[a4_slot(i, j, k, l)] * S[3 * k + l];
+    return s;
+}
+
+// the reference's full a2 (f_ev_c2 body) of the l <= 2 rows
+__device__ __forceinline__ void a2_full(const double* xq, double (&a2)[9]) {
+    const double n00 = xq[0];
+    const double h0 = xq[1] / n00, h1r = xq[2] / n00, h1i = xq[3] / n00, h2r = xq[4] / n00, h2i = xq[5] / n00;
+    const double c = sqrt(2.0 / 3), s = sqrt(2 / 15.0);
+    double e[9];
+    e[0] = -0.5 * c * h0 + h2r;
+    e[4] = -0.5 * c * h0 - h2r;
+    e[8] = c * h0;
+    e[1] = e[3] = -h2i;
+    e[2] = e[6] = -h1r;
+    e[5] = e[7] = h1i;
+#pragma unroll
+    for (int i = 0; i < 9; i++) a2[i] = s * e[i] + ((i % 4 == 0) ? 1.0 / 3.0 : 0.0);
+}
+
+// <A B> for homogeneous polynomials A (degree DA) and B (degree DB) against
+// the moments m of degree DA + DB
+template <int DA, int DB, class M>
+__device__ __forceinline__ double pdot(const double* A, const double* B, const M& m) {
+    double s = 0.0;
+#pragma unroll
+    for (int a1 = DA; a1 >= 0; a1--)
+#pragma unroll
+        for (int b1 = DA - a1; b1 >= 0; b1--) {
+            double t = 0.0;
+#pragma unroll
+            for (int a2 = DB; a2 >= 0; a2--)
+#pragma unroll
+                for (int b2 = DB - a2; b2 >= 0; b2--)
+                    t += B[midx(DB, a2, b2)] * m[midx(DA + DB, a1 + a2, b1 + b2)];
+            s += A[midx(DA, a1, b1)] * t;
+        }
+    return s;
+}
+
+// c.X.c for a general X (its symmetric part) as a degree-2 polynomial
+__device__ __forceinline__ void quad_form_gen(const double (&X)[9], double (&P)[6]) {
+    P[0] = X[0], P[1] = X[1] + X[3], P[2] = X[2] + X[6], P[3] = X[4], P[4] = X[5] + X[7], P[5] = X[8];
+}
+
+// eps(tau) : vw of the n' = 3 Sachs average (src/homogenizations.f90:93-103,
+// 115), as the reference contracts it: doubleinner22(eps, v w) = sum_ij
+// eps_ij w_i v_j (eps is not symmetric: a4's (3,2,1,2) entry).  With
+// Y = w v^T, Z = tau Y + Y tau and E2 = <eta c c>:
+//   eps : Y = c0 (tau : Y) - A (E2 : tau) tr Y + B (A4t : Y) + C (E2 : Z)
+// where <eta> = c0, E2 : X = I2 a2:X + B <(cXc) (c tau c)^2> + 2C a4[X]:tau^2,
+// A4t : Y = I2 a4[Y]:tau + B <(cYc) (c tau c)^3> + 2C <(cYc) (c tau c)(c tau^2 c)>
+template <class U, class M6, class M8>
+__device__ __forceinline__ double sachs3_vw(const double (&tau)[9], const double (&v)[3], const double (&w)[3],
+                                            const double (&a2)[9], const U& u, const M6& m6, const M8& m8,
+                                            double sA, double sB, double sC) {
+    double t2[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            t2[3 * i + j] = tau[3 * i] * tau[j] + tau[3 * i + 1] * tau[3 + j] + tau[3 * i + 2] * tau[6 + j];
+    double I2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) I2 += tau[i] * tau[i];
+    double P1[6], P3[6], P11[15];
+    quad_form(tau, P1);
+    quad_form(t2, P3);
+    poly_mul<2, 2>(P1, P1, P11);
+    double Y[9], Z[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Y[3 * i + j] = w[i] * v[j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double z = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) z += tau[3 * i + k] * Y[3 * k + j] + Y[3 * i + k] * tau[3 * k + j];
+            Z[3 * i + j] = z;
+        }
+    const double trY = v[0] * w[0] + v[1] * w[1] + v[2] * w[2];
+    // <eta>, tau : Y, a4[Y] : tau and the a2 / a4 parts of E2 : tau and E2 : Z
+    double c0 = 0.0, a2t2 = 0.0, tY = 0.0, a4Yt = 0.0, E2t = 0.0, E2Z = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int ij = 3 * i + j;
+            const double a4t = a4_dot(u, i, j, tau), a4t2 = a4_dot(u, i, j, t2);
+            c0 += a4t * tau[ij];
+            a2t2 += a2[ij] * t2[ij];
+            tY += tau[ij] * Y[ij];
+            a4Yt += Y[ij] * a4t;
+            E2t += tau[ij] * (I2 * a2[ij] + 2 * sC * a4t2);
+            E2Z += Z[ij] * (I2 * a2[ij] + 2 * sC * a4t2);
+        }
+    c0 = I2 + sB * c0 + 2 * sC * a2t2;
+    double PX[6];
+    quad_form(tau, PX);
+    E2t += sB * pdot<2, 4>(PX, P11, m6);
+    quad_form_gen(Z, PX);
+    E2Z += sB * pdot<2, 4>(PX, P11, m6);
+    double R[15];
+    quad_form_gen(Y, PX);
+    poly_mul<2, 2>(PX, P1, R);
+    const double A4tY = I2 * a4Yt + sB * pdot<4, 4>(R, P11, m8) + 2 * sC * pdot<4, 2>(R, P3, m6);
+    return c0 * tY - sA * E2t * trY + sB * A4tY + sC * E2Z;
+}
+
+// ev_D4 (src/dynamics.f90:424-449): 35/2 <RSS^4> / (tau:tau)^2 with
+// RSS^2 = c.tau^2.c - (c.tau.c)^2
+template <class U, class M6, class M8>
+__device__ __forceinline__ double ev_D4_of(const double (&tau)[9], const U& u, const M6& m6, const M8& m8) {
+    double t2[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            t2[3 * i + j] = tau[3 * i] * tau[j] + tau[3 * i + 1] * tau[3 + j] + tau[3 * i + 2] * tau[6 + j];
+    const double norm = t2[0] + t2[4] + t2[8];
+    double d = 0.0;                                          // tau^2 : a4 : tau^2
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) d += t2[3 * i + j] * a4_dot(u, i, j, t2);
+    double P1[6], P3[6], P11[15];
+    quad_form(tau, P1);
+    quad_form(t2, P3);
+    poly_mul<2, 2>(P1, P1, P11);
+    const double q4 = pdot<4, 4>(P11, P11, m8);   // <(c tau c)^4>
+    const double q6 = pdot<4, 2>(P11, P3, m6);    // <(c tau c)^2 (c tau^2 c)>
+    d = d + q4 - 2 * q6;
+    return 35 / 2.0 * d / (norm * norm);
+}
+
+// a parcel's l <= 8 rows (missing rows zero) into xq, and its sorted full-a4
+// entries and sixth and eighth moments into the lane's LDS columns
+// (lds[k * 64], k < 15: a4, then 28 sixth, then 45 eighth moments)
+constexpr int kColU = 0, kCol6 = 15, kCol8 = 43, kColDen = 88, kCols = 94;
+__device__ __forceinline__ void nonlin_moments(const double* x, int64_t P, int64_t p, int nrows, const double* A4F,
+                                               double (&xq)[45], double* lds) {
+#pragma unroll
+    for (int q = 0; q < 45; q++) xq[q] = q < nrows ? x[(int64_t)q * P + p] : 0.0;
+    const double ix0 = 1.0 / xq[0];
+#pragma unroll 1
+    for (int e = 0; e < 15; e++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 15; q++) s += A4F[e * 15 + q] * xq[q];
+        lds[(kColU + e) * 64] = s * ix0;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 28; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 28; q++) s += kMOM6[k][q] * xq[q];
+        lds[(kCol6 + k) * 64] = s * ix0;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 45; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 45; q++) s += kMOM8[k][q] * xq[q];
+        lds[(kCol8 + k) * 64] = s * ix0;
+    }
+}
+
+}  // namespace
+
+// Eij with n' = 3 Sachs: (1 - alpha) Sachs + alpha Taylor, the Taylor values
+// (alpha > 0) already in Eij from eij_kernel; writes the frame as eij_kernel
+__global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b) {
+    __shared__ double lds[kCols * 64];
+    __shared__ double iso[88];   // the isotropic state's a4 entries, sixth and eighth moments
+    for (int k = threadIdx.x; k < 88; k += blockDim.x)
+        iso[k] = k < 15 ? b.u_iso[k] : (k < 43 ? kMOM6_ISO[k - 15] : kMOM8_ISO[k - 43]);
+    __syncthreads();
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double* col = lds + threadIdx.x;
+    double xq[45];
+    nonlin_moments(a.x, a.P, p, b.nrows, b.A4F, xq, col);
+    const Col<64> u{col + kColU * 64}, m6{col + kCol6 * 64}, m8{col + kCol8 * 64};
+    const Col<1> ui{iso}, m6i{iso + 15}, m8i{iso + 43};
+    double a2v[6];
+    {
+        double xl[15], a4s[6][6];
+#pragma unroll
+        for (int q = 0; q < 15; q++) xl[q] = xq[q];
+        moment_maps(xl, a2v, a4s);
+        const double ix0 = 1.0 / xq[0];
+#pragma unroll
+        for (int i = 0; i < 6; i++) a2v[i] = a2v[i] * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
+    }
+    double e[9], lam[3];
+    eij_frame(a, p, a2v, e, lam);
+    double a2[9], a2i[9];
+    a2_full(xq, a2);
+#pragma unroll
+    for (int i = 0; i < 9; i++) a2i[i] = (i % 4 == 0) ? 1.0 / 3.0 : 0.0;
+    // the stress state q of the frame: tau_vv = I/3 - v v (q < 3), tau_vw = v w + w v
+    // (src/enhancementfactors.f90:398-413)
+    auto state = [&](int q, double (&v)[3], double (&w)[3], double (&tau)[9]) {
+        const int iv = q < 3 ? q : (q == 3 ? 1 : 0), iw = q < 3 ? q : (q == 5 ? 1 : 2);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            v[i] = iv == 0 ? e[i] : (iv == 1 ? e[3 + i] : e[6 + i]);
+            w[i] = iw == 0 ? e[i] : (iw == 1 ? e[3 + i] : e[6 + i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                tau[3 * i + j] = q < 3 ? (i == j ? 1.0 / 3.0 : 0.0) - v[i] * v[j] : v[i] * w[j] + w[i] * v[j];
+    };
+    // the isotropic normalisations first (their constants are not live in the
+    // second loop), then the parcel's own averages
+#pragma unroll 1
+    for (int q = 0; q < 6; q++) {
+        __asm__ volatile("" ::: "memory");   // reread the shared isotropic values in each pass
+        double v[3], w[3], tau[9];
+        state(q, v, w, tau);
+        col[(kColDen + q) * 64] = sachs3_vw(tau, v, w, a2i, ui, m6i, m8i, a.sA, a.sB, a.sC);
+    }
+    int st = 0;
+#pragma unroll 1
+    for (int q = 0; q < 6; q++) {
+        double v[3], w[3], tau[9];
+        state(q, v, w, tau);
+        const double num = sachs3_vw(tau, v, w, a2, u, m6, m8, a.sA, a.sB, a.sC);
+        const double es = num / col[(kColDen + q) * 64];
+        double* out = a.Eij + p * 6 + q;
+        const double E = a.alpha > 0 ? (1 - a.alpha) * es + a.alpha * *out : es;
+        *out = E;
+        if (!isfinite(E)) st |= 1;
+    }
+    if (a.lami)
+#pragma unroll
+        for (int i = 0; i < 3; i++) a.lami[p * 3 + i] = lam[i];
+    if (a.ei)
+#pragma unroll
+        for (int i = 0; i < 9; i++) a.ei[p * 9 + i] = e[i];
+    if (a.status && st) atomicOr(a.status + p, st);
+}
+
+// <D> with n_RSS = 4 per parcel (tau (P, 3, 3)), optionally mapped to the
+// CAFFE enhancement factor (caffe_kernel)
+__global__ void __launch_bounds__(64) ev_D4_kernel(int64_t P, int nrows, const double* A4F, const double* x,
+                                                   const double* tau, double* out) {
+    __shared__ double lds[kCols * 64];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double* col = lds + threadIdx.x;
+    double xq[45];
+    nonlin_moments(x, P, p, nrows, A4F, xq, col);
+    const Col<64> u{col + kColU * 64}, m6{col + kCol6 * 64}, m8{col + kCol8 * 64};
+    double t[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) t[i] = tau[p * 9 + i];
+    out[p] = ev_D4_of(t, u, m6, m8);
+}
+
+// E_CAFFE (src/enhancementfactors.f90:302-332) from <D> in place
+__global__ void __launch_bounds__(256) caffe_kernel(int64_t P, int n_rss, double Emin, double Emax, double* DE) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const double D = DE[p];
+    const double Dmax = n_rss == 2 ? 5 / 2.0 : 35 / 8.0;
+    const double Dmaxpow = pow(Dmax, 4.0 / n_rss);
+    double E;
+    if (D < 1) {
+        const double gam = (4.0 / n_rss) / Dmaxpow * (Emax - 1) / (1 - Emin);
+        E = Emin + (1 - Emin) * pow(D, gam);
+    } else {
+        E = ((Emax - 1) * pow(D, 4.0 / n_rss) + Dmaxpow - Emax) / (Dmaxpow - 1);
+    }
+    DE[p] = E;
+}
+
 hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
     const int64_t n = args.idx ? args.nidx : args.P;
     if (n == 0) return hipSuccess;
@@ -568,6 +931,32 @@ hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
     else
         hipLaunchKernelGGL(eij_regularized_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 2048)),
                            dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_eij_sachs3(const EijArgs& args, const Sachs3Args& b, hipStream_t s) {
+    if (args.P == 0) return hipSuccess;
+    EijArgs a = args;
+    // rheo_params_tranisotropic (src/rheologies.f90:123-132) of the n' = 3 Sachs branch
+    const double ne = 2.0 / (3.0 + 1);
+    const double pc = std::pow(a.Ecc, ne), pa = std::pow(a.Eca, ne);
+    a.sA = (pc - 1) / 2;
+    a.sB = (3 * (pc + 1) - 2) / 2 - 2 * pa;
+    a.sC = pa - 1;
+    hipLaunchKernelGGL(eij_sachs3_kernel, dim3((unsigned)((a.P + 63) / 64)), dim3(64), 0, s, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x, const double* tau, double* out,
+                        hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(ev_D4_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, s, P, nrows, A4F, x, tau, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_caffe(int64_t P, int n_rss, double Emin, double Emax, double* DE, hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(caffe_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, P, n_rss, Emin, Emax, DE);
     return hipGetLastError();
 }
 

@@ -185,6 +185,50 @@ def ev_D2(nlm, tau):
     return float(out[0])
 
 
+def deformability_tranisotropic(nlm, tau, n_RSS):
+    """<D> = ev_D(nlm, tau, n_RSS), the normalised average basal RSS^n_RSS
+    (n_RSS = 2 or 4): src/specfabpy.f90:557-565 -> dynamics.f90:385-449."""
+    L = _L_of(nlm)
+    out = np.zeros(1)
+    z = _z(nlm)
+    check(lib().sfg_ev_D(_ctx(), L, _zp(z), dptr(_m3(tau)), int(n_RSS), dptr(out)))
+    return float(out[0])
+
+
+def deformability_tranisotropic_arr(nlm, tau, n_RSS):
+    """<D> of every row of nlm (N, n) for one tau: src/specfabpy.f90:567-578,
+    evaluated in one GPU launch."""
+    import torch
+    from .batch import FabricBatch
+    nlm = np.asarray(nlm)
+    b = FabricBatch(_L_of(nlm[0]), nlm=nlm)
+    D = b.deformability(np.tile(_m3(tau), (len(nlm), 1, 1)), n_RSS)
+    torch.cuda.synchronize()
+    return D.cpu().numpy()
+
+
+def E_CAFFE(nlm, eps, Emin, Emax, n_grain):
+    """CAFFE enhancement factor (Placidi et al., 2010) of the strain rate eps:
+    src/specfabpy.f90:461-470 -> enhancementfactors.f90:302-332."""
+    L = _L_of(nlm)
+    out = np.zeros(1)
+    z = _z(nlm)
+    check(lib().sfg_E_CAFFE(_ctx(), L, _zp(z), dptr(_m3(eps)), float(Emin), float(Emax), int(n_grain), dptr(out)))
+    return float(out[0])
+
+
+def E_CAFFE_arr(nlm, eps, Emin, Emax, n_grain):
+    """E_CAFFE of nlm (N, n) with strain rates eps (N, 3, 3):
+    src/specfabpy.f90:543-555, evaluated in one GPU launch."""
+    import torch
+    from .batch import FabricBatch
+    nlm = np.asarray(nlm)
+    b = FabricBatch(_L_of(nlm[0]), nlm=nlm)
+    E = b.E_CAFFE(eps, Emin, Emax, n_grain)
+    torch.cuda.synchronize()
+    return E.cpu().numpy()
+
+
 def ugrad_to_D_and_W(ugrad):
     """src/deformationmodes.f90:29-35 (kinematics helper, host arithmetic only)."""
     G = np.asarray(ugrad, dtype=np.float64)
