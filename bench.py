@@ -425,6 +425,102 @@ def homogenization_block(fb, procs, precision, more_steps):
         return {"error": "%s: %s" % (type(e).__name__, e)}
 
 
+# FP64 flop per parcel of eij_sachs3_kernel: the a4 and eighth-moment maps
+# (2 x 2,250), 12 evaluations of the n' = 3 Sachs contraction (6 stress
+# states, the parcel's and the isotropic state's), the a2 maps and the
+# Jacobi eigenframe, as SQ_INSTS_VALU_FLOPS_FP64 counts them (x 64 lanes per
+# parcel: 24,064); ev_D4_kernel likewise (5,632)
+FLOPS_SACHS3 = 24064
+FLOPS_EVD4 = 5632
+
+
+def bench_nonlinear(a):
+    """Nonlinear-grain homogenization on the 2^20-parcel L = 12 field (the
+    headline field's synthetic states, unevolved): Eij with n' = 3 grains in
+    each parcel's a2 eigenframe (Sachs, src/homogenizations.f90:93-103, and
+    the Sachs/Taylor mix alpha = 0.0125), <D> with n_RSS = 4 and E_CAFFE
+    (n_grain = 3) of per-parcel strain rates."""
+    import torch
+    from specfab_amd import batch as B
+    dev = torch.device("cuda:0")
+    P, L = a.parcels, 12
+    from specfab_amd import _lib
+    G, gam0, x = make_field(P, L, 1, dev)
+    fb = B.FabricBatch.__new__(B.FabricBatch)
+    fb.ctx = _lib.default_context()
+    fb.L, fb.n, fb.dev, fb.P = L, (L + 1) * (L + 2) // 2, dev, P
+    fb.x, fb.ugrad, fb.tau, fb.gam0, fb.lam, fb.steps = x, G, None, gam0, None, None
+    fb.status = torch.zeros(P, dtype=torch.int32, device=dev)
+    D = ((G + G.transpose(1, 2)) / 2).contiguous()
+    Eg = (1.0, 1e3)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        for _ in range(max(1, a.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(a.steps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / a.steps
+
+    ms = {"eij3_sachs": timed(lambda: fb.Eij(Eg, 0.0, 3, frame="eig")),
+          "eij3_mix": timed(lambda: fb.Eij(Eg, 0.0125, 3, frame="eig")),
+          "ev_D4": timed(lambda: fb.deformability(D, 4)),
+          "caffe3": timed(lambda: fb.E_CAFFE(D, 0.1, 10.0, 3))}
+    assert int(fb.status.abs().sum()) == 0
+    ach = FLOPS_SACHS3 * P / (ms["eij3_sachs"] * 1e-3) / 1e12
+    out = {"metric": "Eij evals/s with n'=3 grains (Sachs, a2 eigenframe), FP64",
+           "value": P / (ms["eij3_sachs"] * 1e-3), "unit": "parcel-evals/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms["eij3_sachs"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
+           "config": {"workload": "nonlinear grains on the 2^20-parcel L=12 field: Eij n'=3 (eigenframe), "
+                                  "<D> n_RSS=4, E_CAFFE n'=3", "parcels": P},
+           "ms": ms, "evals_per_s": {k: P / (v * 1e-3) for k, v in ms.items()},
+           "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                        "frac": ach / PEAK_TFLOPS["fp64"], "traffic": None, "kernel": "eij_sachs3_kernel",
+                        "flops_per_parcel": FLOPS_SACHS3},
+           "roofline_ev_D4": {"bound": "valu", "achieved": FLOPS_EVD4 * P / (ms["ev_D4"] * 1e-3) / 1e12,
+                              "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                              "frac": FLOPS_EVD4 * P / (ms["ev_D4"] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+                              "kernel": "ev_D4_kernel", "flops_per_parcel": FLOPS_EVD4},
+           "cpu_baseline": None}
+    if not a.no_cpu:
+        from oracle import ref
+        if ref.available():
+            import numpy as np
+            r = ref.RefLib()
+            r.init(L)
+            X = (x[:, :256].T.cpu().numpy())
+            from specfab_amd import specfabpy as sf
+            nl = sf.init(L)[1]
+            # states as complex nlm for the reference
+            lm = [(l, m) for l in range(0, L + 1, 2) for m in range(-l, l + 1)]
+            Z = np.zeros((X.shape[0], nl), complex)
+            q = 0
+            pos = {v: i for i, v in enumerate(lm)}
+            for l in range(0, L + 1, 2):
+                Z[:, pos[(l, 0)]] = X[:, q]
+                q += 1
+                for m in range(1, l + 1):
+                    z = X[:, q] + 1j * X[:, q + 1]
+                    Z[:, pos[(l, m)]] = z
+                    Z[:, pos[(l, -m)]] = (-1) ** m * np.conj(z)
+                    q += 2
+            k, t0 = 0, time.perf_counter()
+            I = np.eye(3)
+            while time.perf_counter() - t0 < 10.0:
+                r.Eij_tranisotropic(Z[k % len(Z)], I[0], I[1], I[2], Eg, 0.0, 3)
+                k += 1
+            el = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": k / el, "unit": "parcel-evals/s", "cores": 1, "kind": "reference",
+                                   "sample": "%d Eij_tranisotropic(n'=3) calls of the reference FP64 Fortran "
+                                             "(oracle/_ref), %.1f s" % (k, el)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_discrete(a):
     """Discrete-grain (DDM) path of the reference's SDM-vs-DDM harness
     (tests/SDM-DDM-performance/SDM-DDM-performance.py:24-73): ri_LROT
@@ -527,10 +623,12 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="field", choices=["field", "uneven", "surrogate", "export", "discrete"],
+    ap.add_argument("--workload", default="field",
+                    choices=["field", "uneven", "surrogate", "export", "discrete", "nonlinear"],
                     help="field: the L=12 Eulerian field (default, the headline); uneven: 2^18 parcels with mixed "
                          "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs "
-                         "learned; export: operator export; discrete: ri_LROT / ri_to_nlm of 2^20 grains")
+                         "learned; export: operator export; discrete: ri_LROT / ri_to_nlm of 2^20 grains; "
+                         "nonlinear: n'=3 Eij, <D> and E_CAFFE on the field")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -560,6 +658,8 @@ def main():
         return bench_export(a)
     if a.workload == "discrete":
         return bench_discrete(a)
+    if a.workload == "nonlinear":
+        return bench_nonlinear(a)
     dev = torch.device("cuda:0")
     P, L = a.parcels, a.L
     G, gam0, x = make_field(P, L, 1, dev)

@@ -553,7 +553,6 @@ __global__ void __launch_bounds__(256) eij_regularized_kernel(EijArgs a) {
 // moments (tables/make_moment_maps.py; the reference's a6 / a8 bodies carry
 // REAL(4) literal coefficients, so those terms agree with it to ~1e-7).
 
-__constant__ double kMOM6[28][28] = SFG_MOM6_INIT;
 __constant__ double kMOM8[45][45] = SFG_MOM8_INIT;
 __constant__ double kMOM6_ISO[28] = SFG_MOM6_ISO_INIT;
 __constant__ double kMOM8_ISO[45] = SFG_MOM8_ISO_INIT;
@@ -612,6 +611,20 @@ __host__ __device__ constexpr int a4_slot(int i, int j, int k, int l) {
     return -1;
 }
 
+// the 81 slots as a table, so that every use folds to a constant
+struct A4Slots {
+    int s[81];
+};
+__host__ __device__ constexpr A4Slots make_a4_slots() {
+    A4Slots t{};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) t.s[27 * i + 9 * j + 3 * k + l] = a4_slot(i, j, k, l);
+    return t;
+}
+constexpr A4Slots kA4Slots = make_a4_slots();
+
 // sum_kl a4(i,j,k,l) S_kl
 template <class U>
 __device__ __forceinline__ double a4_dot(const U& u, int i, int j, const double (&S)[9]) {
@@ -619,7 +632,7 @@ __device__ __forceinline__ double a4_dot(const U& u, int i, int j, const double 
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
-        for (int l = 0; l < 3; l++) s += u[a4_slot(i, j, k, l)] * S[3 * k + l];
+        for (int l = 0; l < 3; l++) s += u[kA4Slots.s[27 * i + 9 * j + 3 * k + l]] * S[3 * k + l];
     return s;
 }
 
@@ -757,10 +770,36 @@ __device__ __forceinline__ double ev_D4_of(const double (&tau)[9], const U& u, c
     return 35 / 2.0 * d / (norm * norm);
 }
 
+// degree-6 moment k as the sum of three eighth moments (x^2 + y^2 + z^2 = 1)
+struct M6Idx {
+    int i[28][3];
+};
+__host__ __device__ constexpr M6Idx make_m6idx() {
+    M6Idx t{};
+    int k = 0;
+    for (int a = 6; a >= 0; a--)
+        for (int b = 6 - a; b >= 0; b--, k++) {
+            t.i[k][0] = midx(8, a + 2, b);
+            t.i[k][1] = midx(8, a, b + 2);
+            t.i[k][2] = midx(8, a, b);
+        }
+    return t;
+}
+constexpr M6Idx kM6Idx = make_m6idx();
+
+// the sixth moments of a lane's eighth-moment column in LDS
+struct M6From8 {
+    const double* p;
+    __device__ __forceinline__ double operator[](int k) const {
+        return p[kM6Idx.i[k][0] * 64] + p[kM6Idx.i[k][1] * 64] + p[kM6Idx.i[k][2] * 64];
+    }
+};
+
 // a parcel's l <= 8 rows (missing rows zero) into xq, and its sorted full-a4
-// entries and sixth and eighth moments into the lane's LDS columns
-// (lds[k * 64], k < 15: a4, then 28 sixth, then 45 eighth moments)
-constexpr int kColU = 0, kCol6 = 15, kCol8 = 43, kColDen = 88, kCols = 94;
+// entries and eighth moments into the lane's LDS columns (lds[k * 64]: 15 a4
+// entries, then 45 eighth moments; 60 doubles per lane keep four 64-lane
+// workgroups per CU; the sixth moments follow from the eighth)
+constexpr int kColU = 0, kCol8 = 15, kCols = 60;
 __device__ __forceinline__ void nonlin_moments(const double* x, int64_t P, int64_t p, int nrows, const double* A4F,
                                                double (&xq)[45], double* lds) {
 #pragma unroll
@@ -772,13 +811,6 @@ __device__ __forceinline__ void nonlin_moments(const double* x, int64_t P, int64
 #pragma unroll
         for (int q = 0; q < 15; q++) s += A4F[e * 15 + q] * xq[q];
         lds[(kColU + e) * 64] = s * ix0;
-    }
-#pragma unroll 1
-    for (int k = 0; k < 28; k++) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < 28; q++) s += kMOM6[k][q] * xq[q];
-        lds[(kCol6 + k) * 64] = s * ix0;
     }
 #pragma unroll 1
     for (int k = 0; k < 45; k++) {
@@ -804,7 +836,8 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
     double* col = lds + threadIdx.x;
     double xq[45];
     nonlin_moments(a.x, a.P, p, b.nrows, b.A4F, xq, col);
-    const Col<64> u{col + kColU * 64}, m6{col + kCol6 * 64}, m8{col + kCol8 * 64};
+    const Col<64> u{col + kColU * 64}, m8{col + kCol8 * 64};
+    const M6From8 m6{col + kCol8 * 64};
     const Col<1> ui{iso}, m6i{iso + 15}, m8i{iso + 43};
     double a2v[6];
     {
@@ -837,22 +870,15 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
             for (int j = 0; j < 3; j++)
                 tau[3 * i + j] = q < 3 ? (i == j ? 1.0 / 3.0 : 0.0) - v[i] * v[j] : v[i] * w[j] + w[i] * v[j];
     };
-    // the isotropic normalisations first (their constants are not live in the
-    // second loop), then the parcel's own averages
+    int st = 0;
 #pragma unroll 1
     for (int q = 0; q < 6; q++) {
         __asm__ volatile("" ::: "memory");   // reread the shared isotropic values in each pass
         double v[3], w[3], tau[9];
         state(q, v, w, tau);
-        col[(kColDen + q) * 64] = sachs3_vw(tau, v, w, a2i, ui, m6i, m8i, a.sA, a.sB, a.sC);
-    }
-    int st = 0;
-#pragma unroll 1
-    for (int q = 0; q < 6; q++) {
-        double v[3], w[3], tau[9];
-        state(q, v, w, tau);
+        const double den = sachs3_vw(tau, v, w, a2i, ui, m6i, m8i, a.sA, a.sB, a.sC);
         const double num = sachs3_vw(tau, v, w, a2, u, m6, m8, a.sA, a.sB, a.sC);
-        const double es = num / col[(kColDen + q) * 64];
+        const double es = num / den;
         double* out = a.Eij + p * 6 + q;
         const double E = a.alpha > 0 ? (1 - a.alpha) * es + a.alpha * *out : es;
         *out = E;
@@ -877,7 +903,8 @@ __global__ void __launch_bounds__(64) ev_D4_kernel(int64_t P, int nrows, const d
     double* col = lds + threadIdx.x;
     double xq[45];
     nonlin_moments(x, P, p, nrows, A4F, xq, col);
-    const Col<64> u{col + kColU * 64}, m6{col + kCol6 * 64}, m8{col + kCol8 * 64};
+    const Col<64> u{col + kColU * 64}, m8{col + kCol8 * 64};
+    const M6From8 m6{col + kCol8 * 64};
     double t[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) t[i] = tau[p * 9 + i];
