@@ -334,6 +334,42 @@ int sfg_ri_to_nlm(sfg_ctx* ctx, int L, int64_t N, const double* ri, const double
 int sfg_ri_to_state_batch(sfg_ctx* ctx, int L, int64_t E, int64_t N, const double* ri_dev, const double* wi_dev,
                           double* x_dev, void* stream);
 
+/* ---- orthotropic grains (olivine) ----
+ * Three distributions of the slip-system axes q1 = b, q2 = n, q3 = v per
+ * parcel: states x1_dev, x2_dev, x3_dev (n, P); x3 NULL or a parcel's
+ * q3 n00 <= 1e-8 derives v = b x n (ai_orthotropic, src/moments.f90:357-384,
+ * with the joint moments of the product density under the sin^2 kernel of
+ * src/moments.f90:242-310).  Single-parcel forms take host nlm vectors
+ * (nlm3 NULL or zero: derived).  L >= 4. */
+/* Eij_orthotropic(nlm_1, nlm_2, nlm_3, e1, e2, e3, Eij_grain, alpha, n_grain)
+ * (src/enhancementfactors.f90:134-188; specfabpy src/specfabpy.f90:436-446 and
+ * the *_arr loop :488-500): Sachs homogenization only (alpha unused, as the
+ * reference); Eij_grain = (Ebb, Enn, Evv, Env, Ebv, Enb); n_grain = 1 (the
+ * reference returns 0/0 otherwise; here an error).  frame: 1 = frames_dev
+ * (P, 3, 3) rows e1, e2, e3, 2 = Cartesian. */
+int sfg_Eij_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                        const double e1[3], const double e2[3], const double e3[3], const double Eij_grain[6],
+                        double alpha, int n_grain, double Eij[6]);
+int sfg_Eij_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                              const double* x3_dev, int frame, const double* frames_dev, const double Eij_grain[6],
+                              double alpha, int n_grain, double* Eij_dev, void* stream);
+/* Vi_elastic_orthotropic(nlm_1, nlm_2, nlm_3, alpha, Lame_grain, rho, theta,
+ * phi) (src/wavepropagation.f90:41-68,98-115; specfabpy :975-984): Voigt
+ * average only (alpha unused, as the reference); Lame_grain = (lam11, lam22,
+ * lam33, lam23, lam13, lam12, mu1, mu2, mu3).  Vi (ndir, 3) = (qS1, qS2, qP)
+ * row-major; batched Vi_dev (P, ndir, 3). */
+int sfg_Vi_elastic_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                               double alpha, const double Lame[9], double rho, int ndir, const double* theta,
+                               const double* phi, double* Vi);
+int sfg_Vi_elastic_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                                     const double* x3_dev, double alpha, const double Lame[9], double rho, int ndir,
+                                     const double* theta, const double* phi, double* Vi_dev, void* stream);
+
+/* Cij_to_Lame_orthotropic (src/elasticities.f90:122-139): (C11, C22, C33, C44,
+ * C55, C66, C23, C13, C12) -> (lam11, lam22, lam33, lam23, lam13, lam12, mu1,
+ * mu2, mu3).  Host only. */
+int sfg_Cij_to_Lame_orthotropic(const double Cij[9], double Lame[9]);
+
 /* Cij_to_Lame_tranisotropic (src/elasticities.f90:107-120): (C11, C33, C55,
  * C12, C13) -> (lam, mu, Elam, Emu, Egam).  Host only. */
 int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]);

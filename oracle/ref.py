@@ -57,6 +57,14 @@ class RefLib:
         L.sfref_rotate_nlm.argtypes = [_cp, _c.c_double, _c.c_double, _cp]
         L.sfref_a4_eigentensors.argtypes = [_cp, _dp, _dp]
         L.sfref_nlm_isvalid.argtypes = [_c.c_int, _cp, _cp, np.ctypeslib.ndpointer(dtype=np.int32)]
+        L.sfref_ortho_moments.argtypes = [_cp, _cp, _dp, _dp, _dp, _dp]
+        L.sfref_Eij_orthotropic.argtypes = [_cp, _cp, _cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
+        L.sfref_Eij_orthotropic_discrete.argtypes = [_c.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _c.c_double,
+                                                     _c.c_int, _dp]
+        L.sfref_Vi_elastic_orthotropic.argtypes = [_cp, _cp, _cp, _c.c_double, _dp, _c.c_double, _c.c_int, _dp, _dp,
+                                                   _dp]
+        L.sfref_Vi_elastic_orthotropic_discrete.argtypes = [_c.c_int, _dp, _c.c_double, _dp, _c.c_double, _c.c_int,
+                                                            _dp, _dp, _dp]
         L.sfref_ev_ck.argtypes = [_cp, _dp, _dp, _dp, _dp]
         L.sfref_ev_D.argtypes = [_cp, _dp, _c.c_int, _dp]
         L.sfref_E_CAFFE.argtypes = [_cp, _dp, _c.c_double, _c.c_double, _c.c_int, _dp]
@@ -213,6 +221,45 @@ class RefLib:
         ok = np.zeros(len(n20), dtype=np.int32)
         self.lib.sfref_nlm_isvalid(len(n20), n20, n40, ok)
         return ok.astype(bool)
+
+    def ortho_moments(self, b, n):
+        """(a4_joint, a4_jointcross, a2_orth, a4_orth) of the distributions b, n"""
+        out = [np.zeros((3,) * k, order="F") for k in (4, 4, 2, 4)]
+        self.lib.sfref_ortho_moments(self._v(b, np.complex128), self._v(n, np.complex128), *out)
+        return tuple(np.ascontiguousarray(o) for o in out)
+
+    def Eij_orthotropic(self, q1, q2, q3, e1, e2, e3, Eg, alpha, ng):
+        out = np.zeros(6, order="F")
+        self.lib.sfref_Eij_orthotropic(self._v(q1, np.complex128), self._v(q2, np.complex128),
+                                       self._v(q3, np.complex128), self._v(e1), self._v(e2), self._v(e3),
+                                       self._v(Eg), float(alpha), int(ng), out)
+        return out.copy()
+
+    def Eij_orthotropic_discrete(self, bi, ni, vi, e1, e2, e3, Eg, alpha, ng):
+        out = np.zeros(6, order="F")
+        f = lambda a: np.asfortranarray(a, dtype=np.float64)
+        self.lib.sfref_Eij_orthotropic_discrete(len(bi), f(bi), f(ni), f(vi), self._v(e1), self._v(e2), self._v(e3),
+                                                self._v(Eg), float(alpha), int(ng), out)
+        return out.copy()
+
+    def Vi_elastic_orthotropic(self, q1, q2, q3, alpha, Lame, rho, theta, phi):
+        theta = self._v(np.atleast_1d(theta))
+        phi = self._v(np.atleast_1d(phi))
+        out = np.zeros((3, theta.size), order="F")
+        self.lib.sfref_Vi_elastic_orthotropic(self._v(q1, np.complex128), self._v(q2, np.complex128),
+                                              self._v(q3, np.complex128), float(alpha), self._v(Lame), float(rho),
+                                              theta.size, theta, phi, out)
+        return np.ascontiguousarray(out)
+
+    def Vi_elastic_orthotropic_discrete(self, mi, alpha, Lame, rho, theta, phi):
+        """mi (3, 3, N) = (m'_i, xyz, grain), as the reference takes it"""
+        theta = self._v(np.atleast_1d(theta))
+        phi = self._v(np.atleast_1d(phi))
+        out = np.zeros((3, theta.size), order="F")
+        m = np.asfortranarray(mi, dtype=np.float64)
+        self.lib.sfref_Vi_elastic_orthotropic_discrete(m.shape[2], m, float(alpha), self._v(Lame), float(rho),
+                                                       theta.size, theta, phi, out)
+        return np.ascontiguousarray(out)
 
     def ev_ck(self, nlm):
         """(a2, a4, a6, a8) full tensors of f_ev_ck(nlm, 'f') (L >= 8)"""

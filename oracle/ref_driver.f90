@@ -307,6 +307,61 @@ contains
         E = E_CAFFE(nlm, eps, Emin, Emax, ng)
     end subroutine
 
+    ! ---- orthotropic grains ----
+
+    ! joint structure tensors of two distributions (L >= 4 states of length ncur)
+    subroutine sfref_ortho_moments(b, n, a4j, a4jc, a2o, a4o) bind(C, name="sfref_ortho_moments")
+        complex(c_double_complex), intent(in) :: b(ncur), n(ncur)
+        real(c_double), intent(out) :: a4j(3,3,3,3), a4jc(3,3,3,3), a2o(3,3), a4o(3,3,3,3)
+        a4j = a4_joint(b, n)
+        a4jc = a4_jointcross(b, n)
+        a2o = a2_orth(b, n)
+        a4o = a4_orth(b, n)
+    end subroutine
+
+    ! Eij_orthotropic(q1, q2, q3, e1, e2, e3, Eij_grain(6), alpha, n_grain)
+    subroutine sfref_Eij_orthotropic(q1, q2, q3, e1, e2, e3, Eg, alpha, ng, Eij) bind(C, name="sfref_Eij_orthotropic")
+        complex(c_double_complex), intent(in) :: q1(ncur), q2(ncur), q3(ncur)
+        real(c_double), intent(in) :: e1(3), e2(3), e3(3), Eg(6)
+        real(c_double), value :: alpha
+        integer(c_int), value :: ng
+        real(c_double), intent(out) :: Eij(6)
+        Eij = Eij_orthotropic(q1, q2, q3, e1, e2, e3, Eg, alpha, ng)
+    end subroutine
+
+    ! Eij_orthotropic_discrete(bi(N,3), ni(N,3), vi(N,3), e1, e2, e3, Eij_grain(6), alpha, n_grain)
+    subroutine sfref_Eij_orthotropic_discrete(N, bi, ni, vi, e1, e2, e3, Eg, alpha, ng, Eij) &
+            bind(C, name="sfref_Eij_orthotropic_discrete")
+        integer(c_int), value :: N
+        real(c_double), intent(in) :: bi(N,3), ni(N,3), vi(N,3), e1(3), e2(3), e3(3), Eg(6)
+        real(c_double), value :: alpha
+        integer(c_int), value :: ng
+        real(c_double), intent(out) :: Eij(6)
+        Eij = Eij_orthotropic_discrete(bi, ni, vi, e1, e2, e3, Eg, alpha, ng)
+    end subroutine
+
+    ! Vi_elastic_orthotropic(q1, q2, q3, alpha, Lame(9), rho, theta, phi) -> Vi(3, ndir)
+    subroutine sfref_Vi_elastic_orthotropic(q1, q2, q3, alpha, Lame, rho, ndir, theta, phi, Vi) &
+            bind(C, name="sfref_Vi_elastic_orthotropic")
+        complex(c_double_complex), intent(in) :: q1(ncur), q2(ncur), q3(ncur)
+        real(c_double), value :: alpha, rho
+        real(c_double), intent(in) :: Lame(9)
+        integer(c_int), value :: ndir
+        real(c_double), intent(in) :: theta(ndir), phi(ndir)
+        real(c_double), intent(out) :: Vi(3, ndir)
+        Vi = Vi_elastic_orthotropic(q1, q2, q3, alpha, Lame, rho, theta, phi)
+    end subroutine
+
+    ! Vi_elastic_orthotropic__discrete(mi(3,3,N), alpha, Lame(9), rho, theta, phi) -> Vi(3, ndir)
+    subroutine sfref_Vi_elastic_orthotropic_discrete(N, mi, alpha, Lame, rho, ndir, theta, phi, Vi) &
+            bind(C, name="sfref_Vi_elastic_orthotropic_discrete")
+        integer(c_int), value :: N, ndir
+        real(c_double), intent(in) :: mi(3,3,N), Lame(9), theta(ndir), phi(ndir)
+        real(c_double), value :: alpha, rho
+        real(c_double), intent(out) :: Vi(3, ndir)
+        Vi = Vi_elastic_orthotropic__discrete(mi, alpha, Lame, rho, theta, phi)
+    end subroutine
+
     ! ---- discrete grains ----
 
     ! ri_LROT(ri0, dt, Nt, D, W, iota): ri0(N,3), D/W(Nt,3,3) -> ri(Nt,N,3)

@@ -143,6 +143,40 @@ def ri_LROT_batch(ri0, dt, Nt, eps, omg, iota, ctx=None):
     return ri
 
 
+def eij_orthotropic(b1, b2, b3, Eij_grain, alpha=0.0, n_grain=1, frames=None, ctx=None):
+    """Orthotropic Eij (P, 6) (Sachs; src/enhancementfactors.f90:134-188) of the
+    slip-system-axis distributions held by FabricBatch objects b1 (b), b2 (n)
+    and b3 (v; None or zero states derive v = b x n), in the caller frames
+    (P, 3, 3) or the Cartesian frame (frames None)."""
+    ctx = ctx or b1.ctx
+    if b2.P != b1.P or b2.L != b1.L or (b3 is not None and (b3.P != b1.P or b3.L != b1.L)):
+        raise ValueError("the three batches must have the same P and L")
+    fr = b1._f64(frames, (b1.P, 3, 3)) if frames is not None else None
+    E = torch.empty((b1.P, 6), dtype=torch.float64, device=b1.dev)
+    Eg = (_c.c_double * 6)(*np.asarray(Eij_grain, dtype=np.float64).reshape(6))
+    check(lib().sfg_Eij_orthotropic_batch(ctx.handle, b1.L, b1.P, _ptr(b1.x), _ptr(b2.x),
+                                          _ptr(b3.x) if b3 is not None else None, 1 if fr is not None else 2,
+                                          _ptr(fr), Eg, float(alpha), int(n_grain), _ptr(E), _stream()))
+    return E
+
+
+def vi_elastic_orthotropic(b1, b2, b3, lame_grain, rho, theta, phi, alpha=1.0, ctx=None):
+    """Orthotropic elastic phase velocities (P, ndir, 3) = (qS1, qS2, qP)
+    (Voigt; src/wavepropagation.f90:41-68) of the distributions in b1, b2, b3
+    (b3 None or zero: v = b x n) along (theta, phi)."""
+    ctx = ctx or b1.ctx
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+    Vi = torch.empty((b1.P, th.size, 3), dtype=torch.float64, device=b1.dev)
+    Lm = (_c.c_double * 9)(*np.asarray(lame_grain, dtype=np.float64).reshape(9))
+    check(lib().sfg_Vi_elastic_orthotropic_batch(ctx.handle, b1.L, b1.P, _ptr(b1.x), _ptr(b2.x),
+                                                 _ptr(b3.x) if b3 is not None else None, float(alpha), Lm,
+                                                 float(rho), th.size,
+                                                 th.ctypes.data_as(_lib._dp), ph.ctypes.data_as(_lib._dp),
+                                                 _ptr(Vi), _stream()))
+    return Vi
+
+
 class FabricBatch:
     """P parcels at truncation L with per-parcel kinematics.
 

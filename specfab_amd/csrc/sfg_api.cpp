@@ -1263,6 +1263,21 @@ int sfg_Cij_to_Lame_tranisotropic(const double Cij[5], double Lame[5]) {
     return SFG_OK;
 }
 
+int sfg_Cij_to_Lame_orthotropic(const double Cij[9], double Lame[9]) {
+    if (!Cij || !Lame) return fail(SFG_ERR_ARG, "NULL argument");
+    // src/elasticities.f90:122-139
+    Lame[6] = Cij[4] + Cij[5] - Cij[3];
+    Lame[7] = Cij[3] + Cij[5] - Cij[4];
+    Lame[8] = Cij[3] + Cij[4] - Cij[5];
+    Lame[0] = Cij[0] - 2 * Lame[6];
+    Lame[1] = Cij[1] - 2 * Lame[7];
+    Lame[2] = Cij[2] - 2 * Lame[8];
+    Lame[3] = Cij[6];
+    Lame[4] = Cij[7];
+    Lame[5] = Cij[8];
+    return SFG_OK;
+}
+
 static int a4_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4_dev, hipStream_t s) {
     if (L < 4) return fail(SFG_ERR_L, "a4 needs the l <= 4 coefficients (L >= 4)");
     const DevOp* op;
@@ -1413,6 +1428,151 @@ int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm,
                                                  dout, c->stream);
                          },
                          &u);
+}
+
+// ---------------------------------------------------------------- orthotropic grains
+
+extern "C++" {
+namespace {
+// rheo_params_orthotropic (src/rheologies.f90:186-206) -> ci of the Sachs tensor
+// (src/homogenizations.f90:301-303)
+void ortho_ci(const double Eg[6], int n_grain, double ci[6]) {
+    double B[6];
+    for (int i = 0; i < 6; i++) B[i] = std::pow(Eg[i], 2.0 / (n_grain + 1));
+    const double lam[6] = {-B[0] + B[1] + B[2], B[0] - B[1] + B[2], B[0] + B[1] - B[2], B[3], B[4], B[5]};
+    for (int i = 0; i < 3; i++) ci[i] = 4.0 / 3 * lam[i];
+    for (int i = 3; i < 6; i++) ci[i] = 2 * lam[i];
+}
+
+int eij_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const double* x2, const double* x3, int frame,
+                     const double* frames_dev, const double Eg[6], double alpha, int n_grain, double* Eij_dev,
+                     hipStream_t s) {
+    (void)alpha;   // Sachs only, as the reference (src/enhancementfactors.f90:162)
+    if (L < 4) return fail(SFG_ERR_L, "orthotropic Eij needs the l <= 4 coefficients (L >= 4)");
+    if (n_grain != 1)   // the reference returns eps = 0 (Eij = 0/0) for other n'
+        return fail(SFG_ERR_ARG, "orthotropic Sachs homogenization supports n_grain = 1 only");
+    if (frame != 1 && frame != 2) return fail(SFG_ERR_ARG, "frame must be 1 (given frames) or 2 (Cartesian)");
+    if (frame == 1 && !frames_dev) return fail(SFG_ERR_ARG, "frames missing");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::OrthoArgs a{};
+    a.P = P; a.nq = op->nq; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.A2 = op->A2; a.A4F = op->A4F;
+    a.frames = frame == 1 ? frames_dev : nullptr;
+    ortho_ci(Eg, n_grain, a.ci);
+    a.Eij = Eij_dev;
+    void* ws;
+    WsLease lease(ctx, s);
+    rc = lease.acquire(81 * sizeof(double), &ws);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_eij_ortho(a, (double*)ws, s));
+    return lease.release();
+}
+
+int vi_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const double* x2, const double* x3,
+                    const double Lame[9], double rho, int ndir, const double* theta, const double* phi, double* Vi_dev,
+                    hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "orthotropic Vi needs the l <= 4 coefficients (L >= 4)");
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* R;
+    rc = get_dirs(ctx, ndir, theta, phi, &R);
+    if (rc) return rc;
+    sfg::OrthoArgs a{};
+    a.P = P; a.nq = op->nq; a.ndir = ndir; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.A2 = op->A2; a.A4F = op->A4F; a.R = R;
+    for (int i = 0; i < 6; i++) a.lam[i] = Lame[i];
+    for (int i = 0; i < 3; i++) a.mu[i] = Lame[6 + i];
+    a.rho = rho;
+    a.Vi = Vi_dev;
+    HIPCHK(sfg::launch_vi_ortho(a, s));
+    return SFG_OK;
+}
+
+// three host nlm vectors (n complex each; nlm3 may be zero: v = b x n) as
+// device states, then fn on them with P = 1, out (nout doubles) back to the host
+template <class Fn>
+int ortho_single(sfg_ctx* ctx, int L, const double* n1, const double* n2, const double* n3, size_t nout, double* out,
+                 Fn fn) {
+    if (!ctx || !n1 || !n2 || !out) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, ((size_t)9 * n + nout + 9) * sizeof(double)));
+    double* dn = (double*)buf.p;             // 3 x 2n
+    double* dx = dn + 6 * n;                 // 3 x n
+    double* dout = dx + 3 * n;
+    double* dfr = dout + nout;
+    hipStream_t s = ctx->stream;
+    const double* ns[3] = {n1, n2, n3};
+    for (int m = 0; m < 3; m++) {
+        if (ns[m]) HIPCHK(hipMemcpyAsync(dn + 2 * n * m, ns[m], 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        else HIPCHK(hipMemsetAsync(dn + 2 * n * m, 0, 2 * n * sizeof(double), s));
+        HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn + 2 * n * m, dx + n * m, s));
+    }
+    rc = fn(dx, dx + n, dx + 2 * n, dout, dfr, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int sfg_Eij_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                              const double* x3_dev, int frame, const double* frames_dev, const double Eij_grain[6],
+                              double alpha, int n_grain, double* Eij_dev, void* stream) {
+    if (!ctx || !Eij_grain || P < 0 || (P > 0 && (!x1_dev || !x2_dev || !Eij_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    return eij_ortho_locked(ctx, L, P, x1_dev, x2_dev, x3_dev, frame, frames_dev, Eij_grain, alpha, n_grain, Eij_dev,
+                            pick(ctx, stream));
+}
+
+int sfg_Eij_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                        const double e1[3], const double e2[3], const double e3[3], const double Eij_grain[6],
+                        double alpha, int n_grain, double Eij[6]) {
+    if (!e1 || !e2 || !e3 || !Eij_grain || !Eij) return fail(SFG_ERR_ARG, "NULL argument");
+    const double f[9] = {e1[0], e1[1], e1[2], e2[0], e2[1], e2[2], e3[0], e3[1], e3[2]};
+    return ortho_single(ctx, L, nlm1, nlm2, nlm3, 6, Eij,
+                        [&](const double* x1, const double* x2, const double* x3, double* dout, double* dfr,
+                            hipStream_t s) {
+                            HIPCHK(hipMemcpyAsync(dfr, f, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+                            return eij_ortho_locked(ctx, L, 1, x1, x2, x3, 1, dfr, Eij_grain, alpha, n_grain, dout,
+                                                    s);
+                        });
+}
+
+int sfg_Vi_elastic_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                                     const double* x3_dev, double alpha, const double Lame[9], double rho, int ndir,
+                                     const double* theta, const double* phi, double* Vi_dev, void* stream) {
+    (void)alpha;   // Voigt only, as the reference (src/wavepropagation.f90:44)
+    if (!ctx || !Lame || P < 0 || ndir < 0 || (P > 0 && ndir > 0 && (!x1_dev || !x2_dev || !Vi_dev || !theta || !phi)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0 || ndir == 0) return SFG_OK;
+    return vi_ortho_locked(ctx, L, P, x1_dev, x2_dev, x3_dev, Lame, rho, ndir, theta, phi, Vi_dev,
+                           pick(ctx, stream));
+}
+
+int sfg_Vi_elastic_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                               double alpha, const double Lame[9], double rho, int ndir, const double* theta,
+                               const double* phi, double* Vi) {
+    (void)alpha;
+    if (ndir <= 0) return ndir == 0 ? SFG_OK : fail(SFG_ERR_ARG, "ndir < 0");
+    if (!Lame || !theta || !phi) return fail(SFG_ERR_ARG, "NULL argument");
+    return ortho_single(ctx, L, nlm1, nlm2, nlm3, (size_t)3 * ndir, Vi,
+                        [&](const double* x1, const double* x2, const double* x3, double* dout, double*,
+                            hipStream_t s) {
+                            return vi_ortho_locked(ctx, L, 1, x1, x2, x3, Lame, rho, ndir, theta, phi, dout, s);
+                        });
 }
 
 // ---------------------------------------------------------------- rotation, a4 eigentensors, validity

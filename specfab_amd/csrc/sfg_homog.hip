@@ -15,6 +15,7 @@
 
 #include "generated/sfg_moment_maps.h"
 #include "generated/sfg_moments68.h"
+#include "sfg_a4slots.h"
 #include "sfg_device.h"
 #include "sfg_homog.h"
 
@@ -591,39 +592,6 @@ __device__ __forceinline__ void poly_mul(const double* A, const double* B, doubl
                 for (int b2 = DB - a2; b2 >= 0; b2--)
                     C[midx(D, a1 + a2, b1 + b2)] += A[midx(DA, a1, b1)] * B[midx(DB, a2, b2)];
 }
-
-// sorted-entry index (0..14) of a4(i,j,k,l); the reference's body sets
-// a4(3,2,1,2) to its a4(1,2,3,3) entry (src/include/ev_c4__body.f90)
-__host__ __device__ constexpr int a4_slot(int i, int j, int k, int l) {
-    if (i == 2 && j == 1 && k == 0 && l == 1) return 8;
-    int v[4] = {i, j, k, l};
-    for (int p = 0; p < 4; p++)
-        for (int q = p + 1; q < 4; q++)
-            if (v[q] < v[p]) { int t = v[p]; v[p] = v[q]; v[q] = t; }
-    int idx = 0;
-    for (int a0 = 0; a0 < 3; a0++)
-        for (int a1 = a0; a1 < 3; a1++)
-            for (int a2 = a1; a2 < 3; a2++)
-                for (int a3 = a2; a3 < 3; a3++) {
-                    if (a0 == v[0] && a1 == v[1] && a2 == v[2] && a3 == v[3]) return idx;
-                    idx++;
-                }
-    return -1;
-}
-
-// the 81 slots as a table, so that every use folds to a constant
-struct A4Slots {
-    int s[81];
-};
-__host__ __device__ constexpr A4Slots make_a4_slots() {
-    A4Slots t{};
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++)
-                for (int l = 0; l < 3; l++) t.s[27 * i + 9 * j + 3 * k + l] = a4_slot(i, j, k, l);
-    return t;
-}
-constexpr A4Slots kA4Slots = make_a4_slots();
 
 // sum_kl a4(i,j,k,l) S_kl
 template <class U>

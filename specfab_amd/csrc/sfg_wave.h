@@ -50,6 +50,31 @@ struct A4FullArgs {
     double* a4;           // (P, 3, 3, 3, 3) row-major
 };
 
+// Orthotropic grains (olivine): three distributions q1, q2, q3 of the
+// slip-system axes (b, n, v) per parcel, or q1, q2 with v = b x n derived
+// (src/moments.f90:357-384).  Eij (Sachs, src/enhancementfactors.f90:134-188)
+// or elastic Vi (Voigt, src/wavepropagation.f90:41-68,98-115).
+struct OrthoArgs {
+    int64_t P;
+    int nq;               // rows of the l <= 4 block present (15 for L >= 4)
+    int ndir;             // Vi: directions
+    const double* x1;     // (n, P) real states of q1, q2, q3 (x3 null: v = b x n)
+    const double* x2;
+    const double* x3;
+    const double* A2;     // (6, 15) Mandel a2 map
+    const double* A4F;    // (15, 15) full a4 map (REAL(4) body), index-sorted entries
+    const double* R;      // Vi: (ndir, ROT4_STRIDE)
+    const double* frames; // Eij: (P, 3, 3) rows e1, e2, e3, or null (Cartesian)
+    double ci[6];         // Eij: 4/3 lami(1:3), 2 lami(4:6) (rheo_params_orthotropic)
+    const double* Qiso;   // Eij: the isotropic fabric's Sachs tensor Q (81, row-major lkij; device)
+    double lam[6], mu[3], rho;   // Vi: grain Lame parameters, density
+    double* Eij;          // (P, 6)
+    double* Vi;           // (P, ndir, 3): qS1, qS2, qP
+};
+// Eij: Qiso is device scratch of 81 doubles for the isotropic fabric's Q
+hipError_t launch_eij_ortho(const OrthoArgs& a, double* Qiso, hipStream_t s);
+hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s);
+
 hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s);
 hipError_t launch_vi_em(const ViEMArgs& a, hipStream_t s);
 hipError_t launch_a4full(const A4FullArgs& a, hipStream_t s);

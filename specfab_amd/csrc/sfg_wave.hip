@@ -20,6 +20,7 @@
 //      REAL(4) exponents (Qnorm_to_vi, :267-324).
 #include <algorithm>
 
+#include "sfg_a4slots.h"
 #include "sfg_device.h"
 #include "sfg_wave.h"
 
@@ -348,6 +349,356 @@ __global__ void __launch_bounds__(128) a4full_kernel(A4FullArgs a) {
                 }
 }
 
+
+// ---------------------------------------------------------------------------
+// Orthotropic grains.  Each slip-system axis distribution contributes its a2
+// (the reference's a2 body, through the Mandel map) and its full a4 (the
+// REAL(4) body, its (3,2,1,2) entry included); the joint tensors follow from
+// the two distributions' moments under the product density with the kernel
+// sin^2 of the axes' angle, as the reference's generated bodies evaluate them
+// (src/moments.f90:242-310; identified against them to their REAL(4) level):
+//   a4_joint(b, n)      = (a2b (x) a2n - a4b : a4n) / (1 - a2b : a2n)       <b b n n>
+//   a4_jointcross(b, n) = eps eps (a4n (x) a2b) / (1 - a2b : a2n)           <n n v v>
+//   a2_orth(b, n)       = eps eps (a2b (x) a2n) / (1 - a2b : a2n)           <v v>
+//   a4_orth(b, n)       = eps^4 (a4b (x) a4n) / <sin^4> = ... / (1 - 2 a2b:a2n + a4b::a4n)
+// with v = b x n and eps the Levi-Civita symbol.
+
+namespace {
+
+struct OrthoDist {
+    double a2[9];    // a2 (matrix)
+    double u[15];    // a4 body, index-sorted entries
+    bool sym;        // a4 from a formula: symmetric (no (3,2,1,2) entry)
+};
+
+// pair index of (a, b), a symmetric pair: xx yy zz yz xz xy
+__device__ __forceinline__ constexpr int pidx(int a, int b) { return a == b ? a : 6 - a - b; }
+
+__device__ __forceinline__ double levi(int i, int j, int k) {
+    return (i == j || j == k || i == k) ? 0.0 : (((j - i + 3) % 3 == 1) ? 1.0 : -1.0);
+}
+
+// a2 matrix and a4 body entries of the l <= 4 block xr (x0 = xr[0])
+__device__ __forceinline__ void ortho_dist(const double* A2, const double* A4F, const double* xr, OrthoDist& d) {
+    const double x0 = xr[0], r2i = 0.70710678118654752440;
+    double a2v[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) a2v[i] = map_row(A2, i, xr, 15) / x0 + (i < 3 ? 1.0 / 3.0 : 0.0);
+    d.a2[0] = a2v[0], d.a2[4] = a2v[1], d.a2[8] = a2v[2];
+    d.a2[5] = d.a2[7] = a2v[3] * r2i, d.a2[2] = d.a2[6] = a2v[4] * r2i, d.a2[1] = d.a2[3] = a2v[5] * r2i;
+#pragma unroll
+    for (int e = 0; e < 15; e++) d.u[e] = map_row(A4F, e, xr, 15) / x0;
+    d.sym = false;
+}
+
+__device__ __forceinline__ double a4e(const OrthoDist& d, int i, int j, int k, int l) {
+    const int t = 27 * i + 9 * j + 3 * k + l;
+    return d.sym ? d.u[kA4Slots.sym[t]] : d.u[kA4Slots.s[t]];
+}
+__device__ __forceinline__ double a4sym(const OrthoDist& d, int i, int j, int k, int l) {
+    return d.u[kA4Slots.sym[27 * i + 9 * j + 3 * k + l]];
+}
+__device__ __forceinline__ double dd22(const double* A, const double* B) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s += A[i] * B[i];
+    return s;
+}
+
+// J[P][R] (pair indices) of a4_joint(b, n)
+__device__ __forceinline__ void a4_joint6(const OrthoDist& b, const OrthoDist& n, double (&J)[6][6]) {
+    const double nrm = 1.0 - dd22(b.a2, n.a2);
+#pragma unroll
+    for (int P = 0; P < 6; P++)
+#pragma unroll
+        for (int R = 0; R < 6; R++) {
+            const int i = P < 3 ? P : (P == 3 ? 1 : 0), j = P < 3 ? P : (P == 5 ? 1 : 2);
+            const int k = R < 3 ? R : (R == 3 ? 1 : 0), l = R < 3 ? R : (R == 5 ? 1 : 2);
+            double s = b.a2[3 * i + j] * n.a2[3 * k + l];
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) s -= a4sym(b, i, j, p, q) * a4sym(n, k, l, p, q);
+            J[P][R] = s / nrm;
+        }
+}
+
+// J[P][R] of a4_jointcross(b, n) = <n n v v>
+__device__ __forceinline__ void a4_jointcross6(const OrthoDist& b, const OrthoDist& n, double (&J)[6][6]) {
+    const double nrm = 1.0 - dd22(b.a2, n.a2);
+#pragma unroll
+    for (int P = 0; P < 6; P++)
+#pragma unroll
+        for (int R = 0; R < 6; R++) {
+            const int i = P < 3 ? P : (P == 3 ? 1 : 0), j = P < 3 ? P : (P == 5 ? 1 : 2);
+            const int k = R < 3 ? R : (R == 3 ? 1 : 0), l = R < 3 ? R : (R == 5 ? 1 : 2);
+            double s = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+#pragma unroll
+                        for (int t = 0; t < 3; t++) {
+                            const double e = levi(k, p, q) * levi(l, r, t);
+                            if (e != 0.0) s += e * a4sym(n, i, j, q, t) * b.a2[3 * p + r];
+                        }
+            J[P][R] = s / nrm;
+        }
+}
+
+// <v v> and <v^4> of v = b x n: the derived third distribution
+__device__ __forceinline__ void ortho_third(const OrthoDist& b, const OrthoDist& n, OrthoDist& v) {
+    const double n2 = 1.0 - dd22(b.a2, n.a2);
+    double a44 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) a44 += a4sym(b, i, j, k, l) * a4sym(n, i, j, k, l);
+    const double n4 = 1.0 - 2.0 * dd22(b.a2, n.a2) + a44;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+#pragma unroll
+                        for (int t = 0; t < 3; t++) {
+                            const double e = levi(i, p, q) * levi(j, r, t);
+                            if (e != 0.0) s += e * b.a2[3 * p + r] * n.a2[3 * q + t];
+                        }
+            v.a2[3 * i + j] = s / n2;
+        }
+    // the 15 sorted entries (a0 <= a1 <= a2 <= a3) of <v^4>
+    int e = 0;
+#pragma unroll
+    for (int a0 = 0; a0 < 3; a0++)
+#pragma unroll
+        for (int a1 = a0; a1 < 3; a1++)
+#pragma unroll
+            for (int a2 = a1; a2 < 3; a2++)
+#pragma unroll
+                for (int a3 = a2; a3 < 3; a3++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int p0 = 0; p0 < 3; p0++)
+#pragma unroll
+                        for (int q0 = 0; q0 < 3; q0++) {
+                            const double e0 = levi(a0, p0, q0);
+                            if (e0 == 0.0) continue;
+#pragma unroll
+                            for (int p1 = 0; p1 < 3; p1++)
+#pragma unroll
+                                for (int q1 = 0; q1 < 3; q1++) {
+                                    const double e1 = e0 * levi(a1, p1, q1);
+                                    if (e1 == 0.0) continue;
+#pragma unroll
+                                    for (int p2 = 0; p2 < 3; p2++)
+#pragma unroll
+                                        for (int q2 = 0; q2 < 3; q2++) {
+                                            const double e2 = e1 * levi(a2, p2, q2);
+                                            if (e2 == 0.0) continue;
+#pragma unroll
+                                            for (int p3 = 0; p3 < 3; p3++)
+#pragma unroll
+                                                for (int q3 = 0; q3 < 3; q3++) {
+                                                    const double e3 = e2 * levi(a3, p3, q3);
+                                                    if (e3 == 0.0) continue;
+                                                    s += e3 * a4sym(b, p0, p1, p2, p3) * a4sym(n, q0, q1, q2, q3);
+                                                }
+                                        }
+                                }
+                        }
+                    v.u[e++] = s / n4;
+                }
+    v.sym = true;
+}
+
+// the a4_jk triple of ai_orthotropic (src/moments.f90:364-383): jk[0] = <q2 q2 q3 q3>
+// (given) or a4_jointcross(q2, q1) (derived), jk[1] = <q1 q1 q3 q3> or
+// a4_jointcross(q1, q2), jk[2] = <q1 q1 q2 q2>
+__device__ __forceinline__ void ortho_joints(const OrthoDist* q, bool derived, int m, double (&J)[6][6]) {
+    if (m == 2) a4_joint6(q[0], q[1], J);
+    else if (!derived) a4_joint6(m == 0 ? q[1] : q[0], q[2], J);
+    else if (m == 0) a4_jointcross6(q[1], q[0], J);
+    else a4_jointcross6(q[0], q[1], J);
+}
+
+__device__ __forceinline__ double jt(const double (&J)[6][6], int a, int b, int c, int d) {
+    return J[pidx(a, b)][pidx(c, d)];
+}
+
+// Q = sum_m ci_m M2_m + ci_{m+3} H2_m (src/homogenizations.f90:303-314), accumulated into Q
+__device__ __forceinline__ void ortho_Q_add(const OrthoDist* q, int m, const double (&J)[6][6], const double* ci,
+                                            double* Q, int stride) {
+    const int jm = (m + 1) % 3, km = (m + 2) % 3;   // ji, ki of the reference (1-based [2,3,1], [3,1,2])
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const double M2 = (a4e(q[jm], l, k, i, j) + a4e(q[km], l, k, i, j) -
+                                       2 * ((jt(J, l, k, i, j) + jt(J, i, j, l, k)) / 2)) / 4;
+                    const double H2 = (jt(J, l, i, k, j) + jt(J, i, k, l, j) + jt(J, l, j, i, k) + jt(J, j, k, i, l)) / 4;
+                    Q[(27 * l + 9 * k + 3 * i + j) * stride] += ci[m] * M2 + ci[m + 3] * H2;
+                }
+}
+
+// doubleinner22(doubleinner42(Q, tau), v w) = sum Q_lkij tau_ji (v w)_kl
+__device__ __forceinline__ double ortho_eps_vw(const double* Q, int stride, const double (&tau)[9], const double* v,
+                                               const double* w) {
+    double r = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double e = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) e += Q[(27 * l + 9 * k + 3 * i + j) * stride] * tau[3 * j + i];
+            r += e * v[k] * w[l];
+        }
+    return r;
+}
+
+}  // namespace
+
+// Eij of orthotropic grains (Sachs; src/enhancementfactors.f90:134-188): one
+// lane per parcel, Q in the lane's LDS column
+__global__ void __launch_bounds__(64) eij_ortho_kernel(OrthoArgs a) {
+    __shared__ double lds[81 * 64];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double* Q = lds + threadIdx.x;
+    OrthoDist q[3];
+    const double* xs[3] = {a.x1, a.x2, a.x3};
+    // v given (its n00 above 1e-8) or v = b x n (src/moments.f90:371-383)
+    const bool derived = !a.x3 || !(a.x3[p] > 1e-8);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        if (m == 2 && derived) break;
+        double xr[15];
+#pragma unroll
+        for (int r = 0; r < 15; r++) xr[r] = r < a.nq ? xs[m][(int64_t)r * a.P + p] : 0.0;
+        ortho_dist(a.A2, a.A4F, xr, q[m]);
+    }
+    if (derived) ortho_third(q[0], q[1], q[2]);
+#pragma unroll
+    for (int t = 0; t < 81; t++) Q[t * 64] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        double J[6][6];
+        ortho_joints(q, derived, m, J);
+        ortho_Q_add(q, m, J, a.ci, Q, 64);
+    }
+    double e[9];
+    if (a.frames) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    }
+#pragma unroll 1
+    for (int s = 0; s < 6; s++) {
+        const int iv = s < 3 ? s : (s == 3 ? 1 : 0), iw = s < 3 ? s : (s == 5 ? 1 : 2);
+        double v[3], w[3], tau[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            v[i] = iv == 0 ? e[i] : (iv == 1 ? e[3 + i] : e[6 + i]);
+            w[i] = iw == 0 ? e[i] : (iw == 1 ? e[3 + i] : e[6 + i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                tau[3 * i + j] = s < 3 ? (i == j ? 1.0 / 3.0 : 0.0) - v[i] * v[j] : v[i] * w[j] + w[i] * v[j];
+        const double num = ortho_eps_vw(Q, 64, tau, v, w);
+        const double den = ortho_eps_vw(a.Qiso, 1, tau, v, w);   // the isotropic fabric (ortho_Q_kernel)
+        a.Eij[p * 6 + s] = num / den;
+    }
+}
+
+// Q of the isotropic fabric (q1 = q2 = q3 = the isotropic state; the
+// reference's ai_orthotropic(qlm_iso, qlm_iso, qlm_iso)) into Qiso (81)
+__global__ void ortho_Q_kernel(OrthoArgs a, double* Qiso) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    OrthoDist q[3];
+    double xr[15];
+#pragma unroll
+    for (int r = 0; r < 15; r++) xr[r] = r == 0 ? 1.0 : 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) ortho_dist(a.A2, a.A4F, xr, q[m]);
+    for (int t = 0; t < 81; t++) Qiso[t] = 0.0;
+    for (int m = 0; m < 3; m++) {
+        double J[6][6];
+        ortho_joints(q, false, m, J);
+        ortho_Q_add(q, m, J, a.ci, Qiso, 1);
+    }
+}
+
+// elastic Vi of orthotropic grains (Voigt; src/wavepropagation.f90:41-68,
+// 98-115): one lane per (parcel, direction); the distributions rotated as
+// rotate_nlm4 rotates them, the acoustic tensor for k = z, Cardano
+__global__ void __launch_bounds__(128) vi_ortho_kernel(OrthoArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.P * a.ndir) return;
+    const int64_t p = t % a.P;
+    const int d = (int)(t / a.P);
+    const double* R = a.R + (int64_t)d * ROT4_STRIDE;
+    OrthoDist q[3];
+    const double* xs[3] = {a.x1, a.x2, a.x3};
+    const bool derived = !a.x3 || !(a.x3[p] > 1e-8);   // n00 is invariant under rotation
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        if (m == 2 && derived) break;
+        double xr[15];
+        rotated_block(xs[m], a.P, p, a.nq, R, xr);
+        ortho_dist(a.A2, a.A4F, xr, q[m]);
+    }
+    if (derived) ortho_third(q[0], q[1], q[2]);
+    double Qn[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Qn[i] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        double J[6][6];
+        ortho_joints(q, derived, m, J);
+        const double* a2 = q[m].a2;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                // mu/2 (a2 + a2.kk + kk.a2 + (a2:kk) I) + lam a4:kk + lam' 2 sym2(J) ^{first-last} kk
+                const double anti = (j == 2 ? a2[3 * i + 2] : 0.0) + (i == 2 ? a2[3 * 2 + j] : 0.0);
+                double q4 = a4e(q[m], i, j, 2, 2);
+                const double jf = jt(J, 2, i, j, 2) + jt(J, j, 2, 2, i);
+                Qn[3 * i + j] += a.mu[m] / 2 * (a2[3 * i + j] + anti + a2[8] * (i == j ? 1.0 : 0.0)) + a.lam[m] * q4 +
+                                 a.lam[m + 3] * jf;
+            }
+    }
+    double vi[3];
+    qnorm_to_vi(Qn, a.rho, vi);
+    double* o = a.Vi + (p * a.ndir + d) * 3;
+    o[0] = vi[0];
+    o[1] = vi[1];
+    o[2] = vi[2];
+}
+
 #ifndef SFG_HOST_EMUL
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -364,6 +715,21 @@ hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s) {
 hipError_t launch_vi_em(const ViEMArgs& a, hipStream_t s) {
     if (a.P * a.ndir == 0) return hipSuccess;
     hipLaunchKernelGGL(vi_em_kernel, dim3(nblk(a.P * a.ndir, 128)), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_eij_ortho(const OrthoArgs& args, double* Qiso, hipStream_t s) {
+    if (args.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(ortho_Q_kernel, dim3(1), dim3(64), 0, s, args, Qiso);
+    OrthoArgs a = args;
+    a.Qiso = Qiso;
+    hipLaunchKernelGGL(eij_ortho_kernel, dim3(nblk(a.P, 64)), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s) {
+    if (a.P * a.ndir == 0) return hipSuccess;
+    hipLaunchKernelGGL(vi_ortho_kernel, dim3(nblk(a.P * a.ndir, 128)), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

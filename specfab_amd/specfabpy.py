@@ -382,6 +382,75 @@ def Vi_elastic_tranisotropic(nlm, alpha, Lame_grain, rho, theta, phi):
     return out.T.copy()
 
 
+# ---- orthotropic grains (olivine) ----
+
+def Cij_to_Lame_orthotropic(Cij):
+    """(C11,C22,C33,C44,C55,C66,C23,C13,C12) -> (lam11,lam22,lam33,lam23,lam13,lam12,mu1,mu2,mu3),
+    src/elasticities.f90:122-139."""
+    C = np.ascontiguousarray(np.asarray(Cij, dtype=np.float64).reshape(9))
+    out = np.zeros(9)
+    check(lib().sfg_Cij_to_Lame_orthotropic(dptr(C), dptr(out)))
+    return out
+
+
+def Lame_olivine_A2X(Lame, Xtype):
+    """A-type olivine Lame parameters reordered for B- or C-type fabrics,
+    src/elasticities.f90:141-154 (host arithmetic only)."""
+    L = np.asarray(Lame, dtype=np.float64).reshape(9)
+    order = {"B": [2, 1, 0, 5, 4, 3, 8, 7, 6], "C": [2, 0, 1, 5, 3, 4, 8, 6, 7]}
+    if Xtype not in order:
+        raise ValueError("Lame_olivine_A2X(): fabric type not supported")
+    return L[order[Xtype]].copy()
+
+
+def _ortho_states(q1, q2, q3):
+    L = _L_of(q1)
+    z1, z2 = _z(q1), _z(q2)
+    z3 = _z(q3) if q3 is not None else np.zeros_like(z1)
+    if len(z2) != len(z1) or len(z3) != len(z1):
+        raise ValueError("the three distributions must have the same length")
+    return L, z1, z2, z3
+
+
+def Eij_orthotropic(nlm_r1, nlm_r2, nlm_r3, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Enhancement factors of orthotropic grains (Sachs) w.r.t. (e1,e2,e3) from
+    the distributions of the slip-system axes (b, n, v); nlm_r3 = 0 derives v =
+    b x n: src/specfabpy.f90:436-446 -> enhancementfactors.f90:134-188."""
+    L, z1, z2, z3 = _ortho_states(nlm_r1, nlm_r2, nlm_r3)
+    v = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    out = np.zeros(6)
+    check(lib().sfg_Eij_orthotropic(_ctx(), L, _zp(z1), _zp(z2), _zp(z3), dptr(v(e1)), dptr(v(e2)), dptr(v(e3)),
+                                    dptr(v(Eij_grain).reshape(6)), float(alpha), int(n_grain), dptr(out)))
+    return out
+
+
+def Eij_orthotropic_arr(nlm_r1, nlm_r2, nlm_r3, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Batched orthotropic Eij (src/specfabpy.f90:488-500): nlm_ri (N, n), e_i
+    (N, 3) -> (N, 6), one GPU launch."""
+    import torch
+    from .batch import FabricBatch, eij_orthotropic
+    q1, q2 = np.asarray(nlm_r1), np.asarray(nlm_r2)
+    L = _L_of(q1[0])
+    b1, b2 = FabricBatch(L, nlm=q1), FabricBatch(L, nlm=q2)
+    b3 = FabricBatch(L, nlm=np.asarray(nlm_r3)) if nlm_r3 is not None else None
+    frames = np.stack([np.asarray(e1), np.asarray(e2), np.asarray(e3)], axis=1)
+    E = eij_orthotropic(b1, b2, b3, Eij_grain, alpha=alpha, n_grain=n_grain, frames=frames)
+    torch.cuda.synchronize()
+    return E.cpu().numpy()
+
+
+def Vi_elastic_orthotropic(nlm_1, nlm_2, nlm_3, alpha, lame_grain, rho, theta, phi):
+    """Elastic phase velocities (qS1, qS2, qP) x ndir of orthotropic grains (Voigt),
+    src/specfabpy.f90:975-984 -> wavepropagation.f90:41-68.  Returns (3, ndir)."""
+    L, z1, z2, z3 = _ortho_states(nlm_1, nlm_2, nlm_3)
+    th, ph = _dirs(theta, phi)
+    out = np.zeros((th.size, 3))
+    Lm = np.ascontiguousarray(np.asarray(lame_grain, dtype=np.float64).reshape(9))
+    check(lib().sfg_Vi_elastic_orthotropic(_ctx(), L, _zp(z1), _zp(z2), _zp(z3), float(alpha), dptr(Lm), float(rho),
+                                           th.size, dptr(th), dptr(ph), dptr(out)))
+    return out.T.copy()
+
+
 def Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi):
     """EM phase velocities (S1 fast, S2 slow) x ndir, src/specfabpy.f90:1024-1033
     -> src/wavepropagation.f90:217-261.  Returns shape (2, ndir)."""
