@@ -521,6 +521,94 @@ def bench_nonlinear(a):
     print(json.dumps(out), flush=True)
 
 
+# eij_ortho_kernel (v given, Cartesian frame) per parcel: algorithmic HBM
+# bytes (three l <= 4 blocks read, 6 FP64 written) and FP64 flop as
+# SQ_INSTS_VALU_FLOPS_FP64 counts them (x 64 lanes per parcel); vi_ortho_kernel
+# (v given) FP64 flop per (parcel, direction) likewise
+BYTES_ORTHO_EIJ = 3 * 15 * 8 + 6 * 8
+FLOPS_ORTHO_EIJ = 1569
+FLOPS_ORTHO_VI = 3879
+
+
+def bench_ortho(a):
+    """Orthotropic (olivine) grains on 2^20 parcels, each with the l <= 4
+    distributions of its three slip-system axes (b, n, v; seeded perturbed
+    isotropic states): Eij_orthotropic in the Cartesian frame
+    (src/enhancementfactors.f90:134-188) with v given and v = b x n derived,
+    Vi_elastic_orthotropic (src/wavepropagation.f90:41-68) along six
+    directions, Abramson (1997) olivine Lame parameters."""
+    import numpy as np
+    import torch
+    from specfab_amd import batch as B
+    from specfab_amd import workloads as W
+    from specfab_amd import specfabpy as sf
+    P = a.parcels
+    rng = np.random.default_rng(21)
+    bs = [B.FabricBatch(4, nlm=W.perturbed_isotropic(rng, P, 4, sigma=0.15)) for _ in range(3)]
+    Eg = (1.5, 1.0, 0.8, 1.2, 2.0, 10.0)
+    Cij = np.array([320.5e9, 196.5e9, 233.5e9, 64.0e9, 77.0e9, 78.7e9, 76.8e9, 71.6e9, 68.15e9])
+    Lame, rho = sf.Cij_to_Lame_orthotropic(Cij), 3355.0
+    th = np.array([0.0, 0.3, 1.2, np.pi / 2, np.pi / 2, np.deg2rad(70)])
+    ph = np.array([0.0, 0.0, 0.7, 0.0, 2.0, np.deg2rad(10)])
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        for _ in range(max(1, a.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(a.steps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / a.steps
+
+    ms = {"eij_given": timed(lambda: B.eij_orthotropic(bs[0], bs[1], bs[2], Eg)),
+          "eij_derived": timed(lambda: B.eij_orthotropic(bs[0], bs[1], None, Eg)),
+          "vi_given": timed(lambda: B.vi_elastic_orthotropic(bs[0], bs[1], bs[2], Lame, rho, th, ph)),
+          "vi_derived": timed(lambda: B.vi_elastic_orthotropic(bs[0], bs[1], None, Lame, rho, th, ph))}
+    E = B.eij_orthotropic(bs[0], bs[1], bs[2], Eg)
+    assert bool(torch.isfinite(E).all())
+    rate = {k: P * (len(th) if k.startswith("vi") else 1) / (v * 1e-3) for k, v in ms.items()}
+    # the Eij kernel's algorithmic bytes need longer at 8 TB/s than its flop at
+    # 78.6 TFLOP/s: HBM-bound
+    ach = BYTES_ORTHO_EIJ * P / (ms["eij_given"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+            "traffic": measured_traffic("eij_ortho_kernel", P, 1), "kernel": "eij_ortho_kernel",
+            "bytes_per_parcel": BYTES_ORTHO_EIJ, "flops_per_parcel": FLOPS_ORTHO_EIJ,
+            "fp64_frac": FLOPS_ORTHO_EIJ * P / (ms["eij_given"] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
+    out = {"metric": "Eij_orthotropic evals/s (olivine, Sachs, v given), FP64", "value": rate["eij_given"],
+           "unit": "parcel-evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": ms["eij_given"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded perturbed isotropic l<=4 distributions)",
+           "config": {"workload": "orthotropic grains: 2^20 parcels x 3 axis distributions (L=4), Eij "
+                                  "(Cartesian frame, v given / derived), Vi along 6 directions", "parcels": P},
+           "ms": ms, "evals_per_s": rate, "roofline": roof, "cpu_baseline": None}
+    if FLOPS_ORTHO_VI:
+        ach = FLOPS_ORTHO_VI * P * len(th) / (ms["vi_given"] * 1e-3) / 1e12
+        out["roofline_vi"] = {"bound": "fp64 valu", "achieved": ach, "peak": PEAK_TFLOPS["fp64"],
+                              "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["fp64"], "kernel": "vi_ortho_kernel",
+                              "flops_per_item": FLOPS_ORTHO_VI,
+                              "traffic": measured_traffic("vi_ortho_kernel", P, 1)}
+    if not a.no_cpu:
+        from oracle import ref
+        if ref.available():
+            r = ref.RefLib()
+            r.init(4)
+            Q = [W.perturbed_isotropic(np.random.default_rng(22), 64, 4, sigma=0.15) for _ in range(3)]
+            I = np.eye(3)
+            k, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 10.0:
+                j = k % 64
+                r.Eij_orthotropic(Q[0][j], Q[1][j], Q[2][j], I[0], I[1], I[2], Eg, 0.0, 1)
+                k += 1
+            el = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": k / el, "unit": "parcel-evals/s", "cores": 1, "kind": "reference",
+                                   "sample": "%d Eij_orthotropic calls (v given) of the reference FP64 Fortran "
+                                             "(oracle/_ref), %.1f s" % (k, el)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_discrete(a):
     """Discrete-grain (DDM) path of the reference's SDM-vs-DDM harness
     (tests/SDM-DDM-performance/SDM-DDM-performance.py:24-73): ri_LROT
@@ -624,11 +712,11 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="field",
-                    choices=["field", "uneven", "surrogate", "export", "discrete", "nonlinear"],
+                    choices=["field", "uneven", "surrogate", "export", "discrete", "nonlinear", "ortho"],
                     help="field: the L=12 Eulerian field (default, the headline); uneven: 2^18 parcels with mixed "
                          "L and step counts in one graph launch; surrogate: in-loop homogenization, plain vs "
                          "learned; export: operator export; discrete: ri_LROT / ri_to_nlm of 2^20 grains; "
-                         "nonlinear: n'=3 Eij, <D> and E_CAFFE on the field")
+                         "nonlinear: n'=3 Eij, <D> and E_CAFFE on the field; ortho: olivine Eij / Vi")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -660,6 +748,8 @@ def main():
         return bench_discrete(a)
     if a.workload == "nonlinear":
         return bench_nonlinear(a)
+    if a.workload == "ortho":
+        return bench_ortho(a)
     dev = torch.device("cuda:0")
     P, L = a.parcels, a.L
     G, gam0, x = make_field(P, L, 1, dev)

@@ -365,6 +365,31 @@ int sfg_Vi_elastic_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const doubl
                                      const double* x3_dev, double alpha, const double Lame[9], double rho, int ndir,
                                      const double* theta, const double* phi, double* Vi_dev, void* stream);
 
+/* Discrete grains: the axes b_i, n_i, v_i of N grains, (N, 3) each
+ * (ai_orthotropic_discrete, src/moments.f90:386-417, averages over the grains).
+ * Eij_orthotropic_discrete(bi, ni, vi, e1, e2, e3, Eij_grain, alpha, n_grain)
+ * (src/enhancementfactors.f90:191-266; specfabpy's _arr form :502-514 loops
+ * over ensembles): Sachs, normalised by the continuum isotropic fabric.
+ * Vi_elastic_orthotropic__discrete(mi, alpha, Lame_grain, rho, theta, phi)
+ * (src/wavepropagation.f90:70-96; specfabpy :986-995, mi (3, 3, N) =
+ * (axis, xyz, grain)): Voigt, (ndir, 3) = (qS1, qS2, qP) row-major.
+ * Batched: E ensembles of N grains, each axis array (E, N, 3) on the device;
+ * Eij_dev (E, 6), frames_dev (E, 3, 3) for frame 1; Vi_dev (E, ndir, 3);
+ * at most 65535 directions per call. */
+int sfg_Eij_orthotropic_discrete(sfg_ctx* ctx, int64_t N, const double* bi, const double* ni, const double* vi,
+                                 const double e1[3], const double e2[3], const double e3[3],
+                                 const double Eij_grain[6], double alpha, int n_grain, double Eij[6]);
+int sfg_Eij_orthotropic_discrete_batch(sfg_ctx* ctx, int64_t E, int64_t N, const double* b_dev, const double* n_dev,
+                                       const double* v_dev, int frame, const double* frames_dev,
+                                       const double Eij_grain[6], double alpha, int n_grain, double* Eij_dev,
+                                       void* stream);
+int sfg_Vi_elastic_orthotropic_discrete(sfg_ctx* ctx, int64_t N, const double* bi, const double* ni,
+                                        const double* vi, double alpha, const double Lame[9], double rho, int ndir,
+                                        const double* theta, const double* phi, double* Vi);
+int sfg_Vi_elastic_orthotropic_discrete_batch(sfg_ctx* ctx, int64_t E, int64_t N, const double* b_dev,
+                                              const double* n_dev, const double* v_dev, double alpha,
+                                              const double Lame[9], double rho, int ndir, const double* theta,
+                                              const double* phi, double* Vi_dev, void* stream);
 /* Cij_to_Lame_orthotropic (src/elasticities.f90:122-139): (C11, C22, C33, C44,
  * C55, C66, C23, C13, C12) -> (lam11, lam22, lam33, lam23, lam13, lam12, mu1,
  * mu2, mu3).  Host only. */

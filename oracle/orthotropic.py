@@ -1,7 +1,9 @@
 """Numpy restatement of the orthotropic-grain (olivine) homogenization:
 Eij_orthotropic (Sachs, src/enhancementfactors.f90:134-188 with
 src/homogenizations.f90:288-319) and Vi_elastic_orthotropic (Voigt,
-src/wavepropagation.f90:41-68, 98-115).
+src/wavepropagation.f90:41-68, 98-115), and their discrete-grain forms
+(src/enhancementfactors.f90:191-266, src/wavepropagation.f90:70-96,
+src/moments.f90:386-417).
 
 Test infrastructure: imported only by tests/ as the checker; nothing under
 specfab_amd/ imports it.
@@ -139,23 +141,76 @@ def Eij(q1, q2, q3, e1, e2, e3, Eg, oracle):
     return np.array([d22(d42(Q, tau), vw) / d22(d42(Qi, tau), vw) for tau, vw in _states(e1, e2, e3)])
 
 
-def Vi(q1, q2, q3, Lame, rho, theta, phi, oracle):
-    """(ndir, 3) velocities (qS1, qS2, qP) per direction"""
+def _qnorm(a2_i, a4_ii, a4_jk, Lame):
+    """ai_to_Qnorm_orthotropic (src/wavepropagation.f90:98-115), Voigt, k = z"""
     lam, mu = np.asarray(Lame[:6], float), np.asarray(Lame[6:], float)
     kk = np.zeros((3, 3))
     kk[2, 2] = 1
+    Qn = np.zeros((3, 3))
+    for i in range(3):
+        a2 = a2_i[i]
+        Qn += mu[i] / 2 * (a2 + a2 @ kk + kk @ a2 + np.einsum("ij,ji", a2, kk) * np.eye(3))
+        Qn += lam[i] * np.einsum("lkij,ji->lk", a4_ii[i], kk)
+        A = 2 * _sym2(a4_jk[i])
+        Qn += lam[i + 3] * np.einsum("lkij,jl->ki", A, kk)
+    return Qn
+
+
+def Vi(q1, q2, q3, Lame, rho, theta, phi, oracle):
+    """(ndir, 3) velocities (qS1, qS2, qP) per direction"""
     out = []
     for t, p in zip(np.atleast_1d(theta), np.atleast_1d(phi)):
         rot = lambda q: oracle.rotate_nlm4(np.asarray(q)[:15], -t, -p) if q is not None else None
-        a2_i, a4_ii, a4_jk = ai(rot(q1), rot(q2), rot(q3), oracle)
-        Qn = np.zeros((3, 3))
-        for i in range(3):
-            a2 = a2_i[i]
-            Qn += mu[i] / 2 * (a2 + a2 @ kk + kk @ a2 + np.einsum("ij,ji", a2, kk) * np.eye(3))
-            Qn += lam[i] * np.einsum("lkij,ji->lk", a4_ii[i], kk)
-            A = 2 * _sym2(a4_jk[i])
-            Qn += lam[i + 3] * np.einsum("lkij,jl->ki", A, kk)
-        out.append(qnorm_to_vi(Qn, rho))
+        out.append(qnorm_to_vi(_qnorm(*ai(rot(q1), rot(q2), rot(q3), oracle), Lame), rho))
+    return np.array(out)
+
+
+# ---- discrete grains: (N, 3) axes b_i, n_i, v_i of N grains ----
+
+def ai_discrete(bi, ni, vi):
+    """ai_orthotropic_discrete (src/moments.f90:386-417): grain averages of
+    q q, q q q q per axis and <q2 q2 q3 q3>, <q1 q1 q3 q3>, <q1 q1 q2 q2>"""
+    q = [np.asarray(x, float) for x in (bi, ni, vi)]
+    N = len(q[0])
+    a2_i = np.array([np.einsum("gi,gj->ij", x, x) / N for x in q])
+    a4_ii = np.array([np.einsum("gi,gj,gk,gl->ijkl", x, x, x, x) / N for x in q])
+    jk = [(1, 2), (0, 2), (0, 1)]
+    a4_jk = np.array([np.einsum("gi,gj,gk,gl->ijkl", q[j], q[j], q[k], q[k]) / N for j, k in jk])
+    return a2_i, a4_ii, a4_jk
+
+
+def Eij_discrete(bi, ni, vi, e1, e2, e3, Eg, oracle):
+    """Eij_orthotropic_discrete (src/enhancementfactors.f90:191-266): the
+    grains' moments in the Sachs tensor, normalised by the continuum
+    isotropic fabric's"""
+    iso = np.zeros(15, dtype=complex)
+    iso[0] = 1 / np.sqrt(4 * np.pi)
+    _, a4_ii, a4_jk = ai_discrete(bi, ni, vi)
+    Q = sachs_Q(a4_ii, a4_jk, Eg)
+    _, a4i, a4jki = ai(iso, iso, iso, oracle)
+    Qi = sachs_Q(a4i, a4jki, Eg)
+    d42 = lambda A, B: np.einsum("lkij,ji->lk", A, B)
+    d22 = lambda A, B: np.einsum("ij,ji->", A, B)
+    return np.array([d22(d42(Q, tau), vw) / d22(d42(Qi, tau), vw) for tau, vw in _states(e1, e2, e3)])
+
+
+def rotate_vector(v, theta, phi):
+    """src/rotation.f90:70-89: w = Ry(theta) Rz(phi) v"""
+    c, s = np.cos(phi), np.sin(phi)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+    c, s = np.cos(theta), np.sin(theta)
+    Ry = np.array([[c, 0, s], [0, 1.0, 0], [-s, 0, c]])
+    return np.asarray(v, float) @ (Ry @ Rz).T
+
+
+def Vi_discrete(bi, ni, vi, Lame, rho, theta, phi):
+    """Vi_elastic_orthotropic__discrete (src/wavepropagation.f90:70-96): the
+    grains' axes rotated by (-theta, -phi) per direction, their moments, the
+    Voigt acoustic tensor for k = z and the Cardano roots; (ndir, 3)"""
+    out = []
+    for t, p in zip(np.atleast_1d(theta), np.atleast_1d(phi)):
+        r = [rotate_vector(x, -t, -p) for x in (bi, ni, vi)]
+        out.append(qnorm_to_vi(_qnorm(*ai_discrete(*r), Lame), rho))
     return np.array(out)
 
 

@@ -177,6 +177,54 @@ def vi_elastic_orthotropic(b1, b2, b3, lame_grain, rho, theta, phi, alpha=1.0, c
     return Vi
 
 
+def _axes(t, name):
+    t = torch.as_tensor(t).to(torch.float64)
+    if t.dim() == 2:
+        t = t.unsqueeze(0)
+    if t.dim() != 3 or t.shape[2] != 3:
+        raise ValueError("%s must be (E, N, 3) or (N, 3)" % name)
+    return t.contiguous()
+
+
+def eij_orthotropic_discrete(bi, ni, vi, Eij_grain, alpha=0.0, n_grain=1, frames=None, ctx=None):
+    """Orthotropic Eij (E, 6) (Sachs; src/enhancementfactors.f90:191-266) of E
+    ensembles of N grains with slip-system axes bi, ni, vi (E, N, 3) device
+    tensors, in the caller frames (E, 3, 3) or the Cartesian frame."""
+    ctx = ctx or _lib.default_context()
+    b, n, v = _axes(bi, "bi"), _axes(ni, "ni"), _axes(vi, "vi")
+    if n.shape != b.shape or v.shape != b.shape:
+        raise ValueError("bi, ni, vi must have the same shape")
+    E, N = b.shape[0], b.shape[1]
+    fr = None
+    if frames is not None:
+        fr = torch.as_tensor(frames).to(b.device, torch.float64).reshape(E, 3, 3).contiguous()
+    out = torch.empty((E, 6), dtype=torch.float64, device=b.device)
+    Eg = (ctypes.c_double * 6)(*np.asarray(Eij_grain, dtype=np.float64).reshape(6))
+    check(lib().sfg_Eij_orthotropic_discrete_batch(ctx.handle, E, N, _ptr(b), _ptr(n), _ptr(v),
+                                                   1 if fr is not None else 2, _ptr(fr) if fr is not None else None,
+                                                   Eg, float(alpha), int(n_grain), _ptr(out), _stream()))
+    return out
+
+
+def vi_elastic_orthotropic_discrete(bi, ni, vi, lame_grain, rho, theta, phi, alpha=1.0, ctx=None):
+    """Orthotropic elastic phase velocities (E, ndir, 3) = (qS1, qS2, qP)
+    (Voigt; src/wavepropagation.f90:70-96) of E ensembles of N grains with
+    axes bi, ni, vi (E, N, 3) along (theta, phi)."""
+    ctx = ctx or _lib.default_context()
+    b, n, v = _axes(bi, "bi"), _axes(ni, "ni"), _axes(vi, "vi")
+    if n.shape != b.shape or v.shape != b.shape:
+        raise ValueError("bi, ni, vi must have the same shape")
+    E, N = b.shape[0], b.shape[1]
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+    Vi = torch.empty((E, th.size, 3), dtype=torch.float64, device=b.device)
+    Lm = (ctypes.c_double * 9)(*np.asarray(lame_grain, dtype=np.float64).reshape(9))
+    check(lib().sfg_Vi_elastic_orthotropic_discrete_batch(ctx.handle, E, N, _ptr(b), _ptr(n), _ptr(v), float(alpha),
+                                                          Lm, float(rho), th.size, th.ctypes.data_as(_lib._dp),
+                                                          ph.ctypes.data_as(_lib._dp), _ptr(Vi), _stream()))
+    return Vi
+
+
 class FabricBatch:
     """P parcels at truncation L with per-parcel kinematics.
 

@@ -37,6 +37,14 @@ def gen_tables(force=False):
         run([sys.executable, os.path.join(HERE, "tables", "make_gaunt.py"), inc])
     run([sys.executable, os.path.join(HERE, "tables", "make_a6_map.py"), os.path.join(GEN, "sfg_a6_map.h")])
     run([sys.executable, os.path.join(HERE, "tables", "make_moment_maps.py"), os.path.join(GEN, "sfg_moments68.h")])
+    # the l <= 4 a2 / a4 maps as compile-time tables (host program over the map code)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    gen = os.path.join(objdir, "gen_l4maps")
+    run([os.environ.get("CXX_HOST", "g++"), "-std=c++17", "-O1", "-I", CSRC,
+         os.path.join(HERE, "tables", "gen_l4maps.cpp"), os.path.join(CSRC, "sfg_wave_host.cpp"),
+         os.path.join(CSRC, "sfg_basis.cpp"), "-o", gen])
+    run([gen, os.path.join(GEN, "sfg_l4maps.h")])
     gen_fast = os.path.join(HERE, "tables", "gen_fast.py")
     if os.path.exists(gen_fast):
         run([sys.executable, gen_fast, GEN])
@@ -86,7 +94,7 @@ def compile_fortran():
     exe = os.path.join(out, "c1_parcel")
     run([fc, "-O2", "-J", out, "-c", os.path.join(HERE, "fortran", "specfab_gpu.f90"), "-o",
          os.path.join(out, "specfab_gpu.o")])
-    for prog in ("c1_parcel", "homog_faces", "discrete_faces"):
+    for prog in ("c1_parcel", "homog_faces", "discrete_faces", "ortho_faces"):
         run([fc, "-O2", "-I", out, os.path.join(os.path.dirname(HERE), "tests", "fortran", prog + ".f90"),
              os.path.join(out, "specfab_gpu.o"), "-L", HERE, "-lspecfab_gpu", "-L", rocm_lib, "-lamdhip64",
              "-Wl,-rpath," + HERE, "-Wl,-rpath," + rocm_lib, "-o", os.path.join(out, prog)])

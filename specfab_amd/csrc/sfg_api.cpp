@@ -278,7 +278,8 @@ int get_dirs(sfg_ctx* ctx, int ndir, const double* theta, const double* phi, con
             *R = ds.R;
             return SFG_OK;
         }
-    std::vector<double> host((size_t)ndir * sfg::ROT4_STRIDE);
+    // the rotation blocks, then the unit directions (sin t cos p, sin t sin p, cos t)
+    std::vector<double> host((size_t)ndir * (sfg::ROT4_STRIDE + 3));
     for (int d = 0; d < ndir; d++) {
         double R2[5][5], R4[9][9];
         sfg::rotation_l4(-theta[d], -phi[d], R2, R4);
@@ -287,6 +288,10 @@ int get_dirs(sfg_ctx* ctx, int ndir, const double* theta, const double* phi, con
             for (int j = 0; j < 5; j++) o[5 * i + j] = R2[i][j];
         for (int i = 0; i < 9; i++)
             for (int j = 0; j < 9; j++) o[25 + 9 * i + j] = R4[i][j];
+        double* k = host.data() + (size_t)ndir * sfg::ROT4_STRIDE + 3 * d;
+        k[0] = std::sin(theta[d]) * std::cos(phi[d]);
+        k[1] = std::sin(theta[d]) * std::sin(phi[d]);
+        k[2] = std::cos(theta[d]);
     }
     if (ctx->dirs.size() >= 16) {
         // the oldest set may still be read by queued kernels
@@ -300,6 +305,16 @@ int get_dirs(sfg_ctx* ctx, int ndir, const double* theta, const double* phi, con
     HIPCHK(upload(&ds.R, host));
     ctx->dirs.push_back(ds);
     *R = ctx->dirs.back().R;
+    return SFG_OK;
+}
+
+// unit propagation directions (ndir, 3) of the direction set, stored after its
+// rotation blocks
+int get_kdirs(sfg_ctx* ctx, int ndir, const double* theta, const double* phi, const double** K) {
+    const double* R;
+    int rc = get_dirs(ctx, ndir, theta, phi, &R);
+    if (rc) return rc;
+    *K = R + (size_t)ndir * sfg::ROT4_STRIDE;
     return SFG_OK;
 }
 
@@ -1458,16 +1473,12 @@ int eij_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const dou
     if (rc) return rc;
     HIPCHK(hipSetDevice(ctx->device));
     sfg::OrthoArgs a{};
-    a.P = P; a.nq = op->nq; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.A2 = op->A2; a.A4F = op->A4F;
+    a.P = P; a.nq = op->nq; a.x1 = x1; a.x2 = x2; a.x3 = x3;
     a.frames = frame == 1 ? frames_dev : nullptr;
     ortho_ci(Eg, n_grain, a.ci);
     a.Eij = Eij_dev;
-    void* ws;
-    WsLease lease(ctx, s);
-    rc = lease.acquire(81 * sizeof(double), &ws);
-    if (rc) return rc;
-    HIPCHK(sfg::launch_eij_ortho(a, (double*)ws, s));
-    return lease.release();
+    HIPCHK(sfg::launch_eij_ortho(a, s));
+    return SFG_OK;
 }
 
 int vi_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const double* x2, const double* x3,
@@ -1482,7 +1493,7 @@ int vi_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const doub
     rc = get_dirs(ctx, ndir, theta, phi, &R);
     if (rc) return rc;
     sfg::OrthoArgs a{};
-    a.P = P; a.nq = op->nq; a.ndir = ndir; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.A2 = op->A2; a.A4F = op->A4F; a.R = R;
+    a.P = P; a.nq = op->nq; a.ndir = ndir; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.R = R;
     for (int i = 0; i < 6; i++) a.lam[i] = Lame[i];
     for (int i = 0; i < 3; i++) a.mu[i] = Lame[6 + i];
     a.rho = rho;
@@ -1573,6 +1584,148 @@ int sfg_Vi_elastic_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const do
                             hipStream_t s) {
                             return vi_ortho_locked(ctx, L, 1, x1, x2, x3, Lame, rho, ndir, theta, phi, dout, s);
                         });
+}
+
+// ---------------------------------------------------------------- orthotropic grains, discrete axes
+
+extern "C++" {
+namespace {
+// grain averages (E, 3, 57) of E ensembles of N grains into the lease's workspace
+int ortho_disc_moments(sfg_ctx* ctx, int64_t E, int64_t N, const double* b, const double* n, const double* v,
+                       WsLease& lease, size_t extra, double** mom, double** rest, hipStream_t s) {
+    if (N < 1) return fail(SFG_ERR_ARG, "an ensemble needs at least one grain");
+    if (!b || !n || !v) return fail(SFG_ERR_ARG, "NULL axis array");
+    HIPCHK(hipSetDevice(ctx->device));
+    // grain chunks of >= 1024 grains per workgroup, >= ~1024 workgroups overall
+    int64_t G = std::min<int64_t>((N + 1023) / 1024, (1024 + E - 1) / E);
+    G = std::max<int64_t>(G, 1);
+    if (E * G > 0x7fffffff) return fail(SFG_ERR_ARG, "too many ensembles");
+    sfg::OrthoDiscArgs a{};
+    a.E = E; a.N = N; a.G = (int)G; a.b = b; a.n = n; a.v = v;
+    void* ws;
+    const size_t nm = (size_t)E * 3 * sfg::ORTHO_NMOM;
+    int rc = lease.acquire((nm * (G + 1) + extra) * sizeof(double), &ws);
+    if (rc) return rc;
+    a.part = (double*)ws;
+    a.mom = a.part + nm * G;
+    HIPCHK(sfg::launch_ortho_moments(a, s));
+    *mom = a.mom;
+    if (rest) *rest = a.mom + nm;
+    return SFG_OK;
+}
+
+int eij_ortho_disc_locked(sfg_ctx* ctx, int64_t E, int64_t N, const double* b, const double* n, const double* v,
+                          int frame, const double* frames_dev, const double Eg[6], double alpha, int n_grain,
+                          double* Eij_dev, hipStream_t s) {
+    (void)alpha;   // Sachs only, as the reference (src/enhancementfactors.f90:258-266)
+    if (n_grain != 1) return fail(SFG_ERR_ARG, "orthotropic Sachs homogenization supports n_grain = 1 only");
+    if (frame != 1 && frame != 2) return fail(SFG_ERR_ARG, "frame must be 1 (given frames) or 2 (Cartesian)");
+    if (frame == 1 && !frames_dev) return fail(SFG_ERR_ARG, "frames missing");
+    WsLease lease(ctx, s);
+    double* mom;
+    int rc = ortho_disc_moments(ctx, E, N, b, n, v, lease, 0, &mom, nullptr, s);
+    if (rc) return rc;
+    sfg::OrthoMomArgs a{};
+    a.E = E; a.mom = mom; a.frames = frame == 1 ? frames_dev : nullptr; a.Eij = Eij_dev;
+    ortho_ci(Eg, n_grain, a.ci);
+    HIPCHK(sfg::launch_eij_ortho_mom(a, s));
+    return lease.release();
+}
+
+int vi_ortho_disc_locked(sfg_ctx* ctx, int64_t E, int64_t N, const double* b, const double* n, const double* v,
+                         const double Lame[9], double rho, int ndir, const double* theta, const double* phi,
+                         double* Vi_dev, hipStream_t s) {
+    if (ndir > 65535) return fail(SFG_ERR_ARG, "at most 65535 directions per call");
+    const double* K;
+    int rc = get_kdirs(ctx, ndir, theta, phi, &K);
+    if (rc) return rc;
+    WsLease lease(ctx, s);
+    double* mom;
+    rc = ortho_disc_moments(ctx, E, N, b, n, v, lease, 0, &mom, nullptr, s);
+    if (rc) return rc;
+    sfg::OrthoMomArgs a{};
+    a.E = E; a.mom = mom; a.ndir = ndir; a.kdir = K; a.rho = rho; a.Vi = Vi_dev;
+    for (int i = 0; i < 6; i++) a.lam[i] = Lame[i];
+    for (int i = 0; i < 3; i++) a.mu[i] = Lame[6 + i];
+    HIPCHK(sfg::launch_vi_ortho_mom(a, s));
+    return lease.release();
+}
+
+// host axes (N, 3) each -> one device ensemble, fn on it, nout doubles back
+template <class Fn>
+int ortho_disc_single(sfg_ctx* ctx, int64_t N, const double* b, const double* n, const double* v, size_t nout,
+                      double* out, Fn fn) {
+    if (!ctx || !b || !n || !v || !out) return fail(SFG_ERR_ARG, "NULL argument");
+    if (N < 1) return fail(SFG_ERR_ARG, "an ensemble needs at least one grain");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, ((size_t)9 * N + nout + 9) * sizeof(double)));
+    double* d = (double*)buf.p;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(d, b, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d + 3 * N, n, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d + 6 * N, v, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+    double* dout = d + 9 * N;
+    int rc = fn(d, d + 3 * N, d + 6 * N, dout, dout + nout, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int sfg_Eij_orthotropic_discrete_batch(sfg_ctx* ctx, int64_t E, int64_t N, const double* b_dev, const double* n_dev,
+                                       const double* v_dev, int frame, const double* frames_dev,
+                                       const double Eij_grain[6], double alpha, int n_grain, double* Eij_dev,
+                                       void* stream) {
+    if (!ctx || !Eij_grain || E < 0 || (E > 0 && !Eij_dev)) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (E == 0) return SFG_OK;
+    return eij_ortho_disc_locked(ctx, E, N, b_dev, n_dev, v_dev, frame, frames_dev, Eij_grain, alpha, n_grain,
+                                 Eij_dev, pick(ctx, stream));
+}
+
+int sfg_Eij_orthotropic_discrete(sfg_ctx* ctx, int64_t N, const double* bi, const double* ni, const double* vi,
+                                 const double e1[3], const double e2[3], const double e3[3],
+                                 const double Eij_grain[6], double alpha, int n_grain, double Eij[6]) {
+    if (!e1 || !e2 || !e3 || !Eij_grain) return fail(SFG_ERR_ARG, "NULL argument");
+    const double f[9] = {e1[0], e1[1], e1[2], e2[0], e2[1], e2[2], e3[0], e3[1], e3[2]};
+    return ortho_disc_single(ctx, N, bi, ni, vi, 6, Eij,
+                             [&](const double* b, const double* n, const double* v, double* dout, double* dfr,
+                                 hipStream_t s) {
+                                 HIPCHK(hipMemcpyAsync(dfr, f, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+                                 return eij_ortho_disc_locked(ctx, 1, N, b, n, v, 1, dfr, Eij_grain, alpha, n_grain,
+                                                              dout, s);
+                             });
+}
+
+int sfg_Vi_elastic_orthotropic_discrete_batch(sfg_ctx* ctx, int64_t E, int64_t N, const double* b_dev,
+                                              const double* n_dev, const double* v_dev, double alpha,
+                                              const double Lame[9], double rho, int ndir, const double* theta,
+                                              const double* phi, double* Vi_dev, void* stream) {
+    (void)alpha;   // Voigt only, as the reference (src/wavepropagation.f90:73)
+    if (!ctx || !Lame || E < 0 || ndir < 0 || (E > 0 && ndir > 0 && (!Vi_dev || !theta || !phi)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (E == 0 || ndir == 0) return SFG_OK;
+    return vi_ortho_disc_locked(ctx, E, N, b_dev, n_dev, v_dev, Lame, rho, ndir, theta, phi, Vi_dev,
+                                pick(ctx, stream));
+}
+
+int sfg_Vi_elastic_orthotropic_discrete(sfg_ctx* ctx, int64_t N, const double* bi, const double* ni,
+                                        const double* vi, double alpha, const double Lame[9], double rho, int ndir,
+                                        const double* theta, const double* phi, double* Vi) {
+    (void)alpha;
+    if (ndir <= 0) return ndir == 0 ? SFG_OK : fail(SFG_ERR_ARG, "ndir < 0");
+    if (!Lame || !theta || !phi) return fail(SFG_ERR_ARG, "NULL argument");
+    return ortho_disc_single(ctx, N, bi, ni, vi, (size_t)3 * ndir, Vi,
+                             [&](const double* b, const double* n, const double* v, double* dout, double*,
+                                 hipStream_t s) {
+                                 return vi_ortho_disc_locked(ctx, 1, N, b, n, v, Lame, rho, ndir, theta, phi, dout,
+                                                             s);
+                             });
 }
 
 // ---------------------------------------------------------------- rotation, a4 eigentensors, validity

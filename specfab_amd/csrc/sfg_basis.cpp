@@ -71,6 +71,20 @@ void lrot_weights(const cd qe[5], const cd qo[3], cd g0[6], cd gz[6], cd gn[6], 
     gp[0] = gp0 + c56 * qe[3]; gp[1] = 2.0 * qe[1]; gp[2] = c32 * qe[2]; gp[3] = c23 * qe[3]; gp[4] = qe[4]; gp[5] = 0;
 }
 
+// Linear part of the Mandel a2 body over the complex l <= 4 block
+// (src/include/ev_c2__body.f90 without the I/3 term and the 1/n00):
+// v = (xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy).
+void a2_linear(const cd* z, double v[6]) {
+    const double c = std::sqrt(2.0 / 3), s = std::sqrt(2 / 15.0), r2 = std::sqrt(2.0);
+    cd h0 = z[3], h1 = z[4], h2 = z[5];
+    double e11 = -0.5 * c * h0.real() + h2.real();
+    double e22 = -0.5 * c * h0.real() - h2.real();
+    double e33 = c * h0.real();
+    double e12 = -h2.imag(), e13 = -h1.real(), e23 = h1.imag();
+    v[0] = s * e11; v[1] = s * e22; v[2] = s * e33;
+    v[3] = r2 * (s * e23); v[4] = r2 * (s * e13); v[5] = r2 * (s * e12);
+}
+
 namespace {
 
 int regcalib(int L, double* expo, double* nu) {
@@ -89,20 +103,8 @@ int regcalib(int L, double* expo, double* nu) {
     }
 }
 
-// Linear part of the Mandel a2/a4 bodies over the complex l <= 4 block.
-// a2: src/include/ev_c2__body.f90 without the I/3 term and the 1/n00;
-// a4: src/include/ev_c4_Mandel__body.f90 times k/sqrt(4 pi) (src/moments.f90:211-218).
-void a2_linear(const cd* z, double v[6]) {
-    const double c = std::sqrt(2.0 / 3), s = std::sqrt(2 / 15.0), r2 = std::sqrt(2.0);
-    cd h0 = z[3], h1 = z[4], h2 = z[5];
-    double e11 = -0.5 * c * h0.real() + h2.real();
-    double e22 = -0.5 * c * h0.real() - h2.real();
-    double e33 = c * h0.real();
-    double e12 = -h2.imag(), e13 = -h1.real(), e23 = h1.imag();
-    v[0] = s * e11; v[1] = s * e22; v[2] = s * e33;
-    v[3] = r2 * (s * e23); v[4] = r2 * (s * e13); v[5] = r2 * (s * e12);
-}
-
+// Linear part of the Mandel a4 body over the complex l <= 4 block:
+// src/include/ev_c4_Mandel__body.f90 times k/sqrt(4 pi) (src/moments.f90:211-218).
 void a4_linear(const cd* z, double a4v[36]) {
     const double PI = 3.141592653589793;
     double n00 = z[0].real();

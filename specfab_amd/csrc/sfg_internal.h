@@ -71,6 +71,11 @@ void real4_index(int q, int* l, int* am, int* c);
 // (1111,1112,1113,1122,1123,1133,1222,1223,1233,1333,2222,2223,2233,2333,3333)
 // of a state x are (A4F x) / x0.
 void a4full_map(double A4F[15][15]);
+// a2 body (src/include/ev_c2__body.f90) over the complex l <= 4 block:
+// Mandel vector (xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy) without I/3 and 1/n00
+void a2_linear(const std::complex<double>* z, double v[6]);
+// a2 tensor entries (xx, yy, zz, yz, xz, xy) of a state x are (A2T x) / x0 + I/3
+void a2_tensor_map(double A2T[6][15]);
 // rotate_nlm4 (src/rotation.f90:14-27) as real 5x5 / 9x9 blocks.
 void rotation_l4(double theta, double phi, double R2[5][5], double R4[9][9]);
 // Terms of the Wigner small-d entries d^l_{mn}(theta), l = 2, 4, ..., L,

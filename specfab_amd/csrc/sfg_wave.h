@@ -61,18 +61,52 @@ struct OrthoArgs {
     const double* x1;     // (n, P) real states of q1, q2, q3 (x3 null: v = b x n)
     const double* x2;
     const double* x3;
-    const double* A2;     // (6, 15) Mandel a2 map
-    const double* A4F;    // (15, 15) full a4 map (REAL(4) body), index-sorted entries
     const double* R;      // Vi: (ndir, ROT4_STRIDE)
     const double* frames; // Eij: (P, 3, 3) rows e1, e2, e3, or null (Cartesian)
     double ci[6];         // Eij: 4/3 lami(1:3), 2 lami(4:6) (rheo_params_orthotropic)
-    const double* Qiso;   // Eij: the isotropic fabric's Sachs tensor Q (81, row-major lkij; device)
+    double q6iso[36];     // Eij: the isotropic fabric's Sachs tensor, pair form (set by the launch)
+    double dqiso;         //      and its (3,2,1,2) excess
     double lam[6], mu[3], rho;   // Vi: grain Lame parameters, density
     double* Eij;          // (P, 6)
     double* Vi;           // (P, ndir, 3): qS1, qS2, qP
 };
-// Eij: Qiso is device scratch of 81 doubles for the isotropic fabric's Q
-hipError_t launch_eij_ortho(const OrthoArgs& a, double* Qiso, hipStream_t s);
+// Orthotropic grains given as discrete axes (src/moments.f90:386-417): E
+// ensembles of N grains, each axis array (E, N, 3).  The moment pass sums,
+// per ensemble and axis m, the 57 grain averages a2_m (6 pairs), a4_m (15
+// sorted entries) and J_m = <q q q' q'> (6 x 6 pairs; m = 0: <n n v v>,
+// 1: <b b v v>, 2: <b b n n>), in G partial sums per ensemble.
+constexpr int ORTHO_NMOM = 57;
+struct OrthoDiscArgs {
+    int64_t E, N;
+    int G;                  // workgroups per ensemble (grain chunks)
+    const double* b;
+    const double* n;
+    const double* v;
+    double* part;           // (E, G, 3, 57) partial sums
+    double* mom;            // (E, 3, 57) grain averages
+};
+hipError_t launch_ortho_moments(const OrthoDiscArgs& a, hipStream_t s);
+// consumers of the averages: Eij (frames as for OrthoArgs, isotropic
+// denominator of the continuum isotropic fabric) and Vi along unit
+// directions kdir (ndir, 3)
+struct OrthoMomArgs {
+    int64_t E;
+    const double* mom;
+    const double* frames;   // (E, 3, 3) or null (Cartesian)
+    double ci[6];
+    double q6iso[36];
+    double dqiso;
+    double* Eij;            // (E, 6)
+    int ndir;
+    const double* kdir;     // (ndir, 3)
+    double lam[6], mu[3], rho;
+    double* Vi;             // (E, ndir, 3)
+};
+hipError_t launch_eij_ortho_mom(const OrthoMomArgs& a, hipStream_t s);
+hipError_t launch_vi_ortho_mom(const OrthoMomArgs& a, hipStream_t s);
+// Q6 (pair form) and the (3,2,1,2) excess of the isotropic fabric's Sachs tensor
+void ortho_q6_iso(const double ci[6], double Q6[36], double* dq);
+hipError_t launch_eij_ortho(const OrthoArgs& a, hipStream_t s);
 hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s);
 
 hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s);

@@ -20,8 +20,9 @@
 //      REAL(4) exponents (Qnorm_to_vi, :267-324).
 #include <algorithm>
 
-#include "sfg_a4slots.h"
 #include "sfg_device.h"
+#include "generated/sfg_l4maps.h"
+#include "sfg_a4slots.h"
 #include "sfg_wave.h"
 
 namespace sfg {
@@ -352,267 +353,306 @@ __global__ void __launch_bounds__(128) a4full_kernel(A4FullArgs a) {
 
 // ---------------------------------------------------------------------------
 // Orthotropic grains.  Each slip-system axis distribution contributes its a2
-// (the reference's a2 body, through the Mandel map) and its full a4 (the
-// REAL(4) body, its (3,2,1,2) entry included); the joint tensors follow from
-// the two distributions' moments under the product density with the kernel
-// sin^2 of the axes' angle, as the reference's generated bodies evaluate them
-// (src/moments.f90:242-310; identified against them to their REAL(4) level):
+// (the reference's a2 body) and its full a4 (the REAL(4) body, its (3,2,1,2)
+// entry included), both from the compile-time maps of the l <= 4 block; the
+// joint tensors follow from the two distributions' moments under the product
+// density with the kernel sin^2 of the axes' angle, as the reference's
+// generated bodies evaluate them (src/moments.f90:242-310; identified against
+// them to their REAL(4) level):
 //   a4_joint(b, n)      = (a2b (x) a2n - a4b : a4n) / (1 - a2b : a2n)       <b b n n>
 //   a4_jointcross(b, n) = eps eps (a4n (x) a2b) / (1 - a2b : a2n)           <n n v v>
 //   a2_orth(b, n)       = eps eps (a2b (x) a2n) / (1 - a2b : a2n)           <v v>
-//   a4_orth(b, n)       = eps^4 (a4b (x) a4n) / <sin^4> = ... / (1 - 2 a2b:a2n + a4b::a4n)
+//   a4_orth(b, n)       = eps^4 (a4b (x) a4n) / (1 - 2 a2b:a2n + a4b::a4n)   <v v v v>
 // with v = b x n and eps the Levi-Civita symbol.
+//
+// Sachs (src/homogenizations.f90:288-319): Q = sum_m ci_m M2_m + ci_m+3 H2_m
+// with M2_m = (a4_jj + a4_kk - 2 sym2(J_m)) / 4 and H2_m = sym4(J_m).  Apart
+// from the body a4's (3,2,1,2) entry, Q is symmetric within its index pairs
+// (lk) and (ij), so it is held as a 6 x 6 pair matrix Q6 plus that entry's
+// excess dq; the a4 terms combine into one symmetric tensor
+// S = sum_d w_d a4_d, w_d = (ci_0 + ci_1 + ci_2 - ci_d) / 4, and each joint
+// entry is added into Q6 where sym2 and sym4 place it (tabulated below).
+// eps(tau) : vw is then a 6 x 6 contraction per stress state.
 
 namespace {
 
-struct OrthoDist {
-    double a2[9];    // a2 (matrix)
-    double u[15];    // a4 body, index-sorted entries
-    bool sym;        // a4 from a formula: symmetric (no (3,2,1,2) entry)
+// pair <-> indices: xx yy zz yz xz xy
+__host__ __device__ constexpr int pa(int P) { return P < 3 ? P : (P == 3 ? 1 : 0); }
+__host__ __device__ constexpr int pb(int P) { return P < 3 ? P : (P == 5 ? 1 : 2); }
+__host__ __device__ constexpr int pidx(int a, int b) { return a == b ? a : 6 - a - b; }
+__host__ __device__ constexpr double pw(int P) { return P < 3 ? 1.0 : 2.0; }   // orderings of a pair
+__host__ __device__ constexpr int levi(int i, int j, int k) {
+    return (i == j || j == k || i == k) ? 0 : (((j - i + 3) % 3 == 1) ? 1 : -1);
+}
+// sorted slot of the symmetric a4 entry (pair X, pair Y)
+__host__ __device__ constexpr int sxy(int X, int Y) { return kA4Slots.sym[27 * pa(X) + 9 * pb(X) + 3 * pa(Y) + pb(Y)]; }
+
+// Destinations of the pair entry J[X][Y] in sym4(J)(l,k,i,j) =
+// (J(l,i,k,j) + J(i,k,l,j) + J(l,j,i,k) + J(j,k,i,l)) / 4, (l,k) = P,
+// (i,j) = R: at most four pair entries (P, R), each receiving it c times
+struct H2Dst {
+    signed char P[36][4], R[36][4], c[36][4];
+};
+__host__ __device__ constexpr H2Dst make_h2dst() {
+    H2Dst m{};
+    for (int P = 0; P < 6; P++)
+        for (int R = 0; R < 6; R++) {
+            const int l = pa(P), k = pb(P), i = pa(R), j = pb(R);
+            const int xy[4] = {6 * pidx(l, i) + pidx(k, j), 6 * pidx(i, k) + pidx(l, j),
+                               6 * pidx(l, j) + pidx(i, k), 6 * pidx(j, k) + pidx(i, l)};
+            for (int t = 0; t < 4; t++) {
+                const int e = xy[t];
+                int slot = 0;
+                while (slot < 4 && m.c[e][slot] != 0 && !(m.P[e][slot] == P && m.R[e][slot] == R)) slot++;
+                m.P[e][slot] = (signed char)P;
+                m.R[e][slot] = (signed char)R;
+                m.c[e][slot]++;
+            }
+        }
+    return m;
+}
+constexpr H2Dst kH2 = make_h2dst();
+
+struct ODist {
+    double a2[6];   // a2 pairs (xx yy zz yz xz xy)
+    double u[15];   // a4 index-sorted entries
 };
 
-// pair index of (a, b), a symmetric pair: xx yy zz yz xz xy
-__device__ __forceinline__ constexpr int pidx(int a, int b) { return a == b ? a : 6 - a - b; }
-
-__device__ __forceinline__ double levi(int i, int j, int k) {
-    return (i == j || j == k || i == k) ? 0.0 : (((j - i + 3) % 3 == 1) ? 1.0 : -1.0);
-}
-
-// a2 matrix and a4 body entries of the l <= 4 block xr (x0 = xr[0])
-__device__ __forceinline__ void ortho_dist(const double* A2, const double* A4F, const double* xr, OrthoDist& d) {
-    const double x0 = xr[0], r2i = 0.70710678118654752440;
-    double a2v[6];
+// a2 and a4 body of the l <= 4 block x (x[0] = n00 > 0)
+__host__ __device__ __forceinline__ void odist(const double (&x)[15], ODist& d) {
+    const double r = 1.0 / x[0];
 #pragma unroll
-    for (int i = 0; i < 6; i++) a2v[i] = map_row(A2, i, xr, 15) / x0 + (i < 3 ? 1.0 / 3.0 : 0.0);
-    d.a2[0] = a2v[0], d.a2[4] = a2v[1], d.a2[8] = a2v[2];
-    d.a2[5] = d.a2[7] = a2v[3] * r2i, d.a2[2] = d.a2[6] = a2v[4] * r2i, d.a2[1] = d.a2[3] = a2v[5] * r2i;
+    for (int i = 0; i < 6; i++) {
+        double s = 0.0;
 #pragma unroll
-    for (int e = 0; e < 15; e++) d.u[e] = map_row(A4F, e, xr, 15) / x0;
-    d.sym = false;
+        for (int q = 0; q < 15; q++)
+            if (kL4Maps.a2[i][q] != 0.0) s += kL4Maps.a2[i][q] * x[q];
+        d.a2[i] = s * r + (i < 3 ? 1.0 / 3.0 : 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 15; e++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 15; q++)
+            if (kL4Maps.a4[e][q] != 0.0) s += kL4Maps.a4[e][q] * x[q];
+        d.u[e] = s * r;
+    }
 }
 
-__device__ __forceinline__ double a4e(const OrthoDist& d, int i, int j, int k, int l) {
-    const int t = 27 * i + 9 * j + 3 * k + l;
-    return d.sym ? d.u[kA4Slots.sym[t]] : d.u[kA4Slots.s[t]];
-}
-__device__ __forceinline__ double a4sym(const OrthoDist& d, int i, int j, int k, int l) {
-    return d.u[kA4Slots.sym[27 * i + 9 * j + 3 * k + l]];
-}
-__device__ __forceinline__ double dd22(const double* A, const double* B) {
+__host__ __device__ __forceinline__ double a2dot(const ODist& b, const ODist& n) {
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; i++) s += A[i] * B[i];
+    for (int P = 0; P < 6; P++) s += pw(P) * (b.a2[P] * n.a2[P]);
     return s;
 }
 
-// J[P][R] (pair indices) of a4_joint(b, n)
-__device__ __forceinline__ void a4_joint6(const OrthoDist& b, const OrthoDist& n, double (&J)[6][6]) {
-    const double nrm = 1.0 - dd22(b.a2, n.a2);
+// pair entry [X][Y] of a4_joint(b, n) times (1 - a2b : a2n)
+__host__ __device__ __forceinline__ double joint_raw(const ODist& b, const ODist& n, int X, int Y) {
+    double d = 0.0, o = 0.0;
 #pragma unroll
-    for (int P = 0; P < 6; P++)
+    for (int R = 0; R < 3; R++) d += b.u[sxy(X, R)] * n.u[sxy(Y, R)];
 #pragma unroll
-        for (int R = 0; R < 6; R++) {
-            const int i = P < 3 ? P : (P == 3 ? 1 : 0), j = P < 3 ? P : (P == 5 ? 1 : 2);
-            const int k = R < 3 ? R : (R == 3 ? 1 : 0), l = R < 3 ? R : (R == 5 ? 1 : 2);
-            double s = b.a2[3 * i + j] * n.a2[3 * k + l];
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) s -= a4sym(b, i, j, p, q) * a4sym(n, k, l, p, q);
-            J[P][R] = s / nrm;
-        }
+    for (int R = 3; R < 6; R++) o += b.u[sxy(X, R)] * n.u[sxy(Y, R)];
+    return (b.a2[X] * n.a2[Y] - d) - 2.0 * o;
 }
 
-// J[P][R] of a4_jointcross(b, n) = <n n v v>
-__device__ __forceinline__ void a4_jointcross6(const OrthoDist& b, const OrthoDist& n, double (&J)[6][6]) {
-    const double nrm = 1.0 - dd22(b.a2, n.a2);
+// pair entry [X][Y] of a4_jointcross(b, n) = <n n v v> times (1 - a2b : a2n)
+__host__ __device__ __forceinline__ double jointcross_raw(const ODist& b, const ODist& n, int X, int Y) {
+    const int k = pa(Y), l = pb(Y);
+    double s = 0.0;
 #pragma unroll
-    for (int P = 0; P < 6; P++)
+    for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int R = 0; R < 6; R++) {
-            const int i = P < 3 ? P : (P == 3 ? 1 : 0), j = P < 3 ? P : (P == 5 ? 1 : 2);
-            const int k = R < 3 ? R : (R == 3 ? 1 : 0), l = R < 3 ? R : (R == 5 ? 1 : 2);
-            double s = 0.0;
+        for (int q = 0; q < 3; q++)
 #pragma unroll
-            for (int p = 0; p < 3; p++)
+            for (int r = 0; r < 3; r++)
 #pragma unroll
-                for (int q = 0; q < 3; q++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-#pragma unroll
-                        for (int t = 0; t < 3; t++) {
-                            const double e = levi(k, p, q) * levi(l, r, t);
-                            if (e != 0.0) s += e * a4sym(n, i, j, q, t) * b.a2[3 * p + r];
-                        }
-            J[P][R] = s / nrm;
-        }
+                for (int t = 0; t < 3; t++) {
+                    const int e = levi(k, p, q) * levi(l, r, t);
+                    if (e != 0)
+                        s += (double)e * (n.u[kA4Slots.sym[27 * pa(X) + 9 * pb(X) + 3 * q + t]] * b.a2[pidx(p, r)]);
+                }
+    return s;
 }
 
-// <v v> and <v^4> of v = b x n: the derived third distribution
-__device__ __forceinline__ void ortho_third(const OrthoDist& b, const OrthoDist& n, OrthoDist& v) {
-    const double n2 = 1.0 - dd22(b.a2, n.a2);
-    double a44 = 0.0;
+// <v v> of v = b x n (pair P) times (1 - a2b : a2n)
+__host__ __device__ __forceinline__ double orth_a2_raw(const ODist& b, const ODist& n, int P) {
+    const int i = pa(P), j = pb(P);
+    double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int j = 0; j < 3; j++)
+        for (int q = 0; q < 3; q++)
 #pragma unroll
-            for (int k = 0; k < 3; k++)
+            for (int r = 0; r < 3; r++)
 #pragma unroll
-                for (int l = 0; l < 3; l++) a44 += a4sym(b, i, j, k, l) * a4sym(n, i, j, k, l);
-    const double n4 = 1.0 - 2.0 * dd22(b.a2, n.a2) + a44;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            double s = 0.0;
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-#pragma unroll
-                        for (int t = 0; t < 3; t++) {
-                            const double e = levi(i, p, q) * levi(j, r, t);
-                            if (e != 0.0) s += e * b.a2[3 * p + r] * n.a2[3 * q + t];
-                        }
-            v.a2[3 * i + j] = s / n2;
-        }
-    // the 15 sorted entries (a0 <= a1 <= a2 <= a3) of <v^4>
-    int e = 0;
-#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const int e = levi(i, p, q) * levi(j, r, t);
+                    if (e != 0) s += (double)e * (b.a2[pidx(p, r)] * n.a2[pidx(q, t)]);
+                }
+    return s;
+}
+
+// indices (a0 <= a1 <= a2 <= a3) of sorted entry e and its multiplicity
+// 4! / (n0! n1! n2!) among the 81 index tuples (tables, so that uses inside
+// unrolled loops fold)
+struct SortedIdx {
+    int i[15][4];
+    double mult[15];
+};
+__host__ __device__ constexpr SortedIdx make_sorted_idx() {
+    SortedIdx t{};
+    int c = 0;
     for (int a0 = 0; a0 < 3; a0++)
-#pragma unroll
         for (int a1 = a0; a1 < 3; a1++)
-#pragma unroll
             for (int a2 = a1; a2 < 3; a2++)
-#pragma unroll
                 for (int a3 = a2; a3 < 3; a3++) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int p0 = 0; p0 < 3; p0++)
-#pragma unroll
-                        for (int q0 = 0; q0 < 3; q0++) {
-                            const double e0 = levi(a0, p0, q0);
-                            if (e0 == 0.0) continue;
-#pragma unroll
-                            for (int p1 = 0; p1 < 3; p1++)
-#pragma unroll
-                                for (int q1 = 0; q1 < 3; q1++) {
-                                    const double e1 = e0 * levi(a1, p1, q1);
-                                    if (e1 == 0.0) continue;
-#pragma unroll
-                                    for (int p2 = 0; p2 < 3; p2++)
-#pragma unroll
-                                        for (int q2 = 0; q2 < 3; q2++) {
-                                            const double e2 = e1 * levi(a2, p2, q2);
-                                            if (e2 == 0.0) continue;
-#pragma unroll
-                                            for (int p3 = 0; p3 < 3; p3++)
-#pragma unroll
-                                                for (int q3 = 0; q3 < 3; q3++) {
-                                                    const double e3 = e2 * levi(a3, p3, q3);
-                                                    if (e3 == 0.0) continue;
-                                                    s += e3 * a4sym(b, p0, p1, p2, p3) * a4sym(n, q0, q1, q2, q3);
-                                                }
-                                        }
-                                }
-                        }
-                    v.u[e++] = s / n4;
+                    t.i[c][0] = a0, t.i[c][1] = a1, t.i[c][2] = a2, t.i[c][3] = a3;
+                    int n[3] = {0, 0, 0};
+                    n[a0]++, n[a1]++, n[a2]++, n[a3]++;
+                    const int f[5] = {1, 1, 2, 6, 24};
+                    t.mult[c] = 24.0 / (f[n[0]] * f[n[1]] * f[n[2]]);
+                    c++;
                 }
-    v.sym = true;
+    return t;
 }
+constexpr SortedIdx kSorted = make_sorted_idx();
+__host__ __device__ constexpr int sslot(int a, int b, int c, int d) { return kA4Slots.sym[27 * a + 9 * b + 3 * c + d]; }
 
-// the a4_jk triple of ai_orthotropic (src/moments.f90:364-383): jk[0] = <q2 q2 q3 q3>
-// (given) or a4_jointcross(q2, q1) (derived), jk[1] = <q1 q1 q3 q3> or
-// a4_jointcross(q1, q2), jk[2] = <q1 q1 q2 q2>
-__device__ __forceinline__ void ortho_joints(const OrthoDist* q, bool derived, int m, double (&J)[6][6]) {
-    if (m == 2) a4_joint6(q[0], q[1], J);
-    else if (!derived) a4_joint6(m == 0 ? q[1] : q[0], q[2], J);
-    else if (m == 0) a4_jointcross6(q[1], q[0], J);
-    else a4_jointcross6(q[0], q[1], J);
-}
-
-__device__ __forceinline__ double jt(const double (&J)[6][6], int a, int b, int c, int d) {
-    return J[pidx(a, b)][pidx(c, d)];
-}
-
-// Q = sum_m ci_m M2_m + ci_{m+3} H2_m (src/homogenizations.f90:303-314), accumulated into Q
-__device__ __forceinline__ void ortho_Q_add(const OrthoDist* q, int m, const double (&J)[6][6], const double* ci,
-                                            double* Q, int stride) {
-    const int jm = (m + 1) % 3, km = (m + 2) % 3;   // ji, ki of the reference (1-based [2,3,1], [3,1,2])
+// sorted entry e of <v v v v> times its normalisation (1 - 2 a2b:a2n + a4b::a4n)
+__host__ __device__ __forceinline__ double orth_a4_raw(const ODist& b, const ODist& n, int e) {
+    const int i0 = kSorted.i[e][0], i1 = kSorted.i[e][1], i2 = kSorted.i[e][2], i3 = kSorted.i[e][3];
+    double s = 0.0;
 #pragma unroll
-    for (int l = 0; l < 3; l++)
+    for (int c0 = 0; c0 < 2; c0++)
 #pragma unroll
-        for (int k = 0; k < 3; k++)
+        for (int c1 = 0; c1 < 2; c1++)
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int c2 = 0; c2 < 2; c2++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const double M2 = (a4e(q[jm], l, k, i, j) + a4e(q[km], l, k, i, j) -
-                                       2 * ((jt(J, l, k, i, j) + jt(J, i, j, l, k)) / 2)) / 4;
-                    const double H2 = (jt(J, l, i, k, j) + jt(J, i, k, l, j) + jt(J, l, j, i, k) + jt(J, j, k, i, l)) / 4;
-                    Q[(27 * l + 9 * k + 3 * i + j) * stride] += ci[m] * M2 + ci[m + 3] * H2;
+                for (int c3 = 0; c3 < 2; c3++) {
+                    // eps_{i p q}: (p, q) = (i+1, i+2) (+) or (i+2, i+1) (-)
+                    const int p0 = (i0 + 1 + c0) % 3, q0 = (i0 + 2 - c0) % 3;
+                    const int p1 = (i1 + 1 + c1) % 3, q1 = (i1 + 2 - c1) % 3;
+                    const int p2 = (i2 + 1 + c2) % 3, q2 = (i2 + 2 - c2) % 3;
+                    const int p3 = (i3 + 1 + c3) % 3, q3 = (i3 + 2 - c3) % 3;
+                    const double sg = ((c0 + c1 + c2 + c3) & 1) ? -1.0 : 1.0;
+                    s += sg * (b.u[sslot(p0, p1, p2, p3)] * n.u[sslot(q0, q1, q2, q3)]);
                 }
+    return s;
+}
+// a4b :: a4n over all 81 index tuples
+__host__ __device__ __forceinline__ double a4dot(const ODist& b, const ODist& n) {
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < 15; e++) s += kSorted.mult[e] * (b.u[e] * n.u[e]);
+    return s;
 }
 
-// doubleinner22(doubleinner42(Q, tau), v w) = sum Q_lkij tau_ji (v w)_kl
-__device__ __forceinline__ double ortho_eps_vw(const double* Q, int stride, const double (&tau)[9], const double* v,
-                                               const double* w) {
-    double r = 0.0;
+// Q6 += sum over pair entries j = J[X][Y] * js of the joint tensor J:
+// -cm/4 (J + J^T) + cm3 sym4(J); jf(X, Y) returns the unscaled entry
+template <class JF>
+__host__ __device__ __forceinline__ void q6_add_joint(double (&Q)[36], double cm, double cm3, double js, JF jf) {
+    const double a = -0.25 * cm * js, h = 0.25 * cm3 * js;
 #pragma unroll
-    for (int l = 0; l < 3; l++)
+    for (int X = 0; X < 6; X++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double e = 0.0;
+        for (int Y = 0; Y < 6; Y++) {
+            const double j = jf(X, Y);
+            Q[6 * X + Y] += a * j;
+            Q[6 * Y + X] += a * j;
 #pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) e += Q[(27 * l + 9 * k + 3 * i + j) * stride] * tau[3 * j + i];
-            r += e * v[k] * w[l];
+            for (int t = 0; t < 4; t++)
+                if (kH2.c[6 * X + Y][t] != 0)
+                    Q[6 * kH2.P[6 * X + Y][t] + kH2.R[6 * X + Y][t]] += ((double)kH2.c[6 * X + Y][t] * h) * j;
         }
+}
+
+// Q6 += w S-part of one distribution's a4
+__host__ __device__ __forceinline__ void q6_add_a4(double (&Q)[36], double w, const double (&u)[15]) {
+#pragma unroll
+    for (int P = 0; P < 6; P++)
+#pragma unroll
+        for (int R = 0; R < 6; R++) Q[6 * P + R] += w * u[sxy(P, R)];
+}
+
+// body a4 excess at (3,2,1,2) (1-based) over the symmetric entry
+__host__ __device__ __forceinline__ double quirk(const ODist& d) {
+    return d.u[kA4Slots.s[27 * 2 + 9 * 1 + 0 + 1]] - d.u[kA4Slots.sym[27 * 2 + 9 * 1 + 0 + 1]];
+}
+
+struct JointF {
+    const ODist& b;
+    const ODist& n;
+    __host__ __device__ __forceinline__ double operator()(int X, int Y) const { return joint_raw(b, n, X, Y); }
+};
+struct CrossF {
+    const ODist& b;
+    const ODist& n;
+    __host__ __device__ __forceinline__ double operator()(int X, int Y) const { return jointcross_raw(b, n, X, Y); }
+};
+
+// Q6 and dq of the given branch (q1, q2, q3 all distributions)
+__host__ __device__ __forceinline__ void ortho_q6_given(const ODist& d0, const ODist& d1, const ODist& d2,
+                                                        const double* ci, double (&Q)[36], double& dq) {
+    const double cs = ci[0] + ci[1] + ci[2];
+    const double w0 = (cs - ci[0]) * 0.25, w1 = (cs - ci[1]) * 0.25, w2 = (cs - ci[2]) * 0.25;
+#pragma unroll
+    for (int t = 0; t < 36; t++) Q[t] = 0.0;
+    q6_add_a4(Q, w0, d0.u);
+    q6_add_a4(Q, w1, d1.u);
+    q6_add_a4(Q, w2, d2.u);
+    dq = w0 * quirk(d0) + w1 * quirk(d1) + w2 * quirk(d2);
+    q6_add_joint(Q, ci[2], ci[5], 1.0 / (1.0 - a2dot(d0, d1)), JointF{d0, d1});   // <q1 q1 q2 q2>
+    q6_add_joint(Q, ci[0], ci[3], 1.0 / (1.0 - a2dot(d1, d2)), JointF{d1, d2});   // <q2 q2 q3 q3>
+    q6_add_joint(Q, ci[1], ci[4], 1.0 / (1.0 - a2dot(d0, d2)), JointF{d0, d2});   // <q1 q1 q3 q3>
+}
+
+// Q6 and dq of the derived branch (v = b x n)
+__host__ __device__ __forceinline__ void ortho_q6_derived(const ODist& d0, const ODist& d1, const double* ci,
+                                                          double (&Q)[36], double& dq) {
+    const double cs = ci[0] + ci[1] + ci[2];
+    const double w0 = (cs - ci[0]) * 0.25, w1 = (cs - ci[1]) * 0.25, w2 = (cs - ci[2]) * 0.25;
+#pragma unroll
+    for (int t = 0; t < 36; t++) Q[t] = 0.0;
+    q6_add_a4(Q, w0, d0.u);
+    q6_add_a4(Q, w1, d1.u);
+    const double s01 = 1.0 / (1.0 - a2dot(d0, d1));
+    const double n4 = 1.0 / (1.0 - 2.0 * a2dot(d0, d1) + a4dot(d0, d1));
+    {
+        double u2[15];
+#pragma unroll
+        for (int e = 0; e < 15; e++) u2[e] = orth_a4_raw(d0, d1, e) * n4;
+        q6_add_a4(Q, w2, u2);
+    }
+    dq = w0 * quirk(d0) + w1 * quirk(d1);
+    q6_add_joint(Q, ci[2], ci[5], s01, JointF{d0, d1});   // <q1 q1 q2 q2>
+    q6_add_joint(Q, ci[0], ci[3], s01, CrossF{d1, d0});   // a4_jointcross(q2, q1)
+    q6_add_joint(Q, ci[1], ci[4], s01, CrossF{d0, d1});   // a4_jointcross(q1, q2)
+}
+
+// eps(tau) : vw = sum Q_lkij tau_ji (vw)_kl of one stress state
+__host__ __device__ __forceinline__ double q6_state(const double (&Q)[36], double dq, const double (&tau)[9],
+                                                    const double* v, const double* w) {
+    double t6[6];
+#pragma unroll
+    for (int R = 0; R < 6; R++) t6[R] = pw(R) * tau[3 * pa(R) + pb(R)];
+    double r = dq * tau[3] * (v[1] * w[2]);
+#pragma unroll
+    for (int P = 0; P < 6; P++) {
+        double y = 0.0;
+#pragma unroll
+        for (int R = 0; R < 6; R++) y += Q[6 * P + R] * t6[R];
+        const int a = pa(P), b = pb(P);
+        r += (P < 3 ? v[a] * w[a] : v[a] * w[b] + v[b] * w[a]) * y;
+    }
     return r;
 }
 
-}  // namespace
-
-// Eij of orthotropic grains (Sachs; src/enhancementfactors.f90:134-188): one
-// lane per parcel, Q in the lane's LDS column
-__global__ void __launch_bounds__(64) eij_ortho_kernel(OrthoArgs a) {
-    __shared__ double lds[81 * 64];
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
-    double* Q = lds + threadIdx.x;
-    OrthoDist q[3];
-    const double* xs[3] = {a.x1, a.x2, a.x3};
-    // v given (its n00 above 1e-8) or v = b x n (src/moments.f90:371-383)
-    const bool derived = !a.x3 || !(a.x3[p] > 1e-8);
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        if (m == 2 && derived) break;
-        double xr[15];
-#pragma unroll
-        for (int r = 0; r < 15; r++) xr[r] = r < a.nq ? xs[m][(int64_t)r * a.P + p] : 0.0;
-        ortho_dist(a.A2, a.A4F, xr, q[m]);
-    }
-    if (derived) ortho_third(q[0], q[1], q[2]);
-#pragma unroll
-    for (int t = 0; t < 81; t++) Q[t * 64] = 0.0;
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        double J[6][6];
-        ortho_joints(q, derived, m, J);
-        ortho_Q_add(q, m, J, a.ci, Q, 64);
-    }
-    double e[9];
-    if (a.frames) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < 9; i++) e[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    }
+// the six stress states of the frame e (rows e1, e2, e3): tau_vv(e_i) =
+// I/3 - e_i e_i, tau_vw = v w + w v (src/enhancementfactors.f90:398-413)
+__host__ __device__ __forceinline__ void ortho_eij_frame(const double (&Q)[36], double dq, const double (&Qi)[36],
+                                                         double dqi, const double (&e)[9], double* out) {
 #pragma unroll 1
     for (int s = 0; s < 6; s++) {
         const int iv = s < 3 ? s : (s == 3 ? 1 : 0), iw = s < 3 ? s : (s == 5 ? 1 : 2);
@@ -627,69 +667,225 @@ __global__ void __launch_bounds__(64) eij_ortho_kernel(OrthoArgs a) {
 #pragma unroll
             for (int j = 0; j < 3; j++)
                 tau[3 * i + j] = s < 3 ? (i == j ? 1.0 / 3.0 : 0.0) - v[i] * v[j] : v[i] * w[j] + w[i] * v[j];
-        const double num = ortho_eps_vw(Q, 64, tau, v, w);
-        const double den = ortho_eps_vw(a.Qiso, 1, tau, v, w);   // the isotropic fabric (ortho_Q_kernel)
-        a.Eij[p * 6 + s] = num / den;
+        out[s] = q6_state(Q, dq, tau, v, w) / q6_state(Qi, dqi, tau, v, w);
     }
 }
 
-// Q of the isotropic fabric (q1 = q2 = q3 = the isotropic state; the
-// reference's ai_orthotropic(qlm_iso, qlm_iso, qlm_iso)) into Qiso (81)
-__global__ void ortho_Q_kernel(OrthoArgs a, double* Qiso) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    OrthoDist q[3];
-    double xr[15];
+// Cartesian frame: E_ii = Q6 row i against (I/3 - e_i e_i), E_vw = Q6[P][P]
+// (the dq entry meets no nonzero tau there)
+__host__ __device__ __forceinline__ double q6_cart(const double (&Q)[36], int s) {
+    if (s >= 3) return Q[7 * s];
+    return (Q[6 * s + 0] + Q[6 * s + 1] + Q[6 * s + 2]) * (1.0 / 3.0) - Q[7 * s];
+}
+
+// the same pointer, opaque to the optimiser (a second load through it is not
+// merged with an earlier one)
+__device__ __forceinline__ const double* opaque_ptr(const double* ptr) {
+#ifndef SFG_HOST_EMUL
+    asm volatile("" : "+s"(ptr));
+#endif
+    return ptr;
+}
+
+__device__ __forceinline__ void load_l4(const double* __restrict__ x, int64_t P, int64_t p, int nq, double (&xr)[15]) {
 #pragma unroll
-    for (int r = 0; r < 15; r++) xr[r] = r == 0 ? 1.0 : 0.0;
+    for (int r = 0; r < 15; r++) xr[r] = r < nq ? x[(int64_t)r * P + p] : 0.0;
+}
+
+}  // namespace
+
+void ortho_q6_iso(const double ci[6], double Q6[36], double* dq) {
+    double x[15] = {1.0};
+    ODist d;
+    odist(x, d);
+    double Q[36];
+    ortho_q6_given(d, d, d, ci, Q, *dq);
+    for (int t = 0; t < 36; t++) Q6[t] = Q[t];
+}
+
+// Eij of orthotropic grains (Sachs; src/enhancementfactors.f90:134-188): one
+// lane per parcel.  MODE 0: v given for every parcel tested per parcel (its
+// n00 above 1e-8, src/moments.f90:371-383); MODE 1: v = b x n for all.
+template <bool CART, int MODE>
+__global__ void __launch_bounds__(256) eij_ortho_kernel(OrthoArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double Q[36], dq;
+    // all three blocks requested at once (one memory round trip); q3's n00
+    // decides the branch
+    double x3r[15];
+    if (MODE == 0) load_l4(a.x3, a.P, p, a.nq, x3r);
+    const bool given = MODE == 0 && x3r[0] > 1e-8;
+    if (given) {
+        // ortho_q6_given in the order q1 q2 | q3 (q2 dropped) | q1 again, so
+        // that at most two distributions are live at a time
+        const double* ci = a.ci;
+        const double cs = ci[0] + ci[1] + ci[2];
+        const double w0 = (cs - ci[0]) * 0.25, w1 = (cs - ci[1]) * 0.25, w2 = (cs - ci[2]) * 0.25;
 #pragma unroll
-    for (int m = 0; m < 3; m++) ortho_dist(a.A2, a.A4F, xr, q[m]);
-    for (int t = 0; t < 81; t++) Qiso[t] = 0.0;
-    for (int m = 0; m < 3; m++) {
-        double J[6][6];
-        ortho_joints(q, false, m, J);
-        ortho_Q_add(q, m, J, a.ci, Qiso, 1);
+        for (int t = 0; t < 36; t++) Q[t] = 0.0;
+        double xr[15];
+        ODist d0, d2;
+        {
+            load_l4(a.x1, a.P, p, a.nq, xr);
+            odist(xr, d0);
+            ODist d1;
+            load_l4(a.x2, a.P, p, a.nq, xr);
+            odist(xr, d1);
+            q6_add_a4(Q, w0, d0.u);
+            q6_add_a4(Q, w1, d1.u);
+            dq = w0 * quirk(d0) + w1 * quirk(d1);
+            q6_add_joint(Q, ci[2], ci[5], 1.0 / (1.0 - a2dot(d0, d1)), JointF{d0, d1});
+            odist(x3r, d2);
+            q6_add_a4(Q, w2, d2.u);
+            dq += w2 * quirk(d2);
+            q6_add_joint(Q, ci[0], ci[3], 1.0 / (1.0 - a2dot(d1, d2)), JointF{d1, d2});
+        }
+        // q1 evaluated again from a pointer the compiler cannot equate with the
+        // first, so that the first evaluation need not stay live
+        load_l4(opaque_ptr(a.x1), a.P, p, a.nq, xr);
+        odist(xr, d0);
+        q6_add_joint(Q, ci[1], ci[4], 1.0 / (1.0 - a2dot(d0, d2)), JointF{d0, d2});
+    } else {
+        double xr[15];
+        load_l4(a.x1, a.P, p, a.nq, xr);
+        ODist d0;
+        odist(xr, d0);
+        load_l4(a.x2, a.P, p, a.nq, xr);
+        ODist d1;
+        odist(xr, d1);
+        ortho_q6_derived(d0, d1, a.ci, Q, dq);
     }
+    double* o = a.Eij + p * 6;
+    if (CART) {
+#pragma unroll
+        for (int s = 0; s < 6; s++) o[s] = q6_cart(Q, s) / q6_cart(a.q6iso, s);
+    } else {
+        double e[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
+        ortho_eij_frame(Q, dq, a.q6iso, a.dqiso, e, o);
+    }
+}
+
+// the l <= 4 blocks of NS states rotated by one direction's real blocks R
+// (R2 5x5 then R4 9x9, row-major; the same R for the whole workgroup, so its
+// entries are scalar loads, each used for all NS states at once)
+template <int NS>
+__device__ __forceinline__ void rotate_l4(const double* __restrict__ R, double (&x)[3][15]) {
+    double y[NS][15];
+#pragma unroll
+    for (int s = 0; s < NS; s++) y[s][0] = x[s][0];
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 5; j++) acc += R[5 * i + j] * x[s][1 + j];
+            y[s][1 + i] = acc;
+        }
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 9; j++) acc += R[25 + 9 * i + j] * x[s][6 + j];
+            y[s][6 + i] = acc;
+        }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int q = 0; q < 15; q++) x[s][q] = y[s][q];
+}
+
+// Qn += mu/2 (a2 + a2.kk + kk.a2 + a2_zz I) + lam a4(i,j,z,z) of one distribution
+__device__ __forceinline__ void qn_add_dist(double (&Qn)[9], const double (&a2)[6], const double (&zz)[6], double mu,
+                                            double lam) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double anti = (j == 2 ? a2[pidx(i, 2)] : 0.0) + (i == 2 ? a2[pidx(2, j)] : 0.0);
+            Qn[3 * i + j] += mu * 0.5 * (a2[pidx(i, j)] + anti + (i == j ? a2[2] : 0.0)) + lam * zz[pidx(i, j)];
+        }
+}
+
+// Qn += lam' (J(z,i,j,z) + J(j,z,z,i)) of one joint tensor (pair form, scaled by js)
+template <class JF>
+__device__ __forceinline__ void qn_add_joint(double (&Qn)[9], double lam3, double js, JF jf) {
+    const double c = lam3 * js;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Qn[3 * i + j] += c * (jf(pidx(2, i), pidx(j, 2)) + jf(pidx(j, 2), pidx(2, i)));
+}
+
+// a4(i, j, z, z) pairs of a distribution's body a4
+__device__ __forceinline__ void a4_zz(const ODist& d, double (&zz)[6]) {
+#pragma unroll
+    for (int P = 0; P < 6; P++) zz[P] = d.u[kA4Slots.s[27 * pa(P) + 9 * pb(P) + 3 * 2 + 2]];
 }
 
 // elastic Vi of orthotropic grains (Voigt; src/wavepropagation.f90:41-68,
-// 98-115): one lane per (parcel, direction); the distributions rotated as
-// rotate_nlm4 rotates them, the acoustic tensor for k = z, Cardano
-__global__ void __launch_bounds__(128) vi_ortho_kernel(OrthoArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.P * a.ndir) return;
-    const int64_t p = t % a.P;
-    const int d = (int)(t / a.P);
-    const double* R = a.R + (int64_t)d * ROT4_STRIDE;
-    OrthoDist q[3];
-    const double* xs[3] = {a.x1, a.x2, a.x3};
-    const bool derived = !a.x3 || !(a.x3[p] > 1e-8);   // n00 is invariant under rotation
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        if (m == 2 && derived) break;
-        double xr[15];
-        rotated_block(xs[m], a.P, p, a.nq, R, xr);
-        ortho_dist(a.A2, a.A4F, xr, q[m]);
-    }
-    if (derived) ortho_third(q[0], q[1], q[2]);
+// 98-115): one lane per parcel, one direction per grid row (blockIdx.y); the
+// distributions rotated as rotate_nlm4 rotates them, the acoustic tensor for
+// k = z
+//   Qn = sum_m mu_m/2 (a2 + a2.kk + kk.a2 + (a2:kk) I) + lam_m a4_m : kk
+//        + lam_m+3 (2 sym2(J_m)) first-last : kk
+// and the reference's Cardano roots.  MODE as for eij_ortho_kernel.
+template <int MODE>
+__global__ void __launch_bounds__(256) vi_ortho_kernel(OrthoArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    const int d = (int)blockIdx.y;
+    const double* __restrict__ R = a.R + (int64_t)d * ROT4_STRIDE;
+    const bool given = MODE == 0 && a.x3[p] > 1e-8;   // n00 is invariant under rotation
+    double x[3][15];
+    load_l4(a.x1, a.P, p, a.nq, x[0]);
+    load_l4(a.x2, a.P, p, a.nq, x[1]);
     double Qn[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) Qn[i] = 0.0;
+    if (given) {
+        load_l4(a.x3, a.P, p, a.nq, x[2]);
+        rotate_l4<3>(R, x);
+        ODist q0, q1, q2;
+        odist(x[0], q0);
+        odist(x[1], q1);
+        odist(x[2], q2);
+        double zz[6];
+        a4_zz(q0, zz);
+        qn_add_dist(Qn, q0.a2, zz, a.mu[0], a.lam[0]);
+        a4_zz(q1, zz);
+        qn_add_dist(Qn, q1.a2, zz, a.mu[1], a.lam[1]);
+        a4_zz(q2, zz);
+        qn_add_dist(Qn, q2.a2, zz, a.mu[2], a.lam[2]);
+        qn_add_joint(Qn, a.lam[5], 1.0 / (1.0 - a2dot(q0, q1)), JointF{q0, q1});   // <q1 q1 q2 q2>
+        qn_add_joint(Qn, a.lam[3], 1.0 / (1.0 - a2dot(q1, q2)), JointF{q1, q2});   // <q2 q2 q3 q3>
+        qn_add_joint(Qn, a.lam[4], 1.0 / (1.0 - a2dot(q0, q2)), JointF{q0, q2});   // <q1 q1 q3 q3>
+    } else {
+        rotate_l4<2>(R, x);
+        ODist q0, q1;
+        odist(x[0], q0);
+        odist(x[1], q1);
+        double zz[6];
+        a4_zz(q0, zz);
+        qn_add_dist(Qn, q0.a2, zz, a.mu[0], a.lam[0]);
+        a4_zz(q1, zz);
+        qn_add_dist(Qn, q1.a2, zz, a.mu[1], a.lam[1]);
+        const double s01 = 1.0 / (1.0 - a2dot(q0, q1));
+        const double n4 = 1.0 / (1.0 - 2.0 * a2dot(q0, q1) + a4dot(q0, q1));
+        double a2v[6];
 #pragma unroll
-    for (int m = 0; m < 3; m++) {
-        double J[6][6];
-        ortho_joints(q, derived, m, J);
-        const double* a2 = q[m].a2;
+        for (int P = 0; P < 6; P++) a2v[P] = orth_a2_raw(q0, q1, P) * s01;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                // mu/2 (a2 + a2.kk + kk.a2 + (a2:kk) I) + lam a4:kk + lam' 2 sym2(J) ^{first-last} kk
-                const double anti = (j == 2 ? a2[3 * i + 2] : 0.0) + (i == 2 ? a2[3 * 2 + j] : 0.0);
-                double q4 = a4e(q[m], i, j, 2, 2);
-                const double jf = jt(J, 2, i, j, 2) + jt(J, j, 2, 2, i);
-                Qn[3 * i + j] += a.mu[m] / 2 * (a2[3 * i + j] + anti + a2[8] * (i == j ? 1.0 : 0.0)) + a.lam[m] * q4 +
-                                 a.lam[m + 3] * jf;
-            }
+        for (int P = 0; P < 6; P++) zz[P] = orth_a4_raw(q0, q1, kA4Slots.sym[27 * pa(P) + 9 * pb(P) + 3 * 2 + 2]) * n4;
+        qn_add_dist(Qn, a2v, zz, a.mu[2], a.lam[2]);
+        qn_add_joint(Qn, a.lam[5], s01, JointF{q0, q1});   // <q1 q1 q2 q2>
+        qn_add_joint(Qn, a.lam[3], s01, CrossF{q1, q0});   // a4_jointcross(q2, q1)
+        qn_add_joint(Qn, a.lam[4], s01, CrossF{q0, q1});   // a4_jointcross(q1, q2)
     }
     double vi[3];
     qnorm_to_vi(Qn, a.rho, vi);
@@ -698,6 +894,168 @@ __global__ void __launch_bounds__(128) vi_ortho_kernel(OrthoArgs a) {
     o[1] = vi[1];
     o[2] = vi[2];
 }
+
+// ---- discrete grains (src/moments.f90:386-417, src/enhancementfactors.f90:191-266,
+// src/wavepropagation.f90:70-96) ----
+
+#ifndef SFG_HOST_EMUL   // wave shuffles: not emulated lane by lane
+// grain sums of one ensemble chunk: blockIdx.x = e * G + chunk, blockIdx.y = m;
+// per grain the axis m's pair products (a2), their products (a4) and the
+// pair products of the two other axes (J_m); wave shuffles, then the four
+// waves' sums through LDS
+__global__ void __launch_bounds__(256) ortho_moments_kernel(OrthoDiscArgs a) {
+    __shared__ double red[4][ORTHO_NMOM];
+    const int m = (int)blockIdx.y;
+    const int64_t e = blockIdx.x / a.G;
+    const int g = (int)(blockIdx.x % a.G);
+    const int64_t per = (a.N + a.G - 1) / a.G;
+    const int64_t g0 = g * per, g1 = g0 + per < a.N ? g0 + per : a.N;
+    const double* ax[3] = {a.b, a.n, a.v};
+    const double* __restrict__ qm = ax[m] + e * a.N * 3;
+    const double* __restrict__ qa = ax[m == 0 ? 1 : 0] + e * a.N * 3;
+    const double* __restrict__ qb = ax[m == 2 ? 1 : 2] + e * a.N * 3;
+    double acc[ORTHO_NMOM];
+#pragma unroll
+    for (int t = 0; t < ORTHO_NMOM; t++) acc[t] = 0.0;
+    for (int64_t i = g0 + threadIdx.x; i < g1; i += 256) {
+        double q[3], x[3], y[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) q[c] = qm[i * 3 + c], x[c] = qa[i * 3 + c], y[c] = qb[i * 3 + c];
+        double p2[6], px[6], py[6];
+#pragma unroll
+        for (int P = 0; P < 6; P++) {
+            p2[P] = q[pa(P)] * q[pb(P)];
+            px[P] = x[pa(P)] * x[pb(P)];
+            py[P] = y[pa(P)] * y[pb(P)];
+        }
+#pragma unroll
+        for (int P = 0; P < 6; P++) acc[P] += p2[P];
+#pragma unroll
+        for (int t = 0; t < 15; t++)
+            acc[6 + t] += p2[pidx(kSorted.i[t][0], kSorted.i[t][1])] * p2[pidx(kSorted.i[t][2], kSorted.i[t][3])];
+#pragma unroll
+        for (int X = 0; X < 6; X++)
+#pragma unroll
+            for (int Y = 0; Y < 6; Y++) acc[21 + 6 * X + Y] += px[X] * py[Y];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < ORTHO_NMOM; t++) {
+        double v = acc[t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[w][t] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < ORTHO_NMOM) {
+        const int t = threadIdx.x;
+        a.part[((e * a.G + g) * 3 + m) * ORTHO_NMOM + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    }
+}
+
+// the G partial sums of each (ensemble, m, moment) in chunk order, over N
+__global__ void __launch_bounds__(256) ortho_moments_final_kernel(OrthoDiscArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.E * 3 * ORTHO_NMOM) return;
+    const int64_t e = t / (3 * ORTHO_NMOM);
+    const int r = (int)(t % (3 * ORTHO_NMOM));
+    double s = 0.0;
+    for (int g = 0; g < a.G; g++) s += a.part[(e * a.G + g) * 3 * ORTHO_NMOM + r];
+    a.mom[t] = s / (double)a.N;
+}
+#endif
+
+// Eij of an ensemble's grain averages: one lane per ensemble
+__global__ void __launch_bounds__(64) eij_ortho_mom_kernel(OrthoMomArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const double* M = a.mom + e * 3 * ORTHO_NMOM;
+    const double cs = a.ci[0] + a.ci[1] + a.ci[2];
+    double Q[36];
+#pragma unroll
+    for (int t = 0; t < 36; t++) Q[t] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double u[15];
+#pragma unroll
+        for (int t = 0; t < 15; t++) u[t] = M[d * ORTHO_NMOM + 6 + t];
+        q6_add_a4(Q, (cs - a.ci[d]) * 0.25, u);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const double* J = M + m * ORTHO_NMOM + 21;
+        q6_add_joint(Q, a.ci[m], a.ci[m + 3], 1.0, [&](int X, int Y) { return J[6 * X + Y]; });
+    }
+    double* o = a.Eij + e * 6;
+    if (!a.frames) {
+#pragma unroll
+        for (int s = 0; s < 6; s++) o[s] = q6_cart(Q, s) / q6_cart(a.q6iso, s);
+    } else {
+        double f[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) f[i] = a.frames[e * 9 + i];
+        ortho_eij_frame(Q, 0.0, a.q6iso, a.dqiso, f, o);
+    }
+}
+
+// Vi of an ensemble's grain averages along kdir[blockIdx.y]: the reference
+// rotates the grains so that the direction is z and forms the acoustic
+// tensor for k = z; here the tensor is formed in the grains' frame for k =
+// kdir, the same tensor in the other basis (the rotation is orthogonal), whose
+// invariants -- all that the Cardano roots read -- are the same
+__global__ void __launch_bounds__(64) vi_ortho_mom_kernel(OrthoMomArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int d = (int)blockIdx.y;
+    const double k[3] = {a.kdir[3 * d], a.kdir[3 * d + 1], a.kdir[3 * d + 2]};
+    const double* M = a.mom + e * 3 * ORTHO_NMOM;
+    double Qn[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Qn[i] = 0.0;
+#pragma unroll 1
+    for (int m = 0; m < 3; m++) {
+        const double* a2 = M + m * ORTHO_NMOM;
+        const double* u = a2 + 6;
+        const double* J = a2 + 21;
+        double a2k[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) a2k[i] = a2[pidx(i, 0)] * k[0] + a2[pidx(i, 1)] * k[1] + a2[pidx(i, 2)] * k[2];
+        const double kak = a2k[0] * k[0] + a2k[1] * k[1] + a2k[2] * k[2];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                // mu/2 (a2 + a2.kk + kk.a2 + (k.a2.k) I) + lam a4 : kk
+                double q4 = 0.0, jf = 0.0;
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const double kk = k[p] * k[q];
+                        q4 += u[kA4Slots.sym[27 * i + 9 * j + 3 * p + q]] * kk;
+                        // lam' (J(p,i,j,q) + J(j,q,p,i)) k_p k_q
+                        jf += (J[6 * pidx(p, i) + pidx(j, q)] + J[6 * pidx(j, q) + pidx(p, i)]) * kk;
+                    }
+                Qn[3 * i + j] += a.mu[m] * 0.5 * (a2[pidx(i, j)] + a2k[i] * k[j] + k[i] * a2k[j] + (i == j ? kak : 0.0)) +
+                                 a.lam[m] * q4 + a.lam[m + 3] * jf;
+            }
+    }
+    double vi[3];
+    qnorm_to_vi(Qn, a.rho, vi);
+    double* o = a.Vi + (e * a.ndir + d) * 3;
+    o[0] = vi[0];
+    o[1] = vi[1];
+    o[2] = vi[2];
+}
+
+#ifdef SFG_HOST_EMUL
+template void eij_ortho_kernel<false, 0>(OrthoArgs);
+template void eij_ortho_kernel<false, 1>(OrthoArgs);
+template void eij_ortho_kernel<true, 0>(OrthoArgs);
+template void eij_ortho_kernel<true, 1>(OrthoArgs);
+template void vi_ortho_kernel<0>(OrthoArgs);
+template void vi_ortho_kernel<1>(OrthoArgs);
+#endif
 
 #ifndef SFG_HOST_EMUL
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -718,18 +1076,49 @@ hipError_t launch_vi_em(const ViEMArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_eij_ortho(const OrthoArgs& args, double* Qiso, hipStream_t s) {
+hipError_t launch_eij_ortho(const OrthoArgs& args, hipStream_t s) {
     if (args.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(ortho_Q_kernel, dim3(1), dim3(64), 0, s, args, Qiso);
     OrthoArgs a = args;
-    a.Qiso = Qiso;
-    hipLaunchKernelGGL(eij_ortho_kernel, dim3(nblk(a.P, 64)), dim3(64), 0, s, a);
+    ortho_q6_iso(a.ci, a.q6iso, &a.dqiso);
+    const dim3 g(nblk(a.P, 256)), b(256);
+    if (a.frames) {
+        if (a.x3) hipLaunchKernelGGL((eij_ortho_kernel<false, 0>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((eij_ortho_kernel<false, 1>), g, b, 0, s, a);
+    } else {
+        if (a.x3) hipLaunchKernelGGL((eij_ortho_kernel<true, 0>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((eij_ortho_kernel<true, 1>), g, b, 0, s, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s) {
     if (a.P * a.ndir == 0) return hipSuccess;
-    hipLaunchKernelGGL(vi_ortho_kernel, dim3(nblk(a.P * a.ndir, 128)), dim3(128), 0, s, a);
+    if (a.ndir > 65535) return hipErrorInvalidValue;   // one grid row per direction
+    const dim3 g(nblk(a.P, 256), (unsigned)a.ndir), b(256);
+    if (a.x3) hipLaunchKernelGGL((vi_ortho_kernel<0>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((vi_ortho_kernel<1>), g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ortho_moments(const OrthoDiscArgs& a, hipStream_t s) {
+    if (a.E == 0) return hipSuccess;
+    hipLaunchKernelGGL(ortho_moments_kernel, dim3((unsigned)(a.E * a.G), 3), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ortho_moments_final_kernel, dim3(nblk(a.E * 3 * ORTHO_NMOM, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_eij_ortho_mom(const OrthoMomArgs& args, hipStream_t s) {
+    if (args.E == 0) return hipSuccess;
+    OrthoMomArgs a = args;
+    ortho_q6_iso(a.ci, a.q6iso, &a.dqiso);
+    hipLaunchKernelGGL(eij_ortho_mom_kernel, dim3(nblk(a.E, 64)), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vi_ortho_mom(const OrthoMomArgs& a, hipStream_t s) {
+    if (a.E == 0 || a.ndir == 0) return hipSuccess;
+    if (a.ndir > 65535) return hipErrorInvalidValue;   // one grid row per direction
+    hipLaunchKernelGGL(vi_ortho_mom_kernel, dim3(nblk(a.E, 64), (unsigned)a.ndir), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

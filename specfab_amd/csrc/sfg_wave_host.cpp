@@ -121,6 +121,19 @@ void real4_index(int q, int* l, int* am, int* c) {
     *c = (r == 0) ? 0 : ((r - 1) % 2);
 }
 
+// a2 tensor entries (xx, yy, zz, yz, xz, xy) of a state x are
+// (A2T x) / x0 plus I/3 on the diagonal
+void a2_tensor_map(double A2T[6][15]) {
+    const double r2i = 0.70710678118654752440;
+    for (int q = 0; q < 15; q++) {
+        cd z[15];
+        real_unit_to_complex(q, z);
+        double v[6];
+        a2_linear(z, v);
+        for (int i = 0; i < 6; i++) A2T[i][q] = q == 0 ? 0.0 : (i < 3 ? v[i] : v[i] * r2i);
+    }
+}
+
 // A4F[e][q]: index-sorted entry e of f_ev_c4 (src/moments.f90:202-209) for
 // real basis vector q, times k / sqrt(4 pi) with k = Sqrt(Pi/5.)/21., so that
 // the entries of a state x are (A4F x) / x0.

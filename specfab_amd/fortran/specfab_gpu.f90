@@ -203,6 +203,46 @@ module specfab_gpu
             real(c_double), intent(in) :: ri(*), wi(*)
             complex(c_double_complex), intent(out) :: nlm(*)
         end function
+        integer(c_int) function c_Eij_ortho(c, L, q1, q2, q3, e1, e2, e3, Eg, alpha, ng, Eij) &
+                bind(C, name="sfg_Eij_orthotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ng
+            complex(c_double_complex) :: q1(*), q2(*), q3(*)
+            real(c_double) :: e1(3), e2(3), e3(3), Eg(6), Eij(6)
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_Vi_ortho(c, L, q1, q2, q3, alpha, Lame, rho, ndir, th, ph, Vi) &
+                bind(C, name="sfg_Vi_elastic_orthotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ndir
+            complex(c_double_complex) :: q1(*), q2(*), q3(*)
+            real(c_double), value :: alpha, rho
+            real(c_double) :: Lame(9), th(*), ph(*), Vi(*)
+        end function
+        integer(c_int) function c_Eij_ortho_disc(c, ngr, b, nx, v, e1, e2, e3, Eg, alpha, ng, Eij) &
+                bind(C, name="sfg_Eij_orthotropic_discrete")
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int64_t), value :: ngr
+            integer(c_int), value :: ng
+            real(c_double) :: b(*), nx(*), v(*), e1(3), e2(3), e3(3), Eg(6), Eij(6)
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_Vi_ortho_disc(c, ngr, b, nx, v, alpha, Lame, rho, ndir, th, ph, Vi) &
+                bind(C, name="sfg_Vi_elastic_orthotropic_discrete")
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int64_t), value :: ngr
+            integer(c_int), value :: ndir
+            real(c_double) :: b(*), nx(*), v(*), Lame(9), th(*), ph(*), Vi(*)
+            real(c_double), value :: alpha, rho
+        end function
+        integer(c_int) function c_Cij_to_Lame_ortho(Cij, Lame) bind(C, name="sfg_Cij_to_Lame_orthotropic")
+            import :: c_int, c_double
+            real(c_double) :: Cij(9), Lame(9)
+        end function
         integer(c_int) function hipMalloc(p, bytes) bind(C, name="hipMalloc")
             import :: c_int, c_ptr, c_size_t
             type(c_ptr) :: p
@@ -346,6 +386,70 @@ contains
         z = nlm(1:nlm_len); lm = Lame; th = theta; ph = phi
         call check(c_Vi_el(ctx, int(Lcap, c_int), z, alpha, lm, rho, int(size(theta), c_int), th, ph, Vi), &
                    'Vi_elastic_tranisotropic')
+    end function
+
+    ! Eij_orthotropic(nlm_r1, nlm_r2, nlm_r3, e1,e2,e3, Eij_grain, alpha, n_grain):
+    ! src/enhancementfactors.f90:134-188 (nlm_r3 = 0: v = b x n)
+    function Eij_orthotropic(nlm_r1, nlm_r2, nlm_r3, e1, e2, e3, Eij_grain, alpha, n_grain) result(Eij)
+        complex(dp), intent(in) :: nlm_r1(:), nlm_r2(:), nlm_r3(:)
+        real(dp), intent(in) :: e1(3), e2(3), e3(3), Eij_grain(6), alpha
+        integer, intent(in) :: n_grain
+        real(dp) :: Eij(6), a1(3), a2_(3), a3(3), Eg(6)
+        complex(dp) :: z1(nlm_len), z2(nlm_len), z3(nlm_len)
+        z1 = nlm_r1(1:nlm_len); z2 = nlm_r2(1:nlm_len); z3 = nlm_r3(1:nlm_len)
+        a1 = e1; a2_ = e2; a3 = e3; Eg = Eij_grain
+        call check(c_Eij_ortho(ctx, int(Lcap, c_int), z1, z2, z3, a1, a2_, a3, Eg, alpha, int(n_grain, c_int), Eij), &
+                   'Eij_orthotropic')
+    end function
+
+    ! Vi_elastic_orthotropic(qlm_1, qlm_2, qlm_3, alpha, Lame_grain, rho, theta, phi):
+    ! src/wavepropagation.f90:41-68 (Voigt), Vi(3, ndir) = (qS1, qS2, qP)
+    function Vi_elastic_orthotropic(qlm_1, qlm_2, qlm_3, alpha, Lame_grain, rho, theta, phi) result(Vi)
+        complex(dp), intent(in) :: qlm_1(:), qlm_2(:), qlm_3(:)
+        real(dp), intent(in) :: alpha, Lame_grain(9), rho, theta(:), phi(:)
+        real(dp) :: Vi(3, size(theta)), lm(9), th(size(theta)), ph(size(theta))
+        complex(dp) :: z1(nlm_len), z2(nlm_len), z3(nlm_len)
+        z1 = qlm_1(1:nlm_len); z2 = qlm_2(1:nlm_len); z3 = qlm_3(1:nlm_len)
+        lm = Lame_grain; th = theta; ph = phi
+        call check(c_Vi_ortho(ctx, int(Lcap, c_int), z1, z2, z3, alpha, lm, rho, int(size(theta), c_int), th, ph, Vi), &
+                   'Vi_elastic_orthotropic')
+    end function
+
+    ! Eij_orthotropic_discrete(bi, ni, vi, e1,e2,e3, Eij_grain, alpha, n_grain), bi(N,3):
+    ! src/enhancementfactors.f90:191-227
+    function Eij_orthotropic_discrete(bi, ni, vi, e1, e2, e3, Eij_grain, alpha, n_grain) result(Eij)
+        real(dp), intent(in) :: bi(:,:), ni(:,:), vi(:,:), e1(3), e2(3), e3(3), Eij_grain(6), alpha
+        integer, intent(in) :: n_grain
+        real(dp) :: Eij(6), a1(3), a2_(3), a3(3), Eg(6)
+        real(c_double) :: b(3*size(bi,1)), nx(3*size(bi,1)), v(3*size(bi,1))
+        b = reshape(transpose(bi), [3*size(bi,1)])   ! (N,3) column-major -> row-major
+        nx = reshape(transpose(ni), [3*size(bi,1)])
+        v = reshape(transpose(vi), [3*size(bi,1)])
+        a1 = e1; a2_ = e2; a3 = e3; Eg = Eij_grain
+        call check(c_Eij_ortho_disc(ctx, int(size(bi,1), c_int64_t), b, nx, v, a1, a2_, a3, Eg, alpha, &
+                                    int(n_grain, c_int), Eij), 'Eij_orthotropic_discrete')
+    end function
+
+    ! Vi_elastic_orthotropic__discrete(mi, alpha, Lame_grain, rho, theta, phi), mi(3,3,N)
+    ! = (axis, xyz, grain): src/wavepropagation.f90:70-96
+    function Vi_elastic_orthotropic__discrete(mi, alpha, Lame_grain, rho, theta, phi) result(Vi)
+        real(dp), intent(in) :: mi(:,:,:), alpha, Lame_grain(9), rho, theta(:), phi(:)
+        real(dp) :: Vi(3, size(theta)), lm(9), th(size(theta)), ph(size(theta))
+        real(c_double) :: b(3*size(mi,3)), nx(3*size(mi,3)), v(3*size(mi,3))
+        b = reshape(mi(1,:,:), [3*size(mi,3)])        ! grain g's xyz at 3(g-1)+1..3
+        nx = reshape(mi(2,:,:), [3*size(mi,3)])
+        v = reshape(mi(3,:,:), [3*size(mi,3)])
+        lm = Lame_grain; th = theta; ph = phi
+        call check(c_Vi_ortho_disc(ctx, int(size(mi,3), c_int64_t), b, nx, v, alpha, lm, rho, int(size(theta), c_int), &
+                                   th, ph, Vi), 'Vi_elastic_orthotropic__discrete')
+    end function
+
+    ! Cij_to_Lame_orthotropic(Cij): src/elasticities.f90:122-139
+    function Cij_to_Lame_orthotropic(Cij) result(Lame)
+        real(dp), intent(in) :: Cij(9)
+        real(dp) :: Lame(9), c(9)
+        c = Cij
+        call check(c_Cij_to_Lame_ortho(c, Lame), 'Cij_to_Lame_orthotropic')
     end function
 
     ! Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi): src/wavepropagation.f90:217-261

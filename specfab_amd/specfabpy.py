@@ -451,6 +451,49 @@ def Vi_elastic_orthotropic(nlm_1, nlm_2, nlm_3, alpha, lame_grain, rho, theta, p
     return out.T.copy()
 
 
+def Eij_orthotropic_discrete(bi, ni, vi, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Eij of N discrete grains with slip-system axes bi, ni, vi (N, 3) w.r.t.
+    (e1, e2, e3): src/enhancementfactors.f90:191-266 (Sachs)."""
+    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    b, n, v = f(bi), f(ni), f(vi)
+    if b.ndim != 2 or b.shape[1] != 3 or n.shape != b.shape or v.shape != b.shape:
+        raise ValueError("bi, ni, vi must be (N, 3) each")
+    out = np.zeros(6)
+    check(lib().sfg_Eij_orthotropic_discrete(_ctx(), b.shape[0], dptr(b), dptr(n), dptr(v), dptr(f(e1)), dptr(f(e2)),
+                                             dptr(f(e3)), dptr(f(Eij_grain).reshape(6)), float(alpha), int(n_grain),
+                                             dptr(out)))
+    return out
+
+
+def Eij_orthotropic_discrete_arr(bi, ni, vi, e1, e2, e3, Eij_grain, alpha, n_grain):
+    """Eij (Nt, 6) of Nt grain ensembles bi, ni, vi (Nt, N, 3) in frames e_i
+    (Nt, 3) (src/specfabpy.f90:502-514), one GPU launch."""
+    import torch
+    from .batch import eij_orthotropic_discrete
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+    frames = np.stack([np.asarray(e1), np.asarray(e2), np.asarray(e3)], axis=1)
+    E = eij_orthotropic_discrete(t(bi), t(ni), t(vi), Eij_grain, alpha=alpha, n_grain=n_grain, frames=t(frames))
+    torch.cuda.synchronize()
+    return E.cpu().numpy()
+
+
+def Vi_elastic_orthotropic__discrete(mi, alpha, lame_grain, rho, theta, phi):
+    """Elastic phase velocities (qS1, qS2, qP) x ndir of N discrete grains,
+    mi (3, 3, N) = (axis, xyz, grain) (Voigt): src/specfabpy.f90:986-995 ->
+    src/wavepropagation.f90:70-96.  Returns (3, ndir)."""
+    m = np.asarray(mi, dtype=np.float64)
+    if m.ndim != 3 or m.shape[:2] != (3, 3):
+        raise ValueError("mi must be (3, 3, N)")
+    b, n, v = (np.ascontiguousarray(m[i].T) for i in range(3))
+    th, ph = _dirs(theta, phi)
+    out = np.zeros((th.size, 3))
+    Lm = np.ascontiguousarray(np.asarray(lame_grain, dtype=np.float64).reshape(9))
+    check(lib().sfg_Vi_elastic_orthotropic_discrete(_ctx(), b.shape[0], dptr(b), dptr(n), dptr(v), float(alpha),
+                                                    dptr(Lm), float(rho), th.size, dptr(th), dptr(ph), dptr(out)))
+    return out.T.copy()
+
+
 def Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi):
     """EM phase velocities (S1 fast, S2 slow) x ndir, src/specfabpy.f90:1024-1033
     -> src/wavepropagation.f90:217-261.  Returns shape (2, ndir)."""

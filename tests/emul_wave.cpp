@@ -14,6 +14,10 @@ namespace sfg {
 __global__ void vi_elastic_kernel(ViElasticArgs a);
 __global__ void vi_em_kernel(ViEMArgs a);
 __global__ void a4full_kernel(A4FullArgs a);
+template <bool CART, int MODE>
+__global__ void eij_ortho_kernel(OrthoArgs a);
+template <int MODE>
+__global__ void vi_ortho_kernel(OrthoArgs a);
 }
 
 template <class F>
@@ -99,5 +103,40 @@ extern "C" int emul_a4(int L, int64_t P, const double* x, double* a4) {
     sfg::A4FullArgs a{};
     a.P = P; a.nq = t.nq; a.x = x; a.A4F = t.A4F.data(); a.a4 = a4;
     grid(P, 128, [&] { sfg::a4full_kernel(a); });
+    return 0;
+}
+
+// orthotropic grains: x1, x2, x3 real SoA states (n, P) (x3 null: v = b x n);
+// frames (P, 3, 3) or null.  Eij: (P, 6); Vi: (P, ndir, 3).
+extern "C" int emul_eij_ortho(int L, int64_t P, const double* x1, const double* x2, const double* x3,
+                              const double* frames, const double* ci, double* Eij) {
+    const sfg::RealOperator& op = sfg::real_operator(L);
+    sfg::OrthoArgs a{};
+    a.P = P; a.nq = op.n < 15 ? op.n : 15; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.frames = frames; a.Eij = Eij;
+    for (int i = 0; i < 6; i++) a.ci[i] = ci[i];
+    sfg::ortho_q6_iso(a.ci, a.q6iso, &a.dqiso);
+    if (frames) {
+        if (x3) grid(P, 256, [&] { sfg::eij_ortho_kernel<false, 0>(a); });
+        else grid(P, 256, [&] { sfg::eij_ortho_kernel<false, 1>(a); });
+    } else {
+        if (x3) grid(P, 256, [&] { sfg::eij_ortho_kernel<true, 0>(a); });
+        else grid(P, 256, [&] { sfg::eij_ortho_kernel<true, 1>(a); });
+    }
+    return 0;
+}
+
+extern "C" int emul_vi_ortho(int L, int64_t P, const double* x1, const double* x2, const double* x3,
+                             const double* Lame, double rho, int ndir, const double* theta, const double* phi,
+                             double* Vi) {
+    Tables t = tables(L, ndir, theta, phi);
+    sfg::OrthoArgs a{};
+    a.P = P; a.nq = t.nq; a.ndir = ndir; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.R = t.R.data(); a.rho = rho; a.Vi = Vi;
+    for (int i = 0; i < 6; i++) a.lam[i] = Lame[i];
+    for (int i = 0; i < 3; i++) a.mu[i] = Lame[6 + i];
+    for (int d = 0; d < ndir; d++) {
+        // grid row d: the kernel reads blockIdx.y as its direction
+        if (x3) grid(P, 256, [&] { blockIdx.y = (unsigned)d; sfg::vi_ortho_kernel<0>(a); });
+        else grid(P, 256, [&] { blockIdx.y = (unsigned)d; sfg::vi_ortho_kernel<1>(a); });
+    }
     return 0;
 }

@@ -46,6 +46,11 @@ def test_traffic_table_consistent():
             assert e["bytes_per_parcel_step"] == pytest.approx(per, abs=0.06)
             # at least the state in and out once per launch
             assert e["bytes_per_parcel_step"] * e["steps_per_launch"] > 2 * 91 * 8
+        elif "bytes_per_parcel_launch" in e:
+            # one launch per parcel batch (homogenization kernels): copy-probe scale
+            assert fb == pytest.approx(e["fetch_KiB"] * 2048, rel=1e-6)
+            assert wb == pytest.approx(e["write_KiB"] * 1024, rel=1e-6)
+            assert e["bytes_per_parcel_launch"] == pytest.approx((fb + wb) / e["parcels"], abs=0.1)
         else:
             assert e["bytes_per_parcel"] > 0
 

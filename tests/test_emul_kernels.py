@@ -174,6 +174,9 @@ def emul_wave_lib():
     lib.emul_vi_em.argtypes = [c.c_int, c.c_int64, d, c.c_double, c.c_double, c.c_double, c.c_int, d, d, d]
     lib.emul_vi_elastic_k.argtypes = [c.c_int, c.c_int64, d, c.c_double, d, d, c.c_double, c.c_int, d, d, d, i]
     lib.emul_a4.argtypes = [c.c_int, c.c_int64, d, d]
+    dn = ctypes.c_void_p
+    lib.emul_eij_ortho.argtypes = [c.c_int, c.c_int64, d, d, dn, dn, d, d]
+    lib.emul_vi_ortho.argtypes = [c.c_int, c.c_int64, d, d, dn, d, c.c_double, c.c_int, d, d, d]
     _wave["lib"] = lib
     return lib
 
@@ -202,6 +205,61 @@ def test_wave_kernels_vs_reference():
     A = np.zeros((P, 81))
     lib.emul_a4(8, P, X, A)
     assert relerr(A.reshape(P, 3, 3, 3, 3), g["a4"]) < 1e-14
+
+
+def _ortho_ci(Eg):
+    B = np.asarray(Eg, float) ** (2.0 / 2)
+    lam = [-B[0] + B[1] + B[2], B[0] - B[1] + B[2], B[0] + B[1] - B[2], B[3], B[4], B[5]]
+    return np.array([4.0 / 3 * v for v in lam[:3]] + [2.0 * v for v in lam[3:]])
+
+
+def test_ortho_kernels_vs_reference():
+    """Orthotropic Eij (pair-form Sachs tensor, Cartesian and a random frame,
+    v given, derived, and mixed per parcel) and Vi (Voigt, six directions)
+    kernels against the reference fixture (2e-6 / 1e-7: its REAL(4) joint
+    bodies) and the numpy restatement (1e-11)."""
+    from conftest import load_golden
+    from oracle import orthotropic as O
+    from oracle.oracle import Oracle
+    orc = Oracle()
+    g = load_golden("ortho_L4.npz")
+    lib = emul_wave_lib()
+    Q = g["q"]
+    K = len(Q)
+    X = [to_state(Q[:, i], 4) for i in range(3)]
+    R = g["frame"]
+    fr = np.ascontiguousarray(np.tile(R[None], (K, 1, 1)))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+    for gi, Eg in enumerate(g["Eg"]):
+        ci = _ortho_ci(Eg)
+        for tag, x3 in (("given", X[2]), ("derived", None)):
+            for ftag, frames in (("xyz", None), ("rot", fr)):
+                E = np.zeros((K, 6))
+                lib.emul_eij_ortho(4, K, X[0], X[1], p(x3), p(frames), ci, E)
+                assert relerr(E, g["Eij_%s_%s_%d" % (ftag, tag, gi)]) < 2e-6, (gi, tag, ftag)
+                e = np.eye(3) if frames is None else R
+                for k in range(K):
+                    ora = O.Eij(Q[k, 0], Q[k, 1], Q[k, 2] if x3 is not None else 0 * Q[k, 2], e[0], e[1], e[2], Eg,
+                                orc)
+                    assert relerr(E[k], ora) < 1e-11, (gi, tag, ftag, k)
+    # v given for some parcels and zero (derived) for others in one launch
+    x3m = X[2].copy()
+    x3m[:, 1] = 0.0
+    E = np.zeros((K, 6))
+    lib.emul_eij_ortho(4, K, X[0], X[1], p(x3m), None, _ortho_ci(g["Eg"][1]), E)
+    assert relerr(E[[0, 2]], g["Eij_xyz_given_1"][[0, 2]]) < 2e-6
+    assert relerr(E[1], g["Eij_xyz_derived_1"][1]) < 2e-6
+    th, ph = g["theta"], g["phi"]
+    nd = len(th)
+    for tag, x3 in (("given", X[2]), ("derived", None)):
+        V = np.zeros((K, nd, 3))
+        lib.emul_vi_ortho(4, K, X[0], X[1], p(x3), g["Lame"], float(g["rho"]), nd, th, ph, V)
+        ref = np.transpose(g["Vi_" + tag], (0, 2, 1))
+        assert np.abs(V - ref).max() / ref.max() < 1e-7, tag
+        for k in range(K):
+            ora = O.Vi(Q[k, 0], Q[k, 1], Q[k, 2] if x3 is not None else None, g["Lame"], float(g["rho"]), th, ph,
+                       orc)
+            assert np.abs(V[k] - ora).max() / ora.max() < 1e-11, (tag, k)
 
 
 def test_reuss_singular_branch():

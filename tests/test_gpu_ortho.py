@@ -13,7 +13,14 @@ the numpy restatement (oracle/orthotropic.py):
     batched;
   * L = 8 states (only the l <= 4 blocks enter) and a batch of 4096 random
     triples against the restatement on a sample;
-  * Cij_to_Lame_orthotropic, and the errors for mismatched batches.
+  * Cij_to_Lame_orthotropic, and the errors for mismatched batches;
+  * discrete grains (tests/golden/ortho_discrete.npz: ensembles of 64, 64,
+    100 and 1 grains): Eij 1e-7 against the reference (the continuum
+    isotropic normalisation) and 1e-11 against the restatement, Vi 1e-11
+    against both (the acoustic tensor formed in the grains' frame for k =
+    the direction, the reference's rotated grains for k = z: the same
+    invariants); single-ensemble and batched faces, a 4096-grain ensemble
+    split over several workgroups.
 """
 import numpy as np
 import pytest
@@ -126,3 +133,49 @@ def test_cij_to_lame_orthotropic_and_errors(gpu_ctx):
         eij_orthotropic(b4, b4s, None, g["Eg"][0])
     with pytest.raises(ValueError):
         sf.Eij_orthotropic(g["q"][0, 0], g["q8"][0, 1], None, *np.eye(3), g["Eg"][0], 0.0, 1)
+
+
+def test_discrete_grains(gpu_ctx, orc):
+    import torch
+    from specfab_amd import specfabpy as sf
+    from specfab_amd.batch import eij_orthotropic_discrete, vi_elastic_orthotropic_discrete
+    g = load_golden("ortho_discrete.npz")
+    I, R = np.eye(3), g["frame"]
+    Lame, rho, th, ph = g["Lame"], float(g["rho"]), g["theta"], g["phi"]
+    for k in range(4):
+        T = g["grains_%d" % k]
+        b, n, v = T[:, 0], T[:, 1], T[:, 2]
+        for gi, Eg in enumerate(g["Eg"]):
+            for tag, e in (("xyz", I), ("rot", R)):
+                got = sf.Eij_orthotropic_discrete(b, n, v, e[0], e[1], e[2], Eg, 0.0, 1)
+                assert relerr(got, g["Eij_%s_%d" % (tag, gi)][k]) < 1e-7, (k, gi, tag)
+                assert relerr(got, O.Eij_discrete(b, n, v, e[0], e[1], e[2], Eg, orc)) < 1e-11, (k, gi, tag)
+        V = sf.Vi_elastic_orthotropic__discrete(np.transpose(T, (1, 2, 0)), 1.0, Lame, rho, th, ph)
+        assert np.abs(V - g["Vi"][k]).max() / V.max() < 1e-11, k
+    # batched: the two 64-grain ensembles in one launch
+    T = np.stack([g["grains_0"], g["grains_1"]])                      # (2, 64, 3, 3)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    E = eij_orthotropic_discrete(t(T[:, :, 0]), t(T[:, :, 1]), t(T[:, :, 2]), g["Eg"][1],
+                                 frames=t(np.stack([R, R])))
+    V = vi_elastic_orthotropic_discrete(t(T[:, :, 0]), t(T[:, :, 1]), t(T[:, :, 2]), Lame, rho, th, ph)
+    torch.cuda.synchronize()
+    assert relerr(E.cpu().numpy(), g["Eij_rot_1"][:2]) < 1e-7
+    assert np.abs(V.cpu().numpy() - np.transpose(g["Vi"][:2], (0, 2, 1))).max() / V.max().item() < 1e-11
+    Ea = sf.Eij_orthotropic_discrete_arr(T[:, :, 0], T[:, :, 1], T[:, :, 2], np.tile(I[0], (2, 1)),
+                                         np.tile(I[1], (2, 1)), np.tile(I[2], (2, 1)), g["Eg"][0], 0.0, 1)
+    assert relerr(Ea, g["Eij_xyz_0"][:2]) < 1e-7
+    # a large ensemble: grain sums over several workgroups, against the restatement
+    rng = np.random.default_rng(9)
+    Nb = 4096
+    Qs = np.linalg.qr(np.eye(3) + 0.5 * rng.normal(size=(Nb, 3, 3)))[0]
+    Tb = np.transpose(Qs, (0, 2, 1))
+    got = sf.Eij_orthotropic_discrete(Tb[:, 0], Tb[:, 1], Tb[:, 2], R[0], R[1], R[2], g["Eg"][1], 0.0, 1)
+    ora = O.Eij_discrete(Tb[:, 0], Tb[:, 1], Tb[:, 2], R[0], R[1], R[2], g["Eg"][1], orc)
+    assert relerr(got, ora) < 1e-11
+    V = sf.Vi_elastic_orthotropic__discrete(np.transpose(Tb, (1, 2, 0)), 1.0, Lame, rho, th, ph)
+    ora = O.Vi_discrete(Tb[:, 0], Tb[:, 1], Tb[:, 2], Lame, rho, th, ph)
+    assert np.abs(V.T - ora).max() / ora.max() < 1e-11
+    from specfab_amd._lib import SFGError
+    with pytest.raises(SFGError):
+        sf.Eij_orthotropic_discrete(Tb[:0, 0], Tb[:0, 1], Tb[:0, 2], *np.eye(3), g["Eg"][0], 0.0, 1)

@@ -16,7 +16,10 @@ own Fortran by tests/golden/make_golden.py):
     its first pair is a2_orth; an isotropic fabric has Eij = 1 and
     direction-independent Vi;
   * Cij_to_Lame_orthotropic inverts to the Cij of the Voigt tensor of a
-    single crystal along the axes (C11 = lam11 + 2 mu1, C44 = (mu2 + mu3)/2 ...).
+    single crystal along the axes (C11 = lam11 + 2 mu1, C44 = (mu2 + mu3)/2 ...);
+  * the discrete-grain forms (tests/golden/ortho_discrete.npz): Eij 1e-7, Vi
+    1e-12; repeating every grain leaves Eij unchanged; grain moments of a
+    random planar rotation ensemble approach their closed forms.
 With the reference library present (oracle/_ref), the fixture is also
 regenerated and compared bit for bit."""
 import os
@@ -119,6 +122,48 @@ def test_cij_to_lame_orthotropic():
     assert np.array_equal(C[6:], L[3:6])
 
 
+def test_discrete_matches_reference(orc):
+    """Eij_orthotropic_discrete / Vi_elastic_orthotropic__discrete of four
+    grain ensembles (64, 64, 100 and 1 grains): Eij to 1e-7 (the continuum
+    isotropic normalisation carries the REAL(4) joint bodies: a common factor
+    1 + O(1e-8)), Vi to 1e-12 (exact grain moments, rotate_vector, Cardano)"""
+    g = load_golden("ortho_discrete.npz")
+    I, R = np.eye(3), g["frame"]
+    for k in range(4):
+        T = g["grains_%d" % k]
+        for gi, Eg in enumerate(g["Eg"]):
+            for tag, e in (("xyz", I), ("rot", R)):
+                got = O.Eij_discrete(T[:, 0], T[:, 1], T[:, 2], e[0], e[1], e[2], Eg, orc)
+                assert relerr(got, g["Eij_%s_%d" % (tag, gi)][k]) < 1e-7, (k, gi, tag)
+        V = O.Vi_discrete(T[:, 0], T[:, 1], T[:, 2], g["Lame"], float(g["rho"]), g["theta"], g["phi"])
+        assert np.abs(V - g["Vi"][k].T).max() / V.max() < 1e-12, k
+    # an ensemble of one grain repeated five times has that grain's Eij
+    T = g["grains_3"]
+    rep = np.repeat(T, 5, axis=0)
+    for Eg in g["Eg"]:
+        a = O.Eij_discrete(T[:, 0], T[:, 1], T[:, 2], I[0], I[1], I[2], Eg, orc)
+        b = O.Eij_discrete(rep[:, 0], rep[:, 1], rep[:, 2], I[0], I[1], I[2], Eg, orc)
+        assert np.abs(a - b).max() < 1e-13 * np.abs(a).max()
+
+
+def test_discrete_grains_are_a_sample_of_the_distribution(orc):
+    """Grain triads rotated from a fixed crystal by random rotations about z:
+    the ensemble's moments converge to the continuum moments of its axes
+    (b, n uniformly in the x-y plane, v along z), so Vi of many grains equals
+    Vi of the corresponding transversely isotropic moments; checked through
+    the grains' moments themselves against the closed forms <b_x^2> = 1/2,
+    <b_x^4> = 3/8, <v_z^4> = 1"""
+    rng = np.random.default_rng(5)
+    N = 20000
+    t = rng.uniform(0, 2 * np.pi, N)
+    b = np.stack([np.cos(t), np.sin(t), 0 * t], 1)
+    n = np.stack([-np.sin(t), np.cos(t), 0 * t], 1)
+    v = np.cross(b, n)
+    a2, a4, jk = O.ai_discrete(b, n, v)
+    assert abs(a2[0][0, 0] - 0.5) < 2e-2 and abs(a4[0][0, 0, 0, 0] - 3 / 8) < 2e-2
+    assert abs(a4[2][2, 2, 2, 2] - 1) < 1e-14 and abs(jk[1][0, 0, 2, 2] - 0.5) < 2e-2
+
+
 def test_fixture_regenerates_bit_for_bit():
     from oracle import ref
     if not ref.available():
@@ -128,8 +173,9 @@ def test_fixture_regenerates_bit_for_bit():
                                                                               "make_golden.py"))
     mg = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mg)
-    new = mg.ortho_fixture(ref.RefLib())
-    old = load_golden("ortho_L4.npz")
-    assert set(new) == set(old)
-    for k in new:
-        assert np.array_equal(np.asarray(new[k]), old[k]), k
+    for fn, name in ((mg.ortho_fixture, "ortho_L4.npz"), (mg.ortho_discrete_fixture, "ortho_discrete.npz")):
+        new = fn(ref.RefLib())
+        old = load_golden(name)
+        assert set(new) == set(old)
+        for k in new:
+            assert np.array_equal(np.asarray(new[k]), old[k]), (name, k)
