@@ -228,6 +228,11 @@ int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_
 
 /* single-parcel eig(nlm) -> ei (3x3 rows), lami (3): src/frames.f90:14-22 */
 int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]);
+/* eig3(M, e1, e2, e3, lami) (src/frames.f90:62-80, specfabpy :346-355): the
+ * eigenpairs of a symmetric 3x3 M (row-major), ei rows e1, e2, e3, largest
+ * eigenvalue first; batched M_dev (P, 9) -> ei_dev (P, 9), lami_dev (P, 3). */
+int sfg_eig3(sfg_ctx* ctx, const double M[9], double ei[9], double lami[3]);
+int sfg_eig3_batch(sfg_ctx* ctx, int64_t P, const double* M_dev, double* ei_dev, double* lami_dev, void* stream);
 /* single-parcel Eij_tranisotropic(nlm, e1,e2,e3, Eij_grain, alpha, n_grain) */
 int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e1[3], const double e2[3],
                           const double e3[3], const double Eij_grain[2], double alpha, int n_grain, double Eij[6]);
@@ -411,6 +416,15 @@ int sfg_Vi_elastic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double 
 int sfg_Vi_elastic_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
                                        const double Lame[5], double rho, int ndir, const double* theta,
                                        const double* phi, double* Vi_dev, int32_t* status_dev, void* stream);
+
+/* Qnorm_tranisotropic(nlm, alpha, Lame_grain) (src/wavepropagation.f90:146-211,
+ * specfabpy :1010-1018): the (1-alpha) Reuss + alpha Voigt acoustic tensor
+ * for k = z, row-major 3x3 (P, 9) batched; a singular Reuss compliance
+ * gives NaN and SFG_ST_REUSS_SING in status_dev. */
+int sfg_Qnorm_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double alpha, const double Lame[5],
+                            double Qnorm[9]);
+int sfg_Qnorm_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
+                                  const double Lame[5], double* Qn_dev, int32_t* status_dev, void* stream);
 
 /* Vi_electromagnetic_tranisotropic (src/wavepropagation.f90:217-261): EM
  * phase velocities (S1 fast, S2 slow) for grain permittivities eps_m

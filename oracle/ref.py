@@ -46,6 +46,9 @@ class RefLib:
         L.sfref_a4.argtypes = [_cp, _dp]
         L.sfref_ev_ck_Mandel.argtypes = [_cp, _dp, _dp]
         L.sfref_eig.argtypes = [_cp, _dp, _dp]
+        L.sfref_eig3.argtypes = [_dp, _dp, _dp]
+        L.sfref_eigframe.argtypes = [_dp, _c.c_int, _dp, _dp]
+        L.sfref_nlm_LROT.argtypes = [_cp, _c.c_double, _c.c_int, _dp, _dp, _c.c_double, _cp]
         L.sfref_Eij_tranisotropic.argtypes = [_cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
         L.sfref_Eij_eigframe_batch.argtypes = [_c.c_int, _cp, _dp, _c.c_double, _c.c_int, _dp, _dp]
         L.sfref_Cij_to_Lame_tranisotropic.argtypes = [_dp, _dp]
@@ -169,6 +172,26 @@ class RefLib:
         lami = np.zeros(3, order="F")
         self.lib.sfref_eig(self._v(nlm, np.complex128), ei, lami)
         return np.ascontiguousarray(ei), lami.copy()
+
+    def eig3(self, M):
+        """-> ei (3, 3) rows e1, e2, e3, lami (3)"""
+        ei = np.zeros((3, 3), order="F")
+        lami = np.zeros(3, order="F")
+        self.lib.sfref_eig3(np.asfortranarray(M, dtype=np.float64), ei, lami)
+        return np.ascontiguousarray(ei), lami.copy()
+
+    def eigframe(self, M, plane):
+        ei = np.zeros((3, 3), order="F")
+        lami = np.zeros(3, order="F")
+        self.lib.sfref_eigframe(np.asfortranarray(M, dtype=np.float64), {"ij": 0, "xy": 1, "xz": 2}[plane], ei, lami)
+        return np.ascontiguousarray(ei), lami.copy()
+
+    def nlm_LROT(self, nlm0, dt, Nt, D, W, iota):
+        out = np.zeros((Nt, len(nlm0)), dtype=np.complex128, order="F")
+        self.lib.sfref_nlm_LROT(self._v(nlm0, np.complex128), float(dt), int(Nt),
+                                np.asfortranarray(D, dtype=np.float64), np.asfortranarray(W, dtype=np.float64),
+                                float(iota), out)
+        return np.ascontiguousarray(out)
 
     def Eij_tranisotropic(self, nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
         out = np.zeros(6, order="F")

@@ -185,6 +185,39 @@ contains
         call eig(nlm, ei, lami)
     end subroutine
 
+    ! eig3(M, e1,e2,e3, lami) (src/frames.f90:62-80) -> ei(3,3) rows e1, e2, e3
+    subroutine sfref_eig3(M, ei, lami) bind(C, name="sfref_eig3")
+        real(c_double), intent(in) :: M(3,3)
+        real(c_double), intent(out) :: ei(3,3), lami(3)
+        real(dp) :: e1(3), e2(3), e3(3)
+        call eig3(M, e1, e2, e3, lami)
+        ei(1,:) = e1
+        ei(2,:) = e2
+        ei(3,:) = e3
+    end subroutine
+
+    ! eigframe(M, plane, ei, lami) (src/frames.f90:24-60); plane 0 = 'ij', 1 = 'xy', 2 = 'xz'
+    subroutine sfref_eigframe(M, iplane, ei, lami) bind(C, name="sfref_eigframe")
+        real(c_double), intent(in) :: M(3,3)
+        integer(c_int), value :: iplane
+        real(c_double), intent(out) :: ei(3,3), lami(3)
+        character(len=2) :: plane
+        plane = 'ij'
+        if (iplane == 1) plane = 'xy'
+        if (iplane == 2) plane = 'xz'
+        call eigframe(M, plane, ei, lami)
+    end subroutine
+
+    ! nlm_LROT(nlm0, dt, Nt, D, W, iota) (src/dynamics.f90:99-110) -> nlm(Nt, ncur)
+    subroutine sfref_nlm_LROT(nlm0, dt, Nt, D, W, iota, nlm) bind(C, name="sfref_nlm_LROT")
+        complex(c_double_complex), intent(in) :: nlm0(ncur)
+        real(c_double), value :: dt, iota
+        integer(c_int), value :: Nt
+        real(c_double), intent(in) :: D(Nt,3,3), W(Nt,3,3)
+        complex(c_double_complex), intent(out) :: nlm(Nt, ncur)
+        nlm = nlm_LROT(nlm0, dt, Nt, D, W, iota)
+    end subroutine
+
     subroutine sfref_Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain, Eij) &
             bind(C, name="sfref_Eij_tranisotropic")
         complex(c_double_complex), intent(in) :: nlm(ncur)

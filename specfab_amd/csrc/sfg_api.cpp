@@ -1179,6 +1179,31 @@ int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]
     return eij_single(ctx, L, nlm, 0, nullptr, nullptr, 0.0, 1, nullptr, ei, lami);
 }
 
+int sfg_eig3(sfg_ctx* ctx, const double M[9], double ei[9], double lami[3]) {
+    if (!ctx || !M || !ei || !lami) return fail(SFG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, 21 * sizeof(double)));
+    double* d = (double*)buf.p;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(d, M, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_eig3(1, d, d + 9, d + 18, s));
+    HIPCHK(hipMemcpyAsync(ei, d + 9, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(lami, d + 18, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
+int sfg_eig3_batch(sfg_ctx* ctx, int64_t P, const double* M_dev, double* ei_dev, double* lami_dev, void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!M_dev || !ei_dev || !lami_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_eig3(P, M_dev, ei_dev, lami_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
 int sfg_Eij_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double e1[3], const double e2[3],
                           const double e3[3], const double Eij_grain[2], double alpha, int n_grain, double Eij[6]) {
     if (!e1 || !e2 || !e3 || !Eij_grain || !Eij) return fail(SFG_ERR_ARG, "NULL argument");
@@ -1312,7 +1337,7 @@ int sfg_a4_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* a4
 
 static int vi_el_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha, const double Lame[5],
                         double rho, int ndir, const double* theta, const double* phi, double* Vi_dev,
-                        int32_t* status_dev, hipStream_t s) {
+                        int32_t* status_dev, hipStream_t s, double* Qn_dev = nullptr) {
     if (L < 4) return fail(SFG_ERR_L, "wave velocities need the l <= 4 coefficients (L >= 4)");
     if (ndir < 0 || (ndir > 0 && (!theta || !phi)) || !Lame) return fail(SFG_ERR_ARG, "bad direction arguments");
     const DevOp* op;
@@ -1326,7 +1351,7 @@ static int vi_el_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, dou
     a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4; a.A4F = op->A4F;
     a.alpha = alpha; a.rho = rho;
     sfg::elas_params(Lame, a.kf, a.kr);
-    a.Vi = Vi_dev; a.status = status_dev;
+    a.Vi = Vi_dev; a.status = status_dev; a.Qn = Qn_dev;
     HIPCHK(sfg::launch_vi_elastic(a, s));
     return SFG_OK;
 }
@@ -1430,6 +1455,44 @@ int sfg_Vi_elastic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double 
                                                  dout, nullptr, c->stream);
                          },
                          &u);
+}
+
+// Qnorm_tranisotropic: the acoustic tensor for k = z, i.e. the velocity
+// kernel's tensor for the direction (0, 0), whose rotation blocks are the
+// identity (Wigner-D at theta = 0 with the REAL(4) rule is exactly I)
+int sfg_Qnorm_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double alpha, const double Lame[5],
+                            double Qnorm[9]) {
+    if (!Lame || !Qnorm) return fail(SFG_ERR_ARG, "NULL argument");
+    const double zero = 0.0;
+    ViElUser u{alpha, 1.0, Lame, 1, &zero, &zero};
+    double out[12];
+    int rc = single_parcel(ctx, L, nlm, 12, out,
+                           [](sfg_ctx* c, int L_, const double* dx, double* dout, void* up) {
+                               auto* v = (ViElUser*)up;
+                               return vi_el_locked(c, L_, 1, dx, v->alpha, v->Lame, v->rho, 1, v->th, v->ph, dout,
+                                                   nullptr, c->stream, dout + 3);
+                           },
+                           &u);
+    if (rc) return rc;
+    for (int i = 0; i < 9; i++) Qnorm[i] = out[3 + i];
+    return SFG_OK;
+}
+
+int sfg_Qnorm_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double alpha,
+                                  const double Lame[5], double* Qn_dev, int32_t* status_dev, void* stream) {
+    if (!ctx || !Lame || P < 0 || (P > 0 && (!x_dev || !Qn_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    hipStream_t s = pick(ctx, stream);
+    HIPCHK(hipSetDevice(ctx->device));
+    void* ws;
+    WsLease lease(ctx, s);
+    int rc = lease.acquire((size_t)P * 3 * sizeof(double), &ws);   // the velocities, not returned
+    if (rc) return rc;
+    const double zero = 0.0;
+    rc = vi_el_locked(ctx, L, P, x_dev, alpha, Lame, 1.0, 1, &zero, &zero, (double*)ws, status_dev, s, Qn_dev);
+    if (rc) return rc;
+    return lease.release();
 }
 
 int sfg_Vi_electromagnetic_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double eps_m, double eps_t,

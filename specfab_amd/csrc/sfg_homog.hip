@@ -493,6 +493,21 @@ __device__ __forceinline__ int eij_item(const EijArgs& a, int64_t t) {
 
 // two waves per SIMD (at most 256 registers per lane); with wave flags every
 // wavefront writes its own flag, so they need no clearing
+// eigenpairs of P symmetric 3x3 matrices M (P, 9): ei (P, 9) rows e1, e2, e3,
+// lami (P, 3), largest first (eig3, src/frames.f90:62-80)
+__global__ void __launch_bounds__(256) eig3_kernel(int64_t P, const double* M, double* ei, double* lami) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double m[9], e[9], l[3];
+#pragma unroll
+    for (int i = 0; i < 9; i++) m[i] = M[p * 9 + i];
+    d_eig3(m, e, l);
+#pragma unroll
+    for (int i = 0; i < 9; i++) ei[p * 9 + i] = e[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) lami[p * 3 + i] = l[i];
+}
+
 __global__ void __launch_bounds__(128, 2) eij_kernel(EijArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = a.idx ? a.nidx : a.P;
@@ -946,6 +961,12 @@ hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x
                         hipStream_t s) {
     if (P == 0) return hipSuccess;
     hipLaunchKernelGGL(ev_D4_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, s, P, nrows, A4F, x, tau, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_eig3(int64_t P, const double* M, double* ei, double* lami, hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(eig3_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, P, M, ei, lami);
     return hipGetLastError();
 }
 

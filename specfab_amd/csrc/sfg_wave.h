@@ -24,6 +24,7 @@ struct ViElasticArgs {
     double kr[5];         // elas_revparams (Voigt)
     double* Vi;           // (P, ndir, 3): qS1, qS2, qP
     int* status;          // (P) or null
+    double* Qn;           // (P, ndir, 9) the acoustic tensors (row-major), or null
     // optional work list: items t < nidx evaluate state column idx[t] and
     // write Vi at row t; with nidx_dev the item count is min(*nidx_dev,
     // nidx), read on the device (nidx is then the list's capacity)

@@ -241,6 +241,11 @@ __device__ __forceinline__ void vi_elastic_item(const ViElasticArgs& a, int64_t 
 #pragma unroll
         for (int i = 0; i < 9; i++) Q[i] += a.alpha * QV[i];
     }
+    if (a.Qn) {
+        double* q = a.Qn + (item * a.ndir + d) * 9;
+#pragma unroll
+        for (int i = 0; i < 9; i++) q[i] = ok ? Q[i] : __builtin_nan("");
+    }
     double vi[3];
     if (ok) {
         qnorm_to_vi(Q, a.rho, vi);

@@ -87,6 +87,33 @@ def M_LROT(nlm, eps, omg, iota, zeta):
     return M
 
 
+def nlm_LROT(nlm0, dt, Nt, D, W, iota):
+    """Euler trajectory (Nt, n) of the lattice rotation with per-step strain
+    rate D and spin W (Nt, 3, 3): src/specfabpy.f90:260-269 ->
+    src/dynamics.f90:99-110 (nlm[0] = nlm0, then Nt - 1 steps; zeta = 0).
+    Each step is one launch of the stepping kernel with velocity gradient
+    D_j + W_j (D_j symmetric, W_j antisymmetric, as the reference's
+    kinematics give them)."""
+    import torch
+    from .batch import FabricBatch
+    z0 = _z(nlm0)
+    L = _L_of(z0)
+    Nt = int(Nt)
+    D = np.asarray(D, dtype=np.float64)
+    W = np.asarray(W, dtype=np.float64)
+    if Nt < 1 or D.shape[0] < Nt - 1 or W.shape[0] < Nt - 1 or D.shape[1:] != (3, 3) or W.shape[1:] != (3, 3):
+        raise ValueError("nlm_LROT: D and W must be (>= Nt-1, 3, 3)")
+    fb = FabricBatch(L, nlm=z0[None], ugrad=np.zeros((1, 3, 3)))
+    out = np.zeros((Nt, len(z0)), dtype=np.complex128)
+    out[0] = z0
+    G = torch.as_tensor(D + W, device=fb.dev)
+    for j in range(Nt - 1):
+        fb.ugrad.copy_(G[j].reshape(1, 3, 3))
+        fb.evolve(1, dt, scheme="euler", procs=("LROT",), iota=iota, zeta=0.0)
+        out[j + 1] = fb.nlm()[0].cpu().numpy()
+    return out
+
+
 def M_DDRX(nlm, tau):
     """DDRX operator (Gamma/Gamma0), src/specfabpy.f90:182-190 -> dynamics.f90:251-275."""
     L = _L_of(nlm)
@@ -243,6 +270,54 @@ def eig(nlm):
     z = _z(nlm)
     check(lib().sfg_eig(_ctx(), L, _zp(z), dptr(ei), dptr(lam)))
     return ei.reshape(3, 3), lam
+
+
+def eig3(M):
+    """eig3(M) -> (e1, e2, e3, lami) of a symmetric 3x3 M, largest eigenvalue
+    first: src/specfabpy.f90:346-355 -> src/frames.f90:62-80 (on the GPU)."""
+    ei = np.zeros(9)
+    lam = np.zeros(3)
+    check(lib().sfg_eig3(_ctx(), dptr(_m3(M)), dptr(ei), dptr(lam)))
+    e = ei.reshape(3, 3)
+    return e[0].copy(), e[1].copy(), e[2].copy(), lam
+
+
+def _plane_sort(ei, lam, plane):
+    """eigframe's ordering (src/frames.f90:24-60): 'ij' as eig3; 'xy' / 'xz'
+    put the eigenvector most aligned with z / y last"""
+    if plane == "ij":
+        return ei, lam
+    if plane not in ("xy", "xz"):
+        raise ValueError('eigframe(M, plane): plane must be one of "ij", "xy", "xz"')
+    k = 2 if plane == "xy" else 1
+    imax = np.argmax(np.abs(ei[..., :, k]), axis=-1)
+    order = np.array([[1, 2, 0], [0, 2, 1], [0, 1, 2]])[imax]
+    return (np.take_along_axis(ei, order[..., :, None], axis=-2), np.take_along_axis(lam, order, axis=-1))
+
+
+def eigframe(M, plane):
+    """eigframe(M, plane) -> (ei (3, 3) rows, lami): src/specfabpy.f90:322-331."""
+    e1, e2, e3, lam = eig3(M)
+    return _plane_sort(np.stack([e1, e2, e3]), lam, plane)
+
+
+def eigframe_arr(M, plane):
+    """eigframe of every M (N, 3, 3) in one GPU launch: src/specfabpy.f90:333-344."""
+    import torch
+    Mt = torch.as_tensor(np.ascontiguousarray(np.asarray(M, dtype=np.float64).reshape(-1, 9)),
+                         device=torch.device("cuda", torch.cuda.current_device()))
+    N = Mt.shape[0]
+    ei = torch.empty((N, 9), dtype=torch.float64, device=Mt.device)
+    lam = torch.empty((N, 3), dtype=torch.float64, device=Mt.device)
+    check(lib().sfg_eig3_batch(_ctx(), N, Mt.data_ptr(), ei.data_ptr(), lam.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    return _plane_sort(ei.cpu().numpy().reshape(N, 3, 3), lam.cpu().numpy(), plane)
+
+
+def evD(nlm, tau):
+    """<D> = ev_D2(nlm, tau): src/specfabpy.f90:1105-1111 (on the GPU)."""
+    return deformability_tranisotropic(nlm, tau, 2)
 
 
 def Eij_tranisotropic(nlm, e1, e2, e3, Eij_grain, alpha, n_grain):
@@ -492,6 +567,16 @@ def Vi_elastic_orthotropic__discrete(mi, alpha, lame_grain, rho, theta, phi):
     check(lib().sfg_Vi_elastic_orthotropic_discrete(_ctx(), b.shape[0], dptr(b), dptr(n), dptr(v), float(alpha),
                                                     dptr(Lm), float(rho), th.size, dptr(th), dptr(ph), dptr(out)))
     return out.T.copy()
+
+
+def Qnorm_tranisotropic(nlm, alpha, lame_grain):
+    """(1 - alpha) Reuss + alpha Voigt acoustic tensor (3, 3) for k = z:
+    src/specfabpy.f90:1010-1018 -> src/wavepropagation.f90:146-211."""
+    L = _L_of(nlm)
+    out = np.zeros(9)
+    Lm = np.ascontiguousarray(np.asarray(lame_grain, dtype=np.float64).reshape(5))
+    check(lib().sfg_Qnorm_tranisotropic(_ctx(), L, _zp(_z(nlm)), float(alpha), dptr(Lm), dptr(out)))
+    return out.reshape(3, 3)
 
 
 def Vi_electromagnetic_tranisotropic(nlm, eps_m, eps_t, mu, theta, phi):
