@@ -226,6 +226,18 @@ int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_
                                 const double* frames_dev, const double Eij_grain[2], double alpha, int n_grain,
                                 double* Eij_dev, double* lami_dev, double* ei_dev, int32_t* status_dev, void* stream);
 
+/* Evw_tranisotropic(nlm, v, w, tau, Eij_grain, alpha, n_grain)
+ * (src/enhancementfactors.f90:47-69; specfabpy :379-388): the enhancement
+ * factor of one stress state tau (symmetric, row-major) along v (x) w,
+ * (1-alpha) Sachs + alpha Taylor, normalised by the isotropic fabric.
+ * Batched: the nstates states states_dev (S, 15) = v(3), w(3), tau(9) for
+ * every parcel, Evw_dev (P, S); status_dev bits as for Eij. */
+int sfg_Evw_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double v[3], const double w[3],
+                          const double tau[9], const double Eij_grain[2], double alpha, int n_grain, double* Evw);
+int sfg_Evw_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int nstates,
+                                const double* states_dev, const double Eij_grain[2], double alpha, int n_grain,
+                                double* Evw_dev, int32_t* status_dev, void* stream);
+
 /* single-parcel eig(nlm) -> ei (3x3 rows), lami (3): src/frames.f90:14-22 */
 int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]);
 /* eig3(M, e1, e2, e3, lami) (src/frames.f90:62-80, specfabpy :346-355): the
@@ -358,6 +370,16 @@ int sfg_Eij_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* n
 int sfg_Eij_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
                               const double* x3_dev, int frame, const double* frames_dev, const double Eij_grain[6],
                               double alpha, int n_grain, double* Eij_dev, void* stream);
+/* Evw_orthotropic(nlm_r1, nlm_r2, nlm_r3, v, w, tau, Eij_grain, alpha,
+ * n_grain) (src/enhancementfactors.f90:134-160; specfabpy :425-434): one
+ * stress state (tau symmetric); batched over the states states_dev (S, 15)
+ * as sfg_Evw_tranisotropic_batch, Evw_dev (P, S). */
+int sfg_Evw_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                        const double v[3], const double w[3], const double tau[9], const double Eij_grain[6],
+                        double alpha, int n_grain, double* Evw);
+int sfg_Evw_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                              const double* x3_dev, int nstates, const double* states_dev, const double Eij_grain[6],
+                              double alpha, int n_grain, double* Evw_dev, void* stream);
 /* Vi_elastic_orthotropic(nlm_1, nlm_2, nlm_3, alpha, Lame_grain, rho, theta,
  * phi) (src/wavepropagation.f90:41-68,98-115; specfabpy :975-984): Voigt
  * average only (alpha unused, as the reference); Lame_grain = (lam11, lam22,

@@ -47,6 +47,8 @@ class RefLib:
         L.sfref_ev_ck_Mandel.argtypes = [_cp, _dp, _dp]
         L.sfref_eig.argtypes = [_cp, _dp, _dp]
         L.sfref_eig3.argtypes = [_dp, _dp, _dp]
+        L.sfref_Evw_tranisotropic.argtypes = [_cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
+        L.sfref_Evw_orthotropic.argtypes = [_cp, _cp, _cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
         L.sfref_eigframe.argtypes = [_dp, _c.c_int, _dp, _dp]
         L.sfref_nlm_LROT.argtypes = [_cp, _c.c_double, _c.c_int, _dp, _dp, _c.c_double, _cp]
         L.sfref_Eij_tranisotropic.argtypes = [_cp, _dp, _dp, _dp, _dp, _c.c_double, _c.c_int, _dp]
@@ -172,6 +174,20 @@ class RefLib:
         lami = np.zeros(3, order="F")
         self.lib.sfref_eig(self._v(nlm, np.complex128), ei, lami)
         return np.ascontiguousarray(ei), lami.copy()
+
+    def Evw_tranisotropic(self, nlm, v, w, tau, Eg, alpha, ng):
+        out = np.zeros(1, order="F")
+        self.lib.sfref_Evw_tranisotropic(self._v(nlm, np.complex128), self._v(v), self._v(w),
+                                         np.asfortranarray(tau, dtype=np.float64), self._v(Eg), float(alpha), int(ng),
+                                         out)
+        return float(out[0])
+
+    def Evw_orthotropic(self, q1, q2, q3, v, w, tau, Eg, alpha, ng):
+        out = np.zeros(1, order="F")
+        self.lib.sfref_Evw_orthotropic(self._v(q1, np.complex128), self._v(q2, np.complex128),
+                                       self._v(q3, np.complex128), self._v(v), self._v(w),
+                                       np.asfortranarray(tau, dtype=np.float64), self._v(Eg), float(alpha), int(ng), out)
+        return float(out[0])
 
     def eig3(self, M):
         """-> ei (3, 3) rows e1, e2, e3, lami (3)"""

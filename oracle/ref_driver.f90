@@ -185,6 +185,26 @@ contains
         call eig(nlm, ei, lami)
     end subroutine
 
+    ! Evw_tranisotropic(v, w, tau, nlm, Eij_grain, alpha, n_grain) (src/enhancementfactors.f90:47-69)
+    subroutine sfref_Evw_tranisotropic(nlm, v, w, tau, Eg, alpha, ng, E) bind(C, name="sfref_Evw_tranisotropic")
+        complex(c_double_complex), intent(in) :: nlm(ncur)
+        real(c_double), intent(in) :: v(3), w(3), tau(3,3), Eg(2)
+        real(c_double), value :: alpha
+        integer(c_int), value :: ng
+        real(c_double), intent(out) :: E
+        E = Evw_tranisotropic(v, w, tau, nlm, Eg, alpha, ng)
+    end subroutine
+
+    ! Evw_orthotropic(v, w, tau, q1, q2, q3, Eij_grain, alpha, n_grain) (src/enhancementfactors.f90:134-160)
+    subroutine sfref_Evw_orthotropic(q1, q2, q3, v, w, tau, Eg, alpha, ng, E) bind(C, name="sfref_Evw_orthotropic")
+        complex(c_double_complex), intent(in) :: q1(ncur), q2(ncur), q3(ncur)
+        real(c_double), intent(in) :: v(3), w(3), tau(3,3), Eg(6)
+        real(c_double), value :: alpha
+        integer(c_int), value :: ng
+        real(c_double), intent(out) :: E
+        E = Evw_orthotropic(v, w, tau, q1, q2, q3, Eg, alpha, ng)
+    end subroutine
+
     ! eig3(M, e1,e2,e3, lami) (src/frames.f90:62-80) -> ei(3,3) rows e1, e2, e3
     subroutine sfref_eig3(M, ei, lami) bind(C, name="sfref_eig3")
         real(c_double), intent(in) :: M(3,3)

@@ -113,6 +113,10 @@ SIGNATURES = {
     "sfg_Vi_elastic_orthotropic_discrete": (_i, [_vp, _i64, _dp, _dp, _dp, _d, _dp, _d, _i, _dp, _dp, _dp]),
     "sfg_Vi_elastic_orthotropic_discrete_batch": (_i, [_vp, _i64, _i64, _vp, _vp, _vp, _d, _dp, _d, _i, _dp, _dp,
                                                        _vp, _vp]),
+    "sfg_Evw_tranisotropic": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _d, _i, _dp]),
+    "sfg_Evw_tranisotropic_batch": (_i, [_vp, _i, _i64, _vp, _i, _vp, _dp, _d, _i, _vp, _vp, _vp]),
+    "sfg_Evw_orthotropic": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _d, _i, _dp]),
+    "sfg_Evw_orthotropic_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _i, _vp, _dp, _d, _i, _vp, _vp]),
     "sfg_eig3": (_i, [_vp, _dp, _dp, _dp]),
     "sfg_eig3_batch": (_i, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "sfg_Qnorm_tranisotropic": (_i, [_vp, _i, _dp, _d, _dp, _dp]),

@@ -160,6 +160,31 @@ def eij_orthotropic(b1, b2, b3, Eij_grain, alpha=0.0, n_grain=1, frames=None, ct
     return E
 
 
+def _states(v, w, tau, dev):
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    w = np.asarray(w, dtype=np.float64).reshape(-1, 3)
+    tau = np.asarray(tau, dtype=np.float64).reshape(-1, 3, 3)
+    if not (len(v) == len(w) == len(tau)):
+        raise ValueError("v, w, tau must hold the same number of stress states")
+    if np.abs(tau - np.transpose(tau, (0, 2, 1))).max(initial=0) > 1e-12 * np.abs(tau).max(initial=0):
+        raise ValueError("tau must be symmetric (a stress)")
+    st = np.concatenate([v, w, tau.reshape(-1, 9)], axis=1)
+    return torch.as_tensor(np.ascontiguousarray(st), device=dev)
+
+
+def evw_orthotropic(b1, b2, b3, v, w, tau, Eij_grain, alpha=0.0, n_grain=1, ctx=None):
+    """Orthotropic Evw (P, S) of the S stress states tau (S, 3, 3) along
+    v (x) w (S, 3) for every parcel (src/enhancementfactors.f90:134-160)."""
+    ctx = ctx or b1.ctx
+    st = _states(v, w, tau, b1.dev)
+    E = torch.empty((b1.P, st.shape[0]), dtype=torch.float64, device=b1.dev)
+    Eg = (_c.c_double * 6)(*np.asarray(Eij_grain, dtype=np.float64).reshape(6))
+    check(lib().sfg_Evw_orthotropic_batch(ctx.handle, b1.L, b1.P, _ptr(b1.x), _ptr(b2.x),
+                                          _ptr(b3.x) if b3 is not None else None, st.shape[0], _ptr(st), Eg,
+                                          float(alpha), int(n_grain), _ptr(E), _stream()))
+    return E
+
+
 def vi_elastic_orthotropic(b1, b2, b3, lame_grain, rho, theta, phi, alpha=1.0, ctx=None):
     """Orthotropic elastic phase velocities (P, ndir, 3) = (qS1, qS2, qP)
     (Voigt; src/wavepropagation.f90:41-68) of the distributions in b1, b2, b3
@@ -406,6 +431,17 @@ class FabricBatch:
         return (E, lam, ei) if mode == 0 else E
 
     # ---- rotation, a4 eigentensors, validity ----
+    def Evw(self, v, w, tau, Eij_grain=(1.0, 1e3), alpha=0.0125, n_grain=1, status=None):
+        """Evw (P, S) of the S stress states tau (S, 3, 3) along v (x) w (S, 3)
+        for every parcel (src/enhancementfactors.f90:47-69)."""
+        st = _states(v, w, tau, self.dev)
+        E = torch.empty((self.P, st.shape[0]), dtype=torch.float64, device=self.dev)
+        Eg = (_c.c_double * 2)(*np.asarray(Eij_grain, dtype=np.float64).reshape(2))
+        check(lib().sfg_Evw_tranisotropic_batch(self.ctx.handle, self.L, self.P, _ptr(self.x), st.shape[0], _ptr(st),
+                                                Eg, float(alpha), int(n_grain), _ptr(E),
+                                                _ptr(status if status is not None else self.status), _stream()))
+        return E
+
     def rotate(self, theta, phi):
         """Rotate every parcel's state in place by its own (theta, phi) (P)
         or by scalars: rotate_nlm, src/rotation.f90:29-55 (L <= 12)."""

@@ -1140,6 +1140,92 @@ int sfg_Eij_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_
                             status_dev, pick(ctx, stream));
 }
 
+// Evw of nstates stress states states_dev (S, 15) for P parcels: Evw_dev (P, S)
+static int evw_batch_locked(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int nstates,
+                            const double* states_dev, const double Eg[2], double alpha, int n_grain, double* Evw_dev,
+                            int32_t* status_dev, hipStream_t s) {
+    if (L < 4) return fail(SFG_ERR_L, "Evw needs the l <= 4 coefficients (L >= 4)");
+    if (n_grain != 1 && n_grain != -3 && n_grain != 3) return fail(SFG_ERR_ARG, "n_grain must be 1, -3 or 3");
+    if (n_grain == 3 && L < 8)   // src/homogenizations.f90:94-96
+        return fail(SFG_ERR_L, "Sachs homogenization with n'=3 requires L >= 8");
+    if (nstates < 0 || (nstates > 0 && !states_dev)) return fail(SFG_ERR_ARG, "bad stress states");
+    if (nstates == 0) return SFG_OK;
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::EijArgs a{};
+    a.P = P; a.nq = op->nq; a.frame = 2; a.n_grain = n_grain;
+    a.Ecc = Eg[0]; a.Eca = Eg[1]; a.alpha = n_grain == 3 ? 1.0 : alpha;
+    a.x = x_dev; a.A2 = op->A2; a.A4 = op->A4;
+    a.Eij = Evw_dev; a.status = status_dev; a.states = states_dev; a.nstates = nstates;
+    if (n_grain != 3 || alpha > 0) HIPCHK(sfg::launch_evw(a, s));   // (the Taylor part for n' = 3)
+    if (n_grain == 3) {
+        a.alpha = alpha;
+        sfg::Sachs3Args b{};
+        b.A4F = op->A4F;
+        b.nrows = op->n;
+        double m[15][15];
+        sfg::a4full_map(m);
+        for (int e = 0; e < 15; e++) b.u_iso[e] = m[e][0];
+        HIPCHK(sfg::launch_evw_sachs3(a, b, s));
+    }
+    return SFG_OK;
+}
+
+int sfg_Evw_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, int nstates,
+                                const double* states_dev, const double Eij_grain[2], double alpha, int n_grain,
+                                double* Evw_dev, int32_t* status_dev, void* stream) {
+    if (!ctx || !Eij_grain || P < 0 || (P > 0 && nstates > 0 && (!x_dev || !Evw_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    return evw_batch_locked(ctx, L, P, x_dev, nstates, states_dev, Eij_grain, alpha, n_grain, Evw_dev, status_dev,
+                            pick(ctx, stream));
+}
+
+// (v, w, tau) of one stress state as the 15 doubles the kernels read; tau must be symmetric
+static int pack_state(const double v[3], const double w[3], const double tau[9], double st[15]) {
+    if (!v || !w || !tau) return fail(SFG_ERR_ARG, "NULL argument");
+    double nrm = 0, asym = 0;
+    for (int i = 0; i < 9; i++) nrm = std::max(nrm, std::fabs(tau[i]));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) asym = std::max(asym, std::fabs(tau[3 * i + j] - tau[3 * j + i]));
+    if (asym > 1e-12 * nrm) return fail(SFG_ERR_ARG, "tau must be symmetric (a stress)");
+    for (int i = 0; i < 3; i++) st[i] = v[i], st[3 + i] = w[i];
+    for (int i = 0; i < 9; i++) st[6 + i] = tau[i];
+    return SFG_OK;
+}
+
+int sfg_Evw_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, const double v[3], const double w[3],
+                          const double tau[9], const double Eij_grain[2], double alpha, int n_grain, double* Evw) {
+    if (!ctx || !nlm || !Eij_grain || !Evw) return fail(SFG_ERR_ARG, "NULL argument");
+    double st[15];
+    int rc = pack_state(v, w, tau, st);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const DevOp* op;
+    rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = op->n;
+    DevBuf buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)(3 * n + 16) * sizeof(double)));
+    double* dn = (double*)buf.p;
+    double* dx = dn + 2 * n;
+    double* dst = dx + n;
+    double* dout = dst + 15;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dn, nlm, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dst, st, 15 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(sfg::launch_nlm_to_state(1, n, op->r2c, op->rpart, dn, dx, s));
+    rc = evw_batch_locked(ctx, L, 1, dx, 1, dst, Eij_grain, alpha, n_grain, dout, nullptr, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(Evw, dout, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SFG_OK;
+}
+
 // single parcel: host nlm -> device state -> kernel -> host outputs
 static int eij_single(sfg_ctx* ctx, int L, const double* nlm, int frame, const double* frame9, const double Eg[2],
                       double alpha, int ng, double* Eij, double* ei, double* lami) {
@@ -1544,6 +1630,26 @@ int eij_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const dou
     return SFG_OK;
 }
 
+int evw_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const double* x2, const double* x3,
+                     int nstates, const double* states_dev, const double Eg[6], double alpha, int n_grain,
+                     double* Evw_dev, hipStream_t s) {
+    (void)alpha;   // Sachs only, as the reference (src/enhancementfactors.f90:162)
+    if (L < 4) return fail(SFG_ERR_L, "orthotropic Evw needs the l <= 4 coefficients (L >= 4)");
+    if (n_grain != 1) return fail(SFG_ERR_ARG, "orthotropic Sachs homogenization supports n_grain = 1 only");
+    if (nstates < 0 || (nstates > 0 && !states_dev)) return fail(SFG_ERR_ARG, "bad stress states");
+    if (nstates == 0) return SFG_OK;
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    sfg::OrthoArgs a{};
+    a.P = P; a.nq = op->nq; a.x1 = x1; a.x2 = x2; a.x3 = x3;
+    ortho_ci(Eg, n_grain, a.ci);
+    a.Eij = Evw_dev; a.states = states_dev; a.nstates = nstates;
+    HIPCHK(sfg::launch_evw_ortho(a, s));
+    return SFG_OK;
+}
+
 int vi_ortho_locked(sfg_ctx* ctx, int L, int64_t P, const double* x1, const double* x2, const double* x3,
                     const double Lame[9], double rho, int ndir, const double* theta, const double* phi, double* Vi_dev,
                     hipStream_t s) {
@@ -1578,11 +1684,11 @@ int ortho_single(sfg_ctx* ctx, int L, const double* n1, const double* n2, const 
     HIPCHK(hipSetDevice(ctx->device));
     const int n = op->n;
     DevBuf buf;
-    HIPCHK(hipMalloc(&buf.p, ((size_t)9 * n + nout + 9) * sizeof(double)));
+    HIPCHK(hipMalloc(&buf.p, ((size_t)9 * n + nout + 15) * sizeof(double)));
     double* dn = (double*)buf.p;             // 3 x 2n
     double* dx = dn + 6 * n;                 // 3 x n
     double* dout = dx + 3 * n;
-    double* dfr = dout + nout;
+    double* dfr = dout + nout;               // a frame (9) or a stress state (15)
     hipStream_t s = ctx->stream;
     const double* ns[3] = {n1, n2, n3};
     for (int m = 0; m < 3; m++) {
@@ -1620,6 +1726,33 @@ int sfg_Eij_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* n
                             hipStream_t s) {
                             HIPCHK(hipMemcpyAsync(dfr, f, 9 * sizeof(double), hipMemcpyHostToDevice, s));
                             return eij_ortho_locked(ctx, L, 1, x1, x2, x3, 1, dfr, Eij_grain, alpha, n_grain, dout,
+                                                    s);
+                        });
+}
+
+int sfg_Evw_orthotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x1_dev, const double* x2_dev,
+                              const double* x3_dev, int nstates, const double* states_dev, const double Eij_grain[6],
+                              double alpha, int n_grain, double* Evw_dev, void* stream) {
+    if (!ctx || !Eij_grain || P < 0 || (P > 0 && nstates > 0 && (!x1_dev || !x2_dev || !Evw_dev)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    return evw_ortho_locked(ctx, L, P, x1_dev, x2_dev, x3_dev, nstates, states_dev, Eij_grain, alpha, n_grain,
+                            Evw_dev, pick(ctx, stream));
+}
+
+int sfg_Evw_orthotropic(sfg_ctx* ctx, int L, const double* nlm1, const double* nlm2, const double* nlm3,
+                        const double v[3], const double w[3], const double tau[9], const double Eij_grain[6],
+                        double alpha, int n_grain, double* Evw) {
+    if (!Eij_grain || !Evw) return fail(SFG_ERR_ARG, "NULL argument");
+    double st[15];
+    int rc = pack_state(v, w, tau, st);
+    if (rc) return rc;
+    return ortho_single(ctx, L, nlm1, nlm2, nlm3, 1, Evw,
+                        [&](const double* x1, const double* x2, const double* x3, double* dout, double* dfr,
+                            hipStream_t s) {
+                            HIPCHK(hipMemcpyAsync(dfr, st, 15 * sizeof(double), hipMemcpyHostToDevice, s));
+                            return evw_ortho_locked(ctx, L, 1, x1, x2, x3, 1, dfr, Eij_grain, alpha, n_grain, dout,
                                                     s);
                         });
 }

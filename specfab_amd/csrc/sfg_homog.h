@@ -30,6 +30,11 @@ struct EijArgs {
     // the first pass (1: one of its rows needs the regularized second pass),
     // or null (the second pass then scans the rows)
     int* wave_flag;
+    // Evw (launch_evw): nstates stress states (v, w, tau) given directly,
+    // states (nstates, 15) = v(3), w(3), tau(9) row-major (tau symmetric);
+    // Eij then holds (P, nstates)
+    const double* states;
+    int nstates;
     // filled by launch_eij from (Ecc, Eca, n_grain): rheo_params_tranisotropic
     // (src/rheologies.f90:123-132) of the Sachs (ef = +1) and Taylor (ef = -1)
     // branches and the isotropic normalisations (src/homogenizations.f90:207, 235)
@@ -49,6 +54,12 @@ struct Sachs3Args {
 // launch_eij (n_grain = 3, alpha = 1) left in a.Eij; the Sachs parameters are
 // filled here from (Ecc, Eca)
 hipError_t launch_eij_sachs3(const EijArgs& a, const Sachs3Args& b, hipStream_t s);
+// Evw of the given stress states (a.states, a.nstates) per parcel
+// (src/enhancementfactors.f90:47-69): n_grain 1 or -3 (Sachs + Taylor), or
+// the Taylor part with n' = 3 parameters (n_grain 3, alpha = 1) that
+// launch_evw_sachs3 then mixes with the n' = 3 Sachs part
+hipError_t launch_evw(const EijArgs& a, hipStream_t s);
+hipError_t launch_evw_sachs3(const EijArgs& a, const Sachs3Args& b, hipStream_t s);
 // <D> with n_RSS = 4 (src/dynamics.f90:424-449): x (nrows, P), tau (P, 3, 3) -> out (P)
 hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x, const double* tau, double* out,
                         hipStream_t s);

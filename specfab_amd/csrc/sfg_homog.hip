@@ -360,13 +360,35 @@ __device__ __forceinline__ double sachs_ratio(const EijArgs& c, const double (&a
     return esvw / ((lin ? c.sachs_iso : I2) * tvw);
 }
 
-// Eij of a parcel whose Taylor matrix P is not positive definite: the
+// a stress state given directly (st = v(3), w(3), tau(9) row-major, tau
+// symmetric): tv = mandel(tau), uv = mandel(sym(v w)); returns v . w
+__device__ __forceinline__ double given_state(const double* st, double (&tv)[6], double (&uv)[6]) {
+    const double r2 = sqrt(2.0);
+    const double *v = st, *w = st + 3, *t = st + 6;
+    tv[0] = t[0];
+    tv[1] = t[4];
+    tv[2] = t[8];
+    tv[3] = r2 * t[5];
+    tv[4] = r2 * t[2];
+    tv[5] = r2 * t[1];
+    uv[0] = v[0] * w[0];
+    uv[1] = v[1] * w[1];
+    uv[2] = v[2] * w[2];
+    uv[3] = r2 * (0.5 * (v[1] * w[2] + w[1] * v[2]));
+    uv[4] = r2 * (0.5 * (v[0] * w[2] + w[0] * v[2]));
+    uv[5] = r2 * (0.5 * (v[0] * w[1] + w[0] * v[1]));
+    return v[0] * w[0] + v[1] * w[1] + v[2] * w[2];
+}
+
+// Evw of a parcel whose Taylor matrix P is not positive definite: the
 // reference's regularized branch (src/homogenizations.f90:174-185), solving
 // (F^T F + 1e-6 I) x = F^T tau with F the matrix its failed dposv('L') left
-// (the REAL(4) 1e-6), for every stress state.  Rare: run by a second pass
+// (the REAL(4) 1e-6), for the nq stress states state(q, tv, uv) (returning
+// v . w).  Rare: the Eij path runs it in a second pass
 // (eij_regularized_kernel) over the rows the first one marked, so that its
 // registers do not weigh on the common path.
-__device__ __noinline__ int d_Eij_regularized(const EijArgs& c, int64_t p, const double (&e)[9], double* Eij) {
+template <class SF>
+__device__ __noinline__ int d_states_regularized(const EijArgs& c, int64_t p, int nq, SF state, double* Eij) {
     int status = 2;
     double a2v[6], a4v[6][6];
     load_moments(c, p, a2v, a4v);
@@ -385,9 +407,9 @@ __device__ __noinline__ int d_Eij_regularized(const EijArgs& c, int64_t p, const
         }
     const bool ok_reg = d_chol6(R, Lf);
     if (!ok_reg) status |= 4;
-    for (int q = 0; q < 6; q++) {
+    for (int q = 0; q < nq; q++) {
         double tv[6], uv[6], tvw;
-        const double vdw = stress_state(e, q, tv, uv);
+        const double vdw = state(q, tv, uv);
         const double ts = sachs_ratio(c, a2v, Ql, tv, uv, vdw, &tvw);
         double x[6];
         for (int i = 0; i < 6; i++) {
@@ -403,6 +425,11 @@ __device__ __noinline__ int d_Eij_regularized(const EijArgs& c, int64_t p, const
         if (!isfinite(E)) status |= 1;
     }
     return status;
+}
+
+__device__ __forceinline__ int d_Eij_regularized(const EijArgs& c, int64_t p, const double (&e)[9], double* Eij) {
+    return d_states_regularized(
+        c, p, 6, [&](int q, double (&tv)[6], double (&uv)[6]) { return stress_state(e, q, tv, uv); }, Eij);
 }
 
 // Eij of one parcel in frame (e1, e2, e3): src/enhancementfactors.f90:23-69,
@@ -553,6 +580,51 @@ __global__ void __launch_bounds__(256) eij_regularized_kernel(EijArgs a) {
     const int64_t n = (a.idx && a.nidx_dev) ? (*a.nidx_dev < a.nidx ? *a.nidx_dev : a.nidx) : (a.idx ? a.nidx : a.P);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) eij_regularized_item(a, t);
+}
+
+// Evw of the given stress states per parcel (src/enhancementfactors.f90:47-69):
+// one lane per parcel, the Taylor factor formed once for all states; a
+// parcel whose Taylor matrix is not positive definite takes the regularized
+// branch in place (rare)
+__global__ void __launch_bounds__(128, 2) evw_kernel(EijArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double a2v[6], Pl[21], Ql[21];
+    {
+        double a4v[6][6];
+        load_moments(a, p, a2v, a4v);
+        taylor_sachs_lower(a, a2v, a4v, Pl, Ql);
+    }
+    double* out = a.Eij + p * a.nstates;
+    const double* S = a.states;
+    int st = 0;
+    if (!d_chol6_packed(Pl)) {
+        st = d_states_regularized(
+            a, p, a.nstates, [&](int q, double (&tv)[6], double (&uv)[6]) { return given_state(S + 15 * q, tv, uv); },
+            out);
+    } else {
+        double rd[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) rd[i] = 1.0 / Pl[i * (i + 1) / 2 + i];
+#pragma unroll 1
+        for (int q = 0; q < a.nstates; q++) {
+            double tv[6], uv[6], tvw;
+            const double vdw = given_state(S + 15 * q, tv, uv);
+            const double ts = sachs_ratio(a, a2v, Ql, tv, uv, vdw, &tvw);
+            double xs[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) xs[i] = tv[i];
+            d_chol6_solve_rcp(Pl, rd, xs);
+            double etvw = 0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) etvw += xs[i] * uv[i];
+            const double tt = etvw / (a.taylor_iso * tvw);
+            const double E = (1 - a.alpha) * ts + a.alpha * tt;
+            out[q] = E;
+            if (!isfinite(E)) st |= 1;
+        }
+    }
+    if (a.status && st) atomicOr(a.status + p, st);
 }
 
 
@@ -807,7 +879,9 @@ __device__ __forceinline__ void nonlin_moments(const double* x, int64_t P, int64
 }  // namespace
 
 // Eij with n' = 3 Sachs: (1 - alpha) Sachs + alpha Taylor, the Taylor values
-// (alpha > 0) already in Eij from eij_kernel; writes the frame as eij_kernel
+// (alpha > 0) already in Eij from eij_kernel; writes the frame as eij_kernel.
+// STATES: Evw of the given stress states (a.states) instead of the frame's six
+template <bool STATES>
 __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b) {
     __shared__ double lds[kCols * 64];
     __shared__ double iso[88];   // the isotropic state's a4 entries, sixth and eighth moments
@@ -833,7 +907,7 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
         for (int i = 0; i < 6; i++) a2v[i] = a2v[i] * ix0 + (i < 3 ? 1.0 / 3.0 : 0.0);
     }
     double e[9], lam[3];
-    eij_frame(a, p, a2v, e, lam);
+    if (!STATES) eij_frame(a, p, a2v, e, lam);
     double a2[9], a2i[9];
     a2_full(xq, a2);
 #pragma unroll
@@ -841,6 +915,14 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
     // the stress state q of the frame: tau_vv = I/3 - v v (q < 3), tau_vw = v w + w v
     // (src/enhancementfactors.f90:398-413)
     auto state = [&](int q, double (&v)[3], double (&w)[3], double (&tau)[9]) {
+        if (STATES) {
+            const double* st = a.states + 15 * q;
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[i] = st[i], w[i] = st[3 + i];
+#pragma unroll
+            for (int i = 0; i < 9; i++) tau[i] = st[6 + i];
+            return;
+        }
         const int iv = q < 3 ? q : (q == 3 ? 1 : 0), iw = q < 3 ? q : (q == 5 ? 1 : 2);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -854,23 +936,24 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
                 tau[3 * i + j] = q < 3 ? (i == j ? 1.0 / 3.0 : 0.0) - v[i] * v[j] : v[i] * w[j] + w[i] * v[j];
     };
     int st = 0;
+    const int nq = STATES ? a.nstates : 6;
 #pragma unroll 1
-    for (int q = 0; q < 6; q++) {
+    for (int q = 0; q < nq; q++) {
         __asm__ volatile("" ::: "memory");   // reread the shared isotropic values in each pass
         double v[3], w[3], tau[9];
         state(q, v, w, tau);
         const double den = sachs3_vw(tau, v, w, a2i, ui, m6i, m8i, a.sA, a.sB, a.sC);
         const double num = sachs3_vw(tau, v, w, a2, u, m6, m8, a.sA, a.sB, a.sC);
         const double es = num / den;
-        double* out = a.Eij + p * 6 + q;
+        double* out = a.Eij + p * nq + q;
         const double E = a.alpha > 0 ? (1 - a.alpha) * es + a.alpha * *out : es;
         *out = E;
         if (!isfinite(E)) st |= 1;
     }
-    if (a.lami)
+    if (!STATES && a.lami)
 #pragma unroll
         for (int i = 0; i < 3; i++) a.lami[p * 3 + i] = lam[i];
-    if (a.ei)
+    if (!STATES && a.ei)
 #pragma unroll
         for (int i = 0; i < 9; i++) a.ei[p * 9 + i] = e[i];
     if (a.status && st) atomicOr(a.status + p, st);
@@ -911,12 +994,10 @@ __global__ void __launch_bounds__(256) caffe_kernel(int64_t P, int n_rss, double
     DE[p] = E;
 }
 
-hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
-    const int64_t n = args.idx ? args.nidx : args.P;
-    if (n == 0) return hipSuccess;
-    // rheo_params_tranisotropic (src/rheologies.f90:123-132), d = 3: the same
-    // for every parcel, so on the host
-    EijArgs a = args;
+// rheo_params_tranisotropic (src/rheologies.f90:123-132), d = 3, of the Sachs
+// (ef = +1) and Taylor (ef = -1) branches and the isotropic normalisations
+// (src/homogenizations.f90:207, 235): the same for every parcel, so on the host
+static void eij_params(EijArgs& a) {
     auto rheo = [&](int ef, double* cI, double* cM, double* cL) {
         const double ne = (double)(ef * 2) / ((double)a.n_grain + 1);
         const double pc = std::pow(a.Ecc, ne), pa = std::pow(a.Eca, ne);
@@ -928,6 +1009,13 @@ hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
     rheo(-1, &a.tA, &a.tB, &a.tC);
     a.sachs_iso = 1 + 2.0 / 15 * a.sB + 2.0 / 3 * a.sC;            // src/homogenizations.f90:207
     a.taylor_iso = 1.0 / (1 + 2.0 / 15 * a.tB + 2.0 / 3 * a.tC);   // :235
+}
+
+hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
+    const int64_t n = args.idx ? args.nidx : args.P;
+    if (n == 0) return hipSuccess;
+    EijArgs a = args;
+    eij_params(a);
     const int64_t nblk = (n + 127) / 128;
     if (a.idx && a.nidx_dev)   // a device-side count: at most 2048 workgroups
         hipLaunchKernelGGL(eij_list_kernel, dim3((unsigned)std::min<int64_t>(nblk, 2048)), dim3(128), 0, s, a);
@@ -944,16 +1032,36 @@ hipError_t launch_eij(const EijArgs& args, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_eij_sachs3(const EijArgs& args, const Sachs3Args& b, hipStream_t s) {
-    if (args.P == 0) return hipSuccess;
-    EijArgs a = args;
+static void sachs3_params(EijArgs& a) {
     // rheo_params_tranisotropic (src/rheologies.f90:123-132) of the n' = 3 Sachs branch
     const double ne = 2.0 / (3.0 + 1);
     const double pc = std::pow(a.Ecc, ne), pa = std::pow(a.Eca, ne);
     a.sA = (pc - 1) / 2;
     a.sB = (3 * (pc + 1) - 2) / 2 - 2 * pa;
     a.sC = pa - 1;
-    hipLaunchKernelGGL(eij_sachs3_kernel, dim3((unsigned)((a.P + 63) / 64)), dim3(64), 0, s, a, b);
+}
+
+hipError_t launch_eij_sachs3(const EijArgs& args, const Sachs3Args& b, hipStream_t s) {
+    if (args.P == 0) return hipSuccess;
+    EijArgs a = args;
+    sachs3_params(a);
+    hipLaunchKernelGGL(eij_sachs3_kernel<false>, dim3((unsigned)((a.P + 63) / 64)), dim3(64), 0, s, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_evw_sachs3(const EijArgs& args, const Sachs3Args& b, hipStream_t s) {
+    if (args.P == 0 || args.nstates == 0) return hipSuccess;
+    EijArgs a = args;
+    sachs3_params(a);
+    hipLaunchKernelGGL(eij_sachs3_kernel<true>, dim3((unsigned)((a.P + 63) / 64)), dim3(64), 0, s, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_evw(const EijArgs& args, hipStream_t s) {
+    if (args.P == 0 || args.nstates == 0) return hipSuccess;
+    EijArgs a = args;
+    eij_params(a);
+    hipLaunchKernelGGL(evw_kernel, dim3((unsigned)((a.P + 127) / 128)), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

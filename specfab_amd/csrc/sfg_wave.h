@@ -68,8 +68,10 @@ struct OrthoArgs {
     double q6iso[36];     // Eij: the isotropic fabric's Sachs tensor, pair form (set by the launch)
     double dqiso;         //      and its (3,2,1,2) excess
     double lam[6], mu[3], rho;   // Vi: grain Lame parameters, density
-    double* Eij;          // (P, 6)
+    double* Eij;          // (P, 6), or (P, nstates) for Evw
     double* Vi;           // (P, ndir, 3): qS1, qS2, qP
+    const double* states; // Evw: (nstates, 15) = v(3), w(3), tau(9) row-major (tau symmetric), device
+    int nstates;
 };
 // Orthotropic grains given as discrete axes (src/moments.f90:386-417): E
 // ensembles of N grains, each axis array (E, N, 3).  The moment pass sums,
@@ -109,6 +111,7 @@ hipError_t launch_vi_ortho_mom(const OrthoMomArgs& a, hipStream_t s);
 void ortho_q6_iso(const double ci[6], double Q6[36], double* dq);
 hipError_t launch_eij_ortho(const OrthoArgs& a, hipStream_t s);
 hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s);
+hipError_t launch_evw_ortho(const OrthoArgs& a, hipStream_t s);
 
 hipError_t launch_vi_elastic(const ViElasticArgs& a, hipStream_t s);
 hipError_t launch_vi_em(const ViEMArgs& a, hipStream_t s);

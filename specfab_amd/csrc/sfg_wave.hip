@@ -708,14 +708,10 @@ void ortho_q6_iso(const double ci[6], double Q6[36], double* dq) {
     for (int t = 0; t < 36; t++) Q6[t] = Q[t];
 }
 
-// Eij of orthotropic grains (Sachs; src/enhancementfactors.f90:134-188): one
-// lane per parcel.  MODE 0: v given for every parcel tested per parcel (its
-// n00 above 1e-8, src/moments.f90:371-383); MODE 1: v = b x n for all.
-template <bool CART, int MODE>
-__global__ void __launch_bounds__(256) eij_ortho_kernel(OrthoArgs a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
-    double Q[36], dq;
+// Q6 and dq of parcel p: MODE 0: v given when its n00 is above 1e-8
+// (src/moments.f90:371-383), tested per parcel; MODE 1: v = b x n for all
+template <int MODE>
+__device__ __forceinline__ void ortho_q6_parcel(const OrthoArgs& a, int64_t p, double (&Q)[36], double& dq) {
     // all three blocks requested at once (one memory round trip); q3's n00
     // decides the branch
     double x3r[15];
@@ -761,6 +757,16 @@ __global__ void __launch_bounds__(256) eij_ortho_kernel(OrthoArgs a) {
         odist(xr, d1);
         ortho_q6_derived(d0, d1, a.ci, Q, dq);
     }
+}
+
+// Eij of orthotropic grains (Sachs; src/enhancementfactors.f90:134-188): one
+// lane per parcel, MODE as for ortho_q6_parcel
+template <bool CART, int MODE>
+__global__ void __launch_bounds__(256) eij_ortho_kernel(OrthoArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double Q[36], dq;
+    ortho_q6_parcel<MODE>(a, p, Q, dq);
     double* o = a.Eij + p * 6;
     if (CART) {
 #pragma unroll
@@ -770,6 +776,26 @@ __global__ void __launch_bounds__(256) eij_ortho_kernel(OrthoArgs a) {
 #pragma unroll
         for (int i = 0; i < 9; i++) e[i] = a.frames[p * 9 + i];
         ortho_eij_frame(Q, dq, a.q6iso, a.dqiso, e, o);
+    }
+}
+
+// Evw of orthotropic grains for the given stress states (v, w, tau), shared
+// by all parcels (src/enhancementfactors.f90:134-160, the reference's
+// Evw_orthotropic): one lane per parcel
+template <int MODE>
+__global__ void __launch_bounds__(256) evw_ortho_kernel(OrthoArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double Q[36], dq;
+    ortho_q6_parcel<MODE>(a, p, Q, dq);
+    double* o = a.Eij + p * a.nstates;
+#pragma unroll 1
+    for (int q = 0; q < a.nstates; q++) {
+        const double* st = a.states + 15 * q;
+        double tau[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) tau[i] = st[6 + i];
+        o[q] = q6_state(Q, dq, tau, st, st + 3) / q6_state(a.q6iso, a.dqiso, tau, st, st + 3);
     }
 }
 
@@ -1060,6 +1086,8 @@ template void eij_ortho_kernel<true, 0>(OrthoArgs);
 template void eij_ortho_kernel<true, 1>(OrthoArgs);
 template void vi_ortho_kernel<0>(OrthoArgs);
 template void vi_ortho_kernel<1>(OrthoArgs);
+template void evw_ortho_kernel<0>(OrthoArgs);
+template void evw_ortho_kernel<1>(OrthoArgs);
 #endif
 
 #ifndef SFG_HOST_EMUL
@@ -1093,6 +1121,16 @@ hipError_t launch_eij_ortho(const OrthoArgs& args, hipStream_t s) {
         if (a.x3) hipLaunchKernelGGL((eij_ortho_kernel<true, 0>), g, b, 0, s, a);
         else hipLaunchKernelGGL((eij_ortho_kernel<true, 1>), g, b, 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_evw_ortho(const OrthoArgs& args, hipStream_t s) {
+    if (args.P == 0 || args.nstates == 0) return hipSuccess;
+    OrthoArgs a = args;
+    ortho_q6_iso(a.ci, a.q6iso, &a.dqiso);
+    const dim3 g(nblk(a.P, 256)), b(256);
+    if (a.x3) hipLaunchKernelGGL((evw_ortho_kernel<0>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((evw_ortho_kernel<1>), g, b, 0, s, a);
     return hipGetLastError();
 }
 

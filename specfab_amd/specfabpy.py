@@ -272,6 +272,17 @@ def eig(nlm):
     return ei.reshape(3, 3), lam
 
 
+def Evw_tranisotropic(nlm, v, w, tau, Eij_grain, alpha, n_grain):
+    """Enhancement factor of the stress state tau along v (x) w:
+    src/specfabpy.f90:379-388 -> src/enhancementfactors.f90:47-69."""
+    L = _L_of(nlm)
+    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    out = np.zeros(1)
+    check(lib().sfg_Evw_tranisotropic(_ctx(), L, _zp(_z(nlm)), dptr(f(v)), dptr(f(w)), dptr(_m3(tau)),
+                                      dptr(f(Eij_grain).reshape(2)), float(alpha), int(n_grain), dptr(out)))
+    return float(out[0])
+
+
 def eig3(M):
     """eig3(M) -> (e1, e2, e3, lami) of a symmetric 3x3 M, largest eigenvalue
     first: src/specfabpy.f90:346-355 -> src/frames.f90:62-80 (on the GPU)."""
@@ -497,6 +508,17 @@ def Eij_orthotropic(nlm_r1, nlm_r2, nlm_r3, e1, e2, e3, Eij_grain, alpha, n_grai
     check(lib().sfg_Eij_orthotropic(_ctx(), L, _zp(z1), _zp(z2), _zp(z3), dptr(v(e1)), dptr(v(e2)), dptr(v(e3)),
                                     dptr(v(Eij_grain).reshape(6)), float(alpha), int(n_grain), dptr(out)))
     return out
+
+
+def Evw_orthotropic(nlm_r1, nlm_r2, nlm_r3, v, w, tau, Eij_grain, alpha, n_grain):
+    """Orthotropic enhancement factor of the stress state tau along v (x) w:
+    src/specfabpy.f90:425-434 -> src/enhancementfactors.f90:134-160."""
+    L, z1, z2, z3 = _ortho_states(nlm_r1, nlm_r2, nlm_r3)
+    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    out = np.zeros(1)
+    check(lib().sfg_Evw_orthotropic(_ctx(), L, _zp(z1), _zp(z2), _zp(z3), dptr(f(v)), dptr(f(w)), dptr(_m3(tau)),
+                                    dptr(f(Eij_grain).reshape(6)), float(alpha), int(n_grain), dptr(out)))
+    return float(out[0])
 
 
 def Eij_orthotropic_arr(nlm_r1, nlm_r2, nlm_r3, e1, e2, e3, Eij_grain, alpha, n_grain):

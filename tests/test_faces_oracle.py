@@ -74,3 +74,34 @@ def test_fixture_regenerates_bit_for_bit():
     assert set(new) == set(old)
     for k in new:
         assert np.array_equal(np.asarray(new[k]), old[k]), k
+
+
+def test_evw_pins(orc):
+    """Evw of the fixture: orthotropic grains against the restatement
+    (oracle/orthotropic.py's Sachs tensor contracted with the state: 2e-6,
+    the REAL(4) joint bodies), n' = 3 Sachs (alpha = 0) against
+    oracle/nonlinear.py (1e-5: the REAL(4) a6 / a8 bodies' 1e-7, amplified
+    where a random state's contraction nearly cancels; measured 1.8e-6)"""
+    from oracle import nonlinear as N
+    from oracle import orthotropic as O
+    g = load_golden("faces_L8.npz")
+    V, Wv, T = g["evw_v"], g["evw_w"], g["evw_tau"]
+    iso = np.zeros(15, complex)
+    iso[0] = 1 / np.sqrt(4 * np.pi)
+    d42 = lambda A, B: np.einsum("lkij,ji->lk", A, B)
+    d22 = lambda A, B: np.einsum("ij,ji->", A, B)
+    Qi = O.sachs_Q(*O.ai(iso, iso, iso, orc)[1:], g["evwo_Eg"])
+    for tag, use3 in (("given", True), ("derived", False)):
+        for k, q in enumerate(g["evwo_q"]):
+            Q = O.sachs_Q(*O.ai(q[0], q[1], q[2] if use3 else None, orc)[1:], g["evwo_Eg"])
+            for s in range(len(V)):
+                vw = np.outer(V[s], Wv[s])
+                ref = d22(d42(Q, T[s]), vw) / d22(d42(Qi, T[s]), vw)
+                assert abs(g["evwo_" + tag][k, s] - ref) < 2e-6 * max(1, abs(ref)), (tag, k, s)
+    Si = N._structure(N.isotropic(8), orc)
+    for k, x in enumerate(g["evw_nlm"]):
+        S = N._structure(x, orc)
+        for s in range(len(V)):
+            vw = np.outer(V[s], Wv[s])
+            ref = N._d22(N.sachs3(T[s], *S, (1, 1e3)), vw) / N._d22(N.sachs3(T[s], *Si, (1, 1e3)), vw)
+            assert abs(g["evw_3_0_1"][k, s] / ref - 1) < 1e-5, (k, s)

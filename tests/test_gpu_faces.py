@@ -10,7 +10,11 @@ oracle/_ref):
     (projector) where it is double; single and batched (eigframe_arr);
   * Qnorm_tranisotropic (Reuss, Hill, Voigt): 1e-10 (the Reuss branch is
     the 6x6 Mandel solve in place of the reference's 9x9 inverse);
-  * evD is ev_D2 (nonlinear_L8.npz's evD2 column, 1e-13).
+  * evD is ev_D2 (nonlinear_L8.npz's evD2 column, 1e-13);
+  * Evw_tranisotropic / Evw_orthotropic of general stress states (v, w,
+    tau): n' = 1, -3 1e-10, n' = 3 1e-5 (the reference's REAL(4) a6 / a8
+    bodies in nearly cancelling contractions; 1e-10 against the
+    restatement), orthotropic 2e-6; a non-symmetric tau is rejected.
 """
 import numpy as np
 import pytest
@@ -18,6 +22,12 @@ import pytest
 from conftest import load_golden, relerr
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def orc():
+    from oracle.oracle import Oracle
+    return Oracle()
 
 
 def test_nlm_LROT(gpu_ctx):
@@ -73,3 +83,53 @@ def test_qnorm_and_evD(gpu_ctx):
     for k, x in enumerate(n["nlm"][:4]):
         for j, t in enumerate(n["tau"]):
             assert abs(sf.evD(x, t) - n["evD2"][k, j]) <= 1e-13 * abs(n["evD2"][k, j]) + 1e-15
+
+
+def test_evw(gpu_ctx, orc):
+    """Evw_tranisotropic (n' = 1, -3 with the Sachs/Taylor mix: 1e-10; n' = 3:
+    1e-5 against the reference, whose REAL(4) a6 / a8 bodies a nearly
+    cancelling random state amplifies, 1e-10 against the restatement) and
+    Evw_orthotropic (2e-6; 1e-11 against the restatement), single and
+    batched over parcels and states"""
+    import torch
+    from oracle import nonlinear as N
+    from oracle import orthotropic as O
+    from specfab_amd import specfabpy as sf
+    from specfab_amd.batch import FabricBatch, evw_orthotropic
+    g = load_golden("faces_L8.npz")
+    V, Wv, T, X = g["evw_v"], g["evw_w"], g["evw_tau"], g["evw_nlm"]
+    sf.init(8)
+    cases = ((1, 0.0125, (1, 1e3)), (-3, 0.3, (1, 1e3)), (3, 0.0, (1, 1e3)), (3, 0.0125, (1, 1e3)),
+             (1, 1.0, (2.0, 0.5)))
+    b = FabricBatch(8, nlm=X)
+    for ng, alpha, Eg in cases:
+        ref = g["evw_%d_%g_%g" % (ng, alpha, Eg[0])]
+        bar = 1e-5 if ng == 3 else 1e-10
+        for k, x in enumerate(X[:3]):
+            for s in range(len(V)):
+                got = sf.Evw_tranisotropic(x, V[s], Wv[s], T[s], Eg, alpha, ng)
+                assert abs(got - ref[k, s]) <= bar * max(1.0, abs(ref[k, s])), (ng, alpha, k, s)
+        E = b.Evw(V, Wv, T, Eij_grain=Eg, alpha=alpha, n_grain=ng)
+        torch.cuda.synchronize()
+        E = E.cpu().numpy()
+        assert np.all(np.abs(E - ref) <= bar * np.maximum(1.0, np.abs(ref))), (ng, alpha)
+        if ng == 3 and alpha == 0:
+            Si = N._structure(N.isotropic(8), orc)
+            for k, x in enumerate(X[:2]):
+                S = N._structure(x, orc)
+                for s in range(len(V)):
+                    vw = np.outer(V[s], Wv[s])
+                    ora = N._d22(N.sachs3(T[s], *S, Eg), vw) / N._d22(N.sachs3(T[s], *Si, Eg), vw)
+                    assert abs(E[k, s] / ora - 1) < 1e-10, (k, s)
+    Q, Ego = g["evwo_q"], g["evwo_Eg"]
+    for tag, use3 in (("given", True), ("derived", False)):
+        for k, q in enumerate(Q):
+            for s in range(len(V)):
+                got = sf.Evw_orthotropic(q[0], q[1], q[2] if use3 else None, V[s], Wv[s], T[s], Ego, 0.0, 1)
+                assert abs(got - g["evwo_" + tag][k, s]) < 2e-6 * max(1, abs(g["evwo_" + tag][k, s])), (tag, k, s)
+        bs = [FabricBatch(8, nlm=Q[:, i]) for i in range(3)]
+        E = evw_orthotropic(bs[0], bs[1], bs[2] if use3 else None, V, Wv, T, Ego).cpu().numpy()
+        assert np.all(np.abs(E - g["evwo_" + tag]) < 2e-6 * np.maximum(1, np.abs(g["evwo_" + tag]))), tag
+    from specfab_amd._lib import SFGError
+    with pytest.raises(SFGError):
+        sf.Evw_tranisotropic(X[0], V[0], Wv[0], T[0] + np.triu(np.ones((3, 3)), 1), (1, 1e3), 0.0, 1)
