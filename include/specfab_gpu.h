@@ -238,6 +238,16 @@ int sfg_Evw_tranisotropic_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_
                                 const double* states_dev, const double Eij_grain[2], double alpha, int n_grain,
                                 double* Evw_dev, int32_t* status_dev, void* stream);
 
+/* a6(nlm), and a8 of ai(nlm) (src/moments.f90:57-66, 220-236; specfabpy
+ * :601-618): the sixth and eighth moments <x^a y^b z^c> of the ODF as its
+ * 28 / 45 distinct entries, monomials in the order a = d..0, b = d - a..0
+ * (c = d - a - b); exact (the reference's bodies carry REAL(4) literal
+ * coefficients, ~3e-8 apart).  L >= 6; rows above l = 8 do not enter.
+ * Batched: m6_dev (P, 28), m8_dev (P, 45). */
+int sfg_moments68(sfg_ctx* ctx, int L, const double* nlm, double m6[28], double m8[45]);
+int sfg_moments68_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* m6_dev, double* m8_dev,
+                        void* stream);
+
 /* single-parcel eig(nlm) -> ei (3x3 rows), lami (3): src/frames.f90:14-22 */
 int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]);
 /* eig3(M, e1, e2, e3, lami) (src/frames.f90:62-80, specfabpy :346-355): the

@@ -1265,6 +1265,20 @@ int sfg_eig(sfg_ctx* ctx, int L, const double* nlm, double ei[9], double lami[3]
     return eij_single(ctx, L, nlm, 0, nullptr, nullptr, 0.0, 1, nullptr, ei, lami);
 }
 
+int sfg_moments68_batch(sfg_ctx* ctx, int L, int64_t P, const double* x_dev, double* m6_dev, double* m8_dev,
+                        void* stream) {
+    if (!ctx || P < 0 || (P > 0 && (!x_dev || !m6_dev || !m8_dev))) return fail(SFG_ERR_ARG, "bad argument");
+    if (L < 6) return fail(SFG_ERR_L, "sixth moments need the l <= 6 coefficients (L >= 6)");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (P == 0) return SFG_OK;
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(sfg::launch_moments68(P, std::min(op->n, 45), x_dev, m6_dev, m8_dev, pick(ctx, stream)));
+    return SFG_OK;
+}
+
 int sfg_eig3(sfg_ctx* ctx, const double M[9], double ei[9], double lami[3]) {
     if (!ctx || !M || !ei || !lami) return fail(SFG_ERR_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1561,6 +1575,25 @@ int sfg_Qnorm_tranisotropic(sfg_ctx* ctx, int L, const double* nlm, double alpha
                            &u);
     if (rc) return rc;
     for (int i = 0; i < 9; i++) Qnorm[i] = out[3 + i];
+    return SFG_OK;
+}
+
+int sfg_moments68(sfg_ctx* ctx, int L, const double* nlm, double m6[28], double m8[45]) {
+    if (!m6 || !m8) return fail(SFG_ERR_ARG, "NULL argument");
+    if (L < 6) return fail(SFG_ERR_L, "sixth moments need the l <= 6 coefficients (L >= 6)");
+    double out[73];
+    int rc = single_parcel(ctx, L, nlm, 73, out,
+                           [](sfg_ctx* c, int L_, const double* dx, double* dout, void*) {
+                               const DevOp* op;
+                               int r = get_op(c, L_, &op);
+                               if (r) return r;
+                               HIPCHK(sfg::launch_moments68(1, std::min(op->n, 45), dx, dout, dout + 28, c->stream));
+                               return SFG_OK;
+                           },
+                           nullptr);
+    if (rc) return rc;
+    for (int i = 0; i < 28; i++) m6[i] = out[i];
+    for (int i = 0; i < 45; i++) m8[i] = out[28 + i];
     return SFG_OK;
 }
 

@@ -64,6 +64,10 @@ hipError_t launch_evw_sachs3(const EijArgs& a, const Sachs3Args& b, hipStream_t 
 hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x, const double* tau, double* out,
                         hipStream_t s);
 // E_CAFFE (src/enhancementfactors.f90:302-332) of <D> (P), in place
+// exact sixth / eighth moments (P, 28) / (P, 45), monomials <x^a y^b z^c> in
+// the order a = d..0, b = d - a..0, of states with nrows rows (rows above
+// l = 8 do not enter)
+hipError_t launch_moments68(int64_t P, int nrows, const double* x, double* m6, double* m8, hipStream_t s);
 // eigenpairs of P symmetric 3x3 matrices (P, 9): ei (P, 9) rows, lami (P, 3), largest first
 hipError_t launch_eig3(int64_t P, const double* M, double* ei, double* lami, hipStream_t s);
 hipError_t launch_caffe(int64_t P, int n_rss, double Emin, double Emax, double* DE, hipStream_t s);

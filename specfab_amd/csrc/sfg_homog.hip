@@ -644,6 +644,7 @@ __global__ void __launch_bounds__(128, 2) evw_kernel(EijArgs a) {
 __constant__ double kMOM8[45][45] = SFG_MOM8_INIT;
 __constant__ double kMOM6_ISO[28] = SFG_MOM6_ISO_INIT;
 __constant__ double kMOM8_ISO[45] = SFG_MOM8_ISO_INIT;
+__constant__ double kMOM6[28][28] = SFG_MOM6_INIT;
 
 namespace {
 
@@ -959,6 +960,32 @@ __global__ void __launch_bounds__(64) eij_sachs3_kernel(EijArgs a, Sachs3Args b)
     if (a.status && st) atomicOr(a.status + p, st);
 }
 
+// the exact sixth and eighth moments <x^a y^b z^c> (monomials in the order
+// a = d..0, b = d - a..0) of P states: m6 (P, 28), m8 (P, 45)
+__global__ void __launch_bounds__(128) moments68_kernel(int64_t P, int nrows, const double* x, double* m6,
+                                                        double* m8) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double xq[45];
+#pragma unroll
+    for (int q = 0; q < 45; q++) xq[q] = q < nrows ? x[(int64_t)q * P + p] : 0.0;
+    const double ix0 = 1.0 / xq[0];
+#pragma unroll 1
+    for (int k = 0; k < 28; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 28; q++) s += kMOM6[k][q] * xq[q];
+        m6[p * 28 + k] = s * ix0;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 45; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 45; q++) s += kMOM8[k][q] * xq[q];
+        m8[p * 45 + k] = s * ix0;
+    }
+}
+
 // <D> with n_RSS = 4 per parcel (tau (P, 3, 3)), optionally mapped to the
 // CAFFE enhancement factor (caffe_kernel)
 __global__ void __launch_bounds__(64) ev_D4_kernel(int64_t P, int nrows, const double* A4F, const double* x,
@@ -1069,6 +1096,12 @@ hipError_t launch_ev_D4(int64_t P, int nrows, const double* A4F, const double* x
                         hipStream_t s) {
     if (P == 0) return hipSuccess;
     hipLaunchKernelGGL(ev_D4_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, s, P, nrows, A4F, x, tau, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments68(int64_t P, int nrows, const double* x, double* m6, double* m8, hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(moments68_kernel, dim3((unsigned)((P + 127) / 128)), dim3(128), 0, s, P, nrows, x, m6, m8);
     return hipGetLastError();
 }
 

@@ -283,6 +283,44 @@ def Evw_tranisotropic(nlm, v, w, tau, Eij_grain, alpha, n_grain):
     return float(out[0])
 
 
+def _mono_tensor(m, d):
+    """the full symmetric (3,)*d tensor of the distinct moments m (monomials
+    <x^a y^b z^c> in the order a = d..0, b = d - a..0)"""
+    pos = {}
+    k = 0
+    for a in range(d, -1, -1):
+        for b in range(d - a, -1, -1):
+            pos[(a, b)] = k
+            k += 1
+    out = np.empty((3,) * d)
+    for idx in np.ndindex(*(3,) * d):
+        out[idx] = m[pos[(idx.count(0), idx.count(1))]]
+    return out
+
+
+def _moments68(nlm):
+    L = _L_of(nlm)
+    m6, m8 = np.zeros(28), np.zeros(45)
+    check(lib().sfg_moments68(_ctx(), L, _zp(_z(nlm)), dptr(m6), dptr(m8)))
+    return m6, m8
+
+
+def a6(nlm):
+    """Sixth-order structure tensor <c^6> (3,)*6: src/specfabpy.f90:601-608 ->
+    src/moments.f90:57-66 (exact moments; the reference's body carries REAL(4)
+    literals, ~3e-8 apart).  L >= 6."""
+    return _mono_tensor(_moments68(nlm)[0], 6)
+
+
+def ai(nlm):
+    """(a2, a4, a6, a8) of nlm: src/specfabpy.f90:610-618 (a2 and a4 the
+    reference's bodies, a6 and a8 exact).  L >= 8."""
+    if _L_of(nlm) < 8:
+        raise ValueError("ai: the eighth moments need L >= 8")
+    m6, m8 = _moments68(nlm)
+    return a2(nlm), a4(nlm), _mono_tensor(m6, 6), _mono_tensor(m8, 8)
+
+
 def eig3(M):
     """eig3(M) -> (e1, e2, e3, lami) of a symmetric 3x3 M, largest eigenvalue
     first: src/specfabpy.f90:346-355 -> src/frames.f90:62-80 (on the GPU)."""

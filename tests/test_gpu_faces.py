@@ -133,3 +133,22 @@ def test_evw(gpu_ctx, orc):
     from specfab_amd._lib import SFGError
     with pytest.raises(SFGError):
         sf.Evw_tranisotropic(X[0], V[0], Wv[0], T[0] + np.triu(np.ones((3, 3)), 1), (1, 1e3), 0.0, 1)
+
+
+def test_a6_ai(gpu_ctx):
+    """a6 and ai's a6 / a8: the exact moments (oracle/nonlinear.py's
+    quadrature, 1e-14) and the reference's REAL(4) bodies
+    (nonlinear_L8.npz's a6 / a8, <= 3e-8); ai's a2 / a4 are the a2 / a4 faces"""
+    from oracle import nonlinear as N
+    from specfab_amd import specfabpy as sf
+    g = load_golden("nonlinear_L8.npz")
+    sf.init(8)
+    for x, r6, r8 in zip(g["nlm"][[1, 10]], g["a6"], g["a8"]):
+        A2, A4, A6, A8 = sf.ai(x)
+        m = N.moments(x, (6, 8))
+        assert np.abs(A6 - m[6]).max() < 1e-14 and np.abs(A8 - m[8]).max() < 1e-14
+        assert np.abs(A6 - r6).max() < 3e-8 and np.abs(A8 - r8).max() < 3e-8
+        assert np.array_equal(A6, sf.a6(x))
+        assert np.array_equal(A2, sf.a2(x)) and np.array_equal(A4, sf.a4(x))
+    with pytest.raises(ValueError):
+        sf.ai(g["nlm"][0][:28])
