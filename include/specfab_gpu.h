@@ -143,6 +143,16 @@ int sfg_rnlm_to_nlm(sfg_ctx* ctx, int L, const double* rnlm, double* nlm);
  * nlm_dev (P, n) complex <-> rnlm_dev (P, rn) complex */
 int sfg_reduce_M_batch(sfg_ctx* ctx, int L, int64_t P, const double* M_dev, double* Mrr_dev, double* Mri_dev,
                        double* Mir_dev, double* Mii_dev, void* stream);
+/* reduce_M(M_LROT(eps, omg, iota, zeta)) and reduce_M(M_DDRX_src(tau)) of
+ * every node in one kernel, the full (n, n) matrices never formed: the same
+ * values as sfg_M_LROT_batch / sfg_M_DDRX_src_batch followed by
+ * sfg_reduce_M_batch (the Eulerian assembly of src/specfabpy/fenics/CPO.py:
+ * 200-211); Mrr..Mii_dev (P, rn, rn). */
+int sfg_M_LROT_reduced_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, const double* omg_dev,
+                             double iota, double zeta, double* Mrr_dev, double* Mri_dev, double* Mir_dev,
+                             double* Mii_dev, void* stream);
+int sfg_M_DDRX_src_reduced_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* Mrr_dev,
+                                 double* Mri_dev, double* Mir_dev, double* Mii_dev, void* stream);
 int sfg_nlm_to_rnlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* nlm_dev, double* rnlm_dev, void* stream);
 int sfg_rnlm_to_nlm_batch(sfg_ctx* ctx, int L, int64_t P, const double* rnlm_dev, double* nlm_dev, void* stream);
 

@@ -119,6 +119,8 @@ SIGNATURES = {
     "sfg_Evw_orthotropic_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _i, _vp, _dp, _d, _i, _vp, _vp]),
     "sfg_moments68": (_i, [_vp, _i, _dp, _dp, _dp]),
     "sfg_moments68_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp]),
+    "sfg_M_LROT_reduced_batch": (_i, [_vp, _i, _i64, _vp, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+    "sfg_M_DDRX_src_reduced_batch": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sfg_eig3": (_i, [_vp, _dp, _dp, _dp]),
     "sfg_eig3_batch": (_i, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "sfg_Qnorm_tranisotropic": (_i, [_vp, _i, _dp, _d, _dp, _dp]),

@@ -100,6 +100,30 @@ def reduce_M_batch(L, M, ctx=None):
     return out[0], out[1], out[2], out[3]
 
 
+def M_LROT_reduced_batch(L, eps, omg, iota, zeta, ctx=None):
+    """reduce_M(M_LROT) of P nodes, (Mrr, Mri, Mir, Mii) each (P, rn, rn),
+    built without the full (n, n) operators (the same values as
+    reduce_M_batch(L, M_LROT_batch(...)))."""
+    ctx = ctx or _lib.default_context()
+    E, W = eps.to(torch.float64).contiguous(), omg.to(torch.float64).contiguous()
+    P, rn = E.shape[0], rnlm_len(L)
+    out = torch.empty((4, P, rn, rn), dtype=torch.float64, device=E.device)
+    check(lib().sfg_M_LROT_reduced_batch(ctx.handle, L, P, _ptr(E), _ptr(W), float(iota), float(zeta), _ptr(out[0]),
+                                         _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _stream()))
+    return out[0], out[1], out[2], out[3]
+
+
+def M_DDRX_src_reduced_batch(L, tau, ctx=None):
+    """reduce_M(M_DDRX_src) of P nodes without the full operators."""
+    ctx = ctx or _lib.default_context()
+    T = tau.to(torch.float64).contiguous()
+    P, rn = T.shape[0], rnlm_len(L)
+    out = torch.empty((4, P, rn, rn), dtype=torch.float64, device=T.device)
+    check(lib().sfg_M_DDRX_src_reduced_batch(ctx.handle, L, P, _ptr(T), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                             _ptr(out[3]), _stream()))
+    return out[0], out[1], out[2], out[3]
+
+
 def nlm_to_rnlm_batch(L, nlm, ctx=None):
     """(P, n) complex -> (P, rn) complex, src/reducedform.f90:176-183 per node."""
     ctx = ctx or _lib.default_context()

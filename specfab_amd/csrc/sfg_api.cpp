@@ -98,6 +98,8 @@ struct sfg_ctx {
     int4* csr_ent = nullptr;
     double* csr_val = nullptr;
     int2* csr_seg = nullptr;
+    // reduced-block coefficient maps per (L, mode): Cp then Cm, each (K, rn^2)
+    std::map<std::pair<int, int>, double*> rmap;
     // workspace, shared by all calls on this context: every user waits for
     // the previous user's stream through ws_event before touching it
     void* ws = nullptr;
@@ -403,6 +405,7 @@ int sfg_ctx_destroy(sfg_ctx* ctx) {
     (void)hipFree(ctx->csr_ent);
     (void)hipFree(ctx->csr_val);
     (void)hipFree(ctx->csr_seg);
+    for (auto& kv : ctx->rmap) (void)hipFree(kv.second);
     if (ctx->ws) (void)hipFree(ctx->ws);
     for (auto& ds : ctx->dirs) (void)hipFree(ds.R);
     if (ctx->wigner) (void)hipFree(ctx->wigner);
@@ -777,6 +780,106 @@ int sfg_M_LROT_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, cons
     if (!ctx || P < 0 || (P > 0 && (!eps_dev || !omg_dev || !M_dev))) return fail(SFG_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     return dense_batch(ctx, L, P, 0, eps_dev, omg_dev, iota, zeta, M_dev, pick(ctx, stream));
+}
+
+// the coefficient maps Cp, Cm (K, rn^2) of the reduced blocks of mode's
+// operator at truncation L (see launch_reduced_map), from the coupling-table
+// entries: A(i, c)[comp] = sum of the values of the entries of element (i, c)
+// with that weight component
+static int get_rmap(sfg_ctx* ctx, int L, int mode, const double** out) {
+    auto key = std::make_pair(L, mode);
+    auto it = ctx->rmap.find(key);
+    if (it != ctx->rmap.end()) {
+        *out = it->second;
+        return SFG_OK;
+    }
+    const int K = mode == 0 ? 24 : 15, kmax = mode == 0 ? 6 : 15;
+    const double sg = mode == 0 ? -1.0 : 1.0;
+    const int n = sfg::nlm_len(L), rn = sfg::rnlm_len(L), rn2 = rn * rn;
+    // A for the rows at m >= 0 positions: (n, n, K) restricted to needed rows
+    std::vector<double> A((size_t)n * n * K, 0.0);
+    for (auto& e : sfg::gaunt_entries()) {
+        if (e.i >= n || e.j >= n || e.k >= kmax || (mode == 1 && e.t != 0)) continue;
+        const int comp = mode == 0 ? e.t * 6 + e.k : e.k;
+        A[((size_t)e.i * n + e.j) * K + comp] += e.v;
+    }
+    auto lm = [](int k, int* l, int* m) {
+        int h = (int)std::sqrt((double)k);
+        while ((h + 1) * (h + 1) <= k) ++h;
+        while (h * h > k) --h;
+        *l = 2 * h;
+        *m = k - h * h;
+    };
+    std::vector<double> C((size_t)2 * K * rn2, 0.0);
+    for (int i = 0; i < rn; i++) {
+        int li, mi;
+        lm(i, &li, &mi);
+        const int fi = li * (li - 1) / 2 + li + mi;
+        for (int j = 0; j < rn; j++) {
+            int lj, mj;
+            lm(j, &lj, &mj);
+            const int cp = lj * (lj - 1) / 2 + lj + mj, cn = cp - 2 * mj;
+            const double sgn = mj == 0 ? 0.0 : ((mj & 1) ? -1.0 : 1.0);
+            for (int c = 0; c < K; c++) {
+                const double ap = A[((size_t)fi * n + cp) * K + c], an = A[((size_t)fi * n + cn) * K + c];
+                C[(size_t)c * rn2 + i * rn + j] = sg * (ap + sgn * an);
+                C[(size_t)(K + c) * rn2 + i * rn + j] = sg * (ap - sgn * an);
+            }
+        }
+    }
+    double* d = nullptr;
+    HIPCHK(upload(&d, C));
+    ctx->rmap[key] = d;
+    *out = d;
+    return SFG_OK;
+}
+
+// the reduced blocks of M_LROT / M_DDRX_src of every node without the full
+// matrices: the nodes' coupling weights (dense_weights_kernel), then the
+// blocks as a linear map of them (reduced_map_kernel)
+static int dense_reduced_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const double* A_dev, const double* B_dev,
+                               double iota, double zeta, double* Mrr, double* Mri, double* Mir, double* Mii,
+                               hipStream_t s) {
+    if (P < 0 || (P > 0 && (!A_dev || (mode == 0 && !B_dev) || !Mrr || !Mri || !Mir || !Mii)))
+        return fail(SFG_ERR_ARG, "bad argument");
+    if (P == 0) return SFG_OK;
+    const DevOp* op;
+    int rc = get_op(ctx, L, &op);
+    if (rc) return rc;
+    rc = ensure_csr(ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* C;
+    rc = get_rmap(ctx, L, mode, &C);
+    if (rc) return rc;
+    sfg::DenseArgs a{};
+    a.P = P; a.n = op->n; a.mode = mode; a.iota = iota; a.zeta = zeta;
+    a.A = A_dev; a.B = B_dev;
+    void* ws;
+    WsLease lease(ctx, s);
+    rc = lease.acquire((size_t)P * sfg::RMAP_W * sizeof(double), &ws);
+    if (rc) return rc;
+    HIPCHK(sfg::launch_dense_weights(a, (double*)ws, s));
+    const int rn2 = sfg::rnlm_len(L) * sfg::rnlm_len(L), K = mode == 0 ? 24 : 15;
+    HIPCHK(sfg::launch_reduced_map(P, rn2, K, C, C + (size_t)K * rn2, (const double*)ws, Mrr, Mri, Mir, Mii, s));
+    return lease.release();
+}
+
+int sfg_M_LROT_reduced_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, const double* omg_dev,
+                             double iota, double zeta, double* Mrr_dev, double* Mri_dev, double* Mir_dev,
+                             double* Mii_dev, void* stream) {
+    if (!ctx) return fail(SFG_ERR_ARG, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return dense_reduced_batch(ctx, L, P, 0, eps_dev, omg_dev, iota, zeta, Mrr_dev, Mri_dev, Mir_dev, Mii_dev,
+                               pick(ctx, stream));
+}
+
+int sfg_M_DDRX_src_reduced_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* Mrr_dev,
+                                 double* Mri_dev, double* Mir_dev, double* Mii_dev, void* stream) {
+    if (!ctx) return fail(SFG_ERR_ARG, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return dense_reduced_batch(ctx, L, P, 1, tau_dev, nullptr, 0, 0, Mrr_dev, Mri_dev, Mir_dev, Mii_dev,
+                               pick(ctx, stream));
 }
 
 int sfg_M_DDRX_src_batch(sfg_ctx* ctx, int L, int64_t P, const double* tau_dev, double* M_dev, void* stream) {

@@ -114,6 +114,15 @@ struct DenseArgs {
 };
 
 hipError_t launch_params(const ParamArgs& a, hipStream_t s);
+// The reduced blocks as a linear map of each node's coupling weights: with
+// Z = sg sum_e val_e w[t_e][k_e] (sg = -1 LROT, +1 DDRX) every block entry
+// is sum_c C[c] Re w_c or Im w_c, C = sg (A+ +- s A-) over the K weight
+// components c (LROT: t * 6 + k, K = 24; DDRX: k, K = 15).  Cp, Cm (K,
+// rn^2); W (P, 48) the weights' real parts then imaginary parts (stride 24).
+constexpr int RMAP_W = 48;
+hipError_t launch_dense_weights(const DenseArgs& a, double* W, hipStream_t s);
+hipError_t launch_reduced_map(int64_t P, int rn2, int K, const double* Cp, const double* Cm, const double* W,
+                              double* Mrr, double* Mri, double* Mir, double* Mii, hipStream_t s);
 hipError_t launch_evolve_generic(const EvolveArgs& a, hipStream_t s);
 hipError_t launch_nlm_to_state(int64_t P, int n, const int* r2c, const int* rpart, const double* nlm, double* x,
                                hipStream_t s);

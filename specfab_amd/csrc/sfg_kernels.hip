@@ -312,10 +312,9 @@ __global__ void diag_op_kernel(int64_t P, int n, int mode, const int* row_l, con
 // c owning columns c, c + 64, ...: it sums the entries of (row, column) in
 // table order and stores the element once, so every row leaves as contiguous
 // 1 KiB runs and nothing is read back.
-__global__ __launch_bounds__(256) void dense_op_kernel(DenseArgs a) {
-    __shared__ cplx sw[4][15];
-    const int64_t p = blockIdx.x;
-    if (threadIdx.x == 0) {
+// the node's coupling weights into sw (one thread)
+__device__ inline void dense_weights(const DenseArgs& a, int64_t p, cplx (*sw)[15]) {
+    {
         cplx w[4][15];
         if (a.mode == 0) {
             double D[9], W[9];
@@ -331,33 +330,125 @@ __global__ __launch_bounds__(256) void dense_op_kernel(DenseArgs a) {
             for (int k = 0; k < 15; k++) sw[0][k] = w[0][k];
         }
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
+}
+
+// element (i, c) of the node's operator: its table entries in table order
+__device__ inline void dense_element(const DenseArgs& a, const cplx (*sw)[15], int i, int c, double& re, double& im) {
     const int kmax = a.mode == 0 ? 6 : 15;
     const double sg = a.mode == 0 ? -1.0 : 1.0;
+    const int2 r = a.seg[(size_t)i * a.segw + c];
+    re = 0.0;
+    im = 0.0;
+    for (int e = r.x; e < r.y; e++) {
+        int4 ent = a.ent[e];   // (t, j = c, k, -)
+        int t = ent.x, k = ent.z;
+        if (k >= kmax) continue;
+        if (a.mode == 1 && t != 0) continue;
+        double v = a.val[e];
+        re += sg * v * sw[t][k].re;
+        im += sg * v * sw[t][k].im;
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_op_kernel(DenseArgs a) {
+    __shared__ cplx sw[4][15];
+    const int64_t p = blockIdx.x;
+    if (threadIdx.x == 0) dense_weights(a, p, sw);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
     for (int i = threadIdx.x >> 6; i < a.n; i += 4) {
         double* row = a.M + (p * a.n + i) * a.n * 2;
-        const int2* seg = a.seg + (size_t)i * a.segw;
         for (int c = lane; c < a.n; c += 64) {
-            int2 r = seg[c];
-            double re = 0.0, im = 0.0;
-            for (int e = r.x; e < r.y; e++) {
-                int4 ent = a.ent[e];   // (t, j = c, k, -)
-                int t = ent.x, k = ent.z;
-                if (k >= kmax) continue;
-                if (a.mode == 1 && t != 0) continue;
-                double v = a.val[e];
-                re += sg * v * sw[t][k].re;
-                im += sg * v * sw[t][k].im;
-            }
+            double re, im;
+            dense_element(a, sw, i, c, re, im);
             row[2 * c] = re;
             row[2 * c + 1] = im;
         }
     }
 }
 
+// the coupling weights of every node into W (P, 48): real parts of the K
+// components, then (at +24) their imaginary parts
+__global__ __launch_bounds__(256) void dense_weights_kernel(DenseArgs a, double* W) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    cplx sw[4][15];
+    dense_weights(a, p, sw);
+    double* o = W + p * RMAP_W;
+    if (a.mode == 0) {
+        for (int t = 0; t < 4; t++)
+            for (int k = 0; k < 6; k++) {
+                o[t * 6 + k] = sw[t][k].re;
+                o[24 + t * 6 + k] = sw[t][k].im;
+            }
+    } else {
+        for (int k = 0; k < 15; k++) {
+            o[k] = sw[0][k].re;
+            o[24 + k] = sw[0][k].im;
+        }
+    }
+}
+
+// block entries r of NODES consecutive nodes per thread: the K coefficients
+// of entry r in registers, each node's weights scalar loads (the node index
+// is uniform across the workgroup), four coalesced stores per node
+template <int K>
+__global__ __launch_bounds__(256) void reduced_map_kernel(int64_t P, int rn2, int nodes, const double* __restrict__ Cp,
+                                                          const double* __restrict__ Cm, const double* __restrict__ W,
+                                                          double* Mrr, double* Mri, double* Mir, double* Mii) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rn2) return;
+    double cp[K], cm[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+        cp[c] = Cp[(int64_t)c * rn2 + r];
+        cm[c] = Cm[(int64_t)c * rn2 + r];
+    }
+    const int64_t p0 = (int64_t)blockIdx.y * nodes;
+    const int64_t p1 = p0 + nodes < P ? p0 + nodes : P;
+#pragma unroll 4
+    for (int64_t p = p0; p < p1; p++) {
+        const double* w = W + p * RMAP_W;
+        double rr = 0.0, ir = 0.0, ii = 0.0, ri = 0.0;
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            const double wr = w[c], wi = w[24 + c];
+            rr += cp[c] * wr;
+            ir += cp[c] * wi;
+            ii += cm[c] * wr;
+            ri += cm[c] * wi;
+        }
+        const int64_t t = p * rn2 + r;
+        Mrr[t] = rr;
+        Mri[t] = -ri;
+        Mir[t] = ir;
+        Mii[t] = ii;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 #ifndef SFG_HOST_EMUL
+
+hipError_t launch_dense_weights(const DenseArgs& a, double* W, hipStream_t s) {
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(dense_weights_kernel, dim3((unsigned)((a.P + 255) / 256)), dim3(256), 0, s, a, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduced_map(int64_t P, int rn2, int K, const double* Cp, const double* Cm, const double* W,
+                              double* Mrr, double* Mri, double* Mir, double* Mii, hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    int64_t nodes = 16;
+    if ((P + nodes - 1) / nodes > 65535) nodes = (P + 65534) / 65535;   // grid rows at most 65535
+    const dim3 g((unsigned)((rn2 + 255) / 256), (unsigned)((P + nodes - 1) / nodes)), b(256);
+    if (K == 24)
+        hipLaunchKernelGGL(reduced_map_kernel<24>, g, b, 0, s, P, rn2, (int)nodes, Cp, Cm, W, Mrr, Mri, Mir, Mii);
+    else if (K == 15)
+        hipLaunchKernelGGL(reduced_map_kernel<15>, g, b, 0, s, P, rn2, (int)nodes, Cp, Cm, W, Mrr, Mri, Mir, Mii);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 static inline unsigned nblocks(int64_t work, int bs) { return (unsigned)((work + bs - 1) / bs); }
 

@@ -72,6 +72,44 @@ def test_eulerian_export_chain(gpu_ctx, oracle):
             assert relerr(redd[i][p], wantd[i]) < 1e-13
 
 
+@pytest.mark.parametrize("L", [4, 12, 20])
+def test_fused_export_equals_chain(gpu_ctx, oracle, L):
+    """M_LROT_reduced_batch / M_DDRX_src_reduced_batch (the reduced blocks as
+    a linear map of each node's coupling weights, the full matrices never
+    formed) equal the two-kernel chain to 1e-14 relative to the node's
+    largest entry (the same sums in another order), and the oracle's
+    reduce_M of its operators to 1e-13"""
+    import torch
+    from specfab_amd import batch as B
+    print("fused export L=%d" % L, flush=True)
+    P = 300
+    rng = np.random.default_rng(70 + L)
+    eps = rng.standard_normal((P, 3, 3))
+    eps = eps + eps.transpose(0, 2, 1)
+    omg = rng.standard_normal((P, 3, 3))
+    omg = omg - omg.transpose(0, 2, 1)
+    E, W = torch.from_numpy(eps).cuda(), torch.from_numpy(omg).cuda()
+    def same(c, f, tag):
+        c, f = c.cpu().numpy(), f.cpu().numpy()
+        scale = np.abs(c).reshape(len(c), -1).max(axis=1)[:, None, None]
+        assert (np.abs(f - c) <= 1e-14 * scale).all(), tag
+
+    for zeta in (0.0, 0.4):
+        chain = B.reduce_M_batch(L, B.M_LROT_batch(L, E, W, 1.0, zeta))
+        fused = B.M_LROT_reduced_batch(L, E, W, 1.0, zeta)
+        for i in range(4):
+            same(chain[i], fused[i], (zeta, NAMES[i]))
+    chain = B.reduce_M_batch(L, B.M_DDRX_src_batch(L, E))
+    fused = B.M_DDRX_src_reduced_batch(L, E)
+    for i in range(4):
+        same(chain[i], fused[i], NAMES[i])
+    fl = [t.cpu().numpy() for t in B.M_LROT_reduced_batch(L, E, W, 1.0, 0.0)]
+    for p in (0, 151, P - 1):
+        want = O.reduce_M(oracle.M_LROT(L, eps[p], omg[p], 1.0, 0.0), L)
+        for i in range(4):
+            assert relerr(fl[i][p], want[i]) < 1e-13
+
+
 def test_round_trip_field_size(gpu_ctx):
     """rnlm -> nlm -> rnlm is the identity, and the rebuilt nlm satisfies
     n_l^{-m} = (-1)^m conj(n_l^m), over 2^18 nodes at L = 12"""
