@@ -94,7 +94,7 @@ def compile_fortran():
     exe = os.path.join(out, "c1_parcel")
     run([fc, "-O2", "-J", out, "-c", os.path.join(HERE, "fortran", "specfab_gpu.f90"), "-o",
          os.path.join(out, "specfab_gpu.o")])
-    for prog in ("c1_parcel", "homog_faces", "discrete_faces", "ortho_faces"):
+    for prog in ("c1_parcel", "homog_faces", "discrete_faces", "ortho_faces", "faces"):
         run([fc, "-O2", "-I", out, os.path.join(os.path.dirname(HERE), "tests", "fortran", prog + ".f90"),
              os.path.join(out, "specfab_gpu.o"), "-L", HERE, "-lspecfab_gpu", "-L", rocm_lib, "-lamdhip64",
              "-Wl,-rpath," + HERE, "-Wl,-rpath," + rocm_lib, "-o", os.path.join(out, prog)])

@@ -239,6 +239,43 @@ module specfab_gpu
             real(c_double) :: b(*), nx(*), v(*), Lame(9), th(*), ph(*), Vi(*)
             real(c_double), value :: alpha, rho
         end function
+        integer(c_int) function c_Evw(c, L, nlm, v, w, tau, Eg, alpha, ng, E) bind(C, name="sfg_Evw_tranisotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ng
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: v(3), w(3), tau(9), Eg(2), E
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_Evw_ortho(c, L, q1, q2, q3, v, w, tau, Eg, alpha, ng, E) &
+                bind(C, name="sfg_Evw_orthotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L, ng
+            complex(c_double_complex) :: q1(*), q2(*), q3(*)
+            real(c_double) :: v(3), w(3), tau(9), Eg(6), E
+            real(c_double), value :: alpha
+        end function
+        integer(c_int) function c_eig3(c, M, ei, lami) bind(C, name="sfg_eig3")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            real(c_double) :: M(9), ei(9), lami(3)
+        end function
+        integer(c_int) function c_Qnorm(c, L, nlm, alpha, Lame, Q) bind(C, name="sfg_Qnorm_tranisotropic")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*)
+            real(c_double), value :: alpha
+            real(c_double) :: Lame(5), Q(9)
+        end function
+        integer(c_int) function c_moments68(c, L, nlm, m6, m8) bind(C, name="sfg_moments68")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: c
+            integer(c_int), value :: L
+            complex(c_double_complex) :: nlm(*)
+            real(c_double) :: m6(28), m8(45)
+        end function
         integer(c_int) function c_Cij_to_Lame_ortho(Cij, Lame) bind(C, name="sfg_Cij_to_Lame_orthotropic")
             import :: c_int, c_double
             real(c_double) :: Cij(9), Lame(9)
@@ -442,6 +479,77 @@ contains
         lm = Lame_grain; th = theta; ph = phi
         call check(c_Vi_ortho_disc(ctx, int(size(mi,3), c_int64_t), b, nx, v, alpha, lm, rho, int(size(theta), c_int), &
                                    th, ph, Vi), 'Vi_elastic_orthotropic__discrete')
+    end function
+
+    ! Evw_tranisotropic(v, w, tau, nlm, Eij_grain, alpha, n_grain): src/enhancementfactors.f90:47-69
+    ! (the module function's argument order; tau symmetric)
+    function Evw_tranisotropic(v, w, tau, nlm, Eij_grain, alpha, n_grain) result(Evw)
+        real(dp), intent(in) :: v(3), w(3), tau(3,3), Eij_grain(2), alpha
+        complex(dp), intent(in) :: nlm(:)
+        integer, intent(in) :: n_grain
+        real(dp) :: Evw, a(3), b(3), t(9), Eg(2)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len); a = v; b = w; Eg = Eij_grain
+        t = reshape(transpose(tau), [9])
+        call check(c_Evw(ctx, int(Lcap, c_int), z, a, b, t, Eg, alpha, int(n_grain, c_int), Evw), 'Evw_tranisotropic')
+    end function
+
+    ! Evw_orthotropic(v, w, tau, nlm_r1, nlm_r2, nlm_r3, Eij_grain, alpha, n_grain):
+    ! src/enhancementfactors.f90:134-160
+    function Evw_orthotropic(v, w, tau, nlm_r1, nlm_r2, nlm_r3, Eij_grain, alpha, n_grain) result(Evw)
+        real(dp), intent(in) :: v(3), w(3), tau(3,3), Eij_grain(6), alpha
+        complex(dp), intent(in) :: nlm_r1(:), nlm_r2(:), nlm_r3(:)
+        integer, intent(in) :: n_grain
+        real(dp) :: Evw, a(3), b(3), t(9), Eg(6)
+        complex(dp) :: z1(nlm_len), z2(nlm_len), z3(nlm_len)
+        z1 = nlm_r1(1:nlm_len); z2 = nlm_r2(1:nlm_len); z3 = nlm_r3(1:nlm_len)
+        a = v; b = w; Eg = Eij_grain
+        t = reshape(transpose(tau), [9])
+        call check(c_Evw_ortho(ctx, int(Lcap, c_int), z1, z2, z3, a, b, t, Eg, alpha, int(n_grain, c_int), Evw), &
+                   'Evw_orthotropic')
+    end function
+
+    ! eig3(M, e1, e2, e3, lami): src/frames.f90:62-80 (largest eigenvalue first)
+    subroutine eig3(M, e1, e2, e3, lami)
+        real(dp), intent(in) :: M(3,3)
+        real(dp), intent(out) :: e1(3), e2(3), e3(3), lami(3)
+        real(c_double) :: a(9), v(9)
+        a = reshape(transpose(M), [9])
+        call check(c_eig3(ctx, a, v, lami), 'eig3')
+        e1 = v(1:3); e2 = v(4:6); e3 = v(7:9)
+    end subroutine
+
+    ! Qnorm_tranisotropic(nlm, alpha, Lame_grain): src/wavepropagation.f90:146-211
+    function Qnorm_tranisotropic(nlm, alpha, Lame_grain) result(Q)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp), intent(in) :: alpha, Lame_grain(5)
+        real(dp) :: Q(3,3), lm(5)
+        real(c_double) :: v(9)
+        complex(dp) :: z(nlm_len)
+        z = nlm(1:nlm_len); lm = Lame_grain
+        call check(c_Qnorm(ctx, int(Lcap, c_int), z, alpha, lm, v), 'Qnorm_tranisotropic')
+        Q = transpose(reshape(v, [3,3]))
+    end function
+
+    ! a6(nlm): src/moments.f90:57-66 (exact sixth moments; L >= 6)
+    function a6(nlm) result(A)
+        complex(dp), intent(in) :: nlm(:)
+        real(dp) :: A(3,3,3,3,3,3)
+        real(c_double) :: m6(28), m8(45)
+        complex(dp) :: z(nlm_len)
+        integer :: i1, i2, i3, i4, i5, i6, nx, ny, k, ia
+        z = nlm(1:nlm_len)
+        call check(c_moments68(ctx, int(Lcap, c_int), z, m6, m8), 'a6')
+        ! monomial <x^a y^b z^c> at position sum_{a' > a} (7 - a') + (6 - a - b) (order a = 6..0, b = 6 - a..0)
+        do i1 = 1, 3; do i2 = 1, 3; do i3 = 1, 3; do i4 = 1, 3; do i5 = 1, 3; do i6 = 1, 3
+            nx = count([i1, i2, i3, i4, i5, i6] == 1)
+            ny = count([i1, i2, i3, i4, i5, i6] == 2)
+            k = 0
+            do ia = nx + 1, 6
+                k = k + (7 - ia)
+            end do
+            A(i1, i2, i3, i4, i5, i6) = m6(k + (6 - nx - ny) + 1)
+        end do; end do; end do; end do; end do; end do
     end function
 
     ! Cij_to_Lame_orthotropic(Cij): src/elasticities.f90:122-139
