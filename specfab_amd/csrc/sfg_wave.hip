@@ -683,15 +683,6 @@ __host__ __device__ __forceinline__ double q6_cart(const double (&Q)[36], int s)
     return (Q[6 * s + 0] + Q[6 * s + 1] + Q[6 * s + 2]) * (1.0 / 3.0) - Q[7 * s];
 }
 
-// the same pointer, opaque to the optimiser (a second load through it is not
-// merged with an earlier one)
-__device__ __forceinline__ const double* opaque_ptr(const double* ptr) {
-#ifndef SFG_HOST_EMUL
-    asm volatile("" : "+s"(ptr));
-#endif
-    return ptr;
-}
-
 __device__ __forceinline__ void load_l4(const double* __restrict__ x, int64_t P, int64_t p, int nq, double (&xr)[15]) {
 #pragma unroll
     for (int r = 0; r < 15; r++) xr[r] = r < nq ? x[(int64_t)r * P + p] : 0.0;
@@ -727,8 +718,13 @@ __device__ __forceinline__ void ortho_q6_parcel(const OrthoArgs& a, int64_t p, d
         for (int t = 0; t < 36; t++) Q[t] = 0.0;
         double xr[15];
         ODist d0, d2;
+        // q1's block kept in LDS for its second evaluation (one 15-row column
+        // per lane: read back only by the lane that wrote it, so no barrier)
+        __shared__ double q1s[15][256];
         {
             load_l4(a.x1, a.P, p, a.nq, xr);
+#pragma unroll
+            for (int r = 0; r < 15; r++) q1s[r][threadIdx.x] = xr[r];
             odist(xr, d0);
             ODist d1;
             load_l4(a.x2, a.P, p, a.nq, xr);
@@ -742,9 +738,12 @@ __device__ __forceinline__ void ortho_q6_parcel(const OrthoArgs& a, int64_t p, d
             dq += w2 * quirk(d2);
             q6_add_joint(Q, ci[0], ci[3], 1.0 / (1.0 - a2dot(d1, d2)), JointF{d1, d2});
         }
-        // q1 evaluated again from a pointer the compiler cannot equate with the
-        // first, so that the first evaluation need not stay live
-        load_l4(opaque_ptr(a.x1), a.P, p, a.nq, xr);
+        // q1 evaluated again from LDS; the compiler barrier keeps it from
+        // forwarding the stored registers, so the first evaluation need not
+        // stay live
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int r = 0; r < 15; r++) xr[r] = q1s[r][threadIdx.x];
         odist(xr, d0);
         q6_add_joint(Q, ci[1], ci[4], 1.0 / (1.0 - a2dot(d0, d2)), JointF{d0, d2});
     } else {
@@ -859,18 +858,34 @@ __device__ __forceinline__ void a4_zz(const ODist& d, double (&zz)[6]) {
     for (int P = 0; P < 6; P++) zz[P] = d.u[kA4Slots.s[27 * pa(P) + 9 * pb(P) + 3 * 2 + 2]];
 }
 
+// Workgroup w of a one-dimensional grid of n is dispatched to XCD w % 8
+// (round robin over the eight XCDs); its index in an order that gives each
+// XCD one contiguous run of [0, n), in dispatch order.  A bijection for any
+// n, so the mapping is correct whatever the dispatcher does; it only decides
+// which workgroups share an XCD's L2.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t w, int64_t n) {
+    const int64_t q = n >> 3, r = n & 7, x = w & 7;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (w >> 3);
+}
+
 // elastic Vi of orthotropic grains (Voigt; src/wavepropagation.f90:41-68,
-// 98-115): one lane per parcel, one direction per grid row (blockIdx.y); the
+// 98-115): one lane per parcel, one direction per workgroup; the
 // distributions rotated as rotate_nlm4 rotates them, the acoustic tensor for
 // k = z
 //   Qn = sum_m mu_m/2 (a2 + a2.kk + kk.a2 + (a2:kk) I) + lam_m a4_m : kk
 //        + lam_m+3 (2 sym2(J_m)) first-last : kk
-// and the reference's Cardano roots.  MODE as for eij_ortho_kernel.
+// and the reference's Cardano roots.  The grid is (parcel block, direction)
+// with the direction fastest, laid out so that the ndir workgroups of one
+// parcel block run on one XCD one after another: every direction after the
+// first reads the parcels' blocks from that XCD's L2.  MODE as for
+// eij_ortho_kernel.
 template <int MODE>
 __global__ void __launch_bounds__(256) vi_ortho_kernel(OrthoArgs a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lw = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int64_t blk = lw / a.ndir;
+    const int d = (int)(lw - blk * a.ndir);
+    const int64_t p = blk * blockDim.x + threadIdx.x;
     if (p >= a.P) return;
-    const int d = (int)blockIdx.y;
     const double* __restrict__ R = a.R + (int64_t)d * ROT4_STRIDE;
     const bool given = MODE == 0 && a.x3[p] > 1e-8;   // n00 is invariant under rotation
     double x[3][15];
@@ -1136,8 +1151,9 @@ hipError_t launch_evw_ortho(const OrthoArgs& args, hipStream_t s) {
 
 hipError_t launch_vi_ortho(const OrthoArgs& a, hipStream_t s) {
     if (a.P * a.ndir == 0) return hipSuccess;
-    if (a.ndir > 65535) return hipErrorInvalidValue;   // one grid row per direction
-    const dim3 g(nblk(a.P, 256), (unsigned)a.ndir), b(256);
+    const int64_t nw = (int64_t)nblk(a.P, 256) * a.ndir;   // (parcel block, direction) workgroups
+    if (nw > 0x7fffffff) return hipErrorInvalidValue;
+    const dim3 g((unsigned)nw), b(256);
     if (a.x3) hipLaunchKernelGGL((vi_ortho_kernel<0>), g, b, 0, s, a);
     else hipLaunchKernelGGL((vi_ortho_kernel<1>), g, b, 0, s, a);
     return hipGetLastError();

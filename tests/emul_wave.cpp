@@ -133,10 +133,9 @@ extern "C" int emul_vi_ortho(int L, int64_t P, const double* x1, const double* x
     a.P = P; a.nq = t.nq; a.ndir = ndir; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.R = t.R.data(); a.rho = rho; a.Vi = Vi;
     for (int i = 0; i < 6; i++) a.lam[i] = Lame[i];
     for (int i = 0; i < 3; i++) a.mu[i] = Lame[6 + i];
-    for (int d = 0; d < ndir; d++) {
-        // grid row d: the kernel reads blockIdx.y as its direction
-        if (x3) grid(P, 256, [&] { blockIdx.y = (unsigned)d; sfg::vi_ortho_kernel<0>(a); });
-        else grid(P, 256, [&] { blockIdx.y = (unsigned)d; sfg::vi_ortho_kernel<1>(a); });
-    }
+    // one workgroup per (parcel block, direction): the kernel derives both from blockIdx.x
+    const int64_t nw = (P + 255) / 256 * ndir;
+    if (x3) grid(nw * 256, 256, [&] { sfg::vi_ortho_kernel<0>(a); });
+    else grid(nw * 256, 256, [&] { sfg::vi_ortho_kernel<1>(a); });
     return 0;
 }
