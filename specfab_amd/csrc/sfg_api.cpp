@@ -98,8 +98,8 @@ struct sfg_ctx {
     int4* csr_ent = nullptr;
     double* csr_val = nullptr;
     int2* csr_seg = nullptr;
-    // reduced-block coefficient maps per (L, mode): Cp then Cm, each (K, rn^2)
-    std::map<std::pair<int, int>, double*> rmap;
+    // reduced-block coefficient maps per (L, mode), ELL tables (sfg::RMap)
+    std::map<std::pair<int, int>, sfg::RMap> rmap;
     // workspace, shared by all calls on this context: every user waits for
     // the previous user's stream through ws_event before touching it
     void* ws = nullptr;
@@ -405,7 +405,12 @@ int sfg_ctx_destroy(sfg_ctx* ctx) {
     (void)hipFree(ctx->csr_ent);
     (void)hipFree(ctx->csr_val);
     (void)hipFree(ctx->csr_seg);
-    for (auto& kv : ctx->rmap) (void)hipFree(kv.second);
+    for (auto& kv : ctx->rmap) {
+        (void)hipFree((void*)kv.second.ci);
+        (void)hipFree((void*)kv.second.cp);
+        (void)hipFree((void*)kv.second.cm);
+        (void)hipFree((void*)kv.second.nt);
+    }
     if (ctx->ws) (void)hipFree(ctx->ws);
     for (auto& ds : ctx->dirs) (void)hipFree(ds.R);
     if (ctx->wigner) (void)hipFree(ctx->wigner);
@@ -785,12 +790,12 @@ int sfg_M_LROT_batch(sfg_ctx* ctx, int L, int64_t P, const double* eps_dev, cons
 // the coefficient maps Cp, Cm (K, rn^2) of the reduced blocks of mode's
 // operator at truncation L (see launch_reduced_map), from the coupling-table
 // entries: A(i, c)[comp] = sum of the values of the entries of element (i, c)
-// with that weight component
-static int get_rmap(sfg_ctx* ctx, int L, int mode, const double** out) {
+// with that weight component; kept as the ELL table of their nonzero terms
+static int get_rmap(sfg_ctx* ctx, int L, int mode, const sfg::RMap** out) {
     auto key = std::make_pair(L, mode);
     auto it = ctx->rmap.find(key);
     if (it != ctx->rmap.end()) {
-        *out = it->second;
+        *out = &it->second;
         return SFG_OK;
     }
     const int K = mode == 0 ? 24 : 15, kmax = mode == 0 ? 6 : 15;
@@ -827,10 +832,35 @@ static int get_rmap(sfg_ctx* ctx, int L, int mode, const double** out) {
             }
         }
     }
-    double* d = nullptr;
-    HIPCHK(upload(&d, C));
-    ctx->rmap[key] = d;
-    *out = d;
+    // ELL: entry r's components c with Cp or Cm nonzero, in increasing c
+    const int S = mode == 0 ? 10 : 5, nw = (rn2 + 63) / 64;
+    std::vector<int> ci((size_t)S * rn2, 0), nt(nw, 0);
+    std::vector<double> cp((size_t)S * rn2, 0.0), cm((size_t)S * rn2, 0.0);
+    for (int r = 0; r < rn2; r++) {
+        int k = 0;
+        for (int c = 0; c < K; c++) {
+            const double vp = C[(size_t)c * rn2 + r], vm = C[(size_t)(K + c) * rn2 + r];
+            if (vp == 0.0 && vm == 0.0) continue;
+            if (k == S) return fail(SFG_ERR_L, "reduced map: more nonzero terms per entry than slots");
+            ci[(size_t)k * rn2 + r] = c;
+            cp[(size_t)k * rn2 + r] = vp;
+            cm[(size_t)k * rn2 + r] = vm;
+            k++;
+        }
+        nt[r >> 6] = std::max(nt[r >> 6], k);
+    }
+    sfg::RMap m{};
+    m.rn2 = rn2;
+    m.S = S;
+    int *dci = nullptr, *dnt = nullptr;
+    double *dcp = nullptr, *dcm = nullptr;
+    HIPCHK(upload(&dci, ci));
+    HIPCHK(upload(&dcp, cp));
+    HIPCHK(upload(&dcm, cm));
+    HIPCHK(upload(&dnt, nt));
+    m.ci = dci; m.cp = dcp; m.cm = dcm; m.nt = dnt;
+    auto ins = ctx->rmap.emplace(key, m).first;
+    *out = &ins->second;
     return SFG_OK;
 }
 
@@ -849,8 +879,8 @@ static int dense_reduced_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const d
     rc = ensure_csr(ctx);
     if (rc) return rc;
     HIPCHK(hipSetDevice(ctx->device));
-    const double* C;
-    rc = get_rmap(ctx, L, mode, &C);
+    const sfg::RMap* rm;
+    rc = get_rmap(ctx, L, mode, &rm);
     if (rc) return rc;
     sfg::DenseArgs a{};
     a.P = P; a.n = op->n; a.mode = mode; a.iota = iota; a.zeta = zeta;
@@ -860,8 +890,7 @@ static int dense_reduced_batch(sfg_ctx* ctx, int L, int64_t P, int mode, const d
     rc = lease.acquire((size_t)P * sfg::RMAP_W * sizeof(double), &ws);
     if (rc) return rc;
     HIPCHK(sfg::launch_dense_weights(a, (double*)ws, s));
-    const int rn2 = sfg::rnlm_len(L) * sfg::rnlm_len(L), K = mode == 0 ? 24 : 15;
-    HIPCHK(sfg::launch_reduced_map(P, rn2, K, C, C + (size_t)K * rn2, (const double*)ws, Mrr, Mri, Mir, Mii, s));
+    HIPCHK(sfg::launch_reduced_map(P, *rm, (const double*)ws, Mrr, Mri, Mir, Mii, s));
     return lease.release();
 }
 

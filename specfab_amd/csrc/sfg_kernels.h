@@ -119,10 +119,24 @@ hipError_t launch_params(const ParamArgs& a, hipStream_t s);
 // is sum_c C[c] Re w_c or Im w_c, C = sg (A+ +- s A-) over the K weight
 // components c (LROT: t * 6 + k, K = 24; DDRX: k, K = 15).  Cp, Cm (K,
 // rn^2); W (P, 48) the weights' real parts then imaginary parts (stride 24).
+// Most (c, entry) coefficients are structural zeros (M_LROT at L = 12: 2,391
+// of 57,624 nonzero, at most 10 per entry for every L; M_DDRX_src at most 5),
+// so the map is stored as an ELL table: entry r's nonzero terms in
+// increasing c fill slots s < S (slot-major, ci / cp / cm[s * rn2 + r];
+// unused slots have c = 0 and zero coefficients), and nt[r / 64] is the
+// largest term count of the 64 entries of a wave.
 constexpr int RMAP_W = 48;
+constexpr int RMAP_NODES = 16;   // nodes per workgroup chunk (their weights staged in LDS)
+struct RMap {
+    int rn2, S;                  // S: slots per entry (10 LROT, 5 DDRX)
+    const int* ci;               // (S, rn2) weight component of each slot
+    const double* cp;            // (S, rn2) coefficient of Re w (Mrr) and Im w (Mir)
+    const double* cm;            // (S, rn2) coefficient of Re w (Mii) and -Im w (Mri)
+    const int* nt;               // (ceil(rn2 / 64)) trips per wave
+};
 hipError_t launch_dense_weights(const DenseArgs& a, double* W, hipStream_t s);
-hipError_t launch_reduced_map(int64_t P, int rn2, int K, const double* Cp, const double* Cm, const double* W,
-                              double* Mrr, double* Mri, double* Mir, double* Mii, hipStream_t s);
+hipError_t launch_reduced_map(int64_t P, const RMap& m, const double* W, double* Mrr, double* Mri, double* Mir,
+                              double* Mii, hipStream_t s);
 hipError_t launch_evolve_generic(const EvolveArgs& a, hipStream_t s);
 hipError_t launch_nlm_to_state(int64_t P, int n, const int* r2c, const int* rpart, const double* nlm, double* x,
                                hipStream_t s);

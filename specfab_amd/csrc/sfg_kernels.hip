@@ -389,40 +389,59 @@ __global__ __launch_bounds__(256) void dense_weights_kernel(DenseArgs a, double*
     }
 }
 
-// block entries r of NODES consecutive nodes per thread: the K coefficients
-// of entry r in registers, each node's weights scalar loads (the node index
-// is uniform across the workgroup), four coalesced stores per node
-template <int K>
-__global__ __launch_bounds__(256) void reduced_map_kernel(int64_t P, int rn2, int nodes, const double* __restrict__ Cp,
-                                                          const double* __restrict__ Cm, const double* __restrict__ W,
+// block entries r of the nodes of RMAP_NODES-node chunks, one entry per
+// thread: entry r's S slots (weight component, two coefficients) in
+// registers, the chunk's weights staged in LDS, per node nt (wave-uniform)
+// terms and four coalesced stores.  The terms skipped are exact zeros and
+// the rest keep their order, so each sum equals the dense sum over all K
+// components.
+template <int S>
+__global__ __launch_bounds__(256) void reduced_map_kernel(int64_t P, RMap m, const double* __restrict__ W,
                                                           double* Mrr, double* Mri, double* Mir, double* Mii) {
+    __shared__ double ws[RMAP_NODES * RMAP_W];
+    const int rn2 = m.rn2;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rn2) return;
-    double cp[K], cm[K];
+    const bool on = r < rn2;
+    // waves past the last entry have no trips; a partial last wave reads its
+    // own (in-range) count
+    const int nt = (r >> 6) <= (rn2 - 1) >> 6 ? m.nt[r >> 6] : 0;
+    int ci[S];
+    double cp[S], cm[S];
 #pragma unroll
-    for (int c = 0; c < K; c++) {
-        cp[c] = Cp[(int64_t)c * rn2 + r];
-        cm[c] = Cm[(int64_t)c * rn2 + r];
+    for (int s = 0; s < S; s++) {
+        const bool use = on && s < nt;
+        ci[s] = use ? m.ci[s * rn2 + r] : 0;
+        cp[s] = use ? m.cp[s * rn2 + r] : 0.0;
+        cm[s] = use ? m.cm[s * rn2 + r] : 0.0;
     }
-    const int64_t p0 = (int64_t)blockIdx.y * nodes;
-    const int64_t p1 = p0 + nodes < P ? p0 + nodes : P;
-#pragma unroll 4
-    for (int64_t p = p0; p < p1; p++) {
-        const double* w = W + p * RMAP_W;
-        double rr = 0.0, ir = 0.0, ii = 0.0, ri = 0.0;
+    const int64_t nchunk = (P + RMAP_NODES - 1) / RMAP_NODES;
+    for (int64_t ch = blockIdx.y; ch < nchunk; ch += gridDim.y) {
+        const int64_t p0 = ch * RMAP_NODES;
+        const int np = (int)(P - p0 < RMAP_NODES ? P - p0 : RMAP_NODES);
+        __syncthreads();   // the previous chunk's weights are no longer read
+        for (int i = threadIdx.x; i < np * RMAP_W; i += blockDim.x) ws[i] = W[p0 * RMAP_W + i];
+        __syncthreads();
+        for (int q = 0; q < np; q++) {
+            const double* w = ws + q * RMAP_W;
+            double rr = 0.0, ir = 0.0, ii = 0.0, ri = 0.0;
 #pragma unroll
-        for (int c = 0; c < K; c++) {
-            const double wr = w[c], wi = w[24 + c];
-            rr += cp[c] * wr;
-            ir += cp[c] * wi;
-            ii += cm[c] * wr;
-            ri += cm[c] * wi;
+            for (int s = 0; s < S; s++) {
+                if (s < nt) {
+                    const double wr = w[ci[s]], wi = w[24 + ci[s]];
+                    rr += cp[s] * wr;
+                    ir += cp[s] * wi;
+                    ii += cm[s] * wr;
+                    ri += cm[s] * wi;
+                }
+            }
+            if (on) {
+                const int64_t t = (p0 + q) * rn2 + r;
+                Mrr[t] = rr;
+                Mri[t] = -ri;
+                Mir[t] = ir;
+                Mii[t] = ii;
+            }
         }
-        const int64_t t = p * rn2 + r;
-        Mrr[t] = rr;
-        Mri[t] = -ri;
-        Mir[t] = ir;
-        Mii[t] = ii;
     }
 }
 
@@ -435,16 +454,15 @@ hipError_t launch_dense_weights(const DenseArgs& a, double* W, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_reduced_map(int64_t P, int rn2, int K, const double* Cp, const double* Cm, const double* W,
-                              double* Mrr, double* Mri, double* Mir, double* Mii, hipStream_t s) {
+hipError_t launch_reduced_map(int64_t P, const RMap& m, const double* W, double* Mrr, double* Mri, double* Mir,
+                              double* Mii, hipStream_t s) {
     if (P == 0) return hipSuccess;
-    int64_t nodes = 16;
-    if ((P + nodes - 1) / nodes > 65535) nodes = (P + 65534) / 65535;   // grid rows at most 65535
-    const dim3 g((unsigned)((rn2 + 255) / 256), (unsigned)((P + nodes - 1) / nodes)), b(256);
-    if (K == 24)
-        hipLaunchKernelGGL(reduced_map_kernel<24>, g, b, 0, s, P, rn2, (int)nodes, Cp, Cm, W, Mrr, Mri, Mir, Mii);
-    else if (K == 15)
-        hipLaunchKernelGGL(reduced_map_kernel<15>, g, b, 0, s, P, rn2, (int)nodes, Cp, Cm, W, Mrr, Mri, Mir, Mii);
+    const int64_t nchunk = (P + RMAP_NODES - 1) / RMAP_NODES;
+    const dim3 g((unsigned)((m.rn2 + 255) / 256), (unsigned)(nchunk < 65535 ? nchunk : 65535)), b(256);
+    if (m.S == 10)
+        hipLaunchKernelGGL(reduced_map_kernel<10>, g, b, 0, s, P, m, W, Mrr, Mri, Mir, Mii);
+    else if (m.S == 5)
+        hipLaunchKernelGGL(reduced_map_kernel<5>, g, b, 0, s, P, m, W, Mrr, Mri, Mir, Mii);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -110,6 +110,32 @@ def test_fused_export_equals_chain(gpu_ctx, oracle, L):
             assert relerr(fl[i][p], want[i]) < 1e-13
 
 
+def test_fused_export_many_chunks(gpu_ctx):
+    """more 16-node chunks than grid rows (16 x 65535 + 37 nodes at L = 2):
+    workgroups loop over several chunks, the last one partial; equal to the
+    chain to 1e-14 of each node's largest entry"""
+    import torch
+    from specfab_amd import batch as B
+    print("fused export, 16 x 65535 + 37 nodes", flush=True)
+    P = 16 * 65535 + 37
+    g = torch.Generator(device="cuda").manual_seed(5)
+    E = torch.randn((P, 3, 3), device="cuda", dtype=torch.float64, generator=g)
+    E = E + E.transpose(1, 2)
+    W = torch.randn((P, 3, 3), device="cuda", dtype=torch.float64, generator=g)
+    W = W - W.transpose(1, 2)
+    for tag, chain, fused in (("LROT", lambda: B.reduce_M_batch(2, B.M_LROT_batch(2, E, W, 1.0, 0.3)),
+                               lambda: B.M_LROT_reduced_batch(2, E, W, 1.0, 0.3)),
+                              ("DDRX", lambda: B.reduce_M_batch(2, B.M_DDRX_src_batch(2, E)),
+                               lambda: B.M_DDRX_src_reduced_batch(2, E))):
+        c, f = chain(), fused()
+        for i in range(4):
+            scale = c[i].abs().reshape(P, -1).amax(dim=1)[:, None, None]
+            assert bool(((f[i] - c[i]).abs() <= 1e-14 * scale).all()), (tag, NAMES[i])
+        assert any(bool(f[i][-1].abs().max() > 0) for i in range(4)), tag   # the last node written
+        del c, f
+        torch.cuda.synchronize()
+
+
 def test_round_trip_field_size(gpu_ctx):
     """rnlm -> nlm -> rnlm is the identity, and the rebuilt nlm satisfies
     n_l^{-m} = (-1)^m conj(n_l^m), over 2^18 nodes at L = 12"""
